@@ -1,0 +1,26 @@
+"""CPU oracle for the CCD operator and the variable-density FVM PPE.
+
+TEST INFRASTRUCTURE ONLY.  This package is a NumPy/SciPy restatement of the
+reference algorithms, used as the checker by ``tests/``, by
+``__graft_entry__.smoke()`` and by the ``cpu_baseline`` leg of ``bench.py``.
+The product path (``twophaseflow_amd``) never imports it and fails loudly when
+its HIP library is missing.
+
+Each function cites the reference file:line it restates (paths relative to
+``src/twophase`` of t0m0h1r0/TwoPhaseFlow).  Pinning (what checks each
+function against something other than its own formula):
+
+* ``ccd.differentiate_periodic``  -- closed-form Fourier symbol
+  (``ccd.periodic_symbol``), paper table U1-a, ``test_ccd.py:65-81``.
+* ``ccd.differentiate_wall``      -- polynomial exactness (x^0..x^3),
+  ``test_ccd.py:201-221``, D2 null space (``test_ccd.py:257-299``),
+  convergence slopes, block-Thomas vs dense LU.
+* ``ppe.fvm_apply``               -- assembled CSR twin, row sums, symmetry of
+  the CG system, manufactured 1-D exact answers.
+* ``ppe.solve_cg`` / ``ppe.solve_direct`` -- each other, paper table U2-b,
+  and the FP64 true residual of the reference's CG system.
+* ``fields``                       -- closed forms (sphere volumes, ψ bounds).
+
+No function here is "parity unpinned" except the GMRES/CG iteration counts,
+which the reference does not fix (SciPy internals).
+"""
