@@ -1,0 +1,238 @@
+"""Benchmark of the MI355X CCD operator and variable-density PPE.
+
+Workload (one step): the CCD operator, f' and f'' along every axis of a 3-D
+257^3-node periodic band-limited field (|k| <= 32, amplitude |k|^-2, seed 2)
+in FP64, one launch per axis with all lines of that axis.  `value` is
+Mpts/s = grid points x axes / second over the timed steps, inputs resident in
+HBM.  Also reported on the same line:
+
+* roofline of the dominant CCD kernel (strided-axis line solve): algorithmic
+  bytes per launch (read f, write f' and f'': 24 B per point) over its mean
+  launch time from HIP events on the launch stream;
+* 2-D 1025^2 periodic all-axes CCD throughput;
+* the 257^3 density-ratio-1000 PPE (64 gas spheres, r in [0.03, 0.08],
+  eps = 1.5h, rhs = CCD divergence of a seeded smooth field): wall time to
+  ||r||/||b|| <= 1e-8 for plain FP64 Jacobi-PCG and for FP32-inner/FP64-
+  refined PCG, each with its FP64 true residual;
+* cpu_baseline: the reference CPU algorithm (dense LU per axis via SciPy
+  lu_solve, the oracle restatement) on a bounded slab of the same field,
+  timed on this host.
+
+`python bench.py --gpus N --steps K --warmup W`; N > 1 exits: this build is
+single-GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "256^3 PPE time-to-1e-8: learned-FCG vs plain PCG; CCD Mpts/s & HBM %"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_ccd(ccd, f, d1, d2, mask, steps, warmup, ndim):
+    import torch
+    for _ in range(warmup):
+        ccd.apply_into(f, d1, d2, mask)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ccd.apply_into(f, d1, d2, mask)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def per_axis_event_times(ccd, f, d1, d2, reps):
+    """Mean duration of each axis launch from HIP events on torch's stream
+    (the stream the kernels are launched on)."""
+    import torch
+    n = f.numel()
+    out = []
+    for axis in range(f.dim()):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(reps)]
+        for a, b in ev:
+            a.record()
+            ccd.differentiate_into(f, d1[axis], d2[axis], axis)
+            b.record()
+        torch.cuda.synchronize()
+        ms = sum(a.elapsed_time(b) for a, b in ev) / reps
+        out.append({"axis": axis, "ms": ms,
+                    "GBps": 24.0 * n / (ms * 1e-3) / 1e9,
+                    "info": ccd.axis_info(axis)})
+    return out
+
+
+def cpu_baseline_ccd(N=256, slab=16, seed=2):
+    """Reference CPU path (dense LU per axis, scipy lu_solve) on a slab of
+    `slab` planes of the 257^3 periodic field, all three axes."""
+    import numpy as np
+    from oracle import ccd as occd
+    from twophaseflow_amd.fields import band_limited_periodic
+    f = band_limited_periodic(N, 3, kmax=32, seed=seed)[:, :, :slab + 1].copy()
+    h = 1.0 / N
+    infos = [occd.build_periodic_axis(N + 1, h), occd.build_periodic_axis(N + 1, h)]
+    t0 = time.perf_counter()
+    occd.differentiate_periodic(f, 0, infos[0])
+    occd.differentiate_periodic(f, 1, infos[1])
+    t1 = time.perf_counter()
+    pts = f.size
+    # axis 2 of the full field: same per-line cost as axes 0 and 1
+    rate = 2 * pts / (t1 - t0) / 1e6
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        cores = os.cpu_count() or 1
+    threads = int(os.environ.get("OPENBLAS_NUM_THREADS", os.environ.get("OMP_NUM_THREADS", cores)))
+    return {"value": rate, "unit": "Mpts/s", "cores": threads, "kind": "port",
+            "sample": f"dense-LU CCD (scipy lu_solve, oracle restatement) on a 257x257x{slab+1} "
+                      f"slab of the 257^3 periodic field, axes 0 and 1 ({t1 - t0:.1f} s)"}
+
+
+def ppe_case(N=256, nb=64, seed=3, ratio=1e-3):
+    import numpy as np
+    import torch
+    from twophaseflow_amd import CCDSolver, Grid, GridConfig
+    from twophaseflow_amd import fields
+    g = Grid(GridConfig(ndim=3, N=(N, N, N), L=(1.0, 1.0, 1.0)))
+    centres, radii = fields.random_bubbles(nb, seed=seed, ndim=3)
+    psi, rho = fields.bubbles_psi_rho(g.N, g.L, centres, radii, 1.5 / N, 1.0, ratio)
+    ccd = CCDSolver(g, bc_type="wall")
+    vel = []
+    for a in range(3):
+        u = fields.band_limited_periodic(N, 3, kmax=8, seed=4 + a)
+        vel.append(torch.from_numpy(u).cuda())
+    rhs = fields.divergence(ccd, vel)
+    del vel
+    torch.cuda.synchronize()
+    return g, rho, rhs
+
+
+def ppe_solve_timed(g, rho, rhs, precision, tol=1e-8, maxiter=200000):
+    import torch
+    from twophaseflow_amd import PPESolverFVMMatrixFree, SimulationConfig, SolverConfig
+    cfg = SimulationConfig(solver=SolverConfig(pseudo_tol=tol, pseudo_maxiter=maxiter))
+    cfg.grid = None
+    cfg.solver.ppe_iteration_method = "cg"
+    cfg.solver.ppe_precision = precision
+    cfg.solver.ppe_check_every = 64
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    p = torch.empty_like(rhs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    diag = s.solve_into(rhs, rho, p)  # includes operator setup (Jacobi diagonal)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    tr = s.true_relres(rhs, p)
+    return {"seconds": t, "iterations": diag["iterations"], "refinements": diag["refinements"],
+            "true_relres": tr, "converged": bool(tr <= tol)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-ppe", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ppe-n", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from twophaseflow_amd import CCDSolver, Grid, GridConfig
+    from twophaseflow_amd.fields import band_limited_periodic
+    torch.cuda.set_device(0)
+
+    # ---- CCD 3-D 257^3 periodic, all axes, FP64
+    N = 256
+    log("generating 257^3 band-limited field")
+    f = torch.from_numpy(band_limited_periodic(N, 3, kmax=32, seed=2)).cuda()
+    g = Grid(GridConfig(ndim=3, N=(N, N, N), L=(1.0, 1.0, 1.0)))
+    ccd = CCDSolver(g, bc_type="periodic")
+    d1 = torch.empty((3,) + g.shape, dtype=torch.float64, device="cuda")
+    d2 = torch.empty_like(d1)
+    log("timing 3-D CCD")
+    sec = time_ccd(ccd, f, d1, d2, 0b111, args.steps, args.warmup, 3)
+    npts = f.numel()
+    mpts = 3 * npts / sec / 1e6
+    axes = per_axis_event_times(ccd, f, d1, d2, max(args.steps, 5))
+    strided = [a for a in axes if a["axis"] < 2]
+    dom_ms = sum(a["ms"] for a in strided) / len(strided)
+    achieved = 24.0 * npts / (dom_ms * 1e-3) / 1e9
+    all_axes_gbs = 8.0 * 7 * npts / sec / 1e9
+    del d1, d2
+
+    # ---- CCD 2-D 1025^2 periodic, all axes
+    log("timing 2-D CCD")
+    N2 = 1024
+    f2 = torch.from_numpy(band_limited_periodic(N2, 2, kmax=32, seed=2)).cuda()
+    g2 = Grid(GridConfig(ndim=2, N=(N2, N2), L=(1.0, 1.0)))
+    ccd2 = CCDSolver(g2, bc_type="periodic")
+    e1 = torch.empty((2,) + g2.shape, dtype=torch.float64, device="cuda")
+    e2 = torch.empty_like(e1)
+    sec2 = time_ccd(ccd2, f2, e1, e2, 0b11, args.steps, args.warmup, 2)
+    mpts2 = 2 * f2.numel() / sec2 / 1e6
+    del e1, e2
+
+    result = {
+        "metric": METRIC,
+        "value": round(mpts, 2),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded band-limited periodic field, |k|<=32, amp |k|^-2)",
+        "config": {"workload": "CCD f',f'' all axes, 3-D 257^3 nodes periodic, FP64",
+                   "grid": [N + 1] * 3, "axes": 3, "lines_per_axis": (N + 1) ** 2},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": "ccd_line_kernel<16,false> (strided axes 0,1)",
+                     "bytes_per_launch": 24 * npts, "launch_ms": round(dom_ms, 4)},
+        "ccd_all_axes_GBps": round(all_axes_gbs, 1),
+        "ccd_axes": [{"axis": a["axis"], "ms": round(a["ms"], 4), "GBps": round(a["GBps"], 1)}
+                     for a in axes],
+        "ccd_2d_1025": {"Mpts_per_s": round(mpts2, 2), "ms_per_step": round(sec2 * 1e3, 4)},
+    }
+
+    if not args.no_ppe:
+        log(f"building PPE case {args.ppe_n}^3")
+        gp, rho, rhs = ppe_case(N=args.ppe_n)
+        res = {}
+        for prec in ("fp64", "mixed"):
+            log(f"PPE solve {prec}")
+            res[prec] = ppe_solve_timed(gp, rho, rhs, prec)
+            log(f"  {res[prec]}")
+        result["ppe"] = {"grid": [args.ppe_n + 1] * 3, "density_ratio": 1000, "bubbles": 64,
+                         "tol": 1e-8, "plain_pcg_fp64": res["fp64"], "pcg_mixed": res["mixed"]}
+        del rho, rhs
+
+    if not args.no_cpu:
+        log("CPU baseline (dense-LU CCD slab)")
+        result["cpu_baseline"] = cpu_baseline_ccd()
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+/*
+ * tpf.h -- C ABI of the MI355X-native CCD operator and variable-density PPE.
+ *
+ * Plain pointers and sizes only.  Every array argument is a DEVICE pointer
+ * (hipMalloc'd or a torch CUDA tensor's data_ptr) holding a C-order FP64
+ * field of shape n_nodes[0] x ... x n_nodes[ndim-1] (node-centred, N+1 nodes
+ * per axis, as Grid.shape in the reference).  `stream` is a hipStream_t
+ * passed as void* (NULL = default stream).  Calls are asynchronous on that
+ * stream unless stated otherwise.  Handles are not thread-safe.
+ *
+ * Every function returns a tpf_status; tpf_last_error() gives the message of
+ * the last failure on the calling thread.  Reference file paths below are
+ * relative to src/twophase of t0m0h1r0/TwoPhaseFlow.
+ */
+#ifndef TPF_H
+#define TPF_H
+
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define TPF_API __attribute__((visibility("default")))
+#else
+#define TPF_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  TPF_OK = 0,
+  TPF_ERR_VALUE = 1,      /* bad argument/shape   -> ValueError   (ccd_solver.py:459-465, fvm_matrixfree.py:84-89) */
+  TPF_ERR_RUNTIME = 2,    /* no convergence, HIP  -> RuntimeError (fvm_matrixfree.py:250-256) */
+  TPF_ERR_LINALG = 3      /* singular pivot       -> LinAlgError  (block_tridiag.py:167-174) */
+} tpf_status;
+
+enum { TPF_BC_WALL = 0, TPF_BC_PERIODIC = 1 };
+enum { TPF_PREC_FP64 = 0, TPF_PREC_FP32_REFINED = 1 };
+
+TPF_API const char* tpf_last_error(void);
+TPF_API int tpf_version(void);
+
+/* ---------------------------------------------------------------- CCD --- */
+typedef struct tpf_ccd_plan tpf_ccd_plan;
+
+/* Replaces CCDSolver.__init__ + the per-axis factorisations
+ * (ccd/ccd_solver.py:99-120, ccd/ccd_solver_helpers.py:227-314, 406-423).
+ * h[a] = L[a]/N[a]; bc[a] in {TPF_BC_WALL, TPF_BC_PERIODIC}.
+ * Wall axes need n_nodes >= 6 (TPF_ERR_VALUE otherwise); periodic >= 4. */
+TPF_API tpf_status tpf_ccd_plan_create(tpf_ccd_plan** plan, int ndim,
+                               const int64_t* n_nodes, const double* h,
+                               const int* bc, int precision);
+TPF_API tpf_status tpf_ccd_plan_destroy(tpf_ccd_plan* plan);
+
+/* Replaces CCDSolver.differentiate(data, axis) -> (d1, d2)
+ * (ccd/ccd_solver.py:134-172; ccd_solver_helpers.py:38-77 wall,
+ * :426-465 periodic).  Either output may be NULL (first_derivative /
+ * second_derivative, ccd_solver.py:205-237).  f, d1, d2 have the full
+ * field shape; d1/d2 must not alias f. */
+TPF_API tpf_status tpf_ccd_differentiate(tpf_ccd_plan* plan, const double* f,
+                                 double* d1, double* d2, int axis,
+                                 void* stream);
+
+/* All axes whose bit is set in axis_mask, in increasing axis order:
+ * d1[k*n_total ...] and d2[k*n_total ...] hold the k-th selected axis. */
+TPF_API tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double* d1,
+                         double* d2, uint32_t axis_mask, void* stream);
+
+/* Launch geometry / reduced-system facts of one axis (for tests, bench and
+ * the autotuner): partitions, rows per partition, coupling window, and the
+ * largest dropped reduced-system coefficient (0 when untruncated). */
+TPF_API tpf_status tpf_ccd_axis_info(tpf_ccd_plan* plan, int axis, int* partitions,
+                             int* max_rows, int* window, double* trunc_coeff);
+
+/* ---------------------------------------------------------------- PPE --- */
+typedef struct tpf_ppe tpf_ppe;
+
+typedef struct {
+  double tol;          /* ||r||/||b|| target (SolverConfig.pseudo_tol)     */
+  int64_t maxiter;     /* iteration cap (SolverConfig.pseudo_maxiter)      */
+  int precond;         /* 0 none, 1 Jacobi (fvm_matrixfree.py:106-114)     */
+  int check_every;     /* host convergence check period in iterations      */
+  int use_x0;          /* 1: p holds the initial guess (p_init)            */
+  int mixed;           /* 1: FP32 matvec/inner solve + FP64 refinement     */
+} tpf_solve_opts;
+
+typedef struct {
+  int64_t iterations;  /* Krylov iterations performed                        */
+  double relres;       /* recursively updated ||r||/||b|| at exit           */
+  double true_relres;  /* FP64 ||b - A p||/||b|| recomputed on the CG system */
+  int converged;       /* 1 when true_relres <= tol                         */
+  int refinements;     /* outer FP64 refinement sweeps (mixed mode)         */
+} tpf_solve_report;
+
+/* Replaces PPESolverFVMMatrixFree.__init__ (ppe/fvm_matrixfree.py:66-138)
+ * for wall BC: L[a] domain length, pin_dof = centre node
+ * (core/boundary.py:181-191). Allocates the solver workspace. */
+TPF_API tpf_status tpf_ppe_create(tpf_ppe** ppe, int ndim, const int64_t* n_nodes,
+                          const double* L, int64_t pin_dof, void* stream);
+TPF_API tpf_status tpf_ppe_destroy(tpf_ppe* ppe);
+
+/* Replaces prepare_operator + _prepare_preconditioner_state
+ * (fvm_matrixfree.py:262-279, 352-363): keeps rho, builds the Jacobi
+ * diagonal of the CG system. rho must stay valid until the next call. */
+TPF_API tpf_status tpf_ppe_set_density(tpf_ppe* ppe, const double* rho);
+
+/* Replaces PPESolverFVMMatrixFree.apply (fvm_matrixfree.py:281-300):
+ * out = L(rho) p with out[pin] = p[pin]. */
+TPF_API tpf_status tpf_ppe_apply(tpf_ppe* ppe, const double* p, double* out);
+
+/* Replaces PPESolverFVMMatrixFree.solve in CG form
+ * (fvm_matrixfree.py:150-260): solves (-W L) p = -W rhs with the pin
+ * row/column = identity; on exit p[pin] = 0.  Synchronous.  Returns
+ * TPF_ERR_RUNTIME when maxiter is reached above tol (report still filled). */
+TPF_API tpf_status tpf_ppe_solve(tpf_ppe* ppe, const double* rhs, double* p,
+                         const tpf_solve_opts* opts, tpf_solve_report* rep);
+
+/* FP64 ||b - A p||_2 / ||b||_2 on the CG system (the judge of every solve). */
+TPF_API tpf_status tpf_ppe_true_relres(tpf_ppe* ppe, const double* rhs,
+                               const double* p, double* relres);
+
+/* ------------------------------------------------------------- fields --- */
+/* Replaces InitialConditionBuilder.build (gas spheres in liquid) +
+ * update_properties (simulation/initial_conditions/builder.py:100-172,
+ * levelset/heaviside.py:167-187): psi = 1/(1+exp(clip(-phi_gas/eps))),
+ * rho = rho_g + (rho_l-rho_g) psi.  centres: n_spheres*ndim host doubles,
+ * radii: n_spheres host doubles.  Either output may be NULL. */
+TPF_API tpf_status tpf_fields_bubbles(int ndim, const int64_t* n_nodes,
+                              const double* L, int n_spheres,
+                              const double* centres, const double* radii,
+                              double eps, double rho_l, double rho_g,
+                              double* psi, double* rho, void* stream);
+
+/* Mass integral sum(q * dV) in FP64 with Grid.cell_volumes weights
+ * (core/grid.py:1123-1170), synchronous (heaviside.py:118-149). */
+TPF_API tpf_status tpf_fields_mass(int ndim, const int64_t* n_nodes, const double* L,
+                           const int* bc, const double* q, double* mass,
+                           void* stream);
+
+/* u_a -= dt/rho * g with g zeroed on the wall planes normal to `axis`
+ * when bc[axis] is wall (coupling/velocity_corrector.py:39-89). */
+TPF_API tpf_status tpf_fields_correct_velocity(int ndim, const int64_t* n_nodes,
+                                       const int* bc, int axis, double dt,
+                                       const double* rho, const double* grad,
+                                       double* u, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TPF_H */

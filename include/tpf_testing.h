@@ -1,0 +1,24 @@
+/*
+ * tpf_testing.h -- host-only test hooks of libtpf (not part of the product
+ * path; no counterpart in the reference).  They run scalar host models of
+ * the device algorithms so the precomputed tables can be checked on a
+ * machine without a GPU.  Arrays here are HOST pointers.
+ */
+#ifndef TPF_TESTING_H
+#define TPF_TESTING_H
+
+#include "tpf.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scalar model of the CCD line kernel for one axis (same tables, same
+ * partitioning) on host arrays of the plan's full shape. */
+TPF_API tpf_status tpf_test_ccd_host_model(tpf_ccd_plan* plan, const double* f,
+                                           double* d1, double* d2, int axis);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TPF_TESTING_H */

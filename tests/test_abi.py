@@ -1,0 +1,93 @@
+"""CPU checks of the C ABI library: it loads, exports every declared symbol,
+maps errors to the reference's exception types, and its SPIKE tables
+reproduce the dense-LU oracle through the host model of the kernel."""
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import ccd as occd
+from twophaseflow_amd import _native as nat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in ("tpf.h", "tpf_testing.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        names |= set(re.findall(r"TPF_API\s+[\w\s\*]+?\b(tpf_\w+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = nat.lib()
+    names = declared_symbols()
+    assert len(names) >= 17
+    for n in names:
+        assert hasattr(lib, n), n
+    assert names == set(nat.SIGNATURES), names ^ set(nat.SIGNATURES)
+    assert lib.tpf_version() >= 1
+
+
+def _plan(shape, bcs, L=None):
+    L = L or [1.0] * len(shape)
+    h = [L[a] / (shape[a] - 1) for a in range(len(shape))]
+    plan = C.c_void_p()
+    nat.check(nat.lib().tpf_ccd_plan_create(C.byref(plan), len(shape), nat.i64arr(shape),
+                                            nat.darr(h), nat.iarr(bcs), 0))
+    return plan
+
+
+def test_wall_axis_with_five_nodes_raises_value_error():
+    plan = _plan((5, 9), (0, 0))
+    with pytest.raises(ValueError, match="n_pts >= 6"):
+        nat.check(nat.lib().tpf_ccd_axis_info(plan, 0, None, None, None, None))
+    nat.lib().tpf_ccd_plan_destroy(plan)
+
+
+def test_bad_arguments_raise_value_error():
+    plan = C.c_void_p()
+    with pytest.raises(ValueError):
+        nat.check(nat.lib().tpf_ccd_plan_create(C.byref(plan), 4, nat.i64arr([9] * 4),
+                                                nat.darr([0.1] * 4), nat.iarr([0] * 4), 0))
+
+
+CASES = [((9, 7), (0, 0)), ((65, 65), (0, 0)), ((65, 65), (1, 1)), ((33, 20, 9), (0, 1, 0)),
+         ((257, 3), (1, 0)), ((256, 3), (0, 0)), ((1025, 2), (1, 1)), ((1025, 2), (0, 0)),
+         ((6, 6), (0, 0)), ((4, 5), (1, 1)), ((129, 7), (0, 1)), ((100, 8), (1, 0))]
+
+
+@pytest.mark.parametrize("shape,bcs", CASES)
+def test_host_model_matches_dense_lu_oracle(shape, bcs):
+    L = [1.0 + 0.3 * a for a in range(len(shape))]
+    plan = _plan(shape, bcs, L)
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal(shape)
+    kinds = tuple("periodic" if b else "wall" for b in bcs)
+    o = occd.OracleCCD([s - 1 for s in shape], L, kinds)
+    for axis in range(len(shape)):
+        if kinds[axis] == "wall" and shape[axis] < 6:
+            continue
+        if kinds[axis] == "periodic" and shape[axis] < 4:
+            continue
+        d1 = np.full(shape, np.nan)
+        d2 = np.full(shape, np.nan)
+        nat.check(nat.lib().tpf_test_ccd_host_model(plan, f.ctypes.data, d1.ctypes.data,
+                                                    d2.ctypes.data, axis))
+        r1, r2 = o.differentiate(f, axis)
+        assert np.max(np.abs(d1 - r1)) <= 1e-13 * np.max(np.abs(r1))
+        assert np.max(np.abs(d2 - r2)) <= 1e-13 * np.max(np.abs(r2))
+    nat.lib().tpf_ccd_plan_destroy(plan)
+
+
+def test_reduced_system_window_truncation_is_negligible():
+    plan = _plan((257, 257, 257), (1, 1, 1))
+    P, M, W, t = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    nat.check(nat.lib().tpf_ccd_axis_info(plan, 0, C.byref(P), C.byref(M), C.byref(W), C.byref(t)))
+    assert P.value == 16 and M.value == 16
+    assert W.value < 16 and t.value < 1e-18
+    nat.lib().tpf_ccd_plan_destroy(plan)

@@ -1,0 +1,178 @@
+"""GPU parity of the HIP CCD kernels against the CPU oracle (FP64).
+
+Tolerance: max-norm error relative to max|reference| <= 1e-12 for f' and f''
+except f'' on lines of >= 1024 nodes where the
+reference's own dense LU is already off the exact discrete answer by ~2e-12
+(second-difference cancellation): there the bound is
+max(1e-12, 2 x the dense LU's error against the closed-form spectral result).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ccd as occd
+from twophaseflow_amd import CCDSolver, Grid, GridConfig
+from twophaseflow_amd.fields import band_limited_periodic
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(msg):
+    print(f"[gpu-case] {msg}", file=sys.stderr, flush=True)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(np.asarray(b))))
+
+
+def _grid(N, L=None):
+    L = L or [1.0] * len(N)
+    return Grid(GridConfig(ndim=len(N), N=tuple(N), L=tuple(L)))
+
+
+CASES = [((8, 6), "wall"), ((64, 64), "wall"), ((64, 64), "periodic"),
+         ((32, 19, 8), ("wall", "periodic", "wall")), ((256, 2), "periodic"),
+         ((255, 3), "wall"), ((5, 5), "wall"), ((3, 4), "periodic"), ((128, 6), "periodic_wall"),
+         ((40, 40, 40), "periodic"), ((33, 17, 65), "wall"), ((99, 7), "wall_periodic")]
+
+
+@pytest.mark.parametrize("N,bc", CASES)
+def test_ccd_matches_oracle(N, bc):
+    _announce(f"ccd N={N} bc={bc}")
+    L = [1.0 + 0.25 * a for a in range(len(N))]
+    g = _grid(N, L)
+    ccd = CCDSolver(g, bc_type=bc)
+    o = occd.OracleCCD(N, L, bc)
+    f = np.random.default_rng(3).standard_normal(g.shape)
+    for axis in range(len(N)):
+        kind = o.kinds[axis]
+        if (kind == "wall" and N[axis] + 1 < 6) or (kind == "periodic" and N[axis] < 3):
+            with pytest.raises(ValueError):
+                ccd.differentiate(f, axis)
+            continue
+        d1, d2 = ccd.differentiate(f, axis)
+        r1, r2 = o.differentiate(f, axis)
+        assert _rel(d1, r1) <= 1e-12, (axis, _rel(d1, r1))
+        assert _rel(d2, r2) <= 1e-12, (axis, _rel(d2, r2))
+
+
+def test_first_and_second_only_match_full():
+    _announce("ccd first/second only")
+    g = _grid((48, 40))
+    ccd = CCDSolver(g, bc_type="wall")
+    f = torch.randn(g.shape, dtype=torch.float64, device="cuda")
+    for axis in (0, 1):
+        d1, d2 = ccd.differentiate(f, axis)
+        assert torch.equal(ccd.first_derivative(f, axis), d1)
+        assert torch.equal(ccd.second_derivative(f, axis), d2)
+
+
+def test_torch_and_numpy_inputs_agree():
+    _announce("ccd torch vs numpy")
+    g = _grid((64, 33))
+    ccd = CCDSolver(g, bc_type="periodic")
+    f = np.random.default_rng(5).standard_normal(g.shape)
+    a1, a2 = ccd.differentiate(f, 1)
+    b1, b2 = ccd.differentiate(torch.from_numpy(f).cuda(), 1)
+    assert isinstance(a1, np.ndarray) and isinstance(b1, torch.Tensor)
+    np.testing.assert_array_equal(a1, b1.cpu().numpy())
+    np.testing.assert_array_equal(a2, b2.cpu().numpy())
+
+
+def test_paper_table_U1a_through_gpu():
+    """paper/sections/12u1_ccd_operator.tex:46-50."""
+    _announce("ccd U1-a")
+    table = {16: (2.57e-6, 3.68e-5), 32: (3.86e-8, 5.70e-7), 64: (5.97e-10, 8.89e-9),
+             128: (9.36e-12, 1.49e-10)}
+    for N, (r1, r2) in table.items():
+        g = _grid((N, N))
+        x = np.linspace(0, 1, N + 1)
+        X, Y = np.meshgrid(x, x, indexing="ij")
+        f = np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y)
+        d1, d2 = CCDSolver(g, bc_type="periodic").differentiate(f, 0)
+        e1 = np.max(np.abs(d1 - 2 * np.pi * np.cos(2 * np.pi * X) * np.sin(2 * np.pi * Y)))
+        e2 = np.max(np.abs(d2 + (2 * np.pi) ** 2 * f))
+        assert abs(e1 - r1) <= 0.01 * r1 and abs(e2 - r2) <= 0.01 * r2, (N, e1, e2)
+
+
+def test_wall_cubic_exact_and_nullspace():
+    _announce("ccd wall polynomials")
+    N = 20
+    g = _grid((N, 4))
+    x = np.linspace(0, 1, N + 1)
+    f = np.broadcast_to((x ** 3)[:, None], g.shape).copy()
+    d1, d2 = CCDSolver(g).differentiate(f, 0)
+    assert np.max(np.abs(d1[:, 2] - 3 * x ** 2)) < 1e-10
+    assert np.max(np.abs(d2[:, 2] - 6 * x)) < 1e-9
+    g8 = _grid((8, 8))
+    _, D2 = CCDSolver(g8).differentiate(np.eye(9), 0)
+    assert 9 - np.linalg.matrix_rank(D2, tol=1e-8) == 2
+
+
+@pytest.mark.parametrize("N", [1024, 2048])
+def test_long_periodic_lines_against_spectral(N):
+    """2-D 1024^2 class lines: f'' bound is max(1e-12, 2 x dense-LU error)."""
+    _announce(f"ccd long lines N={N}")
+    g = _grid((N, 3))
+    f = band_limited_periodic(N, 1, kmax=32, seed=2)[:, None].repeat(4, axis=1)
+    ccd = CCDSolver(g, bc_type="periodic")
+    d1, d2 = ccd.differentiate(f, 0)
+    s1, s2 = occd.periodic_spectral(f, 0, 1.0 / N)
+    info = occd.build_periodic_axis(N + 1, 1.0 / N)
+    l1, l2 = occd.differentiate_periodic(f, 0, info)
+    assert _rel(d1, l1) <= 1e-12
+    bound = max(1e-12, 2.0 * _rel(l2, s2))
+    assert _rel(d2, s2) <= bound, (_rel(d2, s2), bound)
+    assert _rel(d2, l2) <= bound
+
+
+def test_all_axes_apply_3d_periodic_band_limited():
+    """Config-2 field shape at reduced size: every axis in one call."""
+    _announce("ccd apply all axes 64^3")
+    N = 64
+    f_np = band_limited_periodic(N, 3, kmax=16, seed=2)
+    g = _grid((N, N, N))
+    ccd = CCDSolver(g, bc_type="periodic")
+    f = torch.from_numpy(f_np).cuda()
+    d1 = torch.empty((3,) + g.shape, dtype=torch.float64, device="cuda")
+    d2 = torch.empty_like(d1)
+    ccd.apply_into(f, d1, d2, 0b111)
+    torch.cuda.synchronize()
+    o = occd.OracleCCD((N, N, N), (1, 1, 1), "periodic")
+    for a in range(3):
+        r1, r2 = o.differentiate(f_np, a)
+        s1, s2 = occd.periodic_spectral(f_np, a, 1.0 / N)
+        assert _rel(d1[a].cpu(), r1) <= 1e-12
+        assert _rel(d2[a].cpu(), r2) <= 1e-12
+        assert _rel(d1[a].cpu(), s1) <= 1e-12
+        assert _rel(d2[a].cpu(), s2) <= 1e-12
+
+
+def test_full_size_256_cubed_properties():
+    """Config-2 3-D size: linearity, image-plane equality and spectral parity
+    on a sampled set of lines (size-independent properties)."""
+    _announce("ccd 256^3 properties")
+    N = 256
+    f_np = band_limited_periodic(N, 3, kmax=32, seed=2)
+    g = _grid((N, N, N))
+    ccd = CCDSolver(g, bc_type="periodic")
+    f = torch.from_numpy(f_np).cuda()
+    for axis in (0, 2):
+        d1, d2 = ccd.differentiate(f, axis)
+        e1, e2 = ccd.differentiate(2.0 * f, axis)
+        assert torch.allclose(e1, 2.0 * d1, rtol=0, atol=1e-12 * float(d1.abs().max()))
+        img = [slice(None)] * 3
+        src = [slice(None)] * 3
+        img[axis], src[axis] = -1, 0
+        assert torch.equal(d1[tuple(img)], d1[tuple(src)])
+        # spectral check on a slab of lines
+        sl = [slice(None)] * 3
+        other = 1 if axis != 1 else 0
+        sl[other] = slice(0, 3)
+        sub = f_np[tuple(sl)]
+        s1, s2 = occd.periodic_spectral(sub, axis, 1.0 / N)
+        assert _rel(d1[tuple(sl)].cpu(), s1) <= 1e-12
+        assert _rel(d2[tuple(sl)].cpu(), s2) <= 1e-12

@@ -1,0 +1,172 @@
+"""GPU parity of the HIP FVM PPE against the CPU oracle.
+
+Operator: matrix-free apply equals the oracle to rtol=atol=1e-12
+(test_fvm_matrixfree.py:33-56).  Solves: the FP64 true residual of the
+reference's CG system is <= tol, and the solution matches the oracle's
+direct solve to the reference's solve tolerances (rtol 1e-9 / atol 1e-10 at
+tol 1e-10, test_fvm_matrixfree.py:88-191) scaled by the conditioning at
+density ratio 1000.  Iteration counts are not compared (SciPy internals).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fields as ofields
+from oracle import ppe as oppe
+from oracle import ccd as occd
+from twophaseflow_amd import (CCDSolver, Grid, GridConfig, PPESolverFVMMatrixFree,
+                              SimulationConfig, SolverConfig, create_ppe_solver)
+from twophaseflow_amd import fields
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(msg):
+    print(f"[gpu-case] {msg}", file=sys.stderr, flush=True)
+
+
+def _cfg(N, tol=1e-10, maxiter=5000, precision="fp64"):
+    cfg = SimulationConfig(grid=GridConfig(ndim=len(N), N=tuple(N), L=(1.0,) * len(N)),
+                           solver=SolverConfig(pseudo_tol=tol, pseudo_maxiter=maxiter))
+    cfg.solver.ppe_iteration_method = "cg"
+    cfg.solver.ppe_precision = precision
+    return cfg
+
+
+@pytest.mark.parametrize("N", [(8, 8), (5, 6, 7), (16, 9, 12)])
+def test_apply_matches_oracle(N):
+    _announce(f"ppe apply N={N}")
+    cfg = _cfg(N)
+    g = Grid(cfg.grid)
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    rng = np.random.default_rng(123)
+    rho = 1.0 + rng.uniform(0.0, 1.0, g.shape)
+    p = rng.standard_normal(g.shape)
+    s.prepare_operator(rho)
+    out = s.apply(p)
+    o = oppe.OraclePPE(N)
+    o.prepare_operator(rho)
+    np.testing.assert_allclose(out, o.apply(p), rtol=1e-12, atol=1e-12)
+
+
+def test_solve_matches_direct_2d():
+    """test_fvm_matrixfree.py:153-169 (N=8, seed 789, Jacobi)."""
+    _announce("ppe solve 2d N=8")
+    cfg = _cfg((8, 8), tol=1e-10, maxiter=200)
+    g = Grid(cfg.grid)
+    s = create_ppe_solver(cfg, None, g)
+    rng = np.random.default_rng(789)
+    rho = 1.0 + rng.uniform(0.0, 1.0, g.shape)
+    rhs = rng.standard_normal(g.shape)
+    p = s.solve(rhs, rho, dt=1e-3)
+    ref = oppe.OraclePPE((8, 8)).solve_direct(rhs, rho)
+    np.testing.assert_allclose(p, ref, rtol=1e-9, atol=1e-10)
+    assert s.last_diagnostics["true_relres"] <= 1e-10
+    assert p.ravel()[s._pin_dof] == 0.0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_density_ratio_1000_3d(precision):
+    _announce(f"ppe solve 3d ratio 1000 {precision}")
+    N = (24, 24, 24)
+    cfg = _cfg(N, tol=1e-10, maxiter=20000, precision=precision)
+    g = Grid(cfg.grid)
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    centres, radii = [(0.3, 0.5, 0.5), (0.7, 0.55, 0.4), (0.5, 0.25, 0.7)], [0.18, 0.15, 0.12]
+    psi = ofields.gas_bubbles_psi(N, (1, 1, 1), centres, radii, 1.5 / 24)
+    rho = ofields.density(psi, 1.0, 1e-3)
+    rhs = np.random.default_rng(5).standard_normal(g.shape)
+    p = s.solve(rhs, rho)
+    o = oppe.OraclePPE(N)
+    o.prepare_operator(rho)
+    assert o.relres(rhs, p) <= 1e-10 * 1.05
+    ref = o.solve_direct(rhs, rho)
+    assert np.max(np.abs(p - ref)) <= 1e-6 * np.max(np.abs(ref))
+
+
+def test_warm_start_and_pin():
+    _announce("ppe warm start")
+    N = (12, 12)
+    cfg = _cfg(N, tol=1e-9)
+    g = Grid(cfg.grid)
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    rng = np.random.default_rng(2)
+    rho = 1.0 + rng.uniform(0, 1, g.shape)
+    rhs = rng.standard_normal(g.shape)
+    p0 = s.solve(rhs, rho)
+    it0 = s.last_diagnostics["iterations"]
+    p1 = s.solve(rhs, rho, p_init=p0 + 1e-12)
+    assert s.last_diagnostics["iterations"] < max(it0 // 2, 2)
+    np.testing.assert_allclose(p1, p0, rtol=1e-7, atol=1e-8)
+
+
+def test_nonconvergence_raises_runtime_error():
+    """fvm_matrixfree.py:250-256: no silent fallback."""
+    _announce("ppe maxiter=2")
+    N = (16, 16)
+    cfg = _cfg(N, tol=1e-12, maxiter=2)
+    g = Grid(cfg.grid)
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    rng = np.random.default_rng(4)
+    with pytest.raises(RuntimeError, match="did not converge"):
+        s.solve(rng.standard_normal(g.shape), 1.0 + rng.uniform(0, 1, g.shape))
+
+
+def test_non_wall_bc_rejected():
+    cfg = _cfg((8, 8))
+    with pytest.raises(ValueError, match="wall"):
+        PPESolverFVMMatrixFree(None, cfg, Grid(cfg.grid), bc_type="periodic_wall")
+
+
+def test_fields_match_oracle():
+    _announce("fields bubbles/mass/correct")
+    N, L = (20, 24, 16), (1.0, 1.2, 0.8)
+    centres, radii = fields.random_bubbles(5, seed=3, ndim=3, rmin=0.1, rmax=0.2)
+    eps = 1.5 * min(L[a] / N[a] for a in range(3))
+    psi, rho = fields.bubbles_psi_rho(N, L, centres, radii, eps, 1.0, 1e-3)
+    psi_o = ofields.gas_bubbles_psi(N, L, centres, radii, eps)
+    np.testing.assert_allclose(psi.cpu().numpy(), psi_o, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(rho.cpu().numpy(), ofields.density(psi_o, 1.0, 1e-3),
+                               rtol=1e-13, atol=1e-15)
+    for bc in ("wall", "periodic"):
+        m = fields.mass(psi, N, L, bc)
+        mo = ofields.mass(psi_o, ofields.cell_volumes(N, L, (bc,) * 3))
+        assert abs(m - mo) <= 1e-13 * abs(mo)
+    # velocity correction
+    g = Grid(GridConfig(ndim=3, N=N, L=L))
+    ccd = CCDSolver(g, bc_type="wall")
+    rng = np.random.default_rng(9)
+    p = rng.standard_normal(g.shape)
+    u = [rng.standard_normal(g.shape) for _ in range(3)]
+    ut = [torch.from_numpy(x.copy()).cuda() for x in u]
+    fields.correct_velocity(ccd, ut, torch.from_numpy(p).cuda(), rho, 0.1)
+    uo = ofields.correct_velocity(occd.OracleCCD(N, L, "wall"), u, p, rho.cpu().numpy(), 0.1)
+    for a in range(3):
+        np.testing.assert_allclose(ut[a].cpu().numpy(), uo[a], rtol=1e-12, atol=1e-12)
+
+
+def test_projection_divergence_drops():
+    """rhs = CCD divergence of a smooth wall-compatible field; after the
+    solve and the CCD velocity correction the interior divergence falls."""
+    _announce("projection divergence")
+    N = (32, 32, 32)
+    cfg = _cfg(N, tol=1e-8)
+    g = Grid(cfg.grid)
+    ccd = CCDSolver(g, bc_type="wall")
+    X = [torch.from_numpy(c).cuda() for c in g.meshgrid()]
+    pi = np.pi
+    u = [torch.sin(pi * X[0]) * X[1] * (1 - X[1]) * X[2] * (1 - X[2]),
+         torch.sin(pi * X[1]) * X[0] * (1 - X[0]),
+         torch.sin(pi * X[2]) * X[1] * (1 - X[1]) * 0.5]
+    rho = torch.ones(g.shape, dtype=torch.float64, device="cuda")
+    div0 = fields.divergence(ccd, u)
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    p = torch.empty_like(div0)
+    s.solve_into(div0, rho, p)
+    fields.correct_velocity(ccd, u, p, rho, 1.0)
+    div1 = fields.divergence(ccd, u)
+    inner = (slice(3, -3),) * 3
+    assert float(div1[inner].abs().max()) < 0.01 * float(div0[inner].abs().max())

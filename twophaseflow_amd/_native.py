@@ -1,0 +1,106 @@
+"""ctypes binding of libtpf.so (the C ABI declared in include/tpf.h).
+
+The library is loaded from this package directory only.  There is no CPU
+fallback: when the library is missing every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtpf.so")
+
+TPF_OK, TPF_ERR_VALUE, TPF_ERR_RUNTIME, TPF_ERR_LINALG = 0, 1, 2, 3
+BC_WALL, BC_PERIODIC = 0, 1
+PREC_FP64, PREC_FP32_REFINED = 0, 1
+
+_i64p = C.POINTER(C.c_int64)
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxiter", C.c_int64), ("precond", C.c_int),
+                ("check_every", C.c_int), ("use_x0", C.c_int), ("mixed", C.c_int)]
+
+
+class SolveReport(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("relres", C.c_double),
+                ("true_relres", C.c_double), ("converged", C.c_int),
+                ("refinements", C.c_int)]
+
+
+# name -> (restype, argtypes); every symbol of include/tpf.h and tpf_testing.h
+SIGNATURES = {
+    "tpf_last_error": (C.c_char_p, []),
+    "tpf_version": (C.c_int, []),
+    "tpf_ccd_plan_create": (C.c_int, [C.POINTER(_vp), C.c_int, _i64p, _dp, _ip, C.c_int]),
+    "tpf_ccd_plan_destroy": (C.c_int, [_vp]),
+    "tpf_ccd_differentiate": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "tpf_ccd_apply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "tpf_ccd_axis_info": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _dp]),
+    "tpf_ppe_create": (C.c_int, [C.POINTER(_vp), C.c_int, _i64p, _dp, C.c_int64, _vp]),
+    "tpf_ppe_destroy": (C.c_int, [_vp]),
+    "tpf_ppe_set_density": (C.c_int, [_vp, _vp]),
+    "tpf_ppe_apply": (C.c_int, [_vp, _vp, _vp]),
+    "tpf_ppe_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveReport)]),
+    "tpf_ppe_true_relres": (C.c_int, [_vp, _vp, _vp, _dp]),
+    "tpf_fields_bubbles": (C.c_int, [C.c_int, _i64p, _dp, C.c_int, _dp, _dp, C.c_double,
+                                     C.c_double, C.c_double, _vp, _vp, _vp]),
+    "tpf_fields_mass": (C.c_int, [C.c_int, _i64p, _dp, _ip, _vp, _dp, _vp]),
+    "tpf_fields_correct_velocity": (C.c_int, [C.c_int, _i64p, _ip, C.c_int, C.c_double,
+                                              _vp, _vp, _vp, _vp]),
+    "tpf_test_ccd_host_model": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
+}
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded libtpf (raises LibraryMissing when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(
+                f"{LIB_PATH} is not built; run `python -m twophaseflow_amd.build` "
+                "(there is no CPU fallback)")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def check(status: int) -> None:
+    """Map a tpf_status to the reference's exception types."""
+    if status == TPF_OK:
+        return
+    msg = lib().tpf_last_error().decode(errors="replace")
+    if status == TPF_ERR_VALUE:
+        raise ValueError(msg)
+    if status == TPF_ERR_LINALG:
+        raise np.linalg.LinAlgError(msg)
+    raise RuntimeError(msg)
+
+
+def i64arr(vals):
+    return (C.c_int64 * len(vals))(*[int(v) for v in vals])
+
+
+def darr(vals):
+    return (C.c_double * len(vals))(*[float(v) for v in vals])
+
+
+def iarr(vals):
+    return (C.c_int * len(vals))(*[int(v) for v in vals])

@@ -1,0 +1,152 @@
+"""``CCDSolver`` on MI355X: the reference call surface over the HIP line kernels.
+
+Mirrors ``ccd/ccd_solver.py:79-403`` for uniform grids:
+``differentiate(data, axis) -> (d1, d2)``, ``first_derivative``,
+``second_derivative``, ``differentiate_raw``, ``enforce_wall_neumann`` and
+the attributes ``bc_type, grid, backend, xp, ndim``.  Inputs may be NumPy
+arrays (copied to and from the device) or CUDA torch tensors (zero copy);
+outputs are new arrays of the input's kind.  Errors follow the reference:
+``ValueError`` for a wall axis with fewer than 6 nodes
+(ccd_solver.py:459-465), ``np.linalg.LinAlgError`` for a singular pivot.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .boundary import axis_kinds
+from .device import Handoff, ptr, require_gpu, stream_handle
+
+
+class CCDSolver:
+    def __init__(self, grid, backend=None, bc_type: str = "wall", precision: str = "fp64"):
+        self.grid = grid
+        self.backend = backend
+        self.xp = np
+        self.ndim = grid.ndim
+        self.bc_type = bc_type
+        if hasattr(grid, "set_boundary_type"):
+            grid.set_boundary_type(bc_type)
+        if not getattr(grid, "uniform", True):
+            raise ValueError("twophaseflow_amd CCDSolver supports uniform grids only")
+        self._kinds = axis_kinds(bc_type, self.ndim)
+        if precision not in ("fp64", "fp32_refined"):
+            raise ValueError("precision must be 'fp64' or 'fp32_refined'")
+        self.precision = precision
+        self.shape = tuple(int(n) + 1 for n in grid.N)
+        self._lib = nat.lib()
+        h = [float(grid.L[a]) / float(grid.N[a]) for a in range(self.ndim)]
+        bcs = [nat.BC_PERIODIC if k == "periodic" else nat.BC_WALL for k in self._kinds]
+        plan = C.c_void_p()
+        nat.check(self._lib.tpf_ccd_plan_create(
+            C.byref(plan), self.ndim, nat.i64arr(self.shape), nat.darr(h), nat.iarr(bcs),
+            nat.PREC_FP64 if precision == "fp64" else nat.PREC_FP32_REFINED))
+        self._plan = plan
+        # periodic axes are factorised eagerly in the reference (ccd_solver.py:113-120)
+        for a, k in enumerate(self._kinds):
+            if k == "periodic":
+                self.axis_info(a)
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan is not None and plan.value:
+            self._lib.tpf_ccd_plan_destroy(plan)
+            self._plan = None
+
+    # -- introspection -------------------------------------------------------
+    def axis_info(self, axis: int) -> dict:
+        P, M, W = C.c_int(), C.c_int(), C.c_int()
+        t = C.c_double()
+        nat.check(self._lib.tpf_ccd_axis_info(self._plan, axis, C.byref(P), C.byref(M),
+                                               C.byref(W), C.byref(t)))
+        return {"partitions": P.value, "max_rows": M.value, "window": W.value,
+                "trunc_coeff": t.value}
+
+    # -- device-level entry points ---------------------------------------------
+    def differentiate_into(self, f: torch.Tensor, d1, d2, axis: int) -> None:
+        """Zero-copy form: f, d1, d2 are contiguous CUDA float64 tensors of
+        grid.shape (d1 or d2 may be None).  Asynchronous on torch's stream."""
+        self._check_axis(axis)
+        nat.check(self._lib.tpf_ccd_differentiate(
+            self._plan, ptr(f), ptr(d1), ptr(d2), axis, stream_handle()))
+
+    def apply_into(self, f: torch.Tensor, d1, d2, axis_mask: int) -> None:
+        """All axes in ``axis_mask`` (bit a = axis a); d1/d2 are (k, *shape)."""
+        nat.check(self._lib.tpf_ccd_apply(
+            self._plan, ptr(f), ptr(d1), ptr(d2), int(axis_mask), stream_handle()))
+
+    # -- reference API -----------------------------------------------------------
+    def differentiate(self, data, axis: int, bc_left=None, bc_right=None,
+                      apply_metric: bool = True):
+        return self._run(data, axis, bc_left, bc_right, want1=True, want2=True)
+
+    def first_derivative(self, data, axis: int, bc_left=None, bc_right=None,
+                         apply_metric: bool = True):
+        return self._run(data, axis, bc_left, bc_right, want1=True, want2=False)[0]
+
+    def second_derivative(self, data, axis: int, bc_left=None, bc_right=None,
+                          apply_metric: bool = True):
+        return self._run(data, axis, bc_left, bc_right, want1=False, want2=True)[1]
+
+    def differentiate_raw(self, data, axis: int):
+        """1-D input of length N[axis]+1 (ccd_solver.py:174-203)."""
+        arr = data if isinstance(data, torch.Tensor) else np.asarray(data)
+        n = self.shape[axis]
+        if arr.reshape(-1).shape[0] != n:
+            raise ValueError(f"differentiate_raw expects {n} values")
+        if self._kinds[axis] != "wall":
+            raise ValueError("differentiate_raw is the wall-closure path (ccd_solver.py:202)")
+        full = [self.shape[a] if a == axis else 1 for a in range(self.ndim)]
+        # A plan over a single line: same spacing and closure along `axis`.
+        lib = self._lib
+        h = [float(self.grid.L[a]) / float(self.grid.N[a]) for a in range(self.ndim)]
+        bcs = [nat.BC_WALL] * self.ndim
+        plan = C.c_void_p()
+        nat.check(lib.tpf_ccd_plan_create(C.byref(plan), self.ndim, nat.i64arr(full),
+                                          nat.darr(h), nat.iarr(bcs), nat.PREC_FP64))
+        try:
+            io = Handoff(arr)
+            f = io.to_device(arr, shape=full)
+            d1 = torch.empty_like(f)
+            d2 = torch.empty_like(f)
+            nat.check(lib.tpf_ccd_differentiate(plan, ptr(f), ptr(d1), ptr(d2), axis,
+                                                stream_handle()))
+            return io.back(d1.reshape(-1)), io.back(d2.reshape(-1))
+        finally:
+            lib.tpf_ccd_plan_destroy(plan)
+
+    def enforce_wall_neumann(self, grad, ax: int) -> None:
+        """Zero the wall planes normal to ``ax`` in place
+        (ccd_boundary_helpers.py:52-61)."""
+        if self._kinds[ax] != "wall":
+            return
+        lo = [slice(None)] * grad.ndim
+        hi = [slice(None)] * grad.ndim
+        lo[ax], hi[ax] = 0, -1
+        grad[tuple(lo)] = 0.0
+        grad[tuple(hi)] = 0.0
+
+    # -- internals -----------------------------------------------------------
+    def _check_axis(self, axis):
+        if not 0 <= axis < self.ndim:
+            raise ValueError(f"axis {axis} out of range for ndim={self.ndim}")
+
+    def _run(self, data, axis, bc_left, bc_right, want1, want2):
+        if bc_left is not None or bc_right is not None:
+            raise NotImplementedError(
+                "bc_left/bc_right overrides (testing only in the reference, "
+                "ccd_solver.py:152-153) are not provided by the HIP path")
+        self._check_axis(axis)
+        require_gpu()
+        io = Handoff(data)
+        f = io.to_device(data)
+        if tuple(f.shape) != self.shape:
+            raise ValueError(f"data shape {tuple(f.shape)} != grid.shape {self.shape}")
+        d1 = torch.empty_like(f) if want1 else None
+        d2 = torch.empty_like(f) if want2 else None
+        self.differentiate_into(f, d1, d2, axis)
+        return (io.back(d1) if want1 else None, io.back(d2) if want2 else None)

@@ -1,0 +1,42 @@
+// Host-side CCD plan objects (tables live on the host and on the device).
+#pragma once
+
+#include <vector>
+
+#include "ccd.hpp"
+#include "tpf_common.hpp"
+
+namespace tpf {
+
+struct CcdAxisHost {
+  CcdAxisGeom geom{};           // device pointers filled after upload
+  std::vector<double> rowtab;   // [P][M][kRowTab]
+  std::vector<double> qtab;     // [P][4][NQ*4]
+  double trunc_coeff = 0.0;     // largest dropped reduced-system coefficient (relative)
+  double* d_rowtab = nullptr;
+  double* d_qtab = nullptr;
+};
+
+// Smallest rows-per-partition cap in {8,16,32,64} that fits n unknowns into
+// at most kMaxParts partitions; -1 when none does.
+int choose_rows_cap(int n);
+
+// Builds the SPIKE tables of one axis.  rows_cap / parts_override <= 0 pick
+// the defaults.
+tpf_status build_axis_tables(CcdAxisHost& ax, int n_nodes, double h, int periodic,
+                             int rows_cap, int parts_override);
+
+// Scalar model of the kernel on one line with node stride `stride`.
+void host_model_line(const CcdAxisHost& ax, const double* f, int64_t stride, double* d1,
+                     double* d2);
+
+}  // namespace tpf
+
+struct tpf_ccd_plan {
+  int ndim = 0;
+  int64_t n_nodes[3] = {1, 1, 1};
+  double h[3] = {1, 1, 1};
+  int bc[3] = {0, 0, 0};
+  int precision = 0;
+  tpf::CcdAxisHost axes[3];
+};

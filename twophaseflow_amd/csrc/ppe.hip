@@ -1,0 +1,703 @@
+// Variable-density FVM PPE for gfx950: matrix-free operator, Jacobi-PCG in
+// the reference's CG system form, iteration chunks replayed as a hipGraph.
+//
+// Reference: ppe/fvm_matrixfree.py:46-363 and fvm_matrixfree_helpers.py:
+// 23-103.  The CG system is (-W L) p = -W rhs (fvm_matrixfree.py:163-202)
+// with W the tensor-product control volume (half cells on wall nodes) and
+// the pin row/column replaced by identity.  Row i of -W L is
+//     sum_a w_a(i)/h_a * sum_{nbr along a} 2/(rho_i+rho_nbr) (p_i - p_nbr),
+// w_a(i) = prod_{b != a} dv_b(i_b): the face coefficient is the harmonic
+// mean of 1/rho (fvm_matrixfree_helpers.py:61) and is recomputed from rho
+// on every application, so a matvec moves 24 B/DOF (p, rho in; Ap out).
+//
+// PCG per iteration (three streaming kernels, 13 FP64 words per DOF):
+//   K_p : p = Dinv r + beta p ;  x += alpha_prev p_old   (deferred x update)
+//   K_q : q = A p ;  <p,q>          -> last block: alpha
+//   K_r : r -= alpha q ; <r,Dinv r>, <r,r>  -> last block: beta, stop test
+// Block partial sums are combined by the last block to finish, in block
+// order, so every reduction is deterministic.  Kernels return immediately
+// once the device stop flag is set, which lets a captured chunk of
+// iterations run past convergence at no cost.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "ppe_impl.hpp"
+#include "reduce.hpp"
+
+namespace tpf {
+namespace {
+
+__device__ inline void coords(const PpeGeom& g, int64_t idx, int& i0, int& i1, int& i2) {
+  const uint32_t plane = (uint32_t)(g.n1 * g.n2);
+  const uint32_t u = (uint32_t)idx;
+  i0 = (int)(u / plane);
+  const uint32_t rem = u - (uint32_t)i0 * plane;
+  i1 = (int)(rem / (uint32_t)g.n2);
+  i2 = (int)(rem - (uint32_t)i1 * (uint32_t)g.n2);
+}
+
+__device__ inline double dv(const PpeGeom& g, int a, int i) {
+  const int n = a == 0 ? g.n0 : (a == 1 ? g.n1 : g.n2);
+  if (n == 1) return 1.0;
+  return (i == 0 || i == n - 1) ? 0.5 * g.h[a] : g.h[a];
+}
+
+// (A p)_i of the CG system (pin handled by the caller) and its diagonal,
+// evaluated in T with rho of the same precision.
+template <typename T>
+__device__ inline void stencil(const PpeGeom& g, const T* __restrict__ p,
+                               const T* __restrict__ rho, int64_t idx, int i0, int i1, int i2,
+                               T& ap, T& diag) {
+  const T r0 = rho[idx];
+  const T pc = p[idx];
+  const T dv0 = (T)dv(g, 0, i0), dv1 = (T)dv(g, 1, i1), dv2 = (T)dv(g, 2, i2);
+  const T two = (T)2;
+  T acc = 0, dg = 0;
+  if (g.n2 > 1) {  // slot 2 (contiguous)
+    const T w = dv0 * dv1 * (T)g.ih[2];
+    if (i2 > 0) {
+      const T c = w * two / (r0 + rho[idx - 1]);
+      acc += c * (pc - p[idx - 1]);
+      dg += c;
+    }
+    if (i2 < g.n2 - 1) {
+      const T c = w * two / (r0 + rho[idx + 1]);
+      acc += c * (pc - p[idx + 1]);
+      dg += c;
+    }
+  }
+  if (g.n1 > 1) {
+    const int64_t s = g.n2;
+    const T w = dv0 * dv2 * (T)g.ih[1];
+    if (i1 > 0) {
+      const T c = w * two / (r0 + rho[idx - s]);
+      acc += c * (pc - p[idx - s]);
+      dg += c;
+    }
+    if (i1 < g.n1 - 1) {
+      const T c = w * two / (r0 + rho[idx + s]);
+      acc += c * (pc - p[idx + s]);
+      dg += c;
+    }
+  }
+  if (g.n0 > 1) {
+    const int64_t s = (int64_t)g.n1 * g.n2;
+    const T w = dv1 * dv2 * (T)g.ih[0];
+    if (i0 > 0) {
+      const T c = w * two / (r0 + rho[idx - s]);
+      acc += c * (pc - p[idx - s]);
+      dg += c;
+    }
+    if (i0 < g.n0 - 1) {
+      const T c = w * two / (r0 + rho[idx + s]);
+      acc += c * (pc - p[idx + s]);
+      dg += c;
+    }
+  }
+  ap = acc;
+  diag = dg;
+}
+
+// ---- kernels ---------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_diag(PpeGeom g, const double* __restrict__ rho,
+                                                 double* __restrict__ dinv) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    double ap, dg;
+    stencil<double>(g, rho, rho, idx, i0, i1, i2, ap, dg);
+    // fvm_matrixfree_helpers.py:100-103: diag[pin] = 1, guarded reciprocal
+    if (idx == g.pin || !(fabs(dg) > 1e-30)) dg = 1.0;
+    dinv[idx] = 1.0 / dg;
+  }
+}
+
+// Reference apply (unweighted, fvm_matrixfree.py:281-300): out = L p, out[pin] = p[pin].
+__global__ __launch_bounds__(kBlock) void k_apply_L(PpeGeom g, const double* __restrict__ p,
+                                                    const double* __restrict__ rho,
+                                                    double* __restrict__ out) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    double ap, dg;
+    stencil<double>(g, p, rho, idx, i0, i1, i2, ap, dg);
+    const double W = dv(g, 0, i0) * dv(g, 1, i1) * dv(g, 2, i2);
+    out[idx] = idx == g.pin ? p[idx] : -ap / W;
+  }
+}
+
+// b = -W rhs (b[pin] = 0); r = b - A x; scalars: rr, rz, bb.
+__global__ __launch_bounds__(kBlock) void k_init(PpeGeom g, const double* __restrict__ rhs,
+                                                 const double* __restrict__ rho,
+                                                 const double* __restrict__ dinv,
+                                                 double* __restrict__ x, double* __restrict__ b,
+                                                 double* __restrict__ r, double* __restrict__ p,
+                                                 PcgState* st, double* partials, int zero_x) {
+  __shared__ double sh[3 * kBlock / 64];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    const double W = dv(g, 0, i0) * dv(g, 1, i1) * dv(g, 2, i2);
+    const double bi = idx == g.pin ? 0.0 : -W * rhs[idx];
+    double ap = 0.0, dg;
+    if (zero_x) {
+      x[idx] = 0.0;
+    } else {  // the caller zeroed x[pin] beforehand
+      stencil<double>(g, x, rho, idx, i0, i1, i2, ap, dg);
+      if (idx == g.pin) ap = x[idx];
+    }
+    const double ri = bi - ap;
+    b[idx] = bi;
+    r[idx] = ri;
+    p[idx] = 0.0;
+    v[0] += ri * ri;
+    v[1] += ri * dinv[idx] * ri;
+    v[2] += bi * bi;
+  }
+  if (grid_sum<3>(v, partials, &st->counter, sh) && threadIdx.x == 0) {
+    st->rr = v[0];
+    st->rz = v[1];
+    st->bb = v[2];
+    st->beta = 0.0;
+    st->alpha = 0.0;
+    st->iter = 0;
+    st->done = (v[0] <= st->tol2 * v[2]) ? 1 : 0;
+  }
+}
+
+// p = Dinv r + beta p ; x += alpha p_old
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update_p(PpeGeom g, const T* __restrict__ r,
+                                                     const T* __restrict__ dinv,
+                                                     T* __restrict__ p, T* __restrict__ x,
+                                                     const PcgState* st) {
+  if (st->done) return;
+  const T beta = (T)st->beta, alpha = (T)st->alpha;
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    const T po = p[idx];
+    x[idx] += alpha * po;
+    p[idx] = dinv[idx] * r[idx] + beta * po;
+  }
+}
+
+// q = A p ; <p, q> -> alpha = rz / <p,q>   (reductions in FP64)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_matvec(PpeGeom g, const T* __restrict__ p,
+                                                   const T* __restrict__ rho,
+                                                   T* __restrict__ q, PcgState* st,
+                                                   double* partials) {
+  if (st->done) return;
+  __shared__ double sh[kBlock / 64];
+  double v[1] = {0.0};
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    T ap, dg;
+    stencil<T>(g, p, rho, idx, i0, i1, i2, ap, dg);
+    if (idx == g.pin) ap = p[idx];
+    q[idx] = ap;
+    v[0] += (double)p[idx] * (double)ap;
+  }
+  if (grid_sum<1>(v, partials, &st->counter, sh) && threadIdx.x == 0) {
+    st->pq = v[0];
+    st->alpha = st->rz / v[0];
+  }
+}
+
+// r -= alpha q ; <r, Dinv r>, <r, r> -> beta, stop test  (reductions in FP64)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update_r(PpeGeom g, const T* __restrict__ q,
+                                                     const T* __restrict__ dinv,
+                                                     T* __restrict__ r, PcgState* st,
+                                                     double* partials) {
+  if (st->done) return;
+  __shared__ double sh[2 * kBlock / 64];
+  const T alpha = (T)st->alpha;
+  double v[2] = {0.0, 0.0};
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    const T ri = r[idx] - alpha * q[idx];
+    r[idx] = ri;
+    const double rd = (double)ri;
+    v[0] += rd * (double)dinv[idx] * rd;
+    v[1] += rd * rd;
+  }
+  if (grid_sum<2>(v, partials, &st->counter, sh) && threadIdx.x == 0) {
+    st->beta = v[0] / st->rz;
+    st->rz = v[0];
+    st->rr = v[1];
+    st->iter += 1;
+    if (v[1] <= st->tol2 * st->bb || st->iter >= st->maxiter) st->done = 1;
+  }
+}
+
+// x += alpha p (the update deferred past the last iteration)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_finish_x(PpeGeom g, const T* __restrict__ p,
+                                                     T* __restrict__ x, const PcgState* st) {
+  const T alpha = (T)st->alpha;
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    x[idx] += alpha * p[idx];
+    if (idx == g.pin) x[idx] = (T)0;
+  }
+}
+
+// Mixed mode: FP64 residual of the CG system r = b - A x, handed to the FP32
+// inner solve (r32 = r, x32 = p32 = 0); <r,r>, <r32,Dinv r32>, <b,b> in FP64.
+__global__ __launch_bounds__(kBlock) void k_resid_mixed(
+    PpeGeom g, const double* __restrict__ rhs, const double* __restrict__ rho,
+    const double* __restrict__ x, const float* __restrict__ dinv32, float* __restrict__ r32,
+    float* __restrict__ x32, float* __restrict__ p32, PcgState* st, double* partials) {
+  __shared__ double sh[3 * kBlock / 64];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    const double W = dv(g, 0, i0) * dv(g, 1, i1) * dv(g, 2, i2);
+    const double bi = idx == g.pin ? 0.0 : -W * rhs[idx];
+    double ap, dg;
+    stencil<double>(g, x, rho, idx, i0, i1, i2, ap, dg);
+    if (idx == g.pin) ap = x[idx];
+    const double ri = bi - ap;
+    const float rf = (float)ri;
+    r32[idx] = rf;
+    x32[idx] = 0.0f;
+    p32[idx] = 0.0f;
+    v[0] += ri * ri;
+    v[1] += (double)rf * (double)dinv32[idx] * (double)rf;
+    v[2] += bi * bi;
+  }
+  if (grid_sum<3>(v, partials, &st->counter, sh) && threadIdx.x == 0) {
+    st->true_rr = v[0];
+    st->true_bb = v[2];
+    st->rz = v[1];
+  }
+}
+
+__global__ void k_set_one(double* __restrict__ x, int64_t idx, double v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) x[idx] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_zero(int64_t n, double* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    x[i] = 0.0;
+}
+
+// x64 += x32
+__global__ __launch_bounds__(kBlock) void k_accumulate(PpeGeom g, const float* __restrict__ x32,
+                                                       double* __restrict__ x) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock)
+    x[idx] += (double)x32[idx];
+}
+
+// FP32 copies of rho and the Jacobi diagonal for the inner solve.
+__global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double* __restrict__ a,
+                                                   float* __restrict__ af,
+                                                   const double* __restrict__ b2,
+                                                   float* __restrict__ bf) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    af[i] = (float)a[i];
+    bf[i] = (float)b2[i];
+  }
+}
+
+// FP64 residual of the CG system: returns <r,r> and <b,b>.
+__global__ __launch_bounds__(kBlock) void k_true_res(PpeGeom g, const double* __restrict__ rhs,
+                                                     const double* __restrict__ rho,
+                                                     const double* __restrict__ x,
+                                                     PcgState* st, double* partials) {
+  __shared__ double sh[2 * kBlock / 64];
+  double v[2] = {0.0, 0.0};
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    const double W = dv(g, 0, i0) * dv(g, 1, i1) * dv(g, 2, i2);
+    const double bi = idx == g.pin ? 0.0 : -W * rhs[idx];
+    double ap, dg;
+    stencil<double>(g, x, rho, idx, i0, i1, i2, ap, dg);
+    if (idx == g.pin) ap = x[idx];
+    const double ri = bi - ap;
+    v[0] += ri * ri;
+    v[1] += bi * bi;
+  }
+  if (grid_sum<2>(v, partials, &st->counter, sh) && threadIdx.x == 0) {
+    st->true_rr = v[0];
+    st->true_bb = v[1];
+  }
+}
+
+int grid_for(int64_t total) {
+  int64_t blocks = (total + kBlock - 1) / kBlock;
+  const int64_t cap = 256 * 16;  // 16 blocks of 256 per CU is plenty of parallelism
+  return (int)(blocks < cap ? (blocks < 1 ? 1 : blocks) : cap);
+}
+
+}  // namespace
+}  // namespace tpf
+
+using namespace tpf;
+
+extern "C" tpf_status tpf_ppe_create(tpf_ppe** out, int ndim, const int64_t* n_nodes,
+                                     const double* L, int64_t pin_dof, void* stream) {
+  if (!out || !n_nodes || !L) return fail(TPF_ERR_VALUE, "null argument");
+  *out = nullptr;
+  if (ndim < 1 || ndim > 3) return fail(TPF_ERR_VALUE, "ndim must be 1, 2 or 3");
+  auto* s = new tpf_ppe();
+  PpeGeom& g = s->g;
+  int n[3] = {1, 1, 1};
+  double h[3] = {1.0, 1.0, 1.0};
+  for (int a = 0; a < ndim; ++a) {
+    const int slot = 3 - ndim + a;  // right-align so the last axis is contiguous
+    if (n_nodes[a] < 2 || !(L[a] > 0.0)) {
+      delete s;
+      return fail(TPF_ERR_VALUE, "PPE axis needs >= 2 nodes and L > 0");
+    }
+    n[slot] = (int)n_nodes[a];
+    h[slot] = L[a] / (double)(n_nodes[a] - 1);
+  }
+  g.n0 = n[0];
+  g.n1 = n[1];
+  g.n2 = n[2];
+  g.total = (int64_t)n[0] * n[1] * n[2];
+  if (g.total >= (int64_t)0xffffffffLL) {
+    delete s;
+    return fail(TPF_ERR_VALUE, "PPE grid too large (2^32 nodes)");
+  }
+  for (int a = 0; a < 3; ++a) {
+    g.h[a] = h[a];
+    g.ih[a] = 1.0 / h[a];
+  }
+  if (pin_dof < 0 || pin_dof >= g.total) {
+    delete s;
+    return fail(TPF_ERR_VALUE, "pin_dof out of range");
+  }
+  g.pin = pin_dof;
+  s->user_stream = (hipStream_t)stream;
+  s->grid = grid_for(g.total);
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(TPF_ERR_RUNTIME, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  const size_t bytes = sizeof(double) * (size_t)g.total;
+  double** bufs[] = {&s->dinv, &s->b, &s->r, &s->p, &s->q};
+  for (double** bp : bufs) {
+    e = hipMalloc(bp, bytes);
+    if (e != hipSuccess) {
+      tpf_ppe_destroy(s);
+      return fail(TPF_ERR_RUNTIME, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+  }
+  if (hipMalloc(&s->partials, sizeof(double) * 4 * s->grid) != hipSuccess ||
+      hipMalloc(&s->st, sizeof(PcgState)) != hipSuccess ||
+      hipHostMalloc(&s->st_host, sizeof(PcgState)) != hipSuccess) {
+    tpf_ppe_destroy(s);
+    return fail(TPF_ERR_RUNTIME, "PPE workspace allocation failed");
+  }
+  PcgState zero{};
+  if (hipMemcpy(s->st, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess) {
+    tpf_ppe_destroy(s);
+    return fail(TPF_ERR_RUNTIME, "PPE state init failed");
+  }
+  *out = s;
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ppe_destroy(tpf_ppe* s) {
+  if (!s) return TPF_OK;
+  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+  if (s->graph32_exec) (void)hipGraphExecDestroy(s->graph32_exec);
+  float* fb[] = {s->r32, s->p32, s->q32, s->x32, s->dinv32, s->rho32};
+  for (float* b : fb)
+    if (b) (void)hipFree(b);
+  if (s->st_in) (void)hipFree(s->st_in);
+  if (s->st_in_host) (void)hipHostFree(s->st_in_host);
+  double* bufs[] = {s->dinv, s->b, s->r, s->p, s->q, s->partials};
+  for (double* b : bufs)
+    if (b) (void)hipFree(b);
+  if (s->st) (void)hipFree(s->st);
+  if (s->st_host) (void)hipHostFree(s->st_host);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return TPF_OK;
+}
+
+namespace tpf {
+// Orders s->stream after the caller's stream.
+static tpf_status join_user_stream(tpf_ppe* s) {
+  hipEvent_t ev;
+  TPF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  TPF_HIP(hipEventRecord(ev, s->user_stream));
+  TPF_HIP(hipStreamWaitEvent(s->stream, ev, 0));
+  TPF_HIP(hipEventDestroy(ev));
+  return TPF_OK;
+}
+}  // namespace tpf
+
+extern "C" tpf_status tpf_ppe_set_density(tpf_ppe* s, const double* rho) {
+  if (!s || !rho) return fail(TPF_ERR_VALUE, "null argument");
+  s->rho = rho;
+  TPF_TRY(join_user_stream(s));
+  hipLaunchKernelGGL(k_diag, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g, rho, s->dinv);
+  TPF_HIP(hipGetLastError());
+  TPF_HIP(hipStreamSynchronize(s->stream));
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ppe_apply(tpf_ppe* s, const double* p, double* out) {
+  if (!s || !p || !out) return fail(TPF_ERR_VALUE, "null argument");
+  if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set (call tpf_ppe_set_density)");
+  TPF_TRY(join_user_stream(s));
+  hipLaunchKernelGGL(k_apply_L, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g, p, s->rho, out);
+  TPF_HIP(hipGetLastError());
+  TPF_HIP(hipStreamSynchronize(s->stream));
+  return TPF_OK;
+}
+
+namespace tpf {
+static tpf_status read_state(tpf_ppe* s) {
+  TPF_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(PcgState), hipMemcpyDeviceToHost, s->stream));
+  TPF_HIP(hipStreamSynchronize(s->stream));
+  return TPF_OK;
+}
+
+// Records `chunk` PCG iterations into a graph (once per handle and chunk).
+static tpf_status ensure_graph(tpf_ppe* s, double* x, int chunk) {
+  if (s->graph_exec && s->graph_x == x && s->graph_chunk == chunk && s->graph_rho == s->rho)
+    return TPF_OK;
+  if (s->graph_exec) {
+    TPF_HIP(hipGraphExecDestroy(s->graph_exec));
+    s->graph_exec = nullptr;
+  }
+  hipGraph_t graph;
+  TPF_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  for (int it = 0; it < chunk; ++it) {
+    hipLaunchKernelGGL(k_update_p<double>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
+                       (const double*)s->r, (const double*)s->dinv, s->p, x, s->st);
+    hipLaunchKernelGGL(k_matvec<double>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
+                       (const double*)s->p, s->rho, s->q, s->st, s->partials);
+    hipLaunchKernelGGL(k_update_r<double>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
+                       (const double*)s->q, (const double*)s->dinv, s->r, s->st, s->partials);
+  }
+  TPF_HIP(hipStreamEndCapture(s->stream, &graph));
+  hipError_t e = hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess)
+    return fail(TPF_ERR_RUNTIME, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  s->graph_x = x;
+  s->graph_chunk = chunk;
+  s->graph_rho = s->rho;
+  return TPF_OK;
+}
+}  // namespace tpf
+
+extern "C" tpf_status tpf_ppe_true_relres(tpf_ppe* s, const double* rhs, const double* x,
+                                          double* relres) {
+  if (!s || !rhs || !x || !relres) return fail(TPF_ERR_VALUE, "null argument");
+  if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set");
+  TPF_TRY(join_user_stream(s));
+  hipLaunchKernelGGL(k_true_res, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g, rhs, s->rho, x,
+                     s->st, s->partials);
+  TPF_HIP(hipGetLastError());
+  TPF_TRY(read_state(s));
+  const double bb = s->st_host->true_bb;
+  *relres = bb > 0.0 ? std::sqrt(s->st_host->true_rr / bb) : std::sqrt(s->st_host->true_rr);
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ppe_solve(tpf_ppe* s, const double* rhs, double* x,
+                                    const tpf_solve_opts* opts, tpf_solve_report* rep) {
+  if (!s || !rhs || !x || !opts) return fail(TPF_ERR_VALUE, "null argument");
+  if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set (call tpf_ppe_set_density)");
+  if (opts->precond != 1 && opts->precond != 0)
+    return fail(TPF_ERR_VALUE, "CG supports preconditioner 'jacobi' or 'none'");
+  if (opts->mixed) return tpf_ppe_solve_mixed(s, rhs, x, opts, rep);
+  if (opts->precond == 0) return fail(TPF_ERR_VALUE, "unpreconditioned CG is not built");
+  if (!(opts->tol > 0.0) || opts->maxiter < 1) return fail(TPF_ERR_VALUE, "bad tol/maxiter");
+  TPF_TRY(join_user_stream(s));
+  if (opts->use_x0) {  // x0[pin] = 0 (fvm_matrixfree.py:185-186)
+    hipLaunchKernelGGL(k_set_one, dim3(1), dim3(64), 0, s->stream, x, s->g.pin, 0.0);
+    TPF_HIP(hipGetLastError());
+  }
+  PcgState init{};
+  init.tol2 = opts->tol * opts->tol;
+  init.maxiter = opts->maxiter;
+  *s->st_host = init;  // pinned staging for the async copy
+  TPF_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(init), hipMemcpyHostToDevice, s->stream));
+  hipLaunchKernelGGL(k_init, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g, rhs, s->rho,
+                     s->dinv, x, s->b, s->r, s->p, s->st, s->partials, opts->use_x0 ? 0 : 1);
+  TPF_HIP(hipGetLastError());
+  const int chunk = opts->check_every > 0 ? opts->check_every : 32;
+  TPF_TRY(ensure_graph(s, x, chunk));
+  TPF_TRY(read_state(s));
+  while (!s->st_host->done) {
+    TPF_HIP(hipGraphLaunch(s->graph_exec, s->stream));
+    TPF_TRY(read_state(s));
+  }
+  hipLaunchKernelGGL(k_finish_x<double>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
+                     (const double*)s->p, x, s->st);
+  TPF_HIP(hipGetLastError());
+  const int64_t iters = s->st_host->iter;
+  const double rr = s->st_host->rr, bb = s->st_host->bb;
+  double tr = 0.0;
+  TPF_TRY(tpf_ppe_true_relres(s, rhs, x, &tr));
+  tpf_solve_report r{};
+  r.iterations = iters;
+  r.relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+  r.true_relres = tr;
+  r.converged = tr <= opts->tol ? 1 : 0;
+  r.refinements = 0;
+  if (rep) *rep = r;
+  if (r.relres > opts->tol)
+    return fail(TPF_ERR_RUNTIME, "PPE CG did not converge cleanly (maxiter " +
+                                     std::to_string(opts->maxiter) + " reached, ||r||/||b|| = " +
+                                     std::to_string(r.relres) + ")");
+  return TPF_OK;
+}
+
+// ---- mixed precision: FP32 inner PCG under FP64 iterative refinement -------
+//
+// Outer loop (FP64): r = b - A x on the reference's CG system, stop when
+// ||r||/||b|| <= tol.  Inner loop (FP32 vectors, FP32 rho, FP64 scalars):
+// PCG on A d = r to a relative reduction kInnerTol, then x += d.
+namespace tpf {
+namespace {
+constexpr double kInnerTol = 1e-4;
+constexpr int kMaxOuter = 40;
+
+tpf_status ensure_mixed(tpf_ppe* s) {
+  const size_t n = (size_t)s->g.total;
+  if (!s->r32) {
+    float** fb[] = {&s->r32, &s->p32, &s->q32, &s->x32, &s->dinv32, &s->rho32};
+    for (float** b : fb) TPF_HIP(hipMalloc(b, n * sizeof(float)));
+    TPF_HIP(hipMalloc(&s->st_in, sizeof(PcgState)));
+    TPF_HIP(hipHostMalloc(&s->st_in_host, sizeof(PcgState)));
+    PcgState zero{};
+    TPF_HIP(hipMemcpy(s->st_in, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  }
+  if (s->rho32_src != s->rho) {
+    hipLaunchKernelGGL(k_to_f32, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g.total, s->rho,
+                       s->rho32, (const double*)s->dinv, s->dinv32);
+    TPF_HIP(hipGetLastError());
+    s->rho32_src = s->rho;
+  }
+  return TPF_OK;
+}
+
+tpf_status ensure_graph32(tpf_ppe* s, int chunk) {
+  if (s->graph32_exec && s->graph32_chunk == chunk && s->graph32_rho == s->rho) return TPF_OK;
+  if (s->graph32_exec) {
+    TPF_HIP(hipGraphExecDestroy(s->graph32_exec));
+    s->graph32_exec = nullptr;
+  }
+  hipGraph_t graph;
+  TPF_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  for (int it = 0; it < chunk; ++it) {
+    hipLaunchKernelGGL(k_update_p<float>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
+                       (const float*)s->r32, (const float*)s->dinv32, s->p32, s->x32, s->st_in);
+    hipLaunchKernelGGL(k_matvec<float>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
+                       (const float*)s->p32, (const float*)s->rho32, s->q32, s->st_in,
+                       s->partials);
+    hipLaunchKernelGGL(k_update_r<float>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
+                       (const float*)s->q32, (const float*)s->dinv32, s->r32, s->st_in,
+                       s->partials);
+  }
+  TPF_HIP(hipStreamEndCapture(s->stream, &graph));
+  hipError_t e = hipGraphInstantiate(&s->graph32_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess)
+    return fail(TPF_ERR_RUNTIME, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  s->graph32_chunk = chunk;
+  s->graph32_rho = s->rho;
+  return TPF_OK;
+}
+
+tpf_status read_inner(tpf_ppe* s) {
+  TPF_HIP(hipMemcpyAsync(s->st_in_host, s->st_in, sizeof(PcgState), hipMemcpyDeviceToHost,
+                         s->stream));
+  TPF_HIP(hipStreamSynchronize(s->stream));
+  return TPF_OK;
+}
+}  // namespace
+}  // namespace tpf
+
+tpf_status tpf_ppe_solve_mixed(tpf_ppe* s, const double* rhs, double* x,
+                               const tpf_solve_opts* opts, tpf_solve_report* rep) {
+  if (opts->precond != 1) return fail(TPF_ERR_VALUE, "mixed solve needs the Jacobi preconditioner");
+  if (!(opts->tol > 0.0) || opts->maxiter < 1) return fail(TPF_ERR_VALUE, "bad tol/maxiter");
+  TPF_TRY(join_user_stream(s));
+  TPF_TRY(ensure_mixed(s));
+  if (opts->use_x0) {  // x0[pin] = 0 (fvm_matrixfree.py:185-186)
+    hipLaunchKernelGGL(k_set_one, dim3(1), dim3(64), 0, s->stream, x, s->g.pin, 0.0);
+    TPF_HIP(hipGetLastError());
+  } else {
+    hipLaunchKernelGGL(k_fill_zero, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g.total, x);
+    TPF_HIP(hipGetLastError());
+  }
+  const int chunk = opts->check_every > 0 ? opts->check_every : 32;
+  TPF_TRY(ensure_graph32(s, chunk));
+  int64_t total_inner = 0;
+  int outer = 0;
+  double relres = 0.0;
+  for (;;) {
+    hipLaunchKernelGGL(k_resid_mixed, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g, rhs,
+                       s->rho, (const double*)x, (const float*)s->dinv32, s->r32, s->x32, s->p32,
+                       s->st_in, s->partials);
+    TPF_HIP(hipGetLastError());
+    TPF_TRY(read_inner(s));
+    PcgState st = *s->st_in_host;
+    const double bb = st.true_bb;
+    relres = bb > 0.0 ? std::sqrt(st.true_rr / bb) : std::sqrt(st.true_rr);
+    if (relres <= opts->tol || outer >= kMaxOuter || total_inner >= opts->maxiter) break;
+    st.rr = st.true_rr;
+    st.bb = st.true_rr;  // inner stop: ||r_in|| <= kInnerTol ||r_outer||
+    st.tol2 = kInnerTol * kInnerTol;
+    st.alpha = st.beta = 0.0;
+    st.iter = 0;
+    st.maxiter = opts->maxiter - total_inner;
+    st.done = 0;
+    st.counter = 0;
+    *s->st_in_host = st;  // pinned staging for the async copy
+    TPF_HIP(hipMemcpyAsync(s->st_in, s->st_in_host, sizeof(st), hipMemcpyHostToDevice, s->stream));
+    do {
+      TPF_HIP(hipGraphLaunch(s->graph32_exec, s->stream));
+      TPF_TRY(read_inner(s));
+    } while (!s->st_in_host->done);
+    hipLaunchKernelGGL(k_finish_x<float>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
+                       (const float*)s->p32, s->x32, s->st_in);
+    hipLaunchKernelGGL(k_accumulate, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
+                       (const float*)s->x32, x);
+    TPF_HIP(hipGetLastError());
+    total_inner += s->st_in_host->iter;
+    ++outer;
+  }
+  TPF_HIP(hipStreamSynchronize(s->stream));
+  tpf_solve_report r{};
+  r.iterations = total_inner;
+  r.relres = relres;
+  r.true_relres = relres;
+  r.converged = relres <= opts->tol ? 1 : 0;
+  r.refinements = outer;
+  if (rep) *rep = r;
+  if (!r.converged)
+    return fail(TPF_ERR_RUNTIME, "mixed-precision PPE did not converge (||r||/||b|| = " +
+                                     std::to_string(relres) + " after " +
+                                     std::to_string(total_inner) + " inner iterations)");
+  return TPF_OK;
+}

@@ -1,0 +1,58 @@
+// PPE handle, geometry and device-side solver state.
+#pragma once
+
+#include <cstdint>
+
+#include "tpf_common.hpp"
+
+namespace tpf {
+
+// Node grid right-aligned into 3 slots (missing leading axes have n = 1);
+// slot 2 is the contiguous axis.
+struct PpeGeom {
+  int n0, n1, n2;
+  int64_t total;
+  int64_t pin;
+  double h[3];   // node spacing per slot
+  double ih[3];  // 1 / h
+};
+
+// Device-resident PCG scalars (updated by the last block of each kernel).
+struct PcgState {
+  double rr, rz, bb, pq, alpha, beta, tol2;
+  double true_rr, true_bb;
+  double scale;             // mixed mode: FP64 residual norm carried into FP32
+  int64_t iter, maxiter;
+  int done;
+  unsigned counter;
+};
+
+}  // namespace tpf
+
+struct tpf_ppe {
+  tpf::PpeGeom g{};
+  hipStream_t user_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int grid = 0;
+  const double* rho = nullptr;
+  double *dinv = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
+  double* partials = nullptr;
+  tpf::PcgState* st = nullptr;
+  tpf::PcgState* st_host = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  const double* graph_x = nullptr;
+  const double* graph_rho = nullptr;
+  int graph_chunk = 0;
+  // mixed-precision workspace (FP32 inner solve), allocated on first use
+  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr, *dinv32 = nullptr,
+        *rho32 = nullptr;
+  tpf::PcgState* st_in = nullptr;       // inner (FP32) solve state
+  tpf::PcgState* st_in_host = nullptr;
+  const double* rho32_src = nullptr;    // rho the FP32 copies were made from
+  hipGraphExec_t graph32_exec = nullptr;
+  int graph32_chunk = 0;
+  const double* graph32_rho = nullptr;
+};
+
+tpf_status tpf_ppe_solve_mixed(tpf_ppe* s, const double* rhs, double* x,
+                               const tpf_solve_opts* opts, tpf_solve_report* rep);
