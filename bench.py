@@ -74,30 +74,37 @@ def per_axis_event_times(ccd, f, d1, d2, reps):
     return out
 
 
-def cpu_baseline_ccd(N=256, slab=16, seed=2):
-    """Reference CPU path (dense LU per axis, scipy lu_solve) on a slab of
-    `slab` planes of the 257^3 periodic field, all three axes."""
-    import numpy as np
+def cpu_baseline_ccd(N=256, seed=2):
+    """Reference CPU path (dense LU per axis with SciPy lu_solve, the oracle
+    restatement of ccd_solver_helpers.py:406-465) on the full 257^3 field, all
+    three axes, with every host core; plus a 1-thread figure on a 17-plane slab."""
+    from threadpoolctl import threadpool_limits
     from oracle import ccd as occd
     from twophaseflow_amd.fields import band_limited_periodic
-    f = band_limited_periodic(N, 3, kmax=32, seed=seed)[:, :, :slab + 1].copy()
-    h = 1.0 / N
-    infos = [occd.build_periodic_axis(N + 1, h), occd.build_periodic_axis(N + 1, h)]
-    t0 = time.perf_counter()
-    occd.differentiate_periodic(f, 0, infos[0])
-    occd.differentiate_periodic(f, 1, infos[1])
-    t1 = time.perf_counter()
-    pts = f.size
-    # axis 2 of the full field: same per-line cost as axes 0 and 1
-    rate = 2 * pts / (t1 - t0) / 1e6
+    f = band_limited_periodic(N, 3, kmax=32, seed=seed)
+    info = occd.build_periodic_axis(N + 1, 1.0 / N)
     try:
         cores = len(os.sched_getaffinity(0))
     except Exception:
         cores = os.cpu_count() or 1
-    threads = int(os.environ.get("OPENBLAS_NUM_THREADS", os.environ.get("OMP_NUM_THREADS", cores)))
-    return {"value": rate, "unit": "Mpts/s", "cores": threads, "kind": "port",
-            "sample": f"dense-LU CCD (scipy lu_solve, oracle restatement) on a 257x257x{slab+1} "
-                      f"slab of the 257^3 periodic field, axes 0 and 1 ({t1 - t0:.1f} s)"}
+    cores = min(cores, 16)
+    with threadpool_limits(limits=cores):
+        t0 = time.perf_counter()
+        for axis in range(3):
+            occd.differentiate_periodic(f, axis, info)
+        t_all = time.perf_counter() - t0
+    slab = f[:, :, :17].copy()
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        occd.differentiate_periodic(slab, 0, info)
+        occd.differentiate_periodic(slab, 1, info)
+        t_one = time.perf_counter() - t0
+    return {"value": round(3 * f.size / t_all / 1e6, 2), "unit": "Mpts/s", "cores": cores,
+            "kind": "port",
+            "sample": f"full workload: dense-LU CCD (SciPy lu_solve, oracle restatement) on the "
+                      f"257^3 periodic field, 3 axes, {cores} BLAS threads, {t_all:.2f} s",
+            "one_thread_Mpts_per_s": round(2 * slab.size / t_one / 1e6, 2),
+            "one_thread_sample": f"257x257x17 slab, axes 0 and 1, {t_one:.2f} s"}
 
 
 def ppe_case(N=256, nb=64, seed=3, ratio=1e-3):
@@ -119,7 +126,7 @@ def ppe_case(N=256, nb=64, seed=3, ratio=1e-3):
     return g, rho, rhs
 
 
-def ppe_solve_timed(g, rho, rhs, precision, tol=1e-8, maxiter=200000):
+def ppe_solve_timed(g, rho, rhs, precision, tol=1e-8, maxiter=200000, inner_tol=0.0):
     import torch
     from twophaseflow_amd import PPESolverFVMMatrixFree, SimulationConfig, SolverConfig
     cfg = SimulationConfig(solver=SolverConfig(pseudo_tol=tol, pseudo_maxiter=maxiter))
@@ -127,6 +134,7 @@ def ppe_solve_timed(g, rho, rhs, precision, tol=1e-8, maxiter=200000):
     cfg.solver.ppe_iteration_method = "cg"
     cfg.solver.ppe_precision = precision
     cfg.solver.ppe_check_every = 64
+    cfg.solver.ppe_inner_tol = inner_tol
     s = PPESolverFVMMatrixFree(None, cfg, g)
     p = torch.empty_like(rhs)
     torch.cuda.synchronize()
@@ -139,6 +147,30 @@ def ppe_solve_timed(g, rho, rhs, precision, tol=1e-8, maxiter=200000):
             "true_relres": tr, "converged": bool(tr <= tol)}
 
 
+def profile_mode(args):
+    import torch
+    from twophaseflow_amd import CCDSolver, Grid, GridConfig
+    from twophaseflow_amd.fields import band_limited_periodic
+    if args.profile == "ccd":
+        N = 256
+        f = torch.from_numpy(band_limited_periodic(N, 3, kmax=32, seed=2)).cuda()
+        g = Grid(GridConfig(ndim=3, N=(N, N, N), L=(1.0, 1.0, 1.0)))
+        ccd = CCDSolver(g, bc_type="periodic")
+        d1 = torch.empty((3,) + g.shape, dtype=torch.float64, device="cuda")
+        d2 = torch.empty_like(d1)
+        for _ in range(args.steps):
+            ccd.apply_into(f, d1, d2, 0b111)
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"profile ccd: {args.steps} steps of 3 axis launches on 257^3, then a 1 GiB copy")
+    else:
+        gp, rho, rhs = ppe_case(N=128)
+        r = ppe_solve_timed(gp, rho, rhs, "fp64")
+        log(f"profile ppe 129^3: {r}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +179,10 @@ def main():
     ap.add_argument("--no-ppe", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ppe-n", type=int, default=256)
+    ap.add_argument("--profile", choices=["ccd", "ppe"], default=None,
+                    help="run only one workload (for rocprofv3): 'ccd' = K all-axes CCD steps "
+                         "on 257^3 plus one 1 GiB device copy (counter scale reference); "
+                         "'ppe' = one FP64 PCG solve of the 129^3 ratio-1000 case")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
@@ -157,6 +193,8 @@ def main():
     from twophaseflow_amd import CCDSolver, Grid, GridConfig
     from twophaseflow_amd.fields import band_limited_periodic
     torch.cuda.set_device(0)
+    if args.profile:
+        return profile_mode(args)
 
     # ---- CCD 3-D 257^3 periodic, all axes, FP64
     N = 256
@@ -219,12 +257,15 @@ def main():
         log(f"building PPE case {args.ppe_n}^3")
         gp, rho, rhs = ppe_case(N=args.ppe_n)
         res = {}
-        for prec in ("fp64", "mixed"):
-            log(f"PPE solve {prec}")
-            res[prec] = ppe_solve_timed(gp, rho, rhs, prec)
-            log(f"  {res[prec]}")
+        for key, prec, itol in (("fp64", "fp64", 0.0), ("mixed", "mixed", 1e-4),
+                                ("mixed_1e-3", "mixed", 1e-3), ("mixed_1e-5", "mixed", 1e-5)):
+            log(f"PPE solve {key}")
+            res[key] = ppe_solve_timed(gp, rho, rhs, prec, inner_tol=itol)
+            log(f"  {res[key]}")
         result["ppe"] = {"grid": [args.ppe_n + 1] * 3, "density_ratio": 1000, "bubbles": 64,
-                         "tol": 1e-8, "plain_pcg_fp64": res["fp64"], "pcg_mixed": res["mixed"]}
+                         "tol": 1e-8, "plain_pcg_fp64": res["fp64"], "pcg_mixed": res["mixed"],
+                         "pcg_mixed_inner_1e-3": res["mixed_1e-3"],
+                         "pcg_mixed_inner_1e-5": res["mixed_1e-5"]}
         del rho, rhs
 
     if not args.no_cpu:

@@ -82,6 +82,7 @@ typedef struct {
   int check_every;     /* host convergence check period in iterations      */
   int use_x0;          /* 1: p holds the initial guess (p_init)            */
   int mixed;           /* 1: FP32 matvec/inner solve + FP64 refinement     */
+  double inner_tol;    /* mixed: relative reduction per FP32 inner solve (0 = 1e-4) */
 } tpf_solve_opts;
 
 typedef struct {

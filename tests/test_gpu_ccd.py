@@ -34,7 +34,8 @@ def _grid(N, L=None):
 
 
 CASES = [((8, 6), "wall"), ((64, 64), "wall"), ((64, 64), "periodic"),
-         ((32, 19, 8), ("wall", "periodic", "wall")), ((256, 2), "periodic"),
+         ((32, 19, 8), ("wall", "periodic", "wall")), ((256, 2), "periodic"), ((256, 3), "periodic"),
+         ((1024, 8), "wall"), ((640, 9), "periodic"),
          ((255, 3), "wall"), ((5, 5), "wall"), ((3, 4), "periodic"), ((128, 6), "periodic_wall"),
          ((40, 40, 40), "periodic"), ((33, 17, 65), "wall"), ((99, 7), "wall_periodic")]
 
@@ -44,8 +45,13 @@ def test_ccd_matches_oracle(N, bc):
     _announce(f"ccd N={N} bc={bc}")
     L = [1.0 + 0.25 * a for a in range(len(N))]
     g = _grid(N, L)
-    ccd = CCDSolver(g, bc_type=bc)
     o = occd.OracleCCD(N, L, bc)
+    if any(k == "periodic" and n < 3 for k, n in zip(o.kinds, N)):
+        # periodic axes are factorised at construction (ccd_solver.py:113-120)
+        with pytest.raises(ValueError, match="3 unique nodes"):
+            CCDSolver(g, bc_type=bc)
+        return
+    ccd = CCDSolver(g, bc_type=bc)
     f = np.random.default_rng(3).standard_normal(g.shape)
     for axis in range(len(N)):
         kind = o.kinds[axis]
@@ -112,7 +118,13 @@ def test_wall_cubic_exact_and_nullspace():
     assert 9 - np.linalg.matrix_rank(D2, tol=1e-8) == 2
 
 
-@pytest.mark.parametrize("N", [1024, 2048])
+def test_lines_beyond_1024_unknowns_rejected():
+    """One workgroup holds a whole line: 16 partitions x 64 rows at most."""
+    with pytest.raises(ValueError, match="too long"):
+        CCDSolver(_grid((2048, 3)), bc_type="periodic")
+
+
+@pytest.mark.parametrize("N", [512, 1024])
 def test_long_periodic_lines_against_spectral(N):
     """2-D 1024^2 class lines: f'' bound is max(1e-12, 2 x dense-LU error)."""
     _announce(f"ccd long lines N={N}")

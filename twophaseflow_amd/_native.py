@@ -26,7 +26,8 @@ _vp = C.c_void_p
 
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int64), ("precond", C.c_int),
-                ("check_every", C.c_int), ("use_x0", C.c_int), ("mixed", C.c_int)]
+                ("check_every", C.c_int), ("use_x0", C.c_int), ("mixed", C.c_int),
+                ("inner_tol", C.c_double)]
 
 
 class SolveReport(C.Structure):

@@ -67,6 +67,20 @@ __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdAxisGeom g, CcdLines 
     }
   }
 
+  // ---- per-row LU/spike tables and coupling rows -> LDS (all WGs share them;
+  // reading them from LDS keeps scalar-cache misses off the dependent sweeps)
+  constexpr bool kLdsTab = M <= 32;
+  const int NQ = g.NQ;
+  double* ltab = lds;
+  double* lq = lds + (kLdsTab ? g.P * M * kRowTab : 0);
+  double* G = lq + (kLdsTab ? g.P * 16 * NQ : 0);  // [4P][64]
+  if (kLdsTab) {
+    const int nt = g.P * M * kRowTab, nq = g.P * 16 * NQ;
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) ltab[i] = g.rowtab[i];
+    for (int i = threadIdx.x; i < nq; i += blockDim.x) lq[i] = g.qtab[i];
+    __syncthreads();
+  }
+
   // ---- wall closure right-hand sides (ccd_boundary_helpers.py:64-116)
   double cl0 = 0.0, cl1 = 0.0;  // L . bc_lo  for the first interior row
   double bl0 = 0.0, bl1 = 0.0;  // bc_lo itself (node-0 output)
@@ -91,7 +105,7 @@ __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdAxisGeom g, CcdLines 
   }
 
   // ---- RHS + forward sweep: y_r = R_r - Lf_r y_{r-1}
-  const double* __restrict__ tab = g.rowtab + (size_t)p * M * kRowTab;
+  const double* tab = (kLdsTab ? ltab : g.rowtab) + (size_t)p * M * kRowTab;
   double x[M][2];
 #pragma unroll
   for (int r = 0; r < M; ++r) {
@@ -116,7 +130,6 @@ __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdAxisGeom g, CcdLines 
     }
   }
   // ---- backward sweep: x_r = iD_r (y_r - U x_{r+1}); partition ends -> LDS
-  double* G = lds;  // [4P][64]
 #pragma unroll
   for (int r = M - 1; r >= 0; --r) {
     if (r < m) {
@@ -141,8 +154,7 @@ __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdAxisGeom g, CcdLines 
   // ---- coupling: (left, right) end values of the neighbours = Q_p . G
   double lr0 = 0.0, lr1 = 0.0, lr2 = 0.0, lr3 = 0.0;
   {
-    const int NQ = g.NQ;
-    const double* __restrict__ q = g.qtab + (size_t)p * 4 * NQ * 4;
+    const double* q = (kLdsTab ? lq : g.qtab) + (size_t)p * 4 * NQ * 4;
     for (int t = 0; t < NQ; ++t) {
       int qq = g.rel ? p + g.d0 + t : t;
       if (periodic) qq = qq < 0 ? qq + g.P : (qq >= g.P ? qq - g.P : qq);
@@ -259,9 +271,11 @@ KernelFn pick_kernel(int M, bool contig) {
   return nullptr;
 }
 
-size_t lds_bytes(int P, bool contig) {
-  size_t gbytes = sizeof(double) * 4 * P * 64;
-  size_t sbytes = contig ? sizeof(double) * P * 64 * kStagePitch : 0;
+size_t lds_bytes(const CcdAxisGeom& g, bool contig) {
+  const bool tab = g.M <= 32;  // matches kLdsTab in the kernel
+  size_t gbytes = sizeof(double) * ((tab ? (size_t)g.P * g.M * kRowTab + (size_t)g.P * 16 * g.NQ : 0) +
+                                    (size_t)4 * g.P * 64);
+  size_t sbytes = contig ? sizeof(double) * g.P * 64 * kStagePitch : 0;
   return gbytes > sbytes ? gbytes : sbytes;
 }
 
@@ -321,7 +335,7 @@ tpf_status launch_axis(tpf_ccd_plan* plan, int axis, const double* f, double* d1
   const bool contig = ln.stride == 1;
   KernelFn k = pick_kernel(ax.geom.M, contig);
   if (!k) return fail(TPF_ERR_VALUE, "unsupported CCD rows cap " + std::to_string(ax.geom.M));
-  const size_t shmem = lds_bytes(ax.geom.P, contig);
+  const size_t shmem = lds_bytes(ax.geom, contig);
   TPF_TRY(raise_lds_limit((const void*)k, shmem));
   if (ln.n_lines > (int64_t)0x7fffffff)
     return fail(TPF_ERR_VALUE, "too many CCD lines for one launch");

@@ -45,6 +45,20 @@ __device__ inline double dv(const PpeGeom& g, int a, int i) {
   return (i == 0 || i == n - 1) ? 0.5 * g.h[a] : g.h[a];
 }
 
+// 1/x from the hardware reciprocal refined by Newton steps (x > 0 here:
+// sums of two densities); avoids the long IEEE division sequence.
+__device__ inline double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+__device__ inline float recip(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+
 // (A p)_i of the CG system (pin handled by the caller) and its diagonal,
 // evaluated in T with rho of the same precision.
 template <typename T>
@@ -59,12 +73,12 @@ __device__ inline void stencil(const PpeGeom& g, const T* __restrict__ p,
   if (g.n2 > 1) {  // slot 2 (contiguous)
     const T w = dv0 * dv1 * (T)g.ih[2];
     if (i2 > 0) {
-      const T c = w * two / (r0 + rho[idx - 1]);
+      const T c = w * two * recip(r0 + rho[idx - 1]);
       acc += c * (pc - p[idx - 1]);
       dg += c;
     }
     if (i2 < g.n2 - 1) {
-      const T c = w * two / (r0 + rho[idx + 1]);
+      const T c = w * two * recip(r0 + rho[idx + 1]);
       acc += c * (pc - p[idx + 1]);
       dg += c;
     }
@@ -73,12 +87,12 @@ __device__ inline void stencil(const PpeGeom& g, const T* __restrict__ p,
     const int64_t s = g.n2;
     const T w = dv0 * dv2 * (T)g.ih[1];
     if (i1 > 0) {
-      const T c = w * two / (r0 + rho[idx - s]);
+      const T c = w * two * recip(r0 + rho[idx - s]);
       acc += c * (pc - p[idx - s]);
       dg += c;
     }
     if (i1 < g.n1 - 1) {
-      const T c = w * two / (r0 + rho[idx + s]);
+      const T c = w * two * recip(r0 + rho[idx + s]);
       acc += c * (pc - p[idx + s]);
       dg += c;
     }
@@ -87,12 +101,12 @@ __device__ inline void stencil(const PpeGeom& g, const T* __restrict__ p,
     const int64_t s = (int64_t)g.n1 * g.n2;
     const T w = dv1 * dv2 * (T)g.ih[0];
     if (i0 > 0) {
-      const T c = w * two / (r0 + rho[idx - s]);
+      const T c = w * two * recip(r0 + rho[idx - s]);
       acc += c * (pc - p[idx - s]);
       dg += c;
     }
     if (i0 < g.n0 - 1) {
-      const T c = w * two / (r0 + rho[idx + s]);
+      const T c = w * two * recip(r0 + rho[idx + s]);
       acc += c * (pc - p[idx + s]);
       dg += c;
     }
@@ -667,7 +681,8 @@ tpf_status tpf_ppe_solve_mixed(tpf_ppe* s, const double* rhs, double* x,
     if (relres <= opts->tol || outer >= kMaxOuter || total_inner >= opts->maxiter) break;
     st.rr = st.true_rr;
     st.bb = st.true_rr;  // inner stop: ||r_in|| <= kInnerTol ||r_outer||
-    st.tol2 = kInnerTol * kInnerTol;
+    const double itol = opts->inner_tol > 0.0 ? opts->inner_tol : kInnerTol;
+    st.tol2 = itol * itol;
     st.alpha = st.beta = 0.0;
     st.iter = 0;
     st.maxiter = opts->maxiter - total_inner;

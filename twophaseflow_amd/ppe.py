@@ -89,6 +89,7 @@ class PPESolverFVMMatrixFree(IPPESolver):
         if self.precision not in ("fp64", "mixed"):
             raise ValueError("ppe_precision must be 'fp64' or 'mixed'")
         self.check_every = int(getattr(solver_cfg, "ppe_check_every", 32))
+        self.inner_tol = float(getattr(solver_cfg, "ppe_inner_tol", 0.0))
         spec = bc_spec if bc_spec is not None else BoundarySpec(bc_type, grid.shape, grid.N)
         self._pin_dof = spec.pin_dof
         self.shape = tuple(int(n) + 1 for n in grid.N)
@@ -157,7 +158,8 @@ class PPESolverFVMMatrixFree(IPPESolver):
             raise RuntimeError("no density set")
         opts = nat.SolveOpts(tol=self.tol, maxiter=self.maxiter, precond=1,
                              check_every=self.check_every, use_x0=1 if use_x0 else 0,
-                             mixed=1 if self.precision == "mixed" else 0)
+                             mixed=1 if self.precision == "mixed" else 0,
+                             inner_tol=self.inner_tol)
         rep = nat.SolveReport()
         status = self._lib.tpf_ppe_solve(self._handle, ptr(rhs), ptr(p), C.byref(opts),
                                          C.byref(rep))
