@@ -28,15 +28,33 @@ namespace {
 constexpr int kStageRows = 16;              // rows per LDS re-staging chunk
 constexpr int kStagePitch = kStageRows + 1; // padded line pitch (doubles)
 
-template <int M, bool CONTIG>
-__global__ __launch_bounds__(1024) void ccd_line_kernel(CcdAxisGeom g, CcdLines ln,
-                                                        const double* __restrict__ f,
-                                                        double* __restrict__ d1,
-                                                        double* __restrict__ d2) {
+// One axis slot of a launch: the axis geometry, its line addressing and the
+// arrays it reads and writes.  A launch covers up to 3 slots (all axes of
+// tpf_ccd_apply in one grid); workgroups [block_begin[s], block_begin[s+1])
+// belong to slot s.
+struct CcdSlot {
+  CcdAxisGeom g;
+  CcdLines ln;
+  const double* f;
+  double* d1;
+  double* d2;
+  int contig;  // line nodes are contiguous (last axis)
+};
+struct CcdLaunch {
+  int nslots;
+  unsigned block_begin[4];
+  CcdSlot slot[3];
+};
+
+template <int M>
+__device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& ln,
+                                          const bool CONTIG, const double* __restrict__ f,
+                                          double* __restrict__ d1, double* __restrict__ d2,
+                                          const unsigned block) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x & 63;
   const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t b = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t b = (int64_t)block * 64 + lane;
   const bool active = b < ln.n_lines;
   int64_t base = 0;
   if (active) {
@@ -228,7 +246,7 @@ __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdAxisGeom g, CcdLines 
     // 4 lines x 16 consecutive nodes.
     __syncthreads();  // G no longer read; its space is reused
     double* S = lds + (size_t)p * 64 * kStagePitch;
-    const int64_t line0 = (int64_t)blockIdx.x * 64;
+    const int64_t line0 = (int64_t)block * 64;
 #pragma unroll
     for (int c0 = 0; c0 < M; c0 += kStageRows) {
 #pragma unroll
@@ -254,19 +272,22 @@ __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdAxisGeom g, CcdLines 
   }
 }
 
-using KernelFn = void (*)(CcdAxisGeom, CcdLines, const double*, double*, double*);
-
-template <int M, bool C>
-KernelFn kernel_ptr() {
-  return ccd_line_kernel<M, C>;
+template <int M>
+__global__ __launch_bounds__(1024) void ccd_line_kernel(CcdLaunch L) {
+  const unsigned blk = blockIdx.x;
+  const int sl = (L.nslots > 1 && blk >= L.block_begin[1]) + (L.nslots > 2 && blk >= L.block_begin[2]);
+  const CcdSlot& S = L.slot[sl];
+  ccd_lines<M>(S.g, S.ln, S.contig != 0, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
 }
 
-KernelFn pick_kernel(int M, bool contig) {
+using KernelFn = void (*)(CcdLaunch);
+
+KernelFn pick_kernel(int M) {
   switch (M) {
-    case 8: return contig ? kernel_ptr<8, true>() : kernel_ptr<8, false>();
-    case 16: return contig ? kernel_ptr<16, true>() : kernel_ptr<16, false>();
-    case 32: return contig ? kernel_ptr<32, true>() : kernel_ptr<32, false>();
-    case 64: return contig ? kernel_ptr<64, true>() : kernel_ptr<64, false>();
+    case 8: return ccd_line_kernel<8>;
+    case 16: return ccd_line_kernel<16>;
+    case 32: return ccd_line_kernel<32>;
+    case 64: return ccd_line_kernel<64>;
   }
   return nullptr;
 }
@@ -328,19 +349,38 @@ tpf_status raise_lds_limit(const void* k, size_t bytes) {
   return TPF_OK;
 }
 
-tpf_status launch_axis(tpf_ccd_plan* plan, int axis, const double* f, double* d1, double* d2,
-                       hipStream_t stream) {
-  CcdAxisHost& ax = plan->axes[axis];
-  const CcdLines ln = lines_for(plan, axis);
-  const bool contig = ln.stride == 1;
-  KernelFn k = pick_kernel(ax.geom.M, contig);
-  if (!k) return fail(TPF_ERR_VALUE, "unsupported CCD rows cap " + std::to_string(ax.geom.M));
-  const size_t shmem = lds_bytes(ax.geom, contig);
+// Launches the given axes in one grid (all must share the rows cap M).
+tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const double* f,
+                       double* const* d1, double* const* d2, hipStream_t stream) {
+  CcdLaunch L{};
+  L.nslots = naxes;
+  size_t shmem = 0;
+  unsigned blocks = 0;
+  int M = -1;
+  for (int i = 0; i < naxes; ++i) {
+    CcdAxisHost& ax = plan->axes[axes[i]];
+    CcdSlot& S = L.slot[i];
+    S.g = ax.geom;
+    S.ln = lines_for(plan, axes[i]);
+    S.contig = S.ln.stride == 1;
+    S.f = f;
+    S.d1 = d1[i];
+    S.d2 = d2[i];
+    if (M < 0) M = ax.geom.M;
+    if (ax.geom.M != M || ax.geom.P != plan->axes[axes[0]].geom.P)
+      return fail(TPF_ERR_VALUE, "fused CCD launch needs one rows cap and partition count");
+    const size_t sh = lds_bytes(ax.geom, S.contig);
+    shmem = sh > shmem ? sh : shmem;
+    if (S.ln.n_lines > (int64_t)0x3fffffff)
+      return fail(TPF_ERR_VALUE, "too many CCD lines for one launch");
+    L.block_begin[i] = blocks;
+    blocks += (unsigned)((S.ln.n_lines + 63) / 64);
+  }
+  L.block_begin[naxes] = blocks;
+  KernelFn k = pick_kernel(M);
+  if (!k) return fail(TPF_ERR_VALUE, "unsupported CCD rows cap " + std::to_string(M));
   TPF_TRY(raise_lds_limit((const void*)k, shmem));
-  if (ln.n_lines > (int64_t)0x7fffffff)
-    return fail(TPF_ERR_VALUE, "too many CCD lines for one launch");
-  const unsigned grid = (unsigned)((ln.n_lines + 63) / 64);
-  hipLaunchKernelGGL(k, dim3(grid), dim3(64 * ax.geom.P), shmem, stream, ax.geom, ln, f, d1, d2);
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * L.slot[0].g.P), shmem, stream, L);
   TPF_HIP(hipGetLastError());
   return TPF_OK;
 }
@@ -412,7 +452,7 @@ extern "C" tpf_status tpf_ccd_differentiate(tpf_ccd_plan* plan, const double* f,
   if ((const double*)d1 == f || (const double*)d2 == f || (d1 && d1 == d2))
     return fail(TPF_ERR_VALUE, "CCD outputs must not alias the input or each other");
   TPF_TRY(ensure_axis(plan, axis));
-  return launch_axis(plan, axis, f, d1, d2, (hipStream_t)stream);
+  return launch_axes(plan, &axis, 1, f, &d1, &d2, (hipStream_t)stream);
 }
 
 extern "C" tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double* d1, double* d2,
@@ -421,13 +461,25 @@ extern "C" tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double*
   if (axis_mask >> plan->ndim) return fail(TPF_ERR_VALUE, "axis_mask names a missing axis");
   int64_t total = 1;
   for (int a = 0; a < plan->ndim; ++a) total *= plan->n_nodes[a];
-  int k = 0;
+  int axes[3], n = 0;
+  double *o1[3], *o2[3];
   for (int a = 0; a < plan->ndim; ++a) {
     if (!(axis_mask & (1u << a))) continue;
-    TPF_TRY(tpf_ccd_differentiate(plan, f, d1 ? d1 + k * total : nullptr,
-                                  d2 ? d2 + k * total : nullptr, a, stream));
-    ++k;
+    TPF_TRY(ensure_axis(plan, a));
+    o1[n] = d1 ? d1 + n * total : nullptr;
+    o2[n] = d2 ? d2 + n * total : nullptr;
+    if ((const double*)o1[n] == f || (const double*)o2[n] == f)
+      return fail(TPF_ERR_VALUE, "CCD outputs must not alias the input");
+    axes[n++] = a;
   }
+  if (n == 0 || (!d1 && !d2)) return TPF_OK;
+  bool same = true;  // one grid when every axis has the same launch shape
+  for (int i = 1; i < n; ++i)
+    same = same && plan->axes[axes[i]].geom.M == plan->axes[axes[0]].geom.M &&
+           plan->axes[axes[i]].geom.P == plan->axes[axes[0]].geom.P;
+  if (same) return launch_axes(plan, axes, n, f, o1, o2, (hipStream_t)stream);
+  for (int i = 0; i < n; ++i)
+    TPF_TRY(launch_axes(plan, &axes[i], 1, f, &o1[i], &o2[i], (hipStream_t)stream));
   return TPF_OK;
 }
 

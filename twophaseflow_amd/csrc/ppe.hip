@@ -358,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_true_res(PpeGeom g, const double* __
 
 int grid_for(int64_t total) {
   int64_t blocks = (total + kBlock - 1) / kBlock;
-  const int64_t cap = 256 * 16;  // 16 blocks of 256 per CU is plenty of parallelism
+  const int64_t cap = 256 * 8;  // 8 blocks of 256 per CU; grid-stride loops do the rest
   return (int)(blocks < cap ? (blocks < 1 ? 1 : blocks) : cap);
 }
 

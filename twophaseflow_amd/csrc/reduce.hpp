@@ -34,7 +34,9 @@ __device__ inline void block_sum(double (&v)[K], double* sh) {
 }
 
 // Writes this block's partials; returns true in exactly one (the last) block,
-// whose thread 0 then holds the grid totals in v.
+// whose thread 0 then holds the grid totals in v.  The counter is an
+// acq_rel agent-scope atomic, so after it the last block reads the partials
+// with plain loads; it resets the counter for the next launch.
 template <int K>
 __device__ inline bool grid_sum(double (&v)[K], double* partials, unsigned* counter, double* sh) {
   block_sum<K>(v, sh);
@@ -42,26 +44,25 @@ __device__ inline bool grid_sum(double (&v)[K], double* partials, unsigned* coun
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) partials[blockIdx.x * K + k] = v[k];
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
+    const unsigned t =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     last = (t == gridDim.x - 1);
   }
   __syncthreads();
   if (!last) return false;
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   double acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0.0;
   // fixed assignment of blocks to threads + fixed tree => deterministic
   for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-      acc[k] += __hip_atomic_load(&partials[b * K + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = 0; k < K; ++k) acc[k] += partials[b * K + k];
   block_sum<K>(acc, sh);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = acc[k];
-    *counter = 0u;
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return true;
 }
