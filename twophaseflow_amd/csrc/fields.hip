@@ -107,8 +107,7 @@ struct BcPack {
 
 __global__ __launch_bounds__(kBlock) void k_mass(FieldGeom g, BcPack bp,
                                                  const double* __restrict__ q,
-                                                 double* partials, unsigned* counter,
-                                                 double* out) {
+                                                 double* partials) {
   __shared__ double sh[kBlock / 64];
   double v[1] = {0.0};
   for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
@@ -119,7 +118,15 @@ __global__ __launch_bounds__(kBlock) void k_mass(FieldGeom g, BcPack bp,
     for (int a = 0; a < g.ndim; ++a) w *= cv_width(g, bp.bc, a, i[a]);
     v[0] += q[idx] * w;
   }
-  if (grid_sum<1>(v, partials, counter, sh) && threadIdx.x == 0) *out = v[0];
+  block_partials<1>(v, partials, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_mass(const double* partials, int nblocks,
+                                                     double* out) {
+  __shared__ double sh[kBlock / 64];
+  double v[1];
+  finish_sum<1>(v, partials, nblocks, sh);
+  if (threadIdx.x == 0) *out = v[0];
 }
 
 __global__ __launch_bounds__(kBlock) void k_correct(FieldGeom g, BcPack bp, int axis, double dt,
@@ -189,10 +196,9 @@ extern "C" tpf_status tpf_fields_mass(int ndim, const int64_t* n_nodes, const do
   const int grid = grid_of(g.total);
   hipStream_t st = (hipStream_t)stream;
   double* buf = nullptr;
-  TPF_HIP(hipMalloc(&buf, sizeof(double) * (grid + 2)));
-  unsigned* counter = reinterpret_cast<unsigned*>(buf + grid + 1);
-  TPF_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned), st));
-  hipLaunchKernelGGL(k_mass, dim3(grid), dim3(kBlock), 0, st, g, bp, q, buf, counter, buf + grid);
+  TPF_HIP(hipMalloc(&buf, sizeof(double) * (grid + 1)));
+  hipLaunchKernelGGL(k_mass, dim3(grid), dim3(kBlock), 0, st, g, bp, q, buf);
+  hipLaunchKernelGGL(k_fin_mass, dim3(1), dim3(kBlock), 0, st, (const double*)buf, grid, buf + grid);
   hipError_t e = hipGetLastError();
   double out = 0.0;
   if (e == hipSuccess) e = hipMemcpyAsync(&out, buf + grid, sizeof(double), hipMemcpyDeviceToHost, st);

@@ -176,7 +176,15 @@ __global__ __launch_bounds__(kBlock) void k_init(PpeGeom g, const double* __rest
     v[1] += ri * dinv[idx] * ri;
     v[2] += bi * bi;
   }
-  if (grid_sum<3>(v, partials, &st->counter, sh) && threadIdx.x == 0) {
+  block_partials<3>(v, partials, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_init(PcgState* st, const double* partials,
+                                                     int nblocks) {
+  __shared__ double sh[3 * kBlock / 64];
+  double v[3];
+  finish_sum<3>(v, partials, nblocks, sh);
+  if (threadIdx.x == 0) {
     st->rr = v[0];
     st->rz = v[1];
     st->bb = v[2];
@@ -222,7 +230,17 @@ __global__ __launch_bounds__(kBlock) void k_matvec(PpeGeom g, const T* __restric
     q[idx] = ap;
     v[0] += (double)p[idx] * (double)ap;
   }
-  if (grid_sum<1>(v, partials, &st->counter, sh) && threadIdx.x == 0) {
+  block_partials<1>(v, partials, sh);
+}
+
+// alpha = rz / <p, q>
+__global__ __launch_bounds__(kBlock) void k_fin_alpha(PcgState* st, const double* partials,
+                                                      int nblocks) {
+  if (st->done) return;
+  __shared__ double sh[kBlock / 64];
+  double v[1];
+  finish_sum<1>(v, partials, nblocks, sh);
+  if (threadIdx.x == 0) {
     st->pq = v[0];
     st->alpha = st->rz / v[0];
   }
@@ -246,7 +264,17 @@ __global__ __launch_bounds__(kBlock) void k_update_r(PpeGeom g, const T* __restr
     v[0] += rd * (double)dinv[idx] * rd;
     v[1] += rd * rd;
   }
-  if (grid_sum<2>(v, partials, &st->counter, sh) && threadIdx.x == 0) {
+  block_partials<2>(v, partials, sh);
+}
+
+// beta = rz_new / rz, stop test on the recursively updated ||r||
+__global__ __launch_bounds__(kBlock) void k_fin_beta(PcgState* st, const double* partials,
+                                                     int nblocks) {
+  if (st->done) return;
+  __shared__ double sh[2 * kBlock / 64];
+  double v[2];
+  finish_sum<2>(v, partials, nblocks, sh);
+  if (threadIdx.x == 0) {
     st->beta = v[0] / st->rz;
     st->rz = v[0];
     st->rr = v[1];
@@ -293,7 +321,15 @@ __global__ __launch_bounds__(kBlock) void k_resid_mixed(
     v[1] += (double)rf * (double)dinv32[idx] * (double)rf;
     v[2] += bi * bi;
   }
-  if (grid_sum<3>(v, partials, &st->counter, sh) && threadIdx.x == 0) {
+  block_partials<3>(v, partials, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_mixed(PcgState* st, const double* partials,
+                                                      int nblocks) {
+  __shared__ double sh[3 * kBlock / 64];
+  double v[3];
+  finish_sum<3>(v, partials, nblocks, sh);
+  if (threadIdx.x == 0) {
     st->true_rr = v[0];
     st->true_bb = v[2];
     st->rz = v[1];
@@ -350,7 +386,15 @@ __global__ __launch_bounds__(kBlock) void k_true_res(PpeGeom g, const double* __
     v[0] += ri * ri;
     v[1] += bi * bi;
   }
-  if (grid_sum<2>(v, partials, &st->counter, sh) && threadIdx.x == 0) {
+  block_partials<2>(v, partials, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_true(PcgState* st, const double* partials,
+                                                     int nblocks) {
+  __shared__ double sh[2 * kBlock / 64];
+  double v[2];
+  finish_sum<2>(v, partials, nblocks, sh);
+  if (threadIdx.x == 0) {
     st->true_rr = v[0];
     st->true_bb = v[1];
   }
@@ -506,8 +550,12 @@ static tpf_status ensure_graph(tpf_ppe* s, double* x, int chunk) {
                        (const double*)s->r, (const double*)s->dinv, s->p, x, s->st);
     hipLaunchKernelGGL(k_matvec<double>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
                        (const double*)s->p, s->rho, s->q, s->st, s->partials);
+    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kBlock), 0, s->stream, s->st,
+                       (const double*)s->partials, s->grid);
     hipLaunchKernelGGL(k_update_r<double>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
                        (const double*)s->q, (const double*)s->dinv, s->r, s->st, s->partials);
+    hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(kBlock), 0, s->stream, s->st,
+                       (const double*)s->partials, s->grid);
   }
   TPF_HIP(hipStreamEndCapture(s->stream, &graph));
   hipError_t e = hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0);
@@ -528,6 +576,8 @@ extern "C" tpf_status tpf_ppe_true_relres(tpf_ppe* s, const double* rhs, const d
   TPF_TRY(join_user_stream(s));
   hipLaunchKernelGGL(k_true_res, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g, rhs, s->rho, x,
                      s->st, s->partials);
+  hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(kBlock), 0, s->stream, s->st,
+                     (const double*)s->partials, s->grid);
   TPF_HIP(hipGetLastError());
   TPF_TRY(read_state(s));
   const double bb = s->st_host->true_bb;
@@ -556,6 +606,8 @@ extern "C" tpf_status tpf_ppe_solve(tpf_ppe* s, const double* rhs, double* x,
   TPF_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(init), hipMemcpyHostToDevice, s->stream));
   hipLaunchKernelGGL(k_init, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g, rhs, s->rho,
                      s->dinv, x, s->b, s->r, s->p, s->st, s->partials, opts->use_x0 ? 0 : 1);
+  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(kBlock), 0, s->stream, s->st,
+                     (const double*)s->partials, s->grid);
   TPF_HIP(hipGetLastError());
   const int chunk = opts->check_every > 0 ? opts->check_every : 32;
   TPF_TRY(ensure_graph(s, x, chunk));
@@ -628,9 +680,13 @@ tpf_status ensure_graph32(tpf_ppe* s, int chunk) {
     hipLaunchKernelGGL(k_matvec<float>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
                        (const float*)s->p32, (const float*)s->rho32, s->q32, s->st_in,
                        s->partials);
+    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kBlock), 0, s->stream, s->st_in,
+                       (const double*)s->partials, s->grid);
     hipLaunchKernelGGL(k_update_r<float>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
                        (const float*)s->q32, (const float*)s->dinv32, s->r32, s->st_in,
                        s->partials);
+    hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(kBlock), 0, s->stream, s->st_in,
+                       (const double*)s->partials, s->grid);
   }
   TPF_HIP(hipStreamEndCapture(s->stream, &graph));
   hipError_t e = hipGraphInstantiate(&s->graph32_exec, graph, nullptr, nullptr, 0);
@@ -673,6 +729,8 @@ tpf_status tpf_ppe_solve_mixed(tpf_ppe* s, const double* rhs, double* x,
     hipLaunchKernelGGL(k_resid_mixed, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g, rhs,
                        s->rho, (const double*)x, (const float*)s->dinv32, s->r32, s->x32, s->p32,
                        s->st_in, s->partials);
+    hipLaunchKernelGGL(k_fin_mixed, dim3(1), dim3(kBlock), 0, s->stream, s->st_in,
+                       (const double*)s->partials, s->grid);
     TPF_HIP(hipGetLastError());
     TPF_TRY(read_inner(s));
     PcgState st = *s->st_in_host;

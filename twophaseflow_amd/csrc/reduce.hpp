@@ -33,38 +33,29 @@ __device__ inline void block_sum(double (&v)[K], double* sh) {
   }
 }
 
-// Writes this block's partials; returns true in exactly one (the last) block,
-// whose thread 0 then holds the grid totals in v.  The counter is an
-// acq_rel agent-scope atomic, so after it the last block reads the partials
-// with plain loads; it resets the counter for the next launch.
+// Writes this block's K partial sums to partials[blockIdx.x * K + k].  The
+// grid totals are formed by a following single-block kernel (finish_sum):
+// kernel boundaries order the writes, which is far cheaper on a multi-XCD
+// device than a per-block agent-scope release and atomic.
 template <int K>
-__device__ inline bool grid_sum(double (&v)[K], double* partials, unsigned* counter, double* sh) {
+__device__ inline void block_partials(double (&v)[K], double* partials, double* sh) {
   block_sum<K>(v, sh);
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
     for (int k = 0; k < K; ++k) partials[blockIdx.x * K + k] = v[k];
-    const unsigned t =
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  double acc[K];
+}
+
+// Single-block reduction of `nblocks` partial rows; thread 0 gets the totals.
+// Fixed thread assignment and tree order: deterministic.
+template <int K>
+__device__ inline void finish_sum(double (&v)[K], const double* partials, int nblocks,
+                                  double* sh) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) acc[k] = 0.0;
-  // fixed assignment of blocks to threads + fixed tree => deterministic
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock)
 #pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] += partials[b * K + k];
-  block_sum<K>(acc, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = acc[k];
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  return true;
+    for (int k = 0; k < K; ++k) v[k] += partials[b * K + k];
+  block_sum<K>(v, sh);
 }
 
 }  // namespace tpf
