@@ -14,6 +14,7 @@
 // each store instruction writes whole 128 B segments.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -39,6 +40,7 @@ struct CcdSlot {
   double* d1;
   double* d2;
   int contig;  // line nodes are contiguous (last axis)
+  int tiled;   // contiguous lines staged through LDS as one coalesced tile
 };
 struct CcdLaunch {
   int nslots;
@@ -46,9 +48,12 @@ struct CcdLaunch {
   CcdSlot slot[3];
 };
 
+constexpr int kTilePer = 20;  // tile doubles per thread when staging a contiguous-axis tile
+
 template <int M>
 __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& ln,
-                                          const bool CONTIG, const double* __restrict__ f,
+                                          const bool CONTIG, const bool TILED,
+                                          const double* __restrict__ f,
                                           double* __restrict__ d1, double* __restrict__ d2,
                                           const unsigned block) {
   extern __shared__ double lds[];
@@ -75,19 +80,52 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
   // ---- load f: fv[k] = node (s0 - 1 + k) periodic (wrapped), (s0 + k) wall
   double fv[M + 2];
   const int j0 = periodic ? s0 - 1 : s0;
+  if (M == 16 && TILED) {
+    // Contiguous axis: the workgroup's 64 lines are one contiguous block of
+    // memory; stage it through LDS with coalesced loads, then each lane reads
+    // its partition of its own line.  (Host guarantees 64*nax <= kTilePer*blockDim.)
+    const int nax = (int)ln.inner_stride;
+    const int64_t rem = ln.n_lines - (int64_t)block * 64;
+    const int cnt = (int)(rem < 64 ? rem : 64) * nax;
+    const double* src = f + (int64_t)block * 64 * nax;
+    double tmp[kTilePer];
 #pragma unroll
-  for (int k = 0; k < M + 2; ++k) {
-    fv[k] = 0.0;
-    if (k < m + 2) {
-      int j = j0 + k;
-      if (periodic) j = j < 0 ? j + g.N : (j >= g.N ? j - g.N : j);
-      if (active) fv[k] = f[base + (int64_t)j * st];
+    for (int i = 0; i < kTilePer; ++i) {
+      const int e = threadIdx.x + i * (int)blockDim.x;
+      tmp[i] = e < cnt ? src[e] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < kTilePer; ++i) {
+      const int e = threadIdx.x + i * (int)blockDim.x;
+      if (e < cnt) lds[e] = tmp[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < M + 2; ++k) {
+      fv[k] = 0.0;
+      if (k < m + 2) {
+        int j = j0 + k;
+        if (periodic) j = j < 0 ? j + g.N : (j >= g.N ? j - g.N : j);
+        if (active) fv[k] = lds[lane * nax + j];
+      }
+    }
+    __syncthreads();  // the tile's LDS is reused below
+  } else {
+#pragma unroll
+    for (int k = 0; k < M + 2; ++k) {
+      fv[k] = 0.0;
+      if (k < m + 2) {
+        int j = j0 + k;
+        if (periodic) j = j < 0 ? j + g.N : (j >= g.N ? j - g.N : j);
+        if (active) fv[k] = f[base + (int64_t)j * st];
+      }
     }
   }
 
   // ---- per-row LU/spike tables and coupling rows -> LDS (all WGs share them;
-  // reading them from LDS keeps scalar-cache misses off the dependent sweeps)
-  constexpr bool kLdsTab = M <= 32;
+  // reading them from LDS keeps scalar-cache misses off the dependent sweeps).
+  // A staged contiguous-axis tile already fills LDS; then they stay in global.
+  const bool kLdsTab = M <= 32 && !(M == 16 && TILED);
   const int NQ = g.NQ;
   double* ltab = lds;
   double* lq = lds + (kLdsTab ? g.P * M * kRowTab : 0);
@@ -236,8 +274,9 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
       for (int r = 0; r < M; ++r) {
         if (r < m) {
           const int64_t a = base + (int64_t)(s0 + r + off) * st;
-          if (d1) d1[a] = x[r][0] * g.inv_h;
-          if (d2) d2[a] = x[r][1] * g.inv_h2;
+          // streaming outputs: non-temporal stores keep f resident in the caches
+          if (d1) __builtin_nontemporal_store(x[r][0] * g.inv_h, d1 + a);
+          if (d2) __builtin_nontemporal_store(x[r][1] * g.inv_h2, d2 + a);
         }
       }
     }
@@ -264,7 +303,8 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
           const int l = e >> 4, r = e & 15;
           const int64_t bl = line0 + l;
           if (c0 + r < m && bl < ln.n_lines)
-            dst[bl * ln.inner_stride + (int64_t)(s0 + c0 + r + off)] = S[l * kStagePitch + r];
+            __builtin_nontemporal_store(S[l * kStagePitch + r],
+                                        dst + bl * ln.inner_stride + (int64_t)(s0 + c0 + r + off));
         }
         __syncthreads();
       }
@@ -274,10 +314,15 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
 
 template <int M>
 __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdLaunch L) {
-  const unsigned blk = blockIdx.x;
+  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so give
+  // XCD x a contiguous run of line blocks (neighbouring lines share DRAM pages
+  // and that XCD's L2); the incomplete last round keeps its own index.
+  const unsigned G = gridDim.x, full = G & ~7u;
+  const unsigned blk =
+      blockIdx.x < full ? (blockIdx.x & 7u) * (full >> 3) + (blockIdx.x >> 3) : blockIdx.x;
   const int sl = (L.nslots > 1 && blk >= L.block_begin[1]) + (L.nslots > 2 && blk >= L.block_begin[2]);
   const CcdSlot& S = L.slot[sl];
-  ccd_lines<M>(S.g, S.ln, S.contig != 0, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
+  ccd_lines<M>(S.g, S.ln, S.contig != 0, S.tiled != 0, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
 }
 
 using KernelFn = void (*)(CcdLaunch);
@@ -292,7 +337,20 @@ KernelFn pick_kernel(int M) {
   return nullptr;
 }
 
-size_t lds_bytes(const CcdAxisGeom& g, bool contig) {
+// Contiguous lines are staged as a [64][nax] tile when it fits LDS and the
+// per-thread staging registers (kTilePer).
+bool tile_fits(const CcdAxisGeom& g, int64_t nax) {
+  return g.M == 16 && 64 * nax <= (int64_t)kTilePer * 64 * g.P && 64 * nax * 8 <= 150 * 1024;
+}
+
+size_t lds_bytes(const CcdAxisGeom& g, bool contig, int64_t nax) {
+  const bool tiled = contig && tile_fits(g, nax);
+  if (tiled) {
+    size_t t = sizeof(double) * 64 * (size_t)nax;
+    size_t st = sizeof(double) * g.P * 64 * kStagePitch;
+    size_t gb = sizeof(double) * 4 * g.P * 64;
+    return std::max(t, std::max(st, gb));
+  }
   const bool tab = g.M <= 32;  // matches kLdsTab in the kernel
   size_t gbytes = sizeof(double) * ((tab ? (size_t)g.P * g.M * kRowTab + (size_t)g.P * 16 * g.NQ : 0) +
                                     (size_t)4 * g.P * 64);
@@ -363,13 +421,14 @@ tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const dou
     S.g = ax.geom;
     S.ln = lines_for(plan, axes[i]);
     S.contig = S.ln.stride == 1;
+    S.tiled = S.contig && tile_fits(ax.geom, plan->n_nodes[axes[i]]);
     S.f = f;
     S.d1 = d1[i];
     S.d2 = d2[i];
     if (M < 0) M = ax.geom.M;
     if (ax.geom.M != M || ax.geom.P != plan->axes[axes[0]].geom.P)
       return fail(TPF_ERR_VALUE, "fused CCD launch needs one rows cap and partition count");
-    const size_t sh = lds_bytes(ax.geom, S.contig);
+    const size_t sh = lds_bytes(ax.geom, S.contig, plan->n_nodes[axes[i]]);
     shmem = sh > shmem ? sh : shmem;
     if (S.ln.n_lines > (int64_t)0x3fffffff)
       return fail(TPF_ERR_VALUE, "too many CCD lines for one launch");

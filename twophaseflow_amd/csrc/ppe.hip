@@ -601,28 +601,37 @@ extern "C" tpf_status tpf_ppe_solve(tpf_ppe* s, const double* rhs, double* x,
   }
   PcgState init{};
   init.tol2 = opts->tol * opts->tol;
-  init.maxiter = opts->maxiter;
-  *s->st_host = init;  // pinned staging for the async copy
-  TPF_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(init), hipMemcpyHostToDevice, s->stream));
-  hipLaunchKernelGGL(k_init, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g, rhs, s->rho,
-                     s->dinv, x, s->b, s->r, s->p, s->st, s->partials, opts->use_x0 ? 0 : 1);
-  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(kBlock), 0, s->stream, s->st,
-                     (const double*)s->partials, s->grid);
-  TPF_HIP(hipGetLastError());
+  // Residual replacement: when the recursive ||r|| has met tol but the FP64
+  // true residual has not, restart CG from the current x (at most 4 times).
+  int64_t iters = 0;
+  double rr = 0.0, bb = 0.0, tr = 0.0;
   const int chunk = opts->check_every > 0 ? opts->check_every : 32;
-  TPF_TRY(ensure_graph(s, x, chunk));
-  TPF_TRY(read_state(s));
-  while (!s->st_host->done) {
-    TPF_HIP(hipGraphLaunch(s->graph_exec, s->stream));
+  for (int restart = 0;; ++restart) {
+    const int zero_x = (restart == 0 && !opts->use_x0) ? 1 : 0;
+    init.maxiter = opts->maxiter - iters;
+    *s->st_host = init;  // pinned staging for the async copy
+    TPF_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(init), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_init, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g, rhs, s->rho,
+                       s->dinv, x, s->b, s->r, s->p, s->st, s->partials, zero_x);
+    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(kBlock), 0, s->stream, s->st,
+                       (const double*)s->partials, s->grid);
+    TPF_HIP(hipGetLastError());
+    TPF_TRY(ensure_graph(s, x, chunk));
     TPF_TRY(read_state(s));
+    while (!s->st_host->done) {
+      TPF_HIP(hipGraphLaunch(s->graph_exec, s->stream));
+      TPF_TRY(read_state(s));
+    }
+    hipLaunchKernelGGL(k_finish_x<double>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
+                       (const double*)s->p, x, s->st);
+    TPF_HIP(hipGetLastError());
+    iters += s->st_host->iter;
+    rr = s->st_host->rr;
+    bb = s->st_host->bb;
+    TPF_TRY(tpf_ppe_true_relres(s, rhs, x, &tr));
+    const bool rec_ok = bb > 0 ? std::sqrt(rr / bb) <= opts->tol : true;
+    if (tr <= opts->tol || !rec_ok || iters >= opts->maxiter || restart >= 4) break;
   }
-  hipLaunchKernelGGL(k_finish_x<double>, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g,
-                     (const double*)s->p, x, s->st);
-  TPF_HIP(hipGetLastError());
-  const int64_t iters = s->st_host->iter;
-  const double rr = s->st_host->rr, bb = s->st_host->bb;
-  double tr = 0.0;
-  TPF_TRY(tpf_ppe_true_relres(s, rhs, x, &tr));
   tpf_solve_report r{};
   r.iterations = iters;
   r.relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
