@@ -455,8 +455,9 @@ extern "C" tpf_status tpf_ccd_plan_create(tpf_ccd_plan** out, int ndim, const in
   if (!out || !n_nodes || !h || !bc) return fail(TPF_ERR_VALUE, "null argument");
   *out = nullptr;
   if (ndim < 1 || ndim > 3) return fail(TPF_ERR_VALUE, "ndim must be 1, 2 or 3");
-  if (precision != TPF_PREC_FP64 && precision != TPF_PREC_FP32_REFINED)
-    return fail(TPF_ERR_VALUE, "unknown precision mode");
+  if (precision == TPF_PREC_FP32_REFINED)
+    return fail(TPF_ERR_VALUE, "FP32-refined CCD is not built; use TPF_PREC_FP64");
+  if (precision != TPF_PREC_FP64) return fail(TPF_ERR_VALUE, "unknown precision mode");
   auto* plan = new tpf_ccd_plan();
   plan->ndim = ndim;
   plan->precision = precision;
