@@ -145,6 +145,37 @@ TPF_API tpf_status tpf_fields_correct_velocity(int ndim, const int64_t* n_nodes,
                                        const double* rho, const double* grad,
                                        double* u, void* stream);
 
+/* --------------------------------------------- learned preconditioner --- */
+/* No counterpart in the reference: BF16 MFMA building blocks of the U-Net
+ * preconditioner used inside flexible CG.  Activations are channels-last
+ * [D][H][W][C] BF16 (raw 16-bit), C a multiple of 16; weights Wt[27][Cout][Cin]
+ * BF16 with tap = (dz+1)*9 + (dy+1)*3 + (dx+1); FP32 accumulation.
+ * Forward: Y = act(bias + conv3x3x3(X)), zero padding, same size; Y is BF16
+ * or FP32 (out_f32).  Cout <= 64. */
+TPF_API tpf_status tpf_conv3d_fwd(const void* X, const void* Wt, const float* bias, void* Y,
+                                  int D, int H, int W, int Cin, int Cout, int relu,
+                                  int out_f32, void* stream);
+/* dW[tap][co][ci] += sum_v dY[v][co] X[v+off(tap)][ci] (FP32, accumulates). */
+TPF_API tpf_status tpf_conv3d_wgrad(const void* X, const void* dY, float* dW, int D, int H,
+                                    int W, int Cin, int Cout, void* stream);
+/* b[i] = bf16(a[i]), round to nearest even. */
+TPF_API tpf_status tpf_f32_to_bf16(const float* a, void* b, int64_t n, void* stream);
+
+/* CG-system matvec out = A p (FP64) with the handle's current density, the
+ * same operator tpf_ppe_solve iterates on (used by flexible CG).  Both
+ * system_apply calls are asynchronous on the stream given to tpf_ppe_create. */
+TPF_API tpf_status tpf_ppe_system_apply(tpf_ppe* ppe, const double* p, double* out);
+/* Builds the handle's FP32 copies of rho and of the Jacobi diagonal (once
+ * per density; synchronous). */
+TPF_API tpf_status tpf_ppe_prepare_f32(tpf_ppe* ppe);
+/* out = D^-1, the Jacobi diagonal inverse of the CG system (FP64). */
+TPF_API tpf_status tpf_ppe_jacobi_inv(tpf_ppe* ppe, double* out);
+/* b = -W rhs with b[pin] = 0: the right-hand side of the CG system. */
+TPF_API tpf_status tpf_ppe_system_rhs(tpf_ppe* ppe, const double* rhs, double* b);
+/* FP32 CG-system matvec out = A p on the handle's FP32 density copy
+ * (tpf_ppe_prepare_f32 first). */
+TPF_API tpf_status tpf_ppe_system_apply_f32(tpf_ppe* ppe, const float* p, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,16 @@ SIGNATURES = {
     "tpf_fields_correct_velocity": (C.c_int, [C.c_int, _i64p, _ip, C.c_int, C.c_double,
                                               _vp, _vp, _vp, _vp]),
     "tpf_test_ccd_host_model": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
+    "tpf_conv3d_fwd": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.c_int, C.c_int, C.c_int, _vp]),
+    "tpf_conv3d_wgrad": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   _vp]),
+    "tpf_f32_to_bf16": (C.c_int, [_vp, _vp, C.c_int64, _vp]),
+    "tpf_ppe_system_apply": (C.c_int, [_vp, _vp, _vp]),
+    "tpf_ppe_prepare_f32": (C.c_int, [_vp]),
+    "tpf_ppe_jacobi_inv": (C.c_int, [_vp, _vp]),
+    "tpf_ppe_system_rhs": (C.c_int, [_vp, _vp, _vp]),
+    "tpf_ppe_system_apply_f32": (C.c_int, [_vp, _vp, _vp]),
 }
 
 _lib = None

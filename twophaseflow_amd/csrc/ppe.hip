@@ -336,6 +336,33 @@ __global__ __launch_bounds__(kBlock) void k_fin_mixed(PcgState* st, const double
   }
 }
 
+// out = A p on the CG system (pin row identity), in T
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_system_apply(PpeGeom g, const T* __restrict__ p,
+                                                         const T* __restrict__ rho,
+                                                         T* __restrict__ out) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    T ap, dg;
+    stencil<T>(g, p, rho, idx, i0, i1, i2, ap, dg);
+    out[idx] = idx == g.pin ? p[idx] : ap;
+  }
+}
+
+// b = -W rhs, b[pin] = 0 (fvm_matrixfree.py:163-177)
+__global__ __launch_bounds__(kBlock) void k_system_rhs(PpeGeom g, const double* __restrict__ rhs,
+                                                       double* __restrict__ b) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    const double W = dv(g, 0, i0) * dv(g, 1, i1) * dv(g, 2, i2);
+    b[idx] = idx == g.pin ? 0.0 : -W * rhs[idx];
+  }
+}
+
 __global__ void k_set_one(double* __restrict__ x, int64_t idx, double v) {
   if (blockIdx.x == 0 && threadIdx.x == 0) x[idx] = v;
 }
@@ -781,5 +808,50 @@ tpf_status tpf_ppe_solve_mixed(tpf_ppe* s, const double* rhs, double* x,
     return fail(TPF_ERR_RUNTIME, "mixed-precision PPE did not converge (||r||/||b|| = " +
                                      std::to_string(relres) + " after " +
                                      std::to_string(total_inner) + " inner iterations)");
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ppe_system_apply(tpf_ppe* s, const double* p, double* out) {
+  if (!s || !p || !out) return fail(TPF_ERR_VALUE, "null argument");
+  if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set");
+  // asynchronous on the caller's stream (the one given at create time)
+  hipLaunchKernelGGL(k_system_apply<double>, dim3(s->grid), dim3(kBlock), 0, s->user_stream,
+                     s->g, p, s->rho, out);
+  TPF_HIP(hipGetLastError());
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ppe_prepare_f32(tpf_ppe* s) {
+  if (!s) return fail(TPF_ERR_VALUE, "null argument");
+  if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set");
+  TPF_TRY(join_user_stream(s));
+  TPF_TRY(ensure_mixed(s));
+  TPF_HIP(hipStreamSynchronize(s->stream));
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ppe_jacobi_inv(tpf_ppe* s, double* out) {
+  if (!s || !out) return fail(TPF_ERR_VALUE, "null argument");
+  if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set");
+  TPF_HIP(hipMemcpyAsync(out, s->dinv, sizeof(double) * s->g.total, hipMemcpyDeviceToDevice,
+                         s->user_stream));
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ppe_system_rhs(tpf_ppe* s, const double* rhs, double* b) {
+  if (!s || !rhs || !b) return fail(TPF_ERR_VALUE, "null argument");
+  hipLaunchKernelGGL(k_system_rhs, dim3(s->grid), dim3(kBlock), 0, s->user_stream, s->g, rhs, b);
+  TPF_HIP(hipGetLastError());
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ppe_system_apply_f32(tpf_ppe* s, const float* p, float* out) {
+  if (!s || !p || !out) return fail(TPF_ERR_VALUE, "null argument");
+  if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set");
+  if (!s->rho32 || s->rho32_src != s->rho)
+    return fail(TPF_ERR_RUNTIME, "FP32 operator copies not built (call tpf_ppe_prepare_f32)");
+  hipLaunchKernelGGL(k_system_apply<float>, dim3(s->grid), dim3(kBlock), 0, s->user_stream, s->g,
+                     p, (const float*)s->rho32, out);
+  TPF_HIP(hipGetLastError());
   return TPF_OK;
 }

@@ -1,0 +1,204 @@
+// BF16 MFMA 3x3x3 convolutions (stride 1, zero padding 1, "same" size) for
+// the learned PPE preconditioner, with FP32 accumulation.
+//
+// Layouts: activations channels-last [D][H][W][C] BF16 (C a multiple of 16);
+// weights "tap-major transposed" Wt[27][Cout][Cin] BF16 with tap =
+// (dz+1)*9 + (dy+1)*3 + (dx+1); bias FP32 [Cout].
+//
+//   forward : Y[v][co] = act( bias[co] + sum_tap sum_ci X[v+off(tap)][ci] Wt[tap][co][ci] )
+//   dgrad   : the forward kernel with W2[tap][ci][co] = Wt[26-tap][co][ci]
+//   wgrad   : dW[tap][co][ci] += sum_v dY[v][co] X[v+off(tap)][ci]   (FP32, atomic)
+//
+// MFMA v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)+j],
+// B[4(l/16)+j][l%16] and D[4(l/16)+i][l%16].  The forward GEMM is
+// M = 16 voxels along x, N = 16 output channels per tile, K = 27 x Cin.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "tpf_common.hpp"
+
+namespace tpf {
+namespace {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t f2bf(float f) {  // round to nearest even
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf/nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+
+constexpr int kWaves = 4;  // wavefronts per block
+
+// One wavefront: output voxels (z, y, x0..x0+15) x COT tiles of 16 channels.
+template <int COT>
+__global__ __launch_bounds__(64 * kWaves) void conv_fwd(const uint16_t* __restrict__ X,
+                                                       const uint16_t* __restrict__ Wt,
+                                                       const float* __restrict__ bias,
+                                                       void* __restrict__ Y, int D, int H, int W,
+                                                       int Cin, int Cout, int relu, int out_f32) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int xt = (W + 15) >> 4;
+  const int64_t ntile = (int64_t)D * H * xt;
+  if (wave >= ntile) return;  // uniform per wavefront
+  const int z = wave / (H * xt);
+  const int rem = wave - z * H * xt;
+  const int y = rem / xt;
+  const int x0 = (rem - y * xt) * 16;
+  const int row = lane & 15, kq = lane >> 4;
+  v4f acc[COT];
+#pragma unroll
+  for (int t = 0; t < COT; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 27; ++tap) {
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int zz = z + dz, yy = y + dy, xx = x0 + row + dx;
+    const bool ok = zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W;
+    const uint16_t* xp = X + (((int64_t)zz * H + yy) * W + xx) * Cin + 4 * kq;
+    const uint16_t* wp = Wt + ((int64_t)tap * Cout + row) * Cin + 4 * kq;
+    for (int c0 = 0; c0 < Cin; c0 += 16) {
+      v4s a = v4s{0, 0, 0, 0};
+      if (ok) a = *reinterpret_cast<const v4s*>(xp + c0);
+#pragma unroll
+      for (int t = 0; t < COT; ++t) {
+        const v4s b = *reinterpret_cast<const v4s*>(wp + (int64_t)t * 16 * Cin + c0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  // epilogue: lane holds voxels x0 + 4*kq + i, channel t*16 + row
+#pragma unroll
+  for (int t = 0; t < COT; ++t) {
+    const int co = t * 16 + row;
+    const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int x = x0 + 4 * kq + i;
+      if (x >= W) continue;
+      float v = acc[t][i] + bb;
+      if (relu) v = v > 0.f ? v : 0.f;
+      const int64_t o = (((int64_t)z * H + y) * W + x) * Cout + co;
+      if (out_f32) reinterpret_cast<float*>(Y)[o] = v;
+      else reinterpret_cast<uint16_t*>(Y)[o] = f2bf(v);
+    }
+  }
+}
+
+// dW[tap][co][ci] += sum_v dY[v][co] * X[v + off(tap)][ci].  One wavefront:
+// one tap, one 16x16 (co, ci) tile and a run of `rows_per` (z, y) rows, K =
+// voxels along x in steps of 16; partial tiles are added with FP32 atomics.
+__global__ __launch_bounds__(64 * kWaves) void conv_wgrad(const uint16_t* __restrict__ X,
+                                                         const uint16_t* __restrict__ dY,
+                                                         float* __restrict__ dW, int D, int H,
+                                                         int W, int Cin, int Cout, int rows_per) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int cit = Cin >> 4, cot = Cout >> 4;
+  const int nrows = D * H;
+  const int nchunk = (nrows + rows_per - 1) / rows_per;
+  const int64_t total = (int64_t)27 * cot * cit * nchunk;
+  if (wave >= total) return;
+  int64_t w = wave;
+  const int ch = (int)(w % nchunk);
+  w /= nchunk;
+  const int ct = (int)(w % cit);
+  w /= cit;
+  const int ot = (int)(w % cot);
+  const int tap = (int)(w / cot);
+  const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+  const int row = lane & 15, kq = lane >> 4;
+  v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+  const int r0 = ch * rows_per, r1 = r0 + rows_per < nrows ? r0 + rows_per : nrows;
+  for (int r = r0; r < r1; ++r) {
+    const int z = r / H, y = r - (r / H) * H;
+    const int zz = z + dz, yy = y + dy;
+    const bool rowok = zz >= 0 && zz < D && yy >= 0 && yy < H;
+    const uint16_t* dyr = dY + (int64_t)r * W * Cout + ot * 16 + row;
+    const uint16_t* xr = X + ((int64_t)zz * H + yy) * W * Cin + ct * 16 + row;
+    for (int xb = 0; xb < W; xb += 16) {
+      v4s a, b;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int x = xb + 4 * kq + j;
+        uint16_t av = 0, bv = 0;
+        if (x < W) {
+          av = dyr[(int64_t)x * Cout];  // A[co = row][k = voxel]
+          const int xx = x + dx;
+          if (rowok && xx >= 0 && xx < W) bv = xr[(int64_t)xx * Cin];  // B[k][ci = row]
+        }
+        a[j] = (short)av;
+        b[j] = (short)bv;
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int co = ot * 16 + 4 * kq + i, ci = ct * 16 + row;
+    atomicAdd(&dW[((int64_t)tap * Cout + co) * Cin + ci], acc[i]);
+  }
+}
+
+__global__ void to_bf16(const float* __restrict__ a, uint16_t* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = f2bf(a[i]);
+}
+
+}  // namespace
+}  // namespace tpf
+
+using namespace tpf;
+
+extern "C" tpf_status tpf_conv3d_fwd(const void* X, const void* Wt, const float* bias, void* Y,
+                                     int D, int H, int W, int Cin, int Cout, int relu,
+                                     int out_f32, void* stream) {
+  if (!X || !Wt || !Y) return fail(TPF_ERR_VALUE, "null argument");
+  if (D < 1 || H < 1 || W < 1 || Cin < 16 || Cin % 16 || Cout < 16 || Cout % 16 || Cout > 64)
+    return fail(TPF_ERR_VALUE, "conv3d: Cin, Cout must be multiples of 16 (Cout <= 64)");
+  const int64_t waves = (int64_t)D * H * ((W + 15) / 16);
+  const int64_t blocks = (waves + kWaves - 1) / kWaves;
+  if (blocks > 0x7fffffffLL) return fail(TPF_ERR_VALUE, "conv3d grid too large");
+  const uint16_t* x = (const uint16_t*)X;
+  const uint16_t* w = (const uint16_t*)Wt;
+  hipStream_t s = (hipStream_t)stream;
+  switch (Cout / 16) {
+    case 1: hipLaunchKernelGGL(conv_fwd<1>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
+    case 2: hipLaunchKernelGGL(conv_fwd<2>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
+    case 3: hipLaunchKernelGGL(conv_fwd<3>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
+    case 4: hipLaunchKernelGGL(conv_fwd<4>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
+  }
+  TPF_HIP(hipGetLastError());
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_conv3d_wgrad(const void* X, const void* dY, float* dW, int D, int H,
+                                       int W, int Cin, int Cout, void* stream) {
+  if (!X || !dY || !dW) return fail(TPF_ERR_VALUE, "null argument");
+  if (D < 1 || H < 1 || W < 1 || Cin < 16 || Cin % 16 || Cout < 16 || Cout % 16)
+    return fail(TPF_ERR_VALUE, "conv3d wgrad: Cin, Cout must be multiples of 16");
+  const int nrows = D * H;
+  const int rows_per = (nrows + 255) / 256;  // <= 256 partial sums per dW entry
+  const int nchunk = (nrows + rows_per - 1) / rows_per;
+  const int64_t waves = (int64_t)27 * (Cout / 16) * (Cin / 16) * nchunk;
+  const int64_t blocks = (waves + kWaves - 1) / kWaves;
+  if (blocks > 0x7fffffffLL) return fail(TPF_ERR_VALUE, "conv3d wgrad grid too large");
+  hipLaunchKernelGGL(conv_wgrad, dim3(blocks), dim3(64 * kWaves), 0, (hipStream_t)stream,
+                     (const uint16_t*)X, (const uint16_t*)dY, dW, D, H, W, Cin, Cout, rows_per);
+  TPF_HIP(hipGetLastError());
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_f32_to_bf16(const float* a, void* b, int64_t n, void* stream) {
+  if (!a || !b || n < 0) return fail(TPF_ERR_VALUE, "bad argument");
+  const int64_t blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+  hipLaunchKernelGGL(to_bf16, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, (hipStream_t)stream,
+                     a, (uint16_t*)b, n);
+  TPF_HIP(hipGetLastError());
+  return TPF_OK;
+}
