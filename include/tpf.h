@@ -151,7 +151,7 @@ TPF_API tpf_status tpf_fields_correct_velocity(int ndim, const int64_t* n_nodes,
  * [D][H][W][C] BF16 (raw 16-bit), C a multiple of 16; weights Wt[27][Cout][Cin]
  * BF16 with tap = (dz+1)*9 + (dy+1)*3 + (dx+1); FP32 accumulation.
  * Forward: Y = act(bias + conv3x3x3(X)), zero padding, same size; Y is BF16
- * or FP32 (out_f32).  Cout <= 64. */
+ * or FP32 (out_f32).  Cout <= 128. */
 TPF_API tpf_status tpf_conv3d_fwd(const void* X, const void* Wt, const float* bias, void* Y,
                                   int D, int H, int W, int Cin, int Cout, int relu,
                                   int out_f32, void* stream);

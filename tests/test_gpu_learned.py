@@ -46,7 +46,8 @@ def test_conv_forward_matches_oracle(shape, cout, relu):
     assert _rel(yb, ref) <= 1e-2
 
 
-@pytest.mark.parametrize("shape,cout", [((4, 5, 18, 16), 16), ((3, 3, 11, 32), 16), ((3, 4, 9, 16), 48)])
+@pytest.mark.parametrize("shape,cout", [((4, 5, 18, 16), 16), ((3, 3, 11, 32), 16), ((3, 4, 9, 16), 48),
+                                        ((3, 3, 17, 128), 64)])
 def test_conv_gradients_match_oracle(shape, cout):
     _announce(f"conv grads {shape}->{cout}")
     rng = np.random.default_rng(2)
@@ -86,21 +87,22 @@ def test_fcg_with_untrained_unet_reaches_tolerance():
     assert x.view(-1)[s._pin_dof] == 0.0
 
 
-def test_training_reduces_fcg_iterations():
-    """What the trained network gains over the same preconditioner untrained."""
+def test_training_improves_the_preconditioner():
+    """What the trained network gains over the same preconditioner untrained:
+    residual reached after a fixed number of FCG iterations on a held-out
+    problem (seeded)."""
     _announce("train + fcg 33^3")
     torch.manual_seed(0)
     net = L.UNet(seed=1).cuda()
     g, rho, rhs = L.make_problem(32, seed=999)
     s = _solver(32, rho)
-    _, before = L.fcg_solve(s, rhs, L.TwoLevelPreconditioner(s, net), tol=1e-8, maxiter=600,
-                            fallback=False)
+    _, before = L.fcg_solve(s, rhs, L.TwoLevelPreconditioner(s, net), tol=1e-12, maxiter=30,
+                            fallback=False, restarts=0, stall_window=10 ** 6)
     L.train(net, N=32, problems=16, epochs=3, lr=2e-3, seed=200)
     s.prepare_operator(rho)
-    _, after = L.fcg_solve(s, rhs, L.TwoLevelPreconditioner(s, net), tol=1e-8, maxiter=600,
-                           fallback=False)
-    print(f"[gpu-case] fcg iterations untrained {before['fcg_iterations']} "
-          f"({before['fcg_exit']}) -> trained {after['fcg_iterations']} ({after['fcg_exit']})",
+    _, after = L.fcg_solve(s, rhs, L.TwoLevelPreconditioner(s, net), tol=1e-12, maxiter=30,
+                           fallback=False, restarts=0, stall_window=10 ** 6)
+    rb, ra = min(before["history"]), min(after["history"])
+    print(f"[gpu-case] best ||r||/||b|| in 30 FCG iterations: untrained {rb:.3e} -> trained {ra:.3e}",
           file=sys.stderr, flush=True)
-    assert after["fcg_exit"] == "converged"
-    assert after["fcg_iterations"] < before["fcg_iterations"]
+    assert ra < 0.5 * rb

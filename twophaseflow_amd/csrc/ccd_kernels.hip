@@ -124,8 +124,8 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
 
   // ---- per-row LU/spike tables and coupling rows -> LDS (all WGs share them;
   // reading them from LDS keeps scalar-cache misses off the dependent sweeps).
-  // A staged contiguous-axis tile already fills LDS; then they stay in global.
-  const bool kLdsTab = M <= 32 && !(M == 16 && TILED);
+  // A staged contiguous-axis tile has been consumed by now; its space is reused.
+  const bool kLdsTab = M <= 32;
   const int NQ = g.NQ;
   double* ltab = lds;
   double* lq = lds + (kLdsTab ? g.P * M * kRowTab : 0);
@@ -345,13 +345,14 @@ bool tile_fits(const CcdAxisGeom& g, int64_t nax) {
 
 size_t lds_bytes(const CcdAxisGeom& g, bool contig, int64_t nax) {
   const bool tiled = contig && tile_fits(g, nax);
+  const bool tab = g.M <= 32;  // matches kLdsTab in the kernel
   if (tiled) {
     size_t t = sizeof(double) * 64 * (size_t)nax;
     size_t st = sizeof(double) * g.P * 64 * kStagePitch;
-    size_t gb = sizeof(double) * 4 * g.P * 64;
+    size_t gb = sizeof(double) * ((tab ? (size_t)g.P * g.M * kRowTab + (size_t)g.P * 16 * g.NQ : 0) +
+                                  (size_t)4 * g.P * 64);
     return std::max(t, std::max(st, gb));
   }
-  const bool tab = g.M <= 32;  // matches kLdsTab in the kernel
   size_t gbytes = sizeof(double) * ((tab ? (size_t)g.P * g.M * kRowTab + (size_t)g.P * 16 * g.NQ : 0) +
                                     (size_t)4 * g.P * 64);
   size_t sbytes = contig ? sizeof(double) * g.P * 64 * kStagePitch : 0;

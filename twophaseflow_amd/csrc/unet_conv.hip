@@ -159,8 +159,8 @@ extern "C" tpf_status tpf_conv3d_fwd(const void* X, const void* Wt, const float*
                                      int D, int H, int W, int Cin, int Cout, int relu,
                                      int out_f32, void* stream) {
   if (!X || !Wt || !Y) return fail(TPF_ERR_VALUE, "null argument");
-  if (D < 1 || H < 1 || W < 1 || Cin < 16 || Cin % 16 || Cout < 16 || Cout % 16 || Cout > 64)
-    return fail(TPF_ERR_VALUE, "conv3d: Cin, Cout must be multiples of 16 (Cout <= 64)");
+  if (D < 1 || H < 1 || W < 1 || Cin < 16 || Cin % 16 || Cout < 16 || Cout % 16 || Cout > 128)
+    return fail(TPF_ERR_VALUE, "conv3d: Cin, Cout must be multiples of 16 (Cout <= 128)");
   const int64_t waves = (int64_t)D * H * ((W + 15) / 16);
   const int64_t blocks = (waves + kWaves - 1) / kWaves;
   if (blocks > 0x7fffffffLL) return fail(TPF_ERR_VALUE, "conv3d grid too large");
@@ -172,6 +172,10 @@ extern "C" tpf_status tpf_conv3d_fwd(const void* X, const void* Wt, const float*
     case 2: hipLaunchKernelGGL(conv_fwd<2>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
     case 3: hipLaunchKernelGGL(conv_fwd<3>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
     case 4: hipLaunchKernelGGL(conv_fwd<4>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
+    case 5: hipLaunchKernelGGL(conv_fwd<5>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
+    case 6: hipLaunchKernelGGL(conv_fwd<6>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
+    case 7: hipLaunchKernelGGL(conv_fwd<7>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
+    case 8: hipLaunchKernelGGL(conv_fwd<8>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
   }
   TPF_HIP(hipGetLastError());
   return TPF_OK;

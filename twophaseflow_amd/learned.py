@@ -250,51 +250,67 @@ class TwoLevelPreconditioner:
 
 
 # ----------------------------------------------------------- FCG solve ---
-def fcg_solve(ppe_solver, rhs, precond, tol=1e-8, maxiter=2000, stall_window=40,
-              stall_factor=0.9, fallback=True):
-    """Flexible CG (FCG(1)) on the CG system in FP64 with a nonlinear
-    preconditioner; Jacobi-PCG takes over on stagnation.  Returns (p, info)."""
+def fcg_solve(ppe_solver, rhs, precond, tol=1e-8, maxiter=2000, m=8, stall_window=30,
+              stall_factor=0.7, restarts=3, fallback=True):
+    """Flexible CG on the CG system in FP64 with a nonlinear preconditioner.
+
+    FCG(m) (Notay 2000): each new direction is A-orthogonalised against the
+    last m directions.  Progress is tracked on the best residual seen; when
+    it fails to improve by `stall_factor` over `stall_window` iterations the
+    recursion restarts from the FP64 true residual (at most `restarts` times)
+    and then plain Jacobi-PCG (warm-started from x) takes over.  The FP64
+    true residual judges the result.  Returns (p, info)."""
     lib = _lib()
     h = ppe_solver._handle
-    b = torch.empty_like(rhs)
-    nat.check(lib.tpf_ppe_system_rhs(h, _p(rhs), _p(b)))
 
     def A64(v):
         out = torch.empty_like(v)
         nat.check(lib.tpf_ppe_system_apply(h, _p(v), _p(out)))
         return out
 
+    b = torch.empty_like(rhs)
+    nat.check(lib.tpf_ppe_system_rhs(h, _p(rhs), _p(b)))
     bn = float(b.norm())
     x = torch.zeros_like(b)
     r = b.clone()
-    hist = []
-    it = 0
-    p_old = q_old = pq_old = None
+    hist, best = [], []
+    it, nrestart = 0, 0
+    P, Q, PQ = [], [], []
     reason = "maxiter"
+    t0 = time.perf_counter()
     with torch.no_grad():
         while it < maxiter:
             rel = float(r.norm()) / bn
             hist.append(rel)
+            best.append(min(rel, best[-1]) if best else rel)
             if rel <= tol:
                 reason = "converged"
                 break
-            if len(hist) > stall_window and rel > stall_factor * hist[-1 - stall_window]:
-                reason = "stagnation"
-                break
+            if len(best) > stall_window and best[-1] > stall_factor * best[-1 - stall_window]:
+                if nrestart >= restarts:
+                    reason = "stagnation"
+                    break
+                nrestart += 1
+                r = b - A64(x)  # residual replacement
+                P, Q, PQ = [], [], []
+                best = [float(r.norm()) / bn]
+                continue
             z = precond(r).double()
-            if p_old is None:
-                p = z
-            else:
-                beta = -float(torch.dot(z.view(-1), q_old.view(-1))) / pq_old
-                p = z + beta * p_old
+            p = z
+            for pj, qj, pqj in zip(P, Q, PQ):
+                p = p - (float(torch.dot(z.view(-1), qj.view(-1))) / pqj) * pj
             q = A64(p)
             pq = float(torch.dot(p.view(-1), q.view(-1)))
             alpha = float(torch.dot(p.view(-1), r.view(-1))) / pq
             x += alpha * p
             r -= alpha * q
-            p_old, q_old, pq_old = p, q, pq
+            P.append(p); Q.append(q); PQ.append(pq)
+            if len(P) > m:
+                P.pop(0); Q.pop(0); PQ.pop(0)
             it += 1
-    info = {"fcg_iterations": it, "fcg_exit": reason, "fallback_iterations": 0}
+    torch.cuda.synchronize()
+    info = {"fcg_iterations": it, "fcg_exit": reason, "fcg_restarts": nrestart,
+            "fcg_seconds": time.perf_counter() - t0, "fallback_iterations": 0}
     if reason != "converged" and fallback:
         ppe_solver.solve_into(rhs, None, x, use_x0=True)
         info["fallback_iterations"] = ppe_solver.last_diagnostics["iterations"]
@@ -382,7 +398,7 @@ def train(net, N=64, problems=64, epochs=2, lr=2e-3, seed=100, log=None):
                 loss.backward()
                 torch.nn.utils.clip_grad_norm_(net.parameters(), 1.0)
                 opt.step()
-                hist.append(float(loss))
+                hist.append(float(loss.detach()))
         if log:
             log(f"epoch {ep}: mean energy {np.mean(hist[-len(data) * 4:]):.4f}")
     return {"loss_history": hist, "data_seconds": t_data,
