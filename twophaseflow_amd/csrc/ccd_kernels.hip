@@ -314,12 +314,9 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
 
 template <int M>
 __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdLaunch L) {
-  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so give
-  // XCD x a contiguous run of line blocks (neighbouring lines share DRAM pages
-  // and that XCD's L2); the incomplete last round keeps its own index.
-  const unsigned G = gridDim.x, full = G & ~7u;
-  const unsigned blk =
-      blockIdx.x < full ? (blockIdx.x & 7u) * (full >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+  // Block order = slot order (axis 0 lines first).  An XCD-contiguous
+  // remapping of blocks was measured slower for the fused 3-axis grid.
+  const unsigned blk = blockIdx.x;
   const int sl = (L.nslots > 1 && blk >= L.block_begin[1]) + (L.nslots > 2 && blk >= L.block_begin[2]);
   const CcdSlot& S = L.slot[sl];
   ccd_lines<M>(S.g, S.ln, S.contig != 0, S.tiled != 0, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
