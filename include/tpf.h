@@ -116,6 +116,34 @@ TPF_API tpf_status tpf_ppe_apply(tpf_ppe* ppe, const double* p, double* out);
 TPF_API tpf_status tpf_ppe_solve(tpf_ppe* ppe, const double* rhs, double* p,
                          const tpf_solve_opts* opts, tpf_solve_report* rep);
 
+typedef struct {
+  double tol;          /* rtol, atol = 0 (gmres_helpers.py:16-42)                  */
+  int64_t maxiter;     /* restart cycles, SciPy semantics (pseudo_maxiter)         */
+  int restart;         /* Krylov dimension per cycle (ppe_restart; host default
+                          min(80, max(20, maxiter)), fvm_matrixfree.py:104-105)   */
+  int precond;         /* 0 none, 1 Jacobi, 2 shifted line (line_pcr)              */
+  int pcr_stages;      /* line solves: >0 PCR stages capped at ceil(log2 n)
+                          (GPU backend, ppe_pcr_stages), 0 exact Thomas (CPU)     */
+  int use_x0;          /* 1: p holds the initial guess (p_init)                    */
+  double c_tau;        /* pseudo_c_tau of the line shift 2/(c_tau rho h_min^2)    */
+} tpf_gmres_opts;
+
+/* Replaces PPESolverFVMMatrixFree.solve in its default GMRES form
+ * (fvm_matrixfree.py:150-260, 302-323; fvm_matrixfree_helpers.py:76-101):
+ * left-preconditioned restarted GMRES on L p = rhs with rhs[pin] = 0 and the
+ * pin row = identity, SciPy 1.15 stopping rules (true ||rhs - L p|| <= tol
+ * ||rhs||).  On exit p[pin] = 0.  Synchronous.  rep->iterations counts
+ * Arnoldi steps, rep->refinements restart cycles.  Returns TPF_ERR_RUNTIME
+ * when the cycle budget ends above tol (SciPy info = maxiter; report filled). */
+TPF_API tpf_status tpf_ppe_solve_gmres(tpf_ppe* ppe, const double* rhs, double* p,
+                               const tpf_gmres_opts* opts, tpf_solve_report* rep);
+
+/* z = M^-1 r of the GMRES preconditioner (precond 1 Jacobi: r / diag(L),
+ * 2: per-axis shifted line solves averaged over axes; z[pin] = 0;
+ * fvm_matrixfree.py:302-330).  Ordered on the handle's stream; synchronous. */
+TPF_API tpf_status tpf_ppe_gmres_precond(tpf_ppe* ppe, const double* r, double* z,
+                                 const tpf_gmres_opts* opts);
+
 /* FP64 ||b - A p||_2 / ||b||_2 on the CG system (the judge of every solve). */
 TPF_API tpf_status tpf_ppe_true_relres(tpf_ppe* ppe, const double* rhs,
                                const double* p, double* relres);

@@ -12,6 +12,10 @@ to ``src/twophase``):
                           ``ppe/fvm_matrixfree.py:325-331``
 * CG system form          ``ppe/fvm_matrixfree.py:163-221, 345-350``
 * CG / GMRES calls        ``ppe/gmres_helpers.py:16-80``
+* GMRES form, line / Jacobi preconditioners
+                          ``ppe/fvm_matrixfree.py:150-260, 302-330``,
+                          ``ppe/fvm_matrixfree_helpers.py:76-101``
+* Thomas / PCR line solves ``linalg_backend.py:77-115, 118-163, 270-340``
 * assembled CSR           ``ppe/ppe_builder_helpers.py:50-122, 264-311``
 * pin DOF                 ``core/boundary.py:181-191``
 """
@@ -102,6 +106,74 @@ def jacobi_diag_inv(coeffs, pin, shape):
     return 1.0 / np.where(np.abs(diag) > 1e-30, diag, 1.0)
 
 
+def _front(a, axis):
+    return np.moveaxis(np.asarray(a, dtype=np.float64), axis, 0)
+
+
+def thomas_lines(lo, di, up, r, axis):
+    """Exact Thomas sweep on every line along ``axis``
+    (linalg_backend.py:323-339); lo[0] and up[-1] are ignored."""
+    a, d, c, b = (_front(x, axis) for x in (lo, di, up, r))
+    n = d.shape[0]
+    cp = np.empty_like(d)
+    dp = np.empty_like(d)
+    dp[0] = b[0] / d[0]
+    if n > 1:
+        cp[0] = c[0] / d[0]
+    for i in range(1, n):
+        den = d[i] - a[i] * cp[i - 1]
+        if i < n - 1:
+            cp[i] = c[i] / den
+        dp[i] = (b[i] - a[i] * dp[i - 1]) / den
+    x = np.empty_like(d)
+    x[n - 1] = dp[n - 1]
+    for i in range(n - 2, -1, -1):
+        x[i] = dp[i] - cp[i] * x[i + 1]
+    return np.moveaxis(x, 0, axis)
+
+
+def pcr_stage_count(n, max_stages=None):
+    """ceil(log2 n) stages (at least 1), capped at max(1, max_stages)
+    (linalg_backend.py:146-150)."""
+    full = max(1, int(np.ceil(np.log2(n)))) if n > 1 else 1
+    return full if max_stages is None else min(full, max(1, int(max_stages)))
+
+
+def pcr_reduce(lo, di, up, r, axis, stages):
+    """``stages`` PCR reductions along ``axis`` (linalg_backend.py:77-115).
+
+    Returns the reduced (lo, di, up, rw): row i then couples only unknowns
+    i - 2^stages, i, i + 2^stages.  Division guards |d| > 0 as the reference.
+    """
+    a, d, c, b = (_front(x, axis).copy() for x in (lo, di, up, r))
+    n = d.shape[0]
+    s = 1
+    for _ in range(stages):
+        a2, d2, c2, b2 = a * 0.0, d.copy(), c * 0.0, b.copy()
+        if s < n:
+            dl = d[:-s]
+            al = -a[s:] / np.where(np.abs(dl) > 0, dl, 1.0)       # rows i >= s
+            a2[s:] = al * a[:-s]
+            d2[s:] += al * c[:-s]
+            b2[s:] += al * b[:-s]
+            dr = d[s:]
+            be = -c[:-s] / np.where(np.abs(dr) > 0, dr, 1.0)      # rows i < n - s
+            d2[:-s] += be * a[s:]
+            c2[:-s] = be * c[s:]
+            b2[:-s] += be * b[s:]
+        a, d, c, b = a2, d2, c2, b2
+        s *= 2
+    return tuple(np.moveaxis(x, 0, axis) for x in (a, d, c, b))
+
+
+def pcr_lines(lo, di, up, r, axis, max_stages=None):
+    """PCR solve rw / diag after the capped stage count (exact when
+    2^stages >= n; linalg_backend.py:118-163)."""
+    n = np.asarray(di).shape[axis]
+    _a, d, _c, b = pcr_reduce(lo, di, up, r, axis, pcr_stage_count(n, max_stages))
+    return b / d
+
+
 class OraclePPE:
     """CPU FVM PPE in the reference's CG form (uniform wall grids).
 
@@ -122,6 +194,7 @@ class OraclePPE:
 
     def prepare_operator(self, rho):
         rho = np.asarray(rho, dtype=np.float64)
+        self._rho = rho
         self.coeffs = [line_coeffs(rho, self.N, self.d_face, self.dv_node, a)
                        for a in range(self.ndim)]
         self.diag_inv = jacobi_diag_inv(self.coeffs, self.pin, self.shape)
@@ -185,6 +258,69 @@ class OraclePPE:
         p = sol.reshape(self.shape).copy()
         p.ravel()[self.pin] = 0.0
         return p, {"iterations": it[0], "relres": self.relres(rhs, p)}
+
+    # --- GMRES form (fvm_matrixfree.py:150-260, 302-330)
+    def shifted_line_coeffs(self, c_tau=2.0):
+        """(-lower, shift - main, -upper) per axis, shift = 2/(c_tau rho h_min^2)
+        (fvm_matrixfree_helpers.py:76-90); needs prepare_operator first."""
+        shift = 2.0 / (c_tau * self._rho * self.h_min ** 2)
+        return [(-lo, shift - mn, -up) for lo, mn, up in self.coeffs]
+
+    def gmres_precond(self, r, preconditioner="line_pcr", c_tau=2.0, max_stages=None):
+        """z = M^-1 r of the GMRES form, z[pin] = 0.  line_pcr: per-axis line
+        solves (Thomas when max_stages is None, else capped PCR) averaged
+        over axes; jacobi: r * diag_inv."""
+        r = np.asarray(r, dtype=np.float64).reshape(self.shape)
+        if preconditioner == "jacobi":
+            z = r * self.diag_inv
+        elif preconditioner == "line_pcr":
+            rw = r.copy()
+            rw.ravel()[self.pin] = 0.0
+            z = np.zeros_like(rw)
+            for ax, (lo, di, up) in enumerate(self.shifted_line_coeffs(c_tau)):
+                if max_stages is None:
+                    z += thomas_lines(lo, di, up, rw, ax)
+                else:
+                    z += pcr_lines(lo, di, up, rw, ax, max_stages)
+            z /= float(self.ndim)
+        else:
+            raise ValueError(preconditioner)
+        z = z.copy()
+        z.ravel()[self.pin] = 0.0
+        return z
+
+    def solve_gmres(self, rhs, rho, tol=1e-8, maxiter=500, restart=None,
+                    preconditioner="line_pcr", c_tau=2.0, max_stages=None, p_init=None):
+        """SciPy ``gmres`` on L p = rhs, rhs[pin] = 0 (gmres_helpers.py:16-42).
+        Returns (p, {"info", "iterations", "relres"}); p[pin] = 0."""
+        self.prepare_operator(rho)
+        n = int(np.prod(self.shape))
+        b = np.asarray(rhs, dtype=np.float64).ravel().copy()
+        b[self.pin] = 0.0
+        restart = min(80, max(20, maxiter)) if restart is None else int(restart)
+        A = spla.LinearOperator(
+            (n, n), matvec=lambda v: self.apply(np.asarray(v).reshape(self.shape)).ravel(),
+            dtype=np.float64)
+        M = None
+        if preconditioner in ("jacobi", "line_pcr"):
+            M = spla.LinearOperator(
+                (n, n), matvec=lambda v: self.gmres_precond(v, preconditioner, c_tau,
+                                                            max_stages).ravel(),
+                dtype=np.float64)
+        x0 = np.zeros(n) if p_init is None else np.asarray(p_init, dtype=np.float64).ravel().copy()
+        x0[self.pin] = 0.0
+        it = [0]
+
+        def cb(_pr):
+            it[0] += 1
+
+        sol, info = spla.gmres(A, b, x0=x0, M=M, restart=restart, maxiter=maxiter,
+                               atol=0.0, rtol=tol, callback=cb, callback_type="pr_norm")
+        p = sol.reshape(self.shape).copy()
+        p.ravel()[self.pin] = 0.0
+        res = float(np.linalg.norm(b - self.apply(sol.reshape(self.shape)).ravel())
+                    / np.linalg.norm(b)) if np.linalg.norm(b) > 0 else 0.0
+        return p, {"info": int(info), "iterations": it[0], "relres": res}
 
     # --- assembled twin (ppe_builder_helpers.py:50-122, 264-311)
     def csr(self, rho):

@@ -4,6 +4,7 @@ import json
 import os
 
 import numpy as np
+import pytest
 import scipy.sparse.linalg as spla
 
 from oracle import ppe as oppe
@@ -139,3 +140,103 @@ def test_projection_reduces_divergence():
     u1 = ofields.correct_velocity(ccd, u, p, rho, 1.0)
     div1 = ofields.divergence(ccd, u1)
     assert np.max(np.abs(div1[2:-2, 2:-2])) < 0.01 * np.max(np.abs(div0[2:-2, 2:-2]))
+
+
+# --- GMRES form: line solves, preconditioners, SciPy gmres --------------------
+def _tridiag_dense(a, d, c):
+    n = d.size
+    A = np.diag(d)
+    A[np.arange(1, n), np.arange(n - 1)] = a[1:]
+    A[np.arange(n - 1), np.arange(1, n)] = c[:-1]
+    return A
+
+
+def _random_lines(shape, axis, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-1.0, 0.0, shape)
+    c = rng.uniform(-1.0, 0.0, shape)
+    d = 2.5 + rng.uniform(0.0, 1.0, shape)   # diagonally dominant
+    r = rng.standard_normal(shape)
+    return a, d, c, r
+
+
+@pytest.mark.parametrize("shape,axis", [((37,), 0), ((9, 13), 0), ((9, 13), 1), ((5, 6, 33), 2)])
+def test_thomas_and_full_pcr_match_dense_lines(shape, axis):
+    """Both line solvers against numpy.linalg.solve line by line."""
+    a, d, c, r = _random_lines(shape, axis, 11)
+    xt = oppe.thomas_lines(a, d, c, r, axis)
+    xp_ = oppe.pcr_lines(a, d, c, r, axis, max_stages=None)
+    A_, D_, C_, R_ = (np.moveaxis(x, axis, -1).reshape(-1, shape[axis]) for x in (a, d, c, r))
+    ref = np.stack([np.linalg.solve(_tridiag_dense(A_[k], D_[k], C_[k]), R_[k])
+                    for k in range(A_.shape[0])])
+    ref = np.moveaxis(ref.reshape(np.moveaxis(a, axis, -1).shape), -1, axis)
+    np.testing.assert_allclose(xt, ref, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(xp_, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("stages", [1, 2, 3, 4])
+def test_pcr_reduction_keeps_the_exact_solution(stages):
+    """After k stages row i couples x[i -+ 2^k] only, and the exact solution
+    still satisfies every reduced row (a wrong sign or neighbour fails)."""
+    n = 37
+    a, d, c, r = _random_lines((n,), 0, 5)
+    a[0], c[-1] = 0.0, 0.0   # no couplings past the line ends
+    x = np.linalg.solve(_tridiag_dense(a, d, c), r)
+    a2, d2, c2, b2 = oppe.pcr_reduce(a, d, c, r, 0, stages)
+    s = 2 ** stages
+    lhs = d2 * x
+    lhs[s:] += a2[s:] * x[:-s]
+    lhs[:-s] += c2[:-s] * x[s:]
+    np.testing.assert_allclose(lhs, b2, rtol=1e-12, atol=1e-12)
+    assert np.all(a2[:s] == 0.0) and np.all(c2[n - s:] == 0.0)
+
+
+def test_truncated_pcr_is_exact_on_short_lines():
+    a, d, c, r = _random_lines((16, 7), 0, 9)
+    np.testing.assert_allclose(oppe.pcr_lines(a, d, c, r, 0, max_stages=4),
+                               oppe.thomas_lines(a, d, c, r, 0), rtol=1e-13, atol=1e-13)
+    assert oppe.pcr_stage_count(257, 4) == 4 and oppe.pcr_stage_count(257, None) == 9
+    assert oppe.pcr_stage_count(16, 0) == 1
+
+
+def test_line_preconditioner_uniform_density_textbook_rows():
+    """rho = 1: per-axis rows (shift + 2/h^2) z_i - z_{i+-1}/h^2 with the wall
+    face doubled (dv = h/2); z = mean over axes of the line solves, z[pin] = 0."""
+    N, c_tau = (12, 8), 2.0
+    o = oppe.OraclePPE(N)
+    rho = np.ones(o.shape)
+    o.prepare_operator(rho)
+    r = np.random.default_rng(3).standard_normal(o.shape)
+    rw = r.copy()
+    rw.ravel()[o.pin] = 0.0
+    h_min = 1.0 / max(N)
+    shift = 2.0 / (c_tau * h_min ** 2)
+    z = np.zeros_like(r)
+    for ax in range(2):
+        n, h = o.shape[ax], 1.0 / N[ax]
+        A = np.zeros((n, n))
+        for i in range(n):
+            if i > 0:
+                A[i, i - 1] = -(2.0 if i == n - 1 else 1.0) / h ** 2
+            if i < n - 1:
+                A[i, i + 1] = -(2.0 if i == 0 else 1.0) / h ** 2
+            A[i, i] = shift - A[i].sum()
+        z += np.moveaxis(np.linalg.solve(A, np.moveaxis(rw, ax, 0).reshape(n, -1))
+                         .reshape(np.moveaxis(rw, ax, 0).shape), 0, ax)
+    z /= 2.0
+    z.ravel()[o.pin] = 0.0
+    for stages in (None, 10):
+        np.testing.assert_allclose(o.gmres_precond(r, "line_pcr", c_tau, stages), z,
+                                   rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("pre,seed", [("line_pcr", 456), ("jacobi", 789), ("none", 456)])
+def test_gmres_matches_direct(pre, seed):
+    """test_fvm_matrixfree.py:88-103, 153-169: N=8, tol 1e-10, maxiter 200."""
+    o = oppe.OraclePPE((8, 8))
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + rng.uniform(0.0, 1.0, o.shape)
+    rhs = rng.standard_normal(o.shape)
+    p, info = o.solve_gmres(rhs, rho, tol=1e-10, maxiter=200, preconditioner=pre)
+    assert info["info"] == 0 and info["relres"] <= 1e-10
+    np.testing.assert_allclose(p, o.solve_direct(rhs, rho), rtol=1e-9, atol=1e-10)

@@ -36,6 +36,12 @@ class SolveReport(C.Structure):
                 ("refinements", C.c_int)]
 
 
+class GmresOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxiter", C.c_int64), ("restart", C.c_int),
+                ("precond", C.c_int), ("pcr_stages", C.c_int), ("use_x0", C.c_int),
+                ("c_tau", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol of include/tpf.h and tpf_testing.h
 SIGNATURES = {
     "tpf_last_error": (C.c_char_p, []),
@@ -51,6 +57,9 @@ SIGNATURES = {
     "tpf_ppe_apply": (C.c_int, [_vp, _vp, _vp]),
     "tpf_ppe_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveReport)]),
     "tpf_ppe_true_relres": (C.c_int, [_vp, _vp, _vp, _dp]),
+    "tpf_ppe_solve_gmres": (C.c_int, [_vp, _vp, _vp, C.POINTER(GmresOpts),
+                                      C.POINTER(SolveReport)]),
+    "tpf_ppe_gmres_precond": (C.c_int, [_vp, _vp, _vp, C.POINTER(GmresOpts)]),
     "tpf_fields_bubbles": (C.c_int, [C.c_int, _i64p, _dp, C.c_int, _dp, _dp, C.c_double,
                                      C.c_double, C.c_double, _vp, _vp, _vp]),
     "tpf_fields_mass": (C.c_int, [C.c_int, _i64p, _dp, _ip, _vp, _dp, _vp]),

@@ -1,0 +1,669 @@
+// GMRES form of the FVM PPE with the shifted line preconditioner.
+//
+// Reference: ppe/fvm_matrixfree.py:150-260 (GMRES branch), :302-330
+// (apply_line_preconditioner / apply_jacobi_preconditioner),
+// fvm_matrixfree_helpers.py:47-101 (line coefficients, shift
+// 2/(c_tau rho h_min^2), Jacobi diagonal), linalg_backend.py:77-115 and
+// 270-340 (exact Thomas on the CPU backend, PCR capped at `max_stages` on the
+// GPU backend) and gmres_helpers.py:16-42 (SciPy gmres, atol = 0,
+// rtol = tol).  The system is L p = rhs with rhs[pin] = 0 and the pin row
+// replaced by identity (fvm_matrixfree.py:163-166, 297-299).
+//
+// GMRES(m) follows SciPy 1.15's left-preconditioned restart loop
+// (scipy/sparse/linalg/_isolve/iterative.py, gmres): inner stop on the
+// preconditioned residual estimate <= ptol (adapted per cycle), outer stop on
+// the true ||rhs - L x|| <= tol ||rhs||, `maxiter` counts restart cycles.
+// Orthogonalisation is classical Gram-Schmidt applied twice (CGS2: one fused
+// multi-dot and one multi-axpy kernel per pass, chunks of 16 basis vectors
+// spread over grid.y) instead of SciPy's modified Gram-Schmidt, so iteration
+// histories agree to rounding, not bit for bit.  The small Hessenberg /
+// Givens work runs on the host in FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "ppe_impl.hpp"
+#include "reduce.hpp"
+
+namespace tpf {
+namespace {
+
+constexpr int kChunk = 16;          // basis vectors per multi-dot block row
+constexpr int kDot = kChunk + 1;    // + <w, w>
+
+__device__ inline void coords3(const PpeGeom& g, int64_t idx, int (&c)[3]) {
+  const uint32_t plane = (uint32_t)(g.n1 * g.n2);
+  const uint32_t u = (uint32_t)idx;
+  c[0] = (int)(u / plane);
+  const uint32_t rem = u - (uint32_t)c[0] * plane;
+  c[1] = (int)(rem / (uint32_t)g.n2);
+  c[2] = (int)(rem - (uint32_t)c[1] * (uint32_t)g.n2);
+}
+
+__host__ __device__ inline int slot_n(const PpeGeom& g, int a) {
+  return a == 0 ? g.n0 : (a == 1 ? g.n1 : g.n2);
+}
+__host__ __device__ inline int64_t slot_stride(const PpeGeom& g, int a) {
+  return a == 0 ? (int64_t)g.n1 * g.n2 : (a == 1 ? (int64_t)g.n2 : 1);
+}
+
+// Face weight of node i along slot a: (1/h) / dv_i, dv = h/2 at the walls.
+__device__ inline double face_w(const PpeGeom& g, int a, int i) {
+  const int n = slot_n(g, a);
+  const double dv = (i == 0 || i == n - 1) ? 0.5 * g.h[a] : g.h[a];
+  return g.ih[a] / dv;
+}
+
+// Unweighted L p (fvm_matrixfree.py:281-300): per axis (1/dv_i) sum over the
+// two faces of 2/(rho_i + rho_nbr)/h (p_nbr - p_i); pin row = identity.
+__global__ __launch_bounds__(kBlock) void k_L(PpeGeom g, const double* __restrict__ p,
+                                              const double* __restrict__ rho,
+                                              double* __restrict__ out) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int c[3];
+    coords3(g, idx, c);
+    const double pc = p[idx], rc = rho[idx];
+    double acc = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int n = slot_n(g, a);
+      if (n == 1) continue;
+      const int64_t st = slot_stride(g, a);
+      const double w = face_w(g, a, c[a]);
+      if (c[a] > 0) acc += w * (2.0 / (rc + rho[idx - st])) * (p[idx - st] - pc);
+      if (c[a] < n - 1) acc += w * (2.0 / (rc + rho[idx + st])) * (p[idx + st] - pc);
+    }
+    out[idx] = idx == g.pin ? pc : acc;
+  }
+}
+
+// Jacobi of the GMRES form (fvm_matrixfree_helpers.py:92-100,
+// fvm_matrixfree.py:325-330): z = r / sum_axes(main), diag[pin] = 1,
+// |diag| <= 1e-30 -> 1, then z[pin] = 0.
+__global__ __launch_bounds__(kBlock) void k_jacobi_L(PpeGeom g, const double* __restrict__ rho,
+                                                     const double* __restrict__ r,
+                                                     double* __restrict__ z) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int c[3];
+    coords3(g, idx, c);
+    const double rc = rho[idx];
+    double diag = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int n = slot_n(g, a);
+      if (n == 1) continue;
+      const int64_t st = slot_stride(g, a);
+      const double w = face_w(g, a, c[a]);
+      double lo = 0.0, up = 0.0;
+      if (c[a] > 0) lo = w * (2.0 / (rc + rho[idx - st]));
+      if (c[a] < n - 1) up = w * (2.0 / (rc + rho[idx + st]));
+      diag += -(lo + up);
+    }
+    if (idx == g.pin) diag = 1.0;
+    if (!(fabs(diag) > 1e-30)) diag = 1.0;
+    z[idx] = idx == g.pin ? 0.0 : r[idx] / diag;
+  }
+}
+
+// Shifted line system of node i on one line along slot a
+// (fvm_matrixfree_helpers.py:47-90): lo z_{i-1} + di z_i + up z_{i+1}, with
+// lo = -lower, up = -upper, di = shift - main = shift + lower + upper.
+__device__ inline void line_coef(const PpeGeom& g, const double* __restrict__ rho, int a,
+                                 int64_t o, int64_t st, int n, int i, double c_tau,
+                                 double h_min2, double& lo, double& di, double& up) {
+  const double ri = rho[o];
+  const double w = face_w(g, a, i);
+  const double lower = i > 0 ? w * (2.0 / (ri + rho[o - st])) : 0.0;
+  const double upper = i < n - 1 ? w * (2.0 / (ri + rho[o + st])) : 0.0;
+  const double shift = 2.0 / (c_tau * ri * h_min2);
+  lo = -lower;
+  up = -upper;
+  di = shift - (-(lower + upper));
+}
+
+struct LineArgs {
+  int a;          // slot
+  double c_tau;
+  double h_min2;  // h_min^2
+  double scale;   // 1 / ndim
+};
+
+// Exact Thomas per line (linalg_backend.py:323-339), one thread per line;
+// consecutive threads take neighbouring lines, so the sweeps are coalesced
+// for the two strided slots.  cp / dp are per-node scratch.
+__global__ __launch_bounds__(kBlock) void k_line_thomas(PpeGeom g, LineArgs la,
+                                                        const double* __restrict__ rho,
+                                                        const double* __restrict__ r,
+                                                        double* __restrict__ out,
+                                                        double* __restrict__ cp,
+                                                        double* __restrict__ dp) {
+  const int a = la.a;
+  const int n = slot_n(g, a);
+  const int64_t st = slot_stride(g, a);
+  const int64_t nlines = g.total / n;
+  for (int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x; l < nlines;
+       l += (int64_t)gridDim.x * kBlock) {
+    int64_t base;
+    if (a == 0) base = l;
+    else if (a == 1) base = (l / g.n2) * ((int64_t)g.n1 * g.n2) + (l % g.n2);
+    else base = l * n;
+    double cprev = 0.0, dprev = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const int64_t o = base + i * st;
+      double lo, di, up;
+      line_coef(g, rho, a, o, st, n, i, la.c_tau, la.h_min2, lo, di, up);
+      const double ri = o == g.pin ? 0.0 : r[o];  // r_work[pin] = 0
+      const double den = i == 0 ? di : di - lo * cprev;
+      const double cc = i < n - 1 ? up / den : 0.0;
+      const double dd = (i == 0 ? ri : ri - lo * dprev) / den;
+      cp[o] = cc;
+      dp[o] = dd;
+      cprev = cc;
+      dprev = dd;
+    }
+    double x = 0.0;
+    for (int i = n - 1; i >= 0; --i) {
+      const int64_t o = base + i * st;
+      x = i == n - 1 ? dp[o] : dp[o] - cp[o] * x;
+      out[o] += la.scale * x;
+    }
+  }
+}
+
+// PCR (linalg_backend.py:77-115): coefficient arrays over the whole field,
+// one launch per stage, neighbours at +-stride along slot a.
+__global__ __launch_bounds__(kBlock) void k_pcr_init(PpeGeom g, LineArgs la,
+                                                     const double* __restrict__ rho,
+                                                     const double* __restrict__ r,
+                                                     double* __restrict__ lo,
+                                                     double* __restrict__ di,
+                                                     double* __restrict__ up,
+                                                     double* __restrict__ rw) {
+  const int a = la.a;
+  const int n = slot_n(g, a);
+  const int64_t st = slot_stride(g, a);
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int c[3];
+    coords3(g, idx, c);
+    double l, d, u;
+    line_coef(g, rho, a, idx, st, n, c[a], la.c_tau, la.h_min2, l, d, u);
+    lo[idx] = l;
+    di[idx] = d;
+    up[idx] = u;
+    rw[idx] = idx == g.pin ? 0.0 : r[idx];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcr_stage(PpeGeom g, int a, int stride,
+                                                      const double* __restrict__ lo,
+                                                      const double* __restrict__ di,
+                                                      const double* __restrict__ up,
+                                                      const double* __restrict__ rw,
+                                                      double* __restrict__ lo2,
+                                                      double* __restrict__ di2,
+                                                      double* __restrict__ up2,
+                                                      double* __restrict__ rw2) {
+  const int n = slot_n(g, a);
+  const int64_t st = slot_stride(g, a);
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int c[3];
+    coords3(g, idx, c);
+    const int i = c[a];
+    double alpha = 0.0, beta = 0.0;
+    double uL = 0.0, rL = 0.0, lR = 0.0, rR = 0.0, lL = 0.0, uR = 0.0;
+    if (i >= stride) {
+      const int64_t o = idx - stride * st;
+      const double dL = di[o];
+      alpha = -lo[idx] / (fabs(dL) > 0.0 ? dL : 1.0);
+      lL = lo[o];
+      uL = up[o];
+      rL = rw[o];
+    }
+    if (i < n - stride) {
+      const int64_t o = idx + stride * st;
+      const double dR = di[o];
+      beta = -up[idx] / (fabs(dR) > 0.0 ? dR : 1.0);
+      lR = lo[o];
+      uR = up[o];
+      rR = rw[o];
+    }
+    lo2[idx] = alpha * lL;
+    di2[idx] = di[idx] + alpha * uL + beta * lR;
+    up2[idx] = beta * uR;
+    rw2[idx] = rw[idx] + alpha * rL + beta * rR;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcr_finish(int64_t n, double scale,
+                                                       const double* __restrict__ di,
+                                                       const double* __restrict__ rw,
+                                                       double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    out[i] += scale * (rw[i] / di[i]);
+}
+
+// ---- vector kernels ----------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_fill(int64_t n, double v, double* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    x[i] = v;
+}
+__global__ void k_set_pin(double* __restrict__ x, int64_t pin) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) x[pin] = 0.0;
+}
+// y = a x + b y
+__global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, double a,
+                                                  const double* __restrict__ x, double b,
+                                                  double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    y[i] = a * x[i] + b * y[i];
+}
+// y = x - y
+__global__ __launch_bounds__(kBlock) void k_xmy(int64_t n, const double* __restrict__ x,
+                                                double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    y[i] = x[i] - y[i];
+}
+
+// Block row blockIdx.y: partial sums of <V_k, w> for the basis vectors
+// k = 16*blockIdx.y + j (j < 16, k < kcount) and of <w, w> (slot 16).
+__global__ __launch_bounds__(kBlock) void k_multidot(int64_t n, const double* __restrict__ V,
+                                                     int kcount, const double* __restrict__ w,
+                                                     double* partials) {
+  __shared__ double sh[kDot * (kBlock / 64)];
+  const int k0 = blockIdx.y * kChunk;
+  const int kc = min(kChunk, kcount - k0);
+  double v[kDot];
+#pragma unroll
+  for (int k = 0; k < kDot; ++k) v[k] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const double wi = w[i];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      if (k < kc) v[k] += V[(int64_t)(k0 + k) * n + i] * wi;
+    v[kChunk] += wi * wi;
+  }
+  block_partials<kDot>(v, partials + (size_t)blockIdx.y * gridDim.x * kDot, sh);
+}
+__global__ __launch_bounds__(kBlock) void k_multidot_fin(const double* partials, int nblocks,
+                                                         double* out) {
+  __shared__ double sh[kDot * (kBlock / 64)];
+  double v[kDot];
+  finish_sum<kDot>(v, partials + (size_t)blockIdx.y * nblocks * kDot, nblocks, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < kDot; ++k) out[blockIdx.y * kDot + k] = v[k];
+}
+// w += sign * sum_{k < kc} coef[k] V_k
+__global__ __launch_bounds__(kBlock) void k_multiaxpy(int64_t n, const double* __restrict__ V,
+                                                      int kc, const double* __restrict__ coef,
+                                                      double sign, double* __restrict__ w) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    double acc = 0.0;
+    for (int k = 0; k < kc; ++k) acc += coef[k] * V[(int64_t)k * n + i];
+    w[i] += sign * acc;
+  }
+}
+
+// ---- host side -----------------------------------------------------------
+struct GmresWork {
+  int m = 0;             // restart the basis was sized for
+  int chunks = 0;        // ceil((m + 1) / 16)
+  double* V = nullptr;   // (m + 1) x n basis, row-major
+  double* w = nullptr;   // residual
+  double* t = nullptr;   // L v
+  double* b = nullptr;   // rhs with rhs[pin] = 0
+  double* s8 = nullptr;  // 8 n: Thomas factors or PCR ping-pong arrays
+  double* coef = nullptr;
+  double* part = nullptr;
+  double* dots = nullptr;
+  double* host_dn = nullptr;  // pinned: chunks * 17 results
+  double* host_up = nullptr;  // pinned: m + 1 coefficients
+};
+
+void free_work(GmresWork* gw) {
+  double* bufs[] = {gw->V, gw->w, gw->t, gw->b, gw->s8, gw->coef, gw->part, gw->dots};
+  for (double* p : bufs)
+    if (p) (void)hipFree(p);
+  if (gw->host_dn) (void)hipHostFree(gw->host_dn);
+  if (gw->host_up) (void)hipHostFree(gw->host_up);
+  delete gw;
+}
+
+tpf_status ensure_work(tpf_ppe* s, int m) {
+  GmresWork* gw = static_cast<GmresWork*>(s->gmres_work);
+  if (gw && gw->m >= m) return TPF_OK;
+  if (gw) free_work(gw);
+  gw = new GmresWork();
+  s->gmres_work = gw;
+  const size_t n = (size_t)s->g.total;
+  gw->m = m;
+  gw->chunks = (m + 1 + kChunk - 1) / kChunk;
+  TPF_HIP(hipMalloc(&gw->V, sizeof(double) * n * (size_t)(m + 1)));
+  TPF_HIP(hipMalloc(&gw->w, sizeof(double) * n));
+  TPF_HIP(hipMalloc(&gw->t, sizeof(double) * n));
+  TPF_HIP(hipMalloc(&gw->b, sizeof(double) * n));
+  TPF_HIP(hipMalloc(&gw->s8, sizeof(double) * n * 8));
+  TPF_HIP(hipMalloc(&gw->coef, sizeof(double) * (size_t)(m + 1)));
+  TPF_HIP(hipMalloc(&gw->part, sizeof(double) * kDot * (size_t)s->grid * gw->chunks));
+  TPF_HIP(hipMalloc(&gw->dots, sizeof(double) * kDot * (size_t)gw->chunks));
+  TPF_HIP(hipHostMalloc(&gw->host_dn, sizeof(double) * kDot * (size_t)gw->chunks));
+  TPF_HIP(hipHostMalloc(&gw->host_up, sizeof(double) * (size_t)(m + 1)));
+  return TPF_OK;
+}
+
+struct Ctx {
+  tpf_ppe* s;
+  GmresWork* gw;
+  hipStream_t st;
+  dim3 grid;
+  int64_t n;
+  int precond;
+  int stages;
+  double c_tau, h_min2;
+};
+
+#define TPF_LAUNCH(kern, grid, ...)                                               \
+  do {                                                                            \
+    hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, c.st, __VA_ARGS__);           \
+    TPF_HIP(hipGetLastError());                                                   \
+  } while (0)
+
+tpf_status apply_L(Ctx& c, const double* x, double* y) {
+  TPF_LAUNCH(k_L, c.grid, c.s->g, x, c.s->rho, y);
+  return TPF_OK;
+}
+
+// z = M^-1 r (fvm_matrixfree.py:302-330); z and r must differ.
+tpf_status psolve(Ctx& c, const double* r, double* z) {
+  const PpeGeom& g = c.s->g;
+  if (c.precond == 0) {
+    TPF_HIP(hipMemcpyAsync(z, r, sizeof(double) * c.n, hipMemcpyDeviceToDevice, c.st));
+    return TPF_OK;
+  }
+  if (c.precond == 1) {
+    TPF_LAUNCH(k_jacobi_L, c.grid, g, c.s->rho, r, z);
+    return TPF_OK;
+  }
+  const int ndim = (g.n0 > 1) + (g.n1 > 1) + (g.n2 > 1);
+  TPF_LAUNCH(k_fill, c.grid, c.n, 0.0, z);
+  double* s8 = c.gw->s8;
+  for (int a = 0; a < 3; ++a) {
+    const int na = slot_n(g, a);
+    if (na == 1) continue;
+    const LineArgs la{a, c.c_tau, c.h_min2, 1.0 / ndim};
+    if (c.stages <= 0) {
+      const int64_t nlines = c.n / na;
+      const int64_t lg = std::min<int64_t>((nlines + kBlock - 1) / kBlock, 1 << 20);
+      TPF_LAUNCH(k_line_thomas, dim3((unsigned)lg), g, la, c.s->rho, r, z, s8, s8 + c.n);
+      continue;
+    }
+    double* A[4] = {s8, s8 + c.n, s8 + 2 * c.n, s8 + 3 * c.n};
+    double* B[4] = {s8 + 4 * c.n, s8 + 5 * c.n, s8 + 6 * c.n, s8 + 7 * c.n};
+    TPF_LAUNCH(k_pcr_init, c.grid, g, la, c.s->rho, r, A[0], A[1], A[2], A[3]);
+    // linalg_backend.py:147-150: min(ceil(log2 n), max(1, max_stages)) stages
+    int full = 0;
+    while ((1 << full) < na) ++full;
+    full = std::max(1, full);
+    const int ns = std::min(full, std::max(1, c.stages));
+    for (int k = 0, stride = 1; k < ns; ++k, stride *= 2) {
+      TPF_LAUNCH(k_pcr_stage, c.grid, g, a, stride, A[0], A[1], A[2], A[3], B[0], B[1], B[2],
+                 B[3]);
+      for (int q = 0; q < 4; ++q) std::swap(A[q], B[q]);
+    }
+    TPF_LAUNCH(k_pcr_finish, c.grid, c.n, la.scale, A[1], A[3], z);
+  }
+  TPF_LAUNCH(k_set_pin, dim3(1), z, g.pin);
+  return TPF_OK;
+}
+
+// out[j*17 + k] = <V_{16j+k}, w> for 16j+k < kcount, out[j*17 + 16] = <w,w>.
+// Synchronises the stream: every upload of host_up issued before this call
+// has completed when it returns, so host_up may be rewritten afterwards.
+tpf_status multidot(Ctx& c, const double* V, int kcount, const double* w, double* out) {
+  const int chunks = std::max(1, (kcount + kChunk - 1) / kChunk);
+  hipLaunchKernelGGL(k_multidot, dim3(c.grid.x, chunks), dim3(kBlock), 0, c.st, c.n, V, kcount,
+                     w, c.gw->part);
+  TPF_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_multidot_fin, dim3(1, chunks), dim3(kBlock), 0, c.st,
+                     (const double*)c.gw->part, (int)c.grid.x, c.gw->dots);
+  TPF_HIP(hipGetLastError());
+  TPF_HIP(hipMemcpyAsync(c.gw->host_dn, c.gw->dots, sizeof(double) * kDot * chunks,
+                         hipMemcpyDeviceToHost, c.st));
+  TPF_HIP(hipStreamSynchronize(c.st));
+  std::copy(c.gw->host_dn, c.gw->host_dn + kDot * chunks, out);
+  return TPF_OK;
+}
+
+tpf_status norm2(Ctx& c, const double* w, double& out) {
+  double d[kDot];
+  TPF_TRY(multidot(c, w, 0, w, d));
+  out = std::sqrt(d[kChunk]);
+  return TPF_OK;
+}
+
+// w += sign * sum_k coef[k] V_k.  Asynchronous: the stream synchronisation
+// in the next multidot / norm2 (every caller issues one before the next
+// multiaxpy) completes the upload before host_up is written again.
+tpf_status multiaxpy(Ctx& c, const double* V, int kc, const double* coef, double sign,
+                     double* w) {
+  std::copy(coef, coef + kc, c.gw->host_up);
+  TPF_HIP(hipMemcpyAsync(c.gw->coef, c.gw->host_up, sizeof(double) * kc, hipMemcpyHostToDevice,
+                         c.st));
+  TPF_LAUNCH(k_multiaxpy, c.grid, c.n, V, kc, (const double*)c.gw->coef, sign, w);
+  return TPF_OK;
+}
+
+// Plane rotation with c f + s g = r, -s f + c g = 0 (LAPACK 3.10 dlartg
+// conventions: c >= 0, r carries the sign of f).
+void lartg(double f, double g, double& cs, double& sn, double& r) {
+  if (g == 0.0) {
+    cs = 1.0;
+    sn = 0.0;
+    r = f;
+  } else if (f == 0.0) {
+    cs = 0.0;
+    sn = g > 0.0 ? 1.0 : -1.0;
+    r = std::fabs(g);
+  } else {
+    const double d = std::hypot(f, g);
+    cs = std::fabs(f) / d;
+    r = f > 0.0 ? d : -d;
+    sn = g / r;
+  }
+}
+
+tpf_status check_opts(tpf_ppe* s, const tpf_gmres_opts* o) {
+  if (!s || !o) return fail(TPF_ERR_VALUE, "null argument");
+  if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set (call tpf_ppe_set_density)");
+  if (o->precond < 0 || o->precond > 2)
+    return fail(TPF_ERR_VALUE, "GMRES preconditioner must be 0 (none), 1 (Jacobi), 2 (line)");
+  if (o->precond == 2 && !(o->c_tau > 0.0))
+    return fail(TPF_ERR_VALUE, "line preconditioner needs c_tau > 0");
+  return TPF_OK;
+}
+
+tpf_status make_ctx(tpf_ppe* s, const tpf_gmres_opts* o, int m, Ctx& c) {
+  TPF_TRY(ensure_work(s, m));
+  hipEvent_t ev;
+  TPF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  TPF_HIP(hipEventRecord(ev, s->user_stream));
+  TPF_HIP(hipStreamWaitEvent(s->stream, ev, 0));
+  TPF_HIP(hipEventDestroy(ev));
+  const PpeGeom& g = s->g;
+  double h_min = 0.0;
+  for (int a = 0; a < 3; ++a)
+    if (slot_n(g, a) > 1) h_min = h_min == 0.0 ? g.h[a] : std::min(h_min, g.h[a]);
+  c = Ctx{s, static_cast<GmresWork*>(s->gmres_work), s->stream, dim3((unsigned)s->grid),
+          g.total, o->precond, o->pcr_stages, o->c_tau, h_min * h_min};
+  return TPF_OK;
+}
+
+}  // namespace
+}  // namespace tpf
+
+using namespace tpf;
+
+void tpf_gmres_release(tpf_ppe* s) {
+  if (s->gmres_work) free_work(static_cast<GmresWork*>(s->gmres_work));
+  s->gmres_work = nullptr;
+}
+
+extern "C" tpf_status tpf_ppe_gmres_precond(tpf_ppe* s, const double* r, double* z,
+                                            const tpf_gmres_opts* o) {
+  TPF_TRY(check_opts(s, o));
+  if (!r || !z || r == z) return fail(TPF_ERR_VALUE, "r and z must be distinct device arrays");
+  Ctx c;
+  TPF_TRY(make_ctx(s, o, 1, c));
+  TPF_TRY(psolve(c, r, z));
+  TPF_HIP(hipStreamSynchronize(c.st));
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ppe_solve_gmres(tpf_ppe* s, const double* rhs, double* x,
+                                          const tpf_gmres_opts* o, tpf_solve_report* rep) {
+  TPF_TRY(check_opts(s, o));
+  if (!rhs || !x) return fail(TPF_ERR_VALUE, "null argument");
+  if (!(o->tol > 0.0) || o->maxiter < 1 || o->restart < 1)
+    return fail(TPF_ERR_VALUE, "GMRES needs tol > 0, maxiter >= 1, restart >= 1");
+  const int m = (int)std::min<int64_t>(o->restart, s->g.total);  // SciPy: min(restart, n)
+  Ctx c;
+  TPF_TRY(make_ctx(s, o, m, c));
+  GmresWork* gw = c.gw;
+  const int64_t n = c.n;
+  const int64_t pin = s->g.pin;
+  double* b = gw->b;
+  double* V = gw->V;
+  TPF_HIP(hipMemcpyAsync(b, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, c.st));
+  TPF_LAUNCH(k_set_pin, dim3(1), b, pin);
+  if (o->use_x0) TPF_LAUNCH(k_set_pin, dim3(1), x, pin);
+  else TPF_LAUNCH(k_fill, c.grid, n, 0.0, x);
+  tpf_solve_report r{};
+  double bnrm2 = 0.0;
+  TPF_TRY(norm2(c, b, bnrm2));
+  if (bnrm2 == 0.0) {  // SciPy returns b itself (zero)
+    TPF_LAUNCH(k_fill, c.grid, n, 0.0, x);
+    TPF_HIP(hipStreamSynchronize(c.st));
+    r.converged = 1;
+    if (rep) *rep = r;
+    return TPF_OK;
+  }
+  const double atol = o->tol * bnrm2;
+  const double eps = 2.220446049250313e-16;
+  // Mb_nrm2 = ||M b|| seeds the inner tolerance
+  double Mb = 0.0;
+  TPF_TRY(psolve(c, b, gw->w));
+  TPF_TRY(norm2(c, gw->w, Mb));
+  double ptol_max = 1.0;
+  double ptol = Mb * std::min(ptol_max, atol / bnrm2);
+  double presid = 0.0;
+  // r = b - L x
+  TPF_TRY(apply_L(c, x, gw->w));
+  TPF_LAUNCH(k_xmy, c.grid, n, (const double*)b, gw->w);
+  double rnorm = 0.0;
+  TPF_TRY(norm2(c, gw->w, rnorm));
+  int64_t inner = 0, cycles = 0;
+  const size_t ld = (size_t)m + 1;
+  std::vector<double> h((size_t)m * ld), S(ld), gc(m), gs(m), y(ld), hc(ld);
+  std::vector<double> dots((size_t)kDot * gw->chunks);
+  const bool early = rnorm < atol;  // SciPy's first-cycle check
+  while (!early && cycles < o->maxiter) {
+    ++cycles;
+    TPF_TRY(psolve(c, gw->w, V));  // v0 = M r / ||M r||
+    double t0 = 0.0;
+    TPF_TRY(norm2(c, V, t0));
+    TPF_LAUNCH(k_axpby, c.grid, n, 0.0, (const double*)V, 1.0 / t0, V);
+    std::fill(S.begin(), S.end(), 0.0);
+    S[0] = t0;
+    bool breakdown = false;
+    int col = 0;
+    for (col = 0; col < m; ++col) {
+      double* vc = V + (int64_t)col * n;
+      double* vn = V + (int64_t)(col + 1) * n;
+      TPF_TRY(apply_L(c, vc, gw->t));
+      TPF_TRY(psolve(c, gw->t, vn));  // w, built in place in v_{col+1}
+      std::fill(hc.begin(), hc.begin() + col + 1, 0.0);
+      double h0 = 0.0;
+      for (int pass = 0; pass < 2; ++pass) {
+        TPF_TRY(multidot(c, V, col + 1, vn, dots.data()));
+        if (pass == 0) h0 = std::sqrt(dots[kChunk]);
+        for (int k = 0; k <= col; ++k) y[k] = dots[(k / kChunk) * kDot + k % kChunk];
+        for (int k = 0; k <= col; ++k) hc[k] += y[k];
+        TPF_TRY(multiaxpy(c, V, col + 1, y.data(), -1.0, vn));
+      }
+      double h1 = 0.0;
+      TPF_TRY(norm2(c, vn, h1));
+      double* hcol = &h[(size_t)col * ld];
+      for (int k = 0; k <= col; ++k) hcol[k] = hc[k];
+      hcol[col + 1] = h1;
+      if (h1 <= eps * h0) {  // exact-solution indicator
+        hcol[col + 1] = 0.0;
+        breakdown = true;
+      } else {
+        TPF_LAUNCH(k_axpby, c.grid, n, 0.0, (const double*)vn, 1.0 / h1, vn);
+      }
+      for (int k = 0; k < col; ++k) {
+        const double n0 = hcol[k], n1 = hcol[k + 1];
+        hcol[k] = gc[k] * n0 + gs[k] * n1;
+        hcol[k + 1] = -gs[k] * n0 + gc[k] * n1;
+      }
+      double cs, sn, mag;
+      lartg(hcol[col], hcol[col + 1], cs, sn, mag);
+      gc[col] = cs;
+      gs[col] = sn;
+      hcol[col] = mag;
+      hcol[col + 1] = 0.0;
+      const double tmp = -sn * S[col];
+      S[col] = cs * S[col];
+      S[col + 1] = tmp;
+      presid = std::fabs(tmp);
+      ++inner;
+      if (presid <= ptol || breakdown) break;
+    }
+    if (col == m) col = m - 1;
+    // upper-triangular solve h[:col+1, :col+1]^T y = S[:col+1] (SciPy's loop)
+    if (h[(size_t)col * ld + col] == 0.0) S[col] = 0.0;
+    for (int k = 0; k <= col; ++k) y[k] = S[k];
+    for (int k = col; k > 0; --k) {
+      if (y[k] != 0.0) {
+        y[k] /= h[(size_t)k * ld + k];
+        const double t = y[k];
+        for (int j = 0; j < k; ++j) y[j] -= t * h[(size_t)k * ld + j];
+      }
+    }
+    if (y[0] != 0.0) y[0] /= h[0];
+    TPF_TRY(multiaxpy(c, V, col + 1, y.data(), 1.0, x));
+    TPF_TRY(apply_L(c, x, gw->w));
+    TPF_LAUNCH(k_xmy, c.grid, n, (const double*)b, gw->w);
+    TPF_TRY(norm2(c, gw->w, rnorm));
+    if (rnorm <= atol || breakdown) break;
+    if (presid <= ptol) ptol_max = std::max(eps, 0.25 * ptol_max);
+    else ptol_max = std::min(1.0, 1.5 * ptol_max);
+    ptol = presid * std::min(ptol_max, atol / rnorm);
+  }
+  TPF_LAUNCH(k_set_pin, dim3(1), x, pin);  // sol[pin] = 0
+  TPF_HIP(hipStreamSynchronize(c.st));
+  r.iterations = inner;
+  r.relres = rnorm / bnrm2;
+  r.true_relres = rnorm / bnrm2;
+  r.converged = (early || rnorm <= atol) ? 1 : 0;
+  r.refinements = (int)cycles;
+  if (rep) *rep = r;
+  if (!r.converged)
+    return fail(TPF_ERR_RUNTIME, "PPE GMRES did not converge cleanly (info=" +
+                                     std::to_string(o->maxiter) +
+                                     ", ||rhs - L p||/||rhs|| = " + std::to_string(r.relres) + ")");
+  return TPF_OK;
+}

@@ -52,7 +52,9 @@ struct tpf_ppe {
   hipGraphExec_t graph32_exec = nullptr;
   int graph32_chunk = 0;
   const double* graph32_rho = nullptr;
+  void* gmres_work = nullptr;           // GMRES basis and line-solve scratch
 };
 
+void tpf_gmres_release(tpf_ppe* s);
 tpf_status tpf_ppe_solve_mixed(tpf_ppe* s, const double* rhs, double* x,
                                const tpf_solve_opts* opts, tpf_solve_report* rep);

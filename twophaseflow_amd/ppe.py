@@ -2,17 +2,26 @@
 
 Mirrors the reference plugin interface (``ppe/interfaces.py:24-88``,
 ``ppe/factory.py:31-120``) and ``PPESolverFVMMatrixFree``
-(``ppe/fvm_matrixfree.py:46-363``) in CG form: the system
-``(-W L) p = -W rhs`` with the centre-node pin, Jacobi preconditioning, the
-recursively updated ``||r|| <= tol ||b||`` stop, ``RuntimeError`` when it
-does not converge, and ``p[pin] = 0`` on return.  The Krylov loop runs on
-the device (libtpf, hipGraph chunks); nothing iterates in Python.
+(``ppe/fvm_matrixfree.py:46-363``) in both of its forms:
+
+* GMRES (the reference default, also chosen by "explicit" / "gauss_seidel" /
+  "adi"): ``L p = rhs`` with the pin row = identity, left-preconditioned
+  restarted GMRES with SciPy's stopping rules, preconditioner "line_pcr"
+  (default), "jacobi" or "none";
+* CG: ``(-W L) p = -W rhs``, Jacobi preconditioning, the recursively updated
+  ``||r|| <= tol ||b||`` stop (FP64 or FP32-inner/FP64-refined).
+
+Both raise ``RuntimeError`` when they do not converge and return
+``p[pin] = 0``.  The Krylov loops run on the device (libtpf); nothing
+iterates in Python.
 
 Options read from the config like the reference (``getattr``):
-``pseudo_tol``, ``pseudo_maxiter``, ``ppe_iteration_method`` (only "cg" is
-built: the reference's GMRES + line-PCR route is not), ``ppe_preconditioner``
-("jacobi"), plus ``ppe_precision`` ("fp64" | "mixed") and
-``ppe_check_every``.
+``pseudo_tol``, ``pseudo_maxiter``, ``pseudo_c_tau``, ``ppe_iteration_method``,
+``ppe_preconditioner``, ``ppe_restart`` (default ``min(80, max(20,
+maxiter))``), ``ppe_pcr_stages`` (default 4; the line solves use that many
+PCR stages, as the reference's GPU backend does), plus this build's
+``ppe_line_solver`` ("pcr" | "thomas": exact Thomas, the reference's CPU
+backend), ``ppe_precision`` ("fp64" | "mixed", CG) and ``ppe_check_every``.
 """
 
 from __future__ import annotations
@@ -56,7 +65,7 @@ class IPPESolver(ABC):
 
 
 class PPESolverFVMMatrixFree(IPPESolver):
-    """Matrix-free variable-density FVM PPE on the GPU (CG form, Jacobi)."""
+    """Matrix-free variable-density FVM PPE on the GPU (GMRES or CG form)."""
 
     scheme_names = ("fvm_iterative",)
     _scheme_aliases = {"fvm_matrixfree": "fvm_iterative", "matrixfree": "fvm_iterative"}
@@ -74,20 +83,39 @@ class PPESolverFVMMatrixFree(IPPESolver):
                 f"got {bc_type!r}.")
         self.tol = float(getattr(solver_cfg, "pseudo_tol", 1e-8))
         self.maxiter = int(getattr(solver_cfg, "pseudo_maxiter", 500))
-        method = str(getattr(solver_cfg, "ppe_iteration_method", "cg")).strip().lower()
+        self.c_tau = float(getattr(solver_cfg, "pseudo_c_tau", 2.0))
+        method = str(getattr(solver_cfg, "ppe_iteration_method", "gmres")).strip().lower()
         if method in {"explicit", "gauss_seidel", "adi"}:
-            method = "cg"
-        if method != "cg":
+            method = "gmres"
+        if method not in {"gmres", "cg"}:
             raise ValueError(
-                "twophaseflow_amd builds the CG form of the FVM PPE only; "
-                f"ppe_iteration_method={method!r} is not available")
+                "PPESolverFVMMatrixFree supports ppe_iteration_method='gmres' or 'cg' "
+                f"today, got {method!r}")
         self.iteration_method = method
-        self.preconditioner = str(getattr(solver_cfg, "ppe_preconditioner", "jacobi")).lower()
-        if self.preconditioner != "jacobi":
-            raise ValueError("ppe_iteration_method='cg' supports preconditioner='jacobi' here")
+        restart = getattr(solver_cfg, "ppe_restart", None)
+        self.restart = int(restart) if restart is not None else min(80, max(20, self.maxiter))
+        default_pre = "jacobi" if method == "cg" else "line_pcr"
+        self.preconditioner = str(
+            getattr(solver_cfg, "ppe_preconditioner", default_pre)).strip().lower()
+        if self.preconditioner not in {"jacobi", "line_pcr", "none"}:
+            raise ValueError("ppe_preconditioner must be 'jacobi', 'line_pcr', or 'none', "
+                             f"got {self.preconditioner!r}")
+        if method == "cg" and self.preconditioner == "line_pcr":
+            raise ValueError(
+                "ppe_iteration_method='cg' supports preconditioner='jacobi' or 'none'; "
+                "line_pcr is a nonsymmetric GMRES preconditioner.")
+        if method == "cg" and self.preconditioner == "none":
+            raise ValueError("unpreconditioned CG is not built; use 'jacobi'")
+        pcr_stages = getattr(solver_cfg, "ppe_pcr_stages", 4)
+        self.precond_pcr_stages = 4 if pcr_stages is None else int(pcr_stages)
+        self.line_solver = str(getattr(solver_cfg, "ppe_line_solver", "pcr")).lower()
+        if self.line_solver not in ("pcr", "thomas"):
+            raise ValueError("ppe_line_solver must be 'pcr' or 'thomas'")
         self.precision = str(getattr(solver_cfg, "ppe_precision", "fp64")).lower()
         if self.precision not in ("fp64", "mixed"):
             raise ValueError("ppe_precision must be 'fp64' or 'mixed'")
+        if self.precision == "mixed" and method != "cg":
+            raise ValueError("ppe_precision='mixed' is built for ppe_iteration_method='cg'")
         self.check_every = int(getattr(solver_cfg, "ppe_check_every", 32))
         self.inner_tol = float(getattr(solver_cfg, "ppe_inner_tol", 0.0))
         spec = bc_spec if bc_spec is not None else BoundarySpec(bc_type, grid.shape, grid.N)
@@ -156,16 +184,22 @@ class PPESolverFVMMatrixFree(IPPESolver):
                 self.prepare_operator(rho)
         elif self._rho is None:
             raise RuntimeError("no density set")
-        opts = nat.SolveOpts(tol=self.tol, maxiter=self.maxiter, precond=1,
-                             check_every=self.check_every, use_x0=1 if use_x0 else 0,
-                             mixed=1 if self.precision == "mixed" else 0,
-                             inner_tol=self.inner_tol)
         rep = nat.SolveReport()
-        status = self._lib.tpf_ppe_solve(self._handle, ptr(rhs), ptr(p), C.byref(opts),
-                                         C.byref(rep))
+        if self.iteration_method == "gmres":
+            gopts = self._gmres_opts(use_x0)
+            status = self._lib.tpf_ppe_solve_gmres(self._handle, ptr(rhs), ptr(p),
+                                                   C.byref(gopts), C.byref(rep))
+        else:
+            opts = nat.SolveOpts(tol=self.tol, maxiter=self.maxiter, precond=1,
+                                 check_every=self.check_every, use_x0=1 if use_x0 else 0,
+                                 mixed=1 if self.precision == "mixed" else 0,
+                                 inner_tol=self.inner_tol)
+            status = self._lib.tpf_ppe_solve(self._handle, ptr(rhs), ptr(p), C.byref(opts),
+                                             C.byref(rep))
         key = f"ppe_low_order_{self.iteration_method}"
         self.last_diagnostics = {
-            f"{key}_info": 0 if status == nat.TPF_OK else int(rep.iterations),
+            f"{key}_info": 0 if status == nat.TPF_OK else (
+                self.maxiter if self.iteration_method == "gmres" else int(rep.iterations)),
             f"{key}_tol": self.tol,
             f"{key}_maxiter": self.maxiter,
             "iterations": int(rep.iterations),
@@ -182,6 +216,27 @@ class PPESolverFVMMatrixFree(IPPESolver):
                     "PPE solver was applied.")
             nat.check(status)
         return self.last_diagnostics
+
+    def _gmres_opts(self, use_x0: bool = False) -> "nat.GmresOpts":
+        pre = {"none": 0, "jacobi": 1, "line_pcr": 2}[self.preconditioner]
+        stages = 0 if self.line_solver == "thomas" else max(1, self.precond_pcr_stages)
+        return nat.GmresOpts(tol=self.tol, maxiter=self.maxiter, restart=self.restart,
+                             precond=pre, pcr_stages=stages, use_x0=1 if use_x0 else 0,
+                             c_tau=self.c_tau)
+
+    def apply_preconditioner(self, r):
+        """GMRES-form preconditioner z = M^-1 r with z[pin] = 0
+        (fvm_matrixfree.py:302-330)."""
+        if self._rho is None:
+            raise RuntimeError("prepare_operator(rho) must be called first")
+        if self.iteration_method != "gmres" or self.preconditioner == "none":
+            raise RuntimeError("no GMRES-form preconditioner is configured")
+        io = Handoff(r)
+        t = io.to_device(r)
+        out = torch.empty_like(t)
+        nat.check(self._lib.tpf_ppe_gmres_precond(self._handle, ptr(t), ptr(out),
+                                                  C.byref(self._gmres_opts())))
+        return io.back(out)
 
     def solve(self, rhs, rho, dt: float = 0.0, p_init=None):
         io = Handoff(rhs)
