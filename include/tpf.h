@@ -204,6 +204,43 @@ TPF_API tpf_status tpf_ppe_system_rhs(tpf_ppe* ppe, const double* rhs, double* b
  * (tpf_ppe_prepare_f32 first). */
 TPF_API tpf_status tpf_ppe_system_apply_f32(tpf_ppe* ppe, const float* p, float* out);
 
+/* Geometric multigrid on the CG system (FP32), the backbone of the learned
+ * preconditioner; no counterpart in the reference.  Level 0 is the fine
+ * grid; coarse levels halve every axis (node-centred) with homogenised face
+ * conductances.  All calls below except tpf_mg_setup enqueue on the stream
+ * given to tpf_ppe_create and return without synchronising. */
+/* Builds (or refreshes) the hierarchy for the current density: at most
+ * max_levels levels, stopping at <= 1024 nodes.  Synchronous. */
+TPF_API tpf_status tpf_mg_setup(tpf_ppe* ppe, int max_levels, int* levels);
+/* Node counts (3 slots, leading 1s for 2-D) and pin index of a level. */
+TPF_API tpf_status tpf_mg_level_info(tpf_ppe* ppe, int level, int64_t* n3, int64_t* pin);
+/* z = M(r): one multigrid cycle (gamma 1: V, 2: W; nu_pre / nu_post
+ * red-black sweeps, reverse colour order after the coarse correction) down
+ * to stop_level (-1: the coarsest), where a level of <= 1024 nodes gets
+ * coarse_sweeps forward+reverse sweeps in LDS and a larger stop level
+ * coarse_sweeps forward+reverse grid sweeps.  z[pin] = 0. */
+TPF_API tpf_status tpf_mg_vcycle(tpf_ppe* ppe, const float* r, float* z, int nu_pre,
+                         int nu_post, int coarse_sweeps, int stop_level, int gamma);
+/* Descending half of the V-cycle from r to `level`: bc = that level's
+ * restricted residual right-hand side. */
+TPF_API tpf_status tpf_mg_down(tpf_ppe* ppe, const float* r, int level, int nu_pre,
+                       float* bc);
+/* Ascending half: xc is the correction on `level`; z = fine result. */
+TPF_API tpf_status tpf_mg_up(tpf_ppe* ppe, int level, const float* xc, int nu_post, float* z);
+/* z = P x_c: trilinear interpolation from `level` to the fine grid (the
+ * fine and intermediate pins stay 0). */
+TPF_API tpf_status tpf_mg_prolong(tpf_ppe* ppe, int level, const float* xc, float* z);
+/* bc = R r: full-weighting restriction from the fine grid to `level`
+ * (pins of the coarser levels set to 0). */
+TPF_API tpf_status tpf_mg_restrict(tpf_ppe* ppe, const float* r, int level, float* bc);
+/* y = A_level x (y[pin] = 0). */
+TPF_API tpf_status tpf_mg_apply(tpf_ppe* ppe, int level, const float* x, float* y);
+/* `sweeps` forward then `sweeps` reverse red-black sweeps on A_level x = b. */
+TPF_API tpf_status tpf_mg_smooth(tpf_ppe* ppe, int level, const float* b, float* x,
+                         int sweeps);
+/* out[0..3n) = face conductances of the 3 slots, out[3n..4n) = 1/diag. */
+TPF_API tpf_status tpf_mg_coeffs(tpf_ppe* ppe, int level, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,5 +1,5 @@
 """GPU checks of the learned preconditioner's BF16 MFMA convolutions against
-the FP64 CPU oracle, and of flexible CG with the U-Net preconditioner.
+the FP64 CPU oracle (the preconditioner itself: tests/test_gpu_mg.py).
 
 Tolerances: inputs are BF16-representable and the kernels accumulate in FP32
 over K = 27*Cin <= 3456 terms, so FP32 outputs match the oracle to 1e-4
@@ -74,35 +74,3 @@ def _solver(N, rho, tol=1e-8):
     s = PPESolverFVMMatrixFree(None, cfg, Grid(GridConfig(ndim=3, N=(N,) * 3, L=(1.0,) * 3)))
     s.prepare_operator(rho)
     return s
-
-
-def test_fcg_with_untrained_unet_reaches_tolerance():
-    _announce("fcg untrained 33^3")
-    g, rho, rhs = L.make_problem(32, seed=5)
-    s = _solver(32, rho)
-    net = L.UNet().cuda()
-    M = L.TwoLevelPreconditioner(s, net)
-    x, info = L.fcg_solve(s, rhs, M, tol=1e-8, maxiter=400)
-    assert info["true_relres"] <= 1e-8, info
-    assert x.view(-1)[s._pin_dof] == 0.0
-
-
-def test_training_improves_the_preconditioner():
-    """What the trained network gains over the same preconditioner untrained:
-    residual reached after a fixed number of FCG iterations on a held-out
-    problem (seeded)."""
-    _announce("train + fcg 33^3")
-    torch.manual_seed(0)
-    net = L.UNet(seed=1).cuda()
-    g, rho, rhs = L.make_problem(32, seed=999)
-    s = _solver(32, rho)
-    _, before = L.fcg_solve(s, rhs, L.TwoLevelPreconditioner(s, net), tol=1e-12, maxiter=30,
-                            fallback=False, restarts=0, stall_window=10 ** 6)
-    L.train(net, N=32, problems=16, epochs=3, lr=2e-3, seed=200)
-    s.prepare_operator(rho)
-    _, after = L.fcg_solve(s, rhs, L.TwoLevelPreconditioner(s, net), tol=1e-12, maxiter=30,
-                           fallback=False, restarts=0, stall_window=10 ** 6)
-    rb, ra = min(before["history"]), min(after["history"])
-    print(f"[gpu-case] best ||r||/||b|| in 30 FCG iterations: untrained {rb:.3e} -> trained {ra:.3e}",
-          file=sys.stderr, flush=True)
-    assert ra < 0.5 * rb

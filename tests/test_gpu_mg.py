@@ -1,0 +1,205 @@
+"""GPU parity of the multigrid hierarchy (libtpf tpf_mg_*) against the FP64
+oracle (oracle/mg.py), and the multigrid / learned-deflation flexible CG.
+
+FP32 kernels: conductances, operator and grid transfers agree with the
+oracle to 2e-6 of the field maximum; a full cycle to 1e-4 (FP32 rounding
+accumulated over sweeps and levels).  Solves are judged by the FP64 true
+residual of the reference CG system recomputed by the PPE oracle.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fields as ofields
+from oracle import mg as omg
+from oracle import ppe as oppe
+from twophaseflow_amd import (Grid, GridConfig, PPESolverFVMMatrixFree, SimulationConfig,
+                              SolverConfig)
+from twophaseflow_amd import learned as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(msg):
+    print(f"[gpu-case] {msg}", file=sys.stderr, flush=True)
+
+
+def _bubbles(N, ratio=1e-3):
+    nd = len(N)
+    c = [(0.3, 0.5, 0.5)[:nd], (0.7, 0.55, 0.4)[:nd], (0.5, 0.25, 0.7)[:nd]]
+    psi = ofields.gas_bubbles_psi(N, (1.0,) * nd, c, [0.18, 0.15, 0.12], 1.5 / max(N))
+    return ofields.density(psi, 1.0, ratio)
+
+
+def _setup(N, ratio=1e-3):
+    cfg = SimulationConfig(grid=GridConfig(ndim=len(N), N=tuple(N), L=(1.0,) * len(N)),
+                           solver=SolverConfig(pseudo_tol=1e-8, pseudo_maxiter=10 ** 6))
+    cfg.solver.ppe_iteration_method = "cg"
+    g = Grid(cfg.grid)
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    rho = _bubbles(N, ratio)
+    s.prepare_operator(rho)
+    H = L.MGHierarchy(s)
+    rho3 = rho.reshape((1,) * (3 - len(N)) + rho.shape)
+    h3 = [1.0] * (3 - len(N)) + [1.0 / n for n in N]
+    O = omg.OracleMG(rho3, h3, s._pin_dof)
+    return s, H, O, rho
+
+
+def _close(got, ref, tol):
+    got = got.detach().cpu().double().numpy().reshape(ref.shape)
+    scale = max(np.abs(ref).max(), 1e-300)
+    err = np.abs(got - ref).max() / scale
+    assert err <= tol, err
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+@pytest.mark.parametrize("N", [(32, 32, 32), (64, 64), (24, 20, 16)])
+def test_hierarchy_matches_oracle(N):
+    _announce(f"mg hierarchy N={N}")
+    s, H, O, _ = _setup(N)
+    assert H.shapes == O.shapes and H.pins == O.pins
+    for level in range(H.levels):
+        co = H.coeffs(level)
+        for a in range(3):
+            _close(co[a], O.T[level][a], 2e-6)
+        _close(co[3], O.dinv[level], 2e-6)
+
+
+@pytest.mark.parametrize("N", [(32, 32, 32), (64, 64)])
+def test_operator_and_transfers_match_oracle(N):
+    _announce(f"mg apply/transfers N={N}")
+    s, H, O, _ = _setup(N)
+    rng = np.random.default_rng(3)
+    for level in range(H.levels):
+        x = rng.standard_normal(H.shapes[level])
+        _close(H.apply(level, _t(x)), O.apply(level, x.astype(np.float32).astype(float)), 2e-6)
+    last = H.levels - 1
+    r = rng.standard_normal(H.shapes[0]).astype(np.float32).astype(float)
+    _close(H.restrict(_t(r), last), O.restrict_chain(r, last), 2e-6)
+    xc = rng.standard_normal(H.shapes[last]).astype(np.float32).astype(float)
+    _close(H.prolong(last, _t(xc)), O.prolong_chain(last, xc), 2e-6)
+
+
+@pytest.mark.parametrize("N,gamma,nu", [((32, 32, 32), 1, 2), ((32, 32, 32), 2, 2),
+                                        ((64, 64), 2, 1), ((24, 20, 16), 1, 1)])
+def test_cycle_matches_oracle(N, gamma, nu):
+    _announce(f"mg cycle N={N} gamma={gamma} nu={nu}")
+    s, H, O, _ = _setup(N)
+    r = np.random.default_rng(5).standard_normal(H.shapes[0]).astype(np.float32).astype(float)
+    r.ravel()[H.pins[0]] = 0.0
+    z = H.vcycle(_t(r), nu, nu, 20, -1, gamma)
+    _close(z, O.cycle(r, nu=nu, coarse_sweeps=20, gamma=gamma), 1e-4)
+    assert float(z.view(-1)[H.pins[0]]) == 0.0
+
+
+def test_down_up_split_equals_truncated_cycle():
+    _announce("mg down/up split")
+    s, H, O, _ = _setup((32, 32, 32))
+    r = _t(np.random.default_rng(6).standard_normal(H.shapes[0]))
+    level = 1
+    bc = H.down(r, level, 2)
+    z = H.up(level, torch.zeros_like(bc), 2)
+    ref = H.vcycle(r, 2, 2, 0, level, 1)
+    assert torch.allclose(z, ref, rtol=0, atol=1e-6 * float(ref.abs().max()))
+    assert bc.shape == tuple(H.shapes[level])
+
+
+@pytest.mark.parametrize("N,gamma", [((64, 64, 64), 1), ((64, 64, 64), 2), ((128, 128), 2)])
+def test_multigrid_fcg_converges(N, gamma):
+    _announce(f"mg-fcg N={N} gamma={gamma}")
+    s, H, O, rho = _setup(N)
+    rhs = np.random.default_rng(8).standard_normal(s.shape)
+    rhs_t = torch.from_numpy(rhs).cuda()
+    p, info = L.fcg_solve(s, rhs_t, L.MGPreconditioner(H, nu=2, gamma=gamma), tol=1e-8,
+                          maxiter=100, m=1, fallback=False)
+    assert info["fcg_exit"] == "converged"
+    assert info["fcg_iterations"] <= 30
+    o = oppe.OraclePPE(N)
+    o.prepare_operator(rho)
+    assert o.relres(rhs, p.cpu().numpy()) <= 1e-8 * 1.01
+
+
+def test_multigrid_fcg_random_problems():
+    """Bubble sets of the training distribution (8-128 spheres, density ratio
+    10-1000) all converge with the W-cycle."""
+    for seed in (11, 12, 13):
+        _announce(f"mg-fcg random problem seed={seed}")
+        g, rho, rhs = L.make_problem(64, seed)
+        cfg = SimulationConfig(solver=SolverConfig(pseudo_tol=1e-8, pseudo_maxiter=10 ** 6))
+        cfg.solver.ppe_iteration_method = "cg"
+        s = PPESolverFVMMatrixFree(None, cfg, g)
+        s.prepare_operator(rho)
+        H = L.MGHierarchy(s)
+        p, info = L.fcg_solve(s, rhs, L.MGPreconditioner(H, gamma=2), tol=1e-8, maxiter=60,
+                              m=1, fallback=False)
+        assert info["fcg_exit"] == "converged" and info["true_relres"] <= 1e-8
+
+
+def test_learned_deflation_is_linear_and_converges():
+    _announce("learned deflation (untrained) linearity + solve")
+    s, H, O, rho = _setup((64, 64, 64))
+    net = L.UNet(seed=3).cuda()
+    level = L.level_of(H, 33)
+    M = L.LearnedDeflationPreconditioner(H, net, level)
+    rng = np.random.default_rng(9)
+    a, b = (torch.from_numpy(rng.standard_normal(s.shape)).cuda() for _ in range(2))
+    lhs = M(a + 2.0 * b).double()
+    rhs_ = (M(a).double() + 2.0 * M(b).double())
+    assert float((lhs - rhs_).abs().max()) <= 1e-4 * float(rhs_.abs().max())
+    rhs = torch.from_numpy(rng.standard_normal(s.shape)).cuda()
+    p, info = L.fcg_solve(s, rhs, M, tol=1e-8, maxiter=100, m=1, fallback=False)
+    assert info["fcg_exit"] == "converged" and info["true_relres"] <= 1e-8
+
+
+def test_galerkin_operator_is_symmetric():
+    _announce("galerkin symmetry")
+    s, H, O, rho = _setup((64, 64, 64))
+    level = L.level_of(H, 33)
+    rng = np.random.default_rng(10)
+    w = torch.from_numpy(rng.standard_normal((2,) + tuple(H.shapes[level])).astype(np.float32)
+                         ).cuda()
+    w.view(2, -1)[:, H.pins[level]] = 0.0
+    Aw = L._galerkin(H, level, w)
+    x01 = float((w[0] * Aw[1]).sum())
+    x10 = float((w[1] * Aw[0]).sum())
+    assert abs(x01 - x10) <= 1e-4 * abs(x01)
+    assert float((w[0] * Aw[0]).sum()) > 0.0
+
+
+def test_training_lowers_the_deflation_loss():
+    """train_deflation on solver-generated residuals (N=64, 24 problems)
+    lowers the loss, and the trained space captures more error energy than
+    the untrained one on a held-out problem."""
+    _announce("learned deflation training (24 problems)")
+    torch.manual_seed(0)
+    net = L.UNet(seed=1).cuda()
+    fresh = L.UNet(seed=1).cuda()
+    tr = L.train_deflation(net, N=64, problems=24, level_nodes=33, lr=2e-3, seed=300)
+    h = tr["loss_history"]
+    assert np.mean(h[-6:]) < np.mean(h[:6])
+    g, rho, rhs = L.make_problem(64, 777)
+    cfg = SimulationConfig(solver=SolverConfig(pseudo_tol=1e-10, pseudo_maxiter=10 ** 6))
+    cfg.solver.ppe_iteration_method = "cg"
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    s.prepare_operator(rho)
+    H = L.MGHierarchy(s)
+    level = L.level_of(H, 33)
+    _, info = L.fcg_solve(s, rhs, L.MGPreconditioner(H), tol=1e-10, maxiter=6, m=1,
+                          fallback=False, record=(2, 4))
+    samples = []
+    with torch.no_grad():
+        for r in info["records"]:
+            r32 = r.float().view(H.shapes[0])
+            t = r32 - H.apply(0, H.vcycle(r32))
+            t = t / t.norm()
+            samples.append((t, H.vcycle(t)))
+        lt = float(L.deflation_loss(H, net, level, samples))
+        lf = float(L.deflation_loss(H, fresh, level, samples))
+    assert lt < lf

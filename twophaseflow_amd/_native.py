@@ -76,6 +76,16 @@ SIGNATURES = {
     "tpf_ppe_jacobi_inv": (C.c_int, [_vp, _vp]),
     "tpf_ppe_system_rhs": (C.c_int, [_vp, _vp, _vp]),
     "tpf_ppe_system_apply_f32": (C.c_int, [_vp, _vp, _vp]),
+    "tpf_mg_setup": (C.c_int, [_vp, C.c_int, _ip]),
+    "tpf_mg_level_info": (C.c_int, [_vp, C.c_int, _i64p, _i64p]),
+    "tpf_mg_vcycle": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "tpf_mg_down": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp]),
+    "tpf_mg_up": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp]),
+    "tpf_mg_prolong": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "tpf_mg_restrict": (C.c_int, [_vp, _vp, C.c_int, _vp]),
+    "tpf_mg_apply": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "tpf_mg_smooth": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
+    "tpf_mg_coeffs": (C.c_int, [_vp, C.c_int, _vp]),
 }
 
 _lib = None

@@ -507,6 +507,7 @@ extern "C" tpf_status tpf_ppe_create(tpf_ppe** out, int ndim, const int64_t* n_n
 extern "C" tpf_status tpf_ppe_destroy(tpf_ppe* s) {
   if (!s) return TPF_OK;
   tpf_gmres_release(s);
+  tpf_mg_release(s);
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   if (s->graph32_exec) (void)hipGraphExecDestroy(s->graph32_exec);
   float* fb[] = {s->r32, s->p32, s->q32, s->x32, s->dinv32, s->rho32};

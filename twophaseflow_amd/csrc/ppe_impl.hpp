@@ -53,8 +53,10 @@ struct tpf_ppe {
   int graph32_chunk = 0;
   const double* graph32_rho = nullptr;
   void* gmres_work = nullptr;           // GMRES basis and line-solve scratch
+  void* mg = nullptr;                   // multigrid hierarchy (FP32)
 };
 
 void tpf_gmres_release(tpf_ppe* s);
+void tpf_mg_release(tpf_ppe* s);
 tpf_status tpf_ppe_solve_mixed(tpf_ppe* s, const double* rhs, double* x,
                                const tpf_solve_opts* opts, tpf_solve_report* rep);

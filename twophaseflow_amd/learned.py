@@ -2,6 +2,10 @@
 
 No counterpart in the reference; this is the learned extension.  Structure:
 
+* ``MGHierarchy`` / ``MGPreconditioner``: the FP32 geometric multigrid of
+  libtpf (``tpf_mg_*``) on the reference CG system ``(-W L) p = -W rhs``:
+  homogenised face conductances per level, red-black Gauss-Seidel, full
+  weighting and trilinear interpolation, V- or W-cycles.
 * ``UNet``: 4 levels (16, 32, 64, 64 channels), 3x3x3 convolutions in
   hand-written BF16 MFMA kernels with FP32 accumulation (libtpf
   ``tpf_conv3d_fwd`` / ``tpf_conv3d_wgrad``; the input gradient is the forward
@@ -9,19 +13,19 @@ No counterpart in the reference; this is the learned extension.  Structure:
   (1/4, 1/2, 1/4 per axis) between levels and linear interpolation back;
   skip connections by channel concatenation.  PyTorch provides tensors,
   autograd bookkeeping and Adam; no PyTorch convolution is used.
-* ``TwoLevelPreconditioner``: damped-Jacobi pre-smoothing on the fine grid,
-  restriction of the residual, the U-Net as the coarse-grid solver (input:
-  normalised coarse residual and log10 density), interpolation with an
-  energy-optimal step length, Jacobi post-smoothing.  Fine-grid matvecs are
-  the FP32 CG-system operator of the PPE handle (``tpf_ppe_system_apply_f32``).
-* ``fcg_solve``: flexible CG with one-term orthogonalisation (Notay's
-  FCG(1)) on the reference CG system ``(-W L) p = -W rhs`` in FP64; plain
-  Jacobi-PCG (``tpf_ppe_solve``, warm-started) takes over on stagnation, and
-  the FP64 true residual judges the result.
-* ``train``: the network is trained on residuals produced by the plain PCG on
-  random bubble fields (8-128 spheres, density ratio log-uniform in
-  [10, 1000]) by minimising the A-norm energy z.A z - 2 z.r of the
-  preconditioner output, normalised by r.D^-1 r.
+* ``LearnedDeflationPreconditioner``: once per solve the U-Net maps the
+  conductance field of one coarse level to k coarse vectors W; their
+  prolongations span a deflation space applied after each multigrid cycle
+  in the A-DEF2 form z = V(r) + Z E^-1 Z^T (r - A V(r)), E = Z^T A Z
+  (Galerkin, FP64 Cholesky).  The preconditioner is linear in r during a
+  solve.
+* ``fcg_solve``: flexible CG with m-term orthogonalisation (Notay's FCG(m))
+  in FP64; plain Jacobi-PCG (``tpf_ppe_solve``, warm-started) takes over on
+  stagnation, and the FP64 true residual judges the result.
+* ``train_deflation``: the network is trained on the residuals that the
+  multigrid-FCG solver produces on random bubble fields (8-128 spheres,
+  density ratio log-uniform in [10, 1000]) by maximising the share of the
+  post-cycle error energy that the deflation space captures.
 """
 
 from __future__ import annotations
@@ -145,13 +149,13 @@ class UNet(torch.nn.Module):
 
     CIN = 16
 
-    def __init__(self, ch=(16, 32, 64, 64), seed=0):
+    def __init__(self, ch=(16, 32, 64, 64), seed=0, k_out=16):
         super().__init__()
         g = torch.Generator().manual_seed(seed)
         c0, c1, c2, c3 = ch
         spec = {"e0a": (self.CIN, c0), "e0b": (c0, c0), "e1a": (c0, c1), "e1b": (c1, c1),
                 "e2a": (c1, c2), "e2b": (c2, c2), "ba": (c2, c3), "bb": (c3, c3),
-                "d2": (c3 + c2, c2), "d1": (c2 + c1, c1), "d0": (c1 + c0, c0), "out": (c0, 16)}
+                "d2": (c3 + c2, c2), "d1": (c2 + c1, c1), "d0": (c1 + c0, c0), "out": (c0, k_out)}
         self.spec = spec
         for name, (ci, co) in spec.items():
             std = math.sqrt(2.0 / (27 * ci)) * (0.1 if name == "out" else 1.0)
@@ -162,7 +166,7 @@ class UNet(torch.nn.Module):
     def _c(self, name, x, relu=True, out_f32=False):
         return conv(x, getattr(self, f"{name}_w"), getattr(self, f"{name}_b"), relu, out_f32)
 
-    def forward(self, x):
+    def forward(self, x, all_channels=False):
         e0 = self._c("e0b", self._c("e0a", x))
         e1 = self._c("e1b", self._c("e1a", _down(e0)))
         e2 = self._c("e2b", self._c("e2a", _down(e1)))
@@ -170,7 +174,8 @@ class UNet(torch.nn.Module):
         d2 = self._c("d2", torch.cat([_up(b), e2], dim=3))
         d1 = self._c("d1", torch.cat([_up(d2), e1], dim=3))
         d0 = self._c("d0", torch.cat([_up(d1), e0], dim=3))
-        return self._c("out", d0, relu=False, out_f32=True)[..., 0]
+        y = self._c("out", d0, relu=False, out_f32=True)
+        return y if all_channels else y[..., 0]
 
 
 def _down(x):
@@ -181,77 +186,198 @@ def _up(x):
     return prolong3(x.float()).to(BF16).contiguous()
 
 
-# ------------------------------------------------- fine-grid operator ---
-class _A32(torch.autograd.Function):
-    """FP32 CG-system matvec of a PPE handle; A is symmetric on vectors whose
-    pin entry is zero, so the backward is A itself."""
+# ------------------------------------------------- multigrid backbone ---
+class MGHierarchy:
+    """FP32 geometric multigrid hierarchy of a PPE handle (libtpf ``tpf_mg_*``):
+    homogenised face conductances per level, red-black Gauss-Seidel,
+    full-weighting restriction, trilinear prolongation."""
+
+    def __init__(self, ppe_solver, max_levels=12):
+        self.s = ppe_solver
+        self.h = ppe_solver._handle
+        lib = _lib()
+        lv = C.c_int()
+        nat.check(lib.tpf_mg_setup(self.h, int(max_levels), C.byref(lv)))
+        self.levels = lv.value
+        self.shapes, self.pins = [], []
+        for level in range(self.levels):
+            n3 = (C.c_int64 * 3)()
+            pin = C.c_int64()
+            nat.check(lib.tpf_mg_level_info(self.h, level, n3, C.byref(pin)))
+            self.shapes.append(tuple(int(v) for v in n3))
+            self.pins.append(int(pin.value))
+
+    def _new(self, level):
+        return torch.empty(self.shapes[level], dtype=torch.float32, device="cuda")
+
+    def vcycle(self, r, nu_pre=2, nu_post=2, coarse_sweeps=20, stop_level=-1, gamma=1):
+        r = r.float().contiguous().view(self.shapes[0])
+        z = self._new(0)
+        nat.check(_lib().tpf_mg_vcycle(self.h, _p(r), _p(z), nu_pre, nu_post, coarse_sweeps,
+                                       stop_level, gamma))
+        return z
+
+    def down(self, r, level, nu_pre=2):
+        r = r.float().contiguous().view(self.shapes[0])
+        bc = self._new(level)
+        nat.check(_lib().tpf_mg_down(self.h, _p(r), level, nu_pre, _p(bc)))
+        return bc
+
+    def up(self, level, xc, nu_post=2):
+        z = self._new(0)
+        nat.check(_lib().tpf_mg_up(self.h, level, _p(xc.contiguous()), nu_post, _p(z)))
+        return z
+
+    def prolong(self, level, xc):
+        z = self._new(0)
+        nat.check(_lib().tpf_mg_prolong(self.h, level, _p(xc.contiguous()), _p(z)))
+        return z
+
+    def restrict(self, r, level):
+        bc = self._new(level)
+        nat.check(_lib().tpf_mg_restrict(self.h, _p(r.contiguous()), level, _p(bc)))
+        return bc
+
+    def apply(self, level, x):
+        y = self._new(level)
+        nat.check(_lib().tpf_mg_apply(self.h, level, _p(x.contiguous()), _p(y)))
+        return y
+
+    def smooth(self, level, b, x, sweeps):
+        nat.check(_lib().tpf_mg_smooth(self.h, level, _p(b.contiguous()), _p(x), sweeps))
+        return x
+
+    def coeffs(self, level):
+        out = torch.empty((4,) + self.shapes[level], dtype=torch.float32, device="cuda")
+        nat.check(_lib().tpf_mg_coeffs(self.h, level, _p(out)))
+        return out
+
+
+class MGPreconditioner:
+    """z = M(r): one classical multigrid cycle (no network); gamma 1 = V, 2 = W."""
+
+    def __init__(self, hier: MGHierarchy, nu=2, coarse_sweeps=20, stop_level=-1, gamma=1):
+        self.hier, self.nu, self.cs, self.stop, self.gamma = (hier, nu, coarse_sweeps,
+                                                              stop_level, gamma)
+
+    def __call__(self, r):
+        return self.hier.vcycle(r, self.nu, self.nu, self.cs, self.stop,
+                                self.gamma).view(r.shape)
+
+
+def level_features(hier: MGHierarchy, level: int) -> torch.Tensor:
+    """Per-node network inputs of a level (channels 1..4): log10 of the three
+    face conductances relative to the unit-density value, and of 1/diag,
+    scaled to O(1)."""
+    co = hier.coeffs(level)
+    n = hier.shapes[level]
+    h = 1.0 / (max(n) - 1)
+    feats = torch.empty((4,) + n, dtype=torch.float32, device="cuda")
+    for a in range(3):
+        feats[a] = torch.log10(co[a] / h + 1e-4) / 3.0
+    feats[3] = torch.log10(co[3] * 6.0 * h + 1e-4) / 3.0
+    return feats
+
+
+class _Galerkin(torch.autograd.Function):
+    """A_c w = R A P w for a batch of coarse vectors w [k, *shape] (the
+    Galerkin coarse operator of the fine FP32 operator; symmetric, so the
+    backward applies it again)."""
 
     @staticmethod
-    def forward(ctx, p, handle):
-        out = torch.empty_like(p)
-        nat.check(_lib().tpf_ppe_system_apply_f32(handle, _p(p.contiguous()), _p(out)))
-        ctx.handle = handle
-        return out
+    def forward(ctx, w, hier, level):
+        ctx.hier, ctx.level = hier, level
+        return _galerkin(hier, level, w)
 
     @staticmethod
     def backward(ctx, g):
-        out = torch.empty_like(g)
-        nat.check(_lib().tpf_ppe_system_apply_f32(ctx.handle, _p(g.contiguous()), _p(out)))
-        return out, None
+        return _galerkin(ctx.hier, ctx.level, g.contiguous()), None, None
 
 
-class TwoLevelPreconditioner:
-    """z = M(r): Jacobi, U-Net coarse correction, Jacobi (all FP32 inside)."""
+def _galerkin(hier, level, w):
+    out = torch.empty_like(w)
+    pin0 = hier.pins[0]
+    for j in range(w.shape[0]):
+        f = hier.apply(0, hier.prolong(level, w[j]))
+        f.view(-1)[pin0] = 0.0
+        out[j] = hier.restrict(f, level)
+    return out
 
-    def __init__(self, ppe_solver, net: UNet, omega: float = 0.6):
-        self.s = ppe_solver
-        self.net = net
-        self.omega = omega
-        lib = _lib()
-        h = ppe_solver._handle
-        nat.check(lib.tpf_ppe_prepare_f32(h))
-        dinv = torch.empty(ppe_solver.shape, dtype=torch.float64, device="cuda")
-        nat.check(lib.tpf_ppe_jacobi_inv(h, _p(dinv)))
-        self.dinv = dinv.float()
-        self.pin = ppe_solver._pin_dof
-        self.dinv.view(-1)[self.pin] = 0.0
-        rho = ppe_solver._rho.float()
-        self.logrho_c = (restrict3(torch.log10(rho)) / 3.0).contiguous()
 
-    def A(self, p):
-        return _A32.apply(p, self.s._handle)
+class LearnedDeflationPreconditioner:
+    """Multigrid V-cycle with a learned coarse space (A-DEF2 two-level form).
 
-    def __call__(self, r, train=False):
-        r = r.float() if r.dtype != torch.float32 else r
-        om = self.omega
-        z1 = om * self.dinv * r
-        t = r - self.A(z1)
-        rc = restrict3(t)
-        scale = rc.square().mean().sqrt() + 1e-30
-        inp = torch.zeros(rc.shape + (UNet.CIN,), dtype=torch.float32, device=r.device)
-        inp[..., 0] = rc / scale
-        inp[..., 1] = self.logrho_c
-        ec = self.net(inp.to(BF16).contiguous()) * scale
-        pe = prolong3(ec)
-        pe = pe * self._pin_mask()
-        ape = self.A(pe)
-        gamma = (pe * t).sum() / ((pe * ape).sum() + 1e-300)
-        z2 = z1 + gamma * pe
-        t2 = t - gamma * ape
-        z = z2 + om * self.dinv * t2
-        return z * self._pin_mask()
+    Once per solve, the U-Net maps the conductance field of one coarse level
+    (no residual) to k = 16 coarse vectors W; Z = P W spans the deflation
+    space, E = Z^T A Z = W^T (R A P) W is factored in FP64.  Each application
+    is linear in r:
+        z1 = V(r);  t = r - A z1;  z = z1 + Z E^-1 Z^T t
+    (Z^T t = W^T R t and Z y = P (W y) are evaluated on the coarse level).
+    With net=None it is the V-cycle alone."""
 
-    def _pin_mask(self):
-        if not hasattr(self, "_mask"):
-            m = torch.ones(self.s.shape, dtype=torch.float32, device="cuda")
-            m.view(-1)[self.pin] = 0.0
-            self._mask = m
-        return self._mask
+    def __init__(self, hier: MGHierarchy, net, level: int, nu=2, coarse_sweeps=20, ridge=1e-6):
+        self.hier, self.net, self.level = hier, net, level
+        self.nu, self.cs = nu, coarse_sweeps
+        if net is not None:
+            with torch.no_grad():
+                W = deflation_vectors(hier, net, level)
+                E = (W.reshape(W.shape[0], -1) @ _galerkin(hier, level, W)
+                     .reshape(W.shape[0], -1).T).double()
+            E = 0.5 * (E + E.T)
+            E = E + ridge * torch.trace(E) / E.shape[0] * torch.eye(E.shape[0], device=E.device,
+                                                                 dtype=E.dtype)
+            self.W = W.reshape(W.shape[0], -1)
+            self.L = torch.linalg.cholesky(E)
+
+    def __call__(self, r):
+        shape = r.shape
+        r32 = r.float().contiguous().view(self.hier.shapes[0])
+        z1 = self.hier.vcycle(r32, self.nu, self.nu, self.cs)
+        if self.net is None:
+            return z1.view(shape)
+        t = r32 - self.hier.apply(0, z1)
+        c = (self.W @ self.hier.restrict(t, self.level).reshape(-1)).double()
+        y = torch.cholesky_solve(c[:, None], self.L)[:, 0].float()
+        zc = (y @ self.W).view(self.hier.shapes[self.level])
+        return (z1 + self.hier.prolong(self.level, zc)).view(shape)
+
+
+def deflation_vectors(hier: MGHierarchy, net, level: int) -> torch.Tensor:
+    """The network's k coarse vectors [k, *shape] (coarse pin zeroed,
+    unit-norm columns); differentiable in the network parameters."""
+    feats = level_features(hier, level)
+    inp = torch.zeros(hier.shapes[level] + (UNet.CIN,), dtype=torch.float32, device="cuda")
+    inp[..., 0] = 1.0
+    inp[..., 1:5] = feats.permute(1, 2, 3, 0)
+    y = net(inp.to(BF16).contiguous(), all_channels=True)        # [*shape, k]
+    W = y.permute(3, 0, 1, 2).contiguous()
+    mask = torch.ones(hier.shapes[level], dtype=torch.float32, device="cuda")
+    mask.view(-1)[hier.pins[level]] = 0.0
+    W = W * mask
+    return W / (W.reshape(W.shape[0], -1).norm(dim=1).view(-1, 1, 1, 1) + 1e-30)
+
+
+def deflation_loss(hier, net, level, samples, ridge=1e-6):
+    """Minus the mean fraction of each sample's error energy captured by the
+    deflation space: -t^T Q t / t^T V(t) with Q = Z E^-1 Z^T, over
+    post-V-cycle residual samples t (pairs (t, V(t)))."""
+    W = deflation_vectors(hier, net, level)
+    k = W.shape[0]
+    AW = _Galerkin.apply(W, hier, level)
+    E = (W.reshape(k, -1) @ AW.reshape(k, -1).T).double()
+    E = 0.5 * (E + E.T)
+    E = E + ridge * torch.trace(E).detach() / k * torch.eye(k, device=E.device, dtype=E.dtype)
+    loss = 0.0
+    for t, vt in samples:
+        c = (W.reshape(k, -1) @ hier.restrict(t, level).reshape(-1)).double()
+        cap = (c * torch.linalg.solve(E, c)).sum()
+        loss = loss - cap / float((t * vt).sum())
+    return loss / len(samples)
 
 
 # ----------------------------------------------------------- FCG solve ---
 def fcg_solve(ppe_solver, rhs, precond, tol=1e-8, maxiter=2000, m=8, stall_window=30,
-              stall_factor=0.7, restarts=3, fallback=True):
+              stall_factor=0.7, restarts=3, fallback=True, record=()):
     """Flexible CG on the CG system in FP64 with a nonlinear preconditioner.
 
     FCG(m) (Notay 2000): each new direction is A-orthogonalised against the
@@ -259,7 +385,8 @@ def fcg_solve(ppe_solver, rhs, precond, tol=1e-8, maxiter=2000, m=8, stall_windo
     it fails to improve by `stall_factor` over `stall_window` iterations the
     recursion restarts from the FP64 true residual (at most `restarts` times)
     and then plain Jacobi-PCG (warm-started from x) takes over.  The FP64
-    true residual judges the result.  Returns (p, info)."""
+    true residual judges the result.  Residuals of the iterations listed in
+    `record` are kept in info["records"].  Returns (p, info)."""
     lib = _lib()
     h = ppe_solver._handle
 
@@ -276,10 +403,13 @@ def fcg_solve(ppe_solver, rhs, precond, tol=1e-8, maxiter=2000, m=8, stall_windo
     hist, best = [], []
     it, nrestart = 0, 0
     P, Q, PQ = [], [], []
+    records = []
     reason = "maxiter"
     t0 = time.perf_counter()
     with torch.no_grad():
         while it < maxiter:
+            if it in record:
+                records.append(r.clone())
             rel = float(r.norm()) / bn
             hist.append(rel)
             best.append(min(rel, best[-1]) if best else rel)
@@ -316,6 +446,7 @@ def fcg_solve(ppe_solver, rhs, precond, tol=1e-8, maxiter=2000, m=8, stall_windo
         info["fallback_iterations"] = ppe_solver.last_diagnostics["iterations"]
     info["true_relres"] = ppe_solver.true_relres(rhs, x)
     info["history"] = hist
+    info["records"] = records
     return x, info
 
 
@@ -343,63 +474,52 @@ def make_problem(N, seed, nb_range=(8, 128), ratio_range=(10.0, 1000.0)):
     return g, rho, rhs
 
 
-def pcg_residual_samples(ppe_solver, rhs, rho, iters=(0, 10, 40, 120)):
-    """System residuals r_k = b - A x_k of the plain PCG after k iterations."""
-    lib = _lib()
-    h = ppe_solver._handle
-    ppe_solver.prepare_operator(rho)
-    b = torch.empty_like(rhs)
-    nat.check(lib.tpf_ppe_system_rhs(h, _p(rhs), _p(b)))
-    out = []
-    for k in iters:
-        x = torch.zeros_like(rhs)
-        if k > 0:
-            opts = nat.SolveOpts(tol=1e-30, maxiter=k, precond=1, check_every=min(k, 32),
-                                 use_x0=0, mixed=0, inner_tol=0.0)
-            rep = nat.SolveReport()
-            lib.tpf_ppe_solve(h, _p(rhs), _p(x), C.byref(opts), C.byref(rep))  # stops at maxiter
-        ax = torch.empty_like(x)
-        nat.check(lib.tpf_ppe_system_apply(h, _p(x), _p(ax)))
-        out.append((b - ax).float())
-    return out
+def level_of(hier: MGHierarchy, nodes: int) -> int:
+    """Index of the level whose grid has `nodes` nodes per axis."""
+    for level, shp in enumerate(hier.shapes):
+        if max(shp) == nodes:
+            return level
+    raise ValueError(f"no level with {nodes} nodes per axis in {hier.shapes}")
 
 
-def energy_loss(M, r):
-    z = M(r)
-    az = M.A(z)
-    num = (z * az).sum() - 2.0 * (z * r).sum()
-    den = (r * M.dinv * r).sum() + 1e-300
-    return num / den
-
-
-def train(net, N=64, problems=64, epochs=2, lr=2e-3, seed=100, log=None):
-    """Trains `net` in place on PCG residuals of random bubble problems."""
+def train_deflation(net, N=128, problems=200, record=(0, 2, 5, 9), level_nodes=33, lr=1e-3,
+                    seed=1000, log=None, nu=2):
+    """Trains `net` in place to produce the coarse space of
+    LearnedDeflationPreconditioner from residuals that the multigrid-FCG
+    solver itself produces: for each random bubble problem (8-128 spheres,
+    density ratio log-uniform in [10, 1000]) the FCG residuals r_k of the
+    listed iterations give samples t = r - A V(r), and deflation_loss is
+    minimised with Adam (one step per problem)."""
     from .config import SimulationConfig, SolverConfig
     from .ppe import PPESolverFVMMatrixFree
     opt = torch.optim.Adam(net.parameters(), lr=lr)
     hist = []
-    data = []
     t0 = time.perf_counter()
+    cfg = SimulationConfig(solver=SolverConfig(pseudo_tol=1e-10, pseudo_maxiter=10 ** 6))
+    cfg.solver.ppe_iteration_method = "cg"
+    s = None
     for i in range(problems):
         g, rho, rhs = make_problem(N, seed + i)
-        cfg = SimulationConfig(solver=SolverConfig(pseudo_tol=1e-8, pseudo_maxiter=10 ** 6))
-        cfg.solver.ppe_iteration_method = "cg"
-        s = PPESolverFVMMatrixFree(None, cfg, g)
-        samples = pcg_residual_samples(s, rhs, rho)
-        data.append((s, rho, samples))
-    t_data = time.perf_counter() - t0
-    for ep in range(epochs):
-        for (s, rho, samples) in data:
-            M = TwoLevelPreconditioner(s, net)
-            for r in samples:
-                r = r / (r.norm() + 1e-30)
-                loss = energy_loss(M, r)
-                opt.zero_grad(set_to_none=True)
-                loss.backward()
-                torch.nn.utils.clip_grad_norm_(net.parameters(), 1.0)
-                opt.step()
-                hist.append(float(loss.detach()))
-        if log:
-            log(f"epoch {ep}: mean energy {np.mean(hist[-len(data) * 4:]):.4f}")
-    return {"loss_history": hist, "data_seconds": t_data,
-            "train_seconds": time.perf_counter() - t0 - t_data}
+        if s is None:
+            s = PPESolverFVMMatrixFree(None, cfg, g)
+        s.prepare_operator(rho)
+        H = MGHierarchy(s)
+        level = level_of(H, level_nodes)
+        _, info = fcg_solve(s, rhs, MGPreconditioner(H, nu=nu), tol=1e-10,
+                            maxiter=max(record) + 1, m=1, fallback=False, record=record)
+        samples = []
+        with torch.no_grad():
+            for r in info["records"]:
+                r32 = r.float().view(H.shapes[0])
+                t = r32 - H.apply(0, H.vcycle(r32, nu, nu))
+                t = t / (t.norm() + 1e-30)
+                samples.append((t, H.vcycle(t, nu, nu)))
+        loss = deflation_loss(H, net, level, samples)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        torch.nn.utils.clip_grad_norm_(net.parameters(), 1.0)
+        opt.step()
+        hist.append(float(loss.detach()))
+        if log and (i + 1) % max(1, problems // 10) == 0:
+            log(f"problem {i + 1}/{problems}: mean loss (last 20) {np.mean(hist[-20:]):.4f}")
+    return {"loss_history": hist, "seconds": time.perf_counter() - t0}
