@@ -154,6 +154,37 @@ def ppe_solve_timed(g, rho, rhs, precision, tol=1e-8, maxiter=200000, inner_tol=
             "true_relres": tr, "converged": bool(tr <= tol)}
 
 
+def mg_solve_timed(g, rho, rhs, gamma=2, net=None, tol=1e-8, reps=2):
+    """Multigrid-FCG (net=None) or learned-deflation FCG on the CG system;
+    time includes the hierarchy setup (and the network + Galerkin setup)."""
+    import torch
+    from twophaseflow_amd import PPESolverFVMMatrixFree, SimulationConfig, SolverConfig
+    from twophaseflow_amd import learned as L
+    cfg = SimulationConfig(solver=SolverConfig(pseudo_tol=tol, pseudo_maxiter=10 ** 6))
+    cfg.solver.ppe_iteration_method = "cg"
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    out = None
+    for _ in range(reps):
+        s.invalidate_cache()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.prepare_operator(rho)
+        H = L.MGHierarchy(s)
+        if net is None:
+            M = L.MGPreconditioner(H, nu=2, gamma=gamma)
+        else:
+            M = L.LearnedDeflationPreconditioner(H, net, L.level_of(H, 33), nu=2, gamma=gamma)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        p, info = L.fcg_solve(s, rhs, M, tol=tol, maxiter=500, m=1)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        out = {"seconds": t, "setup_seconds": t_setup, "iterations": info["fcg_iterations"],
+               "fallback_iterations": info["fallback_iterations"], "exit": info["fcg_exit"],
+               "true_relres": info["true_relres"], "converged": bool(info["true_relres"] <= tol)}
+    return out
+
+
 def profile_mode(args):
     import torch
     from twophaseflow_amd import CCDSolver, Grid, GridConfig
@@ -186,6 +217,8 @@ def main():
     ap.add_argument("--no-ppe", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ppe-n", type=int, default=256)
+    ap.add_argument("--train-problems", type=int, default=100,
+                    help="random 129^3 bubble problems used to train the learned coarse space")
     ap.add_argument("--profile", choices=["ccd", "ppe"], default=None,
                     help="run only one workload (for rocprofv3): 'ccd' = K all-axes CCD steps "
                          "on 257^3 plus one 1 GiB device copy (counter scale reference); "
@@ -278,11 +311,46 @@ def main():
             log(f"PPE solve {key}")
             res[key] = ppe_solve_timed(gp, rho, rhs, prec, inner_tol=itol)
             log(f"  {res[key]}")
-        result["ppe"] = {"grid": [args.ppe_n + 1] * 3, "density_ratio": 1000, "bubbles": 64,
-                         "tol": 1e-8, "plain_pcg_fp64": res["fp64"], "pcg_mixed": res["mixed"],
-                         "pcg_mixed_inner_1e-3": res["mixed_1e-3"],
-                         "pcg_mixed_inner_1e-5": res["mixed_1e-5"]}
+        for key, gam in (("mg_fcg_V22", 1), ("mg_fcg_W22", 2)):
+            log(f"PPE solve {key}")
+            res[key] = mg_solve_timed(gp, rho, rhs, gamma=gam)
+            log(f"  {res[key]}")
+        import numpy as np
+        from twophaseflow_amd import learned as L
+        log(f"training the learned coarse space on {args.train_problems} problems (129^3)")
+        torch.manual_seed(0)
+        net = L.UNet(seed=0).cuda()
+        tr = L.train_deflation(net, N=128, problems=args.train_problems, lr=1e-3, seed=1000,
+                               gamma=2)
+        log(f"  trained in {tr['seconds']:.1f} s")
+        res["learned"] = mg_solve_timed(gp, rho, rhs, gamma=2, net=net)
+        log(f"  learned: {res['learned']}")
+        res["untrained"] = mg_solve_timed(gp, rho, rhs, gamma=2, net=L.UNet(seed=5).cuda())
+        lh = tr["loss_history"]
+        pcg_s = res["fp64"]["seconds"]
+        result["ppe"] = {
+            "grid": [args.ppe_n + 1] * 3, "density_ratio": 1000, "bubbles": 64, "tol": 1e-8,
+            "plain_pcg_fp64": res["fp64"], "pcg_mixed": res["mixed"],
+            "pcg_mixed_inner_1e-3": res["mixed_1e-3"], "pcg_mixed_inner_1e-5": res["mixed_1e-5"],
+            "mg_fcg_V22": res["mg_fcg_V22"], "mg_fcg_W22": res["mg_fcg_W22"],
+            "learned_fcg": dict(res["learned"], backbone="W(2,2) multigrid",
+                                coarse_space="16 U-Net vectors on the 33^3 level",
+                                train_problems=args.train_problems,
+                                train_seconds=round(tr["seconds"], 1),
+                                train_loss_first10=float(np.mean(lh[:10])),
+                                train_loss_last10=float(np.mean(lh[-10:]))),
+            "untrained_net_fcg": res["untrained"],
+            "speedup_learned_vs_plain_pcg": round(pcg_s / res["learned"]["seconds"], 2),
+            "speedup_mg_W22_vs_plain_pcg": round(pcg_s / res["mg_fcg_W22"]["seconds"], 2),
+            "learned_vs_same_cycle_without_net": round(
+                res["mg_fcg_W22"]["seconds"] / res["learned"]["seconds"], 3),
+        }
         del rho, rhs
+        log("GMRES + line_pcr (reference default route) on 65^3")
+        gq, rhoq, rhsq = ppe_case(N=64)
+        result["ppe"]["gmres_line_pcr_65"] = ppe_solve_timed(gq, rhoq, rhsq, "fp64", maxiter=500,
+                                                             method="gmres")
+        del rhoq, rhsq
 
     if not args.no_cpu:
         log("CPU baseline (dense-LU CCD slab)")

@@ -315,9 +315,10 @@ class LearnedDeflationPreconditioner:
     (Z^T t = W^T R t and Z y = P (W y) are evaluated on the coarse level).
     With net=None it is the V-cycle alone."""
 
-    def __init__(self, hier: MGHierarchy, net, level: int, nu=2, coarse_sweeps=20, ridge=1e-6):
+    def __init__(self, hier: MGHierarchy, net, level: int, nu=2, coarse_sweeps=20, ridge=1e-6,
+                 gamma=1):
         self.hier, self.net, self.level = hier, net, level
-        self.nu, self.cs = nu, coarse_sweeps
+        self.nu, self.cs, self.gamma = nu, coarse_sweeps, gamma
         if net is not None:
             with torch.no_grad():
                 W = deflation_vectors(hier, net, level)
@@ -332,7 +333,7 @@ class LearnedDeflationPreconditioner:
     def __call__(self, r):
         shape = r.shape
         r32 = r.float().contiguous().view(self.hier.shapes[0])
-        z1 = self.hier.vcycle(r32, self.nu, self.nu, self.cs)
+        z1 = self.hier.vcycle(r32, self.nu, self.nu, self.cs, -1, self.gamma)
         if self.net is None:
             return z1.view(shape)
         t = r32 - self.hier.apply(0, z1)
@@ -483,7 +484,7 @@ def level_of(hier: MGHierarchy, nodes: int) -> int:
 
 
 def train_deflation(net, N=128, problems=200, record=(0, 2, 5, 9), level_nodes=33, lr=1e-3,
-                    seed=1000, log=None, nu=2):
+                    seed=1000, log=None, nu=2, gamma=1):
     """Trains `net` in place to produce the coarse space of
     LearnedDeflationPreconditioner from residuals that the multigrid-FCG
     solver itself produces: for each random bubble problem (8-128 spheres,
@@ -505,15 +506,15 @@ def train_deflation(net, N=128, problems=200, record=(0, 2, 5, 9), level_nodes=3
         s.prepare_operator(rho)
         H = MGHierarchy(s)
         level = level_of(H, level_nodes)
-        _, info = fcg_solve(s, rhs, MGPreconditioner(H, nu=nu), tol=1e-10,
+        _, info = fcg_solve(s, rhs, MGPreconditioner(H, nu=nu, gamma=gamma), tol=1e-10,
                             maxiter=max(record) + 1, m=1, fallback=False, record=record)
         samples = []
         with torch.no_grad():
             for r in info["records"]:
                 r32 = r.float().view(H.shapes[0])
-                t = r32 - H.apply(0, H.vcycle(r32, nu, nu))
+                t = r32 - H.apply(0, H.vcycle(r32, nu, nu, 20, -1, gamma))
                 t = t / (t.norm() + 1e-30)
-                samples.append((t, H.vcycle(t, nu, nu)))
+                samples.append((t, H.vcycle(t, nu, nu, 20, -1, gamma)))
         loss = deflation_loss(H, net, level, samples)
         opt.zero_grad(set_to_none=True)
         loss.backward()
