@@ -185,6 +185,75 @@ def mg_solve_timed(g, rho, rhs, gamma=2, net=None, tol=1e-8, reps=2):
     return out
 
 
+def copy_bandwidth_GBps(reps=10):
+    """Device-to-device copy of 1 GiB (read + write counted), event-timed."""
+    import torch
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return 2 * src.numel() * 8 / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline_ppe(rho_dev, rhs_dev, gpu_iterations, k_iters=10):
+    """The oracle's SciPy Jacobi-CG (the reference's CG form) on the host:
+    K iterations on the 257^3 case (s/iteration, extrapolated to the GPU's
+    iteration count) and a full solve of the same construction at 65^3."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    from oracle import ppe as oppe
+    cores = len(os.sched_getaffinity(0))
+    rho = rho_dev.cpu().numpy()
+    rhs = rhs_dev.cpu().numpy()
+    N = rho.shape[0] - 1
+    o = oppe.OraclePPE((N,) * 3)
+    with threadpool_limits(cores):
+        t0 = time.perf_counter()
+        try:
+            o.solve_cg(rhs, rho, tol=1e-30, maxiter=k_iters)
+        except RuntimeError:
+            pass  # stops at maxiter by design
+        t_k = time.perf_counter() - t0
+    per_it = t_k / k_iters
+    return {"kind": "port", "cores": cores,
+            "sample": f"oracle SciPy Jacobi-CG, {k_iters} iterations on the {N + 1}^3 case "
+                      f"({t_k:.1f} s incl. operator setup)",
+            "seconds_per_iteration": round(per_it, 4),
+            "extrapolated_seconds_to_1e-8": round(per_it * gpu_iterations, 1),
+            "extrapolation_iterations": gpu_iterations}
+
+
+def unet_mfma(reps=10):
+    """BF16 MFMA 3x3x3 convolution throughput (forward kernel) against the
+    dense BF16 peak, at the U-Net's coarse-level shape and at a larger one."""
+    import torch
+    from twophaseflow_amd import learned as L
+    out = []
+    for (n, cin, cout) in ((33, 16, 16), (65, 64, 64), (129, 32, 32)):
+        x = torch.randn((n, n, n, cin), device="cuda").to(L.BF16).contiguous()
+        w = (torch.randn((27, cout, cin), device="cuda") * 0.05).to(L.BF16).contiguous()
+        b = torch.zeros(cout, device="cuda")
+        L.conv_fwd(x, w, b, True, False)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            L.conv_fwd(x, w, b, True, False)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        flops = 2.0 * 27 * cin * cout * n ** 3
+        tf = flops / (ms * 1e-3) / 1e12
+        out.append({"grid": n, "cin": cin, "cout": cout, "ms": round(ms, 4),
+                    "TFLOPs": round(tf, 1), "frac_of_bf16_dense_peak": round(tf / 2500.0, 4)})
+    return out
+
+
 def profile_mode(args):
     import torch
     from twophaseflow_amd import CCDSolver, Grid, GridConfig
@@ -296,6 +365,7 @@ def main():
                      "traffic": None,
                      "kernel": "ccd_line_kernel<16> (one grid, all 3 axes)",
                      "bytes_per_launch": bytes_per_launch, "launch_ms": round(dom_ms, 4)},
+        "hbm_copy_GBps": round(copy_bandwidth_GBps(), 1),
         "ccd_all_axes_GBps": round(all_axes_gbs, 1),
         "ccd_axes": [{"axis": a["axis"], "ms": round(a["ms"], 4), "GBps": round(a["GBps"], 1)}
                      for a in axes],
@@ -345,7 +415,13 @@ def main():
             "learned_vs_same_cycle_without_net": round(
                 res["mg_fcg_W22"]["seconds"] / res["learned"]["seconds"], 3),
         }
+        if not args.no_cpu:
+            log("CPU baseline (oracle SciPy Jacobi-CG, 10 iterations at 257^3)")
+            result["ppe"]["cpu_baseline"] = cpu_baseline_ppe(rho, rhs,
+                                                             res["fp64"]["iterations"])
         del rho, rhs
+        log("U-Net BF16 MFMA convolution throughput")
+        result["unet_conv_mfma"] = unet_mfma()
         log("GMRES + line_pcr (reference default route) on 65^3")
         gq, rhoq, rhsq = ppe_case(N=64)
         result["ppe"]["gmres_line_pcr_65"] = ppe_solve_timed(gq, rhoq, rhsq, "fp64", maxiter=500,
