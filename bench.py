@@ -272,10 +272,14 @@ def profile_mode(args):
         dst.copy_(src)
         torch.cuda.synchronize()
         log(f"profile ccd: {args.steps} steps of 3 axis launches on 257^3, then a 1 GiB copy")
-    else:
+    elif args.profile == "ppe":
         gp, rho, rhs = ppe_case(N=128)
         r = ppe_solve_timed(gp, rho, rhs, "fp64")
         log(f"profile ppe 129^3: {r}")
+    else:
+        gp, rho, rhs = ppe_case(N=256)
+        r = mg_solve_timed(gp, rho, rhs, gamma=2, reps=1)
+        log(f"profile mg 257^3 W(2,2)-FCG: {r}")
 
 
 def main():
@@ -288,10 +292,11 @@ def main():
     ap.add_argument("--ppe-n", type=int, default=256)
     ap.add_argument("--train-problems", type=int, default=100,
                     help="random 129^3 bubble problems used to train the learned coarse space")
-    ap.add_argument("--profile", choices=["ccd", "ppe"], default=None,
+    ap.add_argument("--profile", choices=["ccd", "ppe", "mg"], default=None,
                     help="run only one workload (for rocprofv3): 'ccd' = K all-axes CCD steps "
                          "on 257^3 plus one 1 GiB device copy (counter scale reference); "
-                         "'ppe' = one FP64 PCG solve of the 129^3 ratio-1000 case")
+                         "'ppe' = one FP64 PCG solve of the 129^3 ratio-1000 case; "
+                         "'mg' = one W(2,2) multigrid-FCG solve of the 257^3 case")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",

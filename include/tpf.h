@@ -144,6 +144,26 @@ TPF_API tpf_status tpf_ppe_solve_gmres(tpf_ppe* ppe, const double* rhs, double* 
 TPF_API tpf_status tpf_ppe_gmres_precond(tpf_ppe* ppe, const double* r, double* z,
                                  const tpf_gmres_opts* opts);
 
+typedef struct {
+  double tol;          /* ||r||/||b|| target on the CG system                       */
+  int64_t maxiter;     /* FCG iteration cap                                         */
+  int use_x0;          /* 1: p holds the initial guess                              */
+  int nu;              /* red-black smoothing sweeps before and after each coarse step */
+  int gamma;           /* 1: V-cycle, 2: W-cycle                                    */
+  int coarse_sweeps;   /* forward+reverse sweeps on the coarsest level (<= 1024 nodes) */
+  int max_levels;      /* hierarchy depth cap (0: as deep as the grid allows)       */
+} tpf_mg_opts;
+
+/* Solves the reference CG system (-W L) p = -W rhs (fvm_matrixfree.py:163-221)
+ * with flexible CG (FCG(1)) preconditioned by one FP32 multigrid cycle of
+ * the tpf_mg_* hierarchy (no counterpart in the reference; selected by
+ * ppe_preconditioner="mg").  FP64 vectors, matvec and reductions; the
+ * FP64 true residual judges convergence (residual replacement otherwise).
+ * On exit p[pin] = 0.  Synchronous.  TPF_ERR_RUNTIME when maxiter is reached
+ * above tol (report filled). */
+TPF_API tpf_status tpf_ppe_solve_mg(tpf_ppe* ppe, const double* rhs, double* p,
+                            const tpf_mg_opts* opts, tpf_solve_report* rep);
+
 /* FP64 ||b - A p||_2 / ||b||_2 on the CG system (the judge of every solve). */
 TPF_API tpf_status tpf_ppe_true_relres(tpf_ppe* ppe, const double* rhs,
                                const double* p, double* relres);

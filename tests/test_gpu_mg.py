@@ -203,3 +203,80 @@ def test_training_lowers_the_deflation_loss():
         lt = float(L.deflation_loss(H, net, level, samples))
         lf = float(L.deflation_loss(H, fresh, level, samples))
     assert lt < lf
+
+
+# --- native multigrid-FCG behind the PPE plugin (ppe_preconditioner="mg") ----
+def _mg_cfg(N, tol=1e-10, maxiter=200, cycle="W"):
+    cfg = SimulationConfig(grid=GridConfig(ndim=len(N), N=tuple(N), L=(1.0,) * len(N)),
+                           solver=SolverConfig(pseudo_tol=tol, pseudo_maxiter=maxiter))
+    cfg.solver.ppe_iteration_method = "cg"
+    cfg.solver.ppe_preconditioner = "mg"
+    cfg.solver.ppe_mg_cycle = cycle
+    return cfg
+
+
+@pytest.mark.parametrize("seed", [456, 789])
+def test_native_mg_fcg_matches_direct(seed):
+    """test_fvm_matrixfree.py:88-103 setup (N=8, tol 1e-10) through the
+    multigrid-preconditioned route."""
+    _announce(f"native mg-fcg N=8 seed={seed}")
+    cfg = _mg_cfg((8, 8))
+    g = Grid(cfg.grid)
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + rng.uniform(0.0, 1.0, g.shape)
+    rhs = rng.standard_normal(g.shape)
+    p = s.solve(rhs, rho, dt=1e-3)
+    np.testing.assert_allclose(p, oppe.OraclePPE((8, 8)).solve_direct(rhs, rho), rtol=1e-9,
+                               atol=1e-10)
+    assert p.ravel()[s._pin_dof] == 0.0
+    assert s.last_diagnostics["true_relres"] <= 1e-10
+
+
+@pytest.mark.parametrize("N,cycle", [((64, 64, 64), "W"), ((64, 64, 64), "V"),
+                                     ((256, 256), "W"), ((48, 40, 32), "W")])
+def test_native_mg_fcg_ratio_1000(N, cycle):
+    _announce(f"native mg-fcg N={N} {cycle}")
+    cfg = _mg_cfg(N, tol=1e-8, maxiter=100, cycle=cycle)
+    g = Grid(cfg.grid)
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    rho = _bubbles(N)
+    rhs = np.random.default_rng(8).standard_normal(g.shape)
+    p = s.solve(rhs, rho)
+    d = s.last_diagnostics
+    assert d["iterations"] <= 40, d
+    o = oppe.OraclePPE(N)
+    o.prepare_operator(rho)
+    assert o.relres(rhs, p) <= 1e-8 * 1.01
+    # warm start from the solution: done at once
+    s.solve(rhs, rho, p_init=p)
+    assert s.last_diagnostics["iterations"] == 0
+
+
+def test_native_mg_fcg_density_change_rebuilds_hierarchy():
+    _announce("native mg-fcg density change")
+    N = (32, 32, 32)
+    cfg = _mg_cfg(N, tol=1e-9)
+    g = Grid(cfg.grid)
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    rhs = np.random.default_rng(3).standard_normal(g.shape)
+    o = oppe.OraclePPE(N)
+    rho_t = torch.from_numpy(_bubbles(N)).cuda()
+    s.solve(rhs, rho_t)
+    rho_t.copy_(torch.from_numpy(_bubbles(N, 1e-1)))  # same buffer, new values
+    p = s.solve(rhs, rho_t)
+    o.prepare_operator(rho_t.cpu().numpy())
+    assert o.relres(rhs, p) <= 1e-9 * 1.01
+
+
+def test_native_mg_fcg_nonconvergence_raises():
+    _announce("native mg-fcg maxiter=1")
+    cfg = _mg_cfg((32, 32, 32), tol=1e-14, maxiter=1)
+    g = Grid(cfg.grid)
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    with pytest.raises(RuntimeError, match="did not converge"):
+        s.solve(np.random.default_rng(1).standard_normal(g.shape), _bubbles((32, 32, 32)))
+    with pytest.raises(ValueError, match="cg"):
+        c2 = _mg_cfg((8, 8))
+        c2.solver.ppe_iteration_method = "gmres"
+        PPESolverFVMMatrixFree(None, c2, Grid(c2.grid))

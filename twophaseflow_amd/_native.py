@@ -42,6 +42,12 @@ class GmresOpts(C.Structure):
                 ("c_tau", C.c_double)]
 
 
+class MgOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxiter", C.c_int64), ("use_x0", C.c_int),
+                ("nu", C.c_int), ("gamma", C.c_int), ("coarse_sweeps", C.c_int),
+                ("max_levels", C.c_int)]
+
+
 # name -> (restype, argtypes); every symbol of include/tpf.h and tpf_testing.h
 SIGNATURES = {
     "tpf_last_error": (C.c_char_p, []),
@@ -60,6 +66,7 @@ SIGNATURES = {
     "tpf_ppe_solve_gmres": (C.c_int, [_vp, _vp, _vp, C.POINTER(GmresOpts),
                                       C.POINTER(SolveReport)]),
     "tpf_ppe_gmres_precond": (C.c_int, [_vp, _vp, _vp, C.POINTER(GmresOpts)]),
+    "tpf_ppe_solve_mg": (C.c_int, [_vp, _vp, _vp, C.POINTER(MgOpts), C.POINTER(SolveReport)]),
     "tpf_fields_bubbles": (C.c_int, [C.c_int, _i64p, _dp, C.c_int, _dp, _dp, C.c_double,
                                      C.c_double, C.c_double, _vp, _vp, _vp]),
     "tpf_fields_mass": (C.c_int, [C.c_int, _i64p, _dp, _ip, _vp, _dp, _vp]),

@@ -366,7 +366,11 @@ struct MgState {
   int nlev = 0;
   MgLevel lv[kMaxLevels];
   const double* rho_src = nullptr;
-  int64_t rho_version = -1;
+  int max_levels = 0;
+  // captured cycle (tpf::mg_cycle)
+  hipGraphExec_t cyc = nullptr;
+  int cyc_key[3] = {0, 0, 0};
+  hipStream_t cyc_stream = nullptr;
 };
 
 MgState* mg_of(tpf_ppe* s) { return static_cast<MgState*>(s->mg); }
@@ -377,6 +381,7 @@ unsigned grid_of(int64_t total) {
 }
 
 void free_mg(MgState* m) {
+  if (m->cyc) (void)hipGraphExecDestroy(m->cyc);
   for (int l = 0; l < m->nlev; ++l) {
     MgLevel& L = m->lv[l];
     float* bufs[] = {L.T[0], L.T[1], L.T[2], L.dinv, L.x, L.b, L.r};
@@ -547,9 +552,53 @@ extern "C" tpf_status tpf_mg_setup(tpf_ppe* s, int max_levels, int* levels) {
   }
   TPF_HIP(hipStreamSynchronize(st));
   m->rho_src = s->rho;
+  m->max_levels = max_levels;
   *levels = m->nlev;
   return TPF_OK;
 }
+
+namespace tpf {
+tpf_status mg_ensure(tpf_ppe* s, int max_levels) {
+  MgState* m = mg_of(s);
+  if (m && m->nlev > 0 && m->rho_src == s->rho && m->max_levels == max_levels) return TPF_OK;
+  int lv = 0;
+  return tpf_mg_setup(s, max_levels, &lv);
+}
+
+void mg_invalidate(tpf_ppe* s) {
+  if (MgState* m = mg_of(s)) m->rho_src = nullptr;
+}
+
+float* mg_fine_b(tpf_ppe* s) { return mg_of(s)->lv[0].b; }
+float* mg_fine_x(tpf_ppe* s) { return mg_of(s)->lv[0].x; }
+
+tpf_status mg_cycle(tpf_ppe* s, hipStream_t st, int nu, int gamma, int coarse_sweeps) {
+  MgState* m = mg_of(s);
+  if (!m || m->nlev == 0) return fail(TPF_ERR_RUNTIME, "multigrid hierarchy not built");
+  const int key[3] = {nu, gamma, coarse_sweeps};
+  if (!(m->cyc && m->cyc_stream == st && key[0] == m->cyc_key[0] && key[1] == m->cyc_key[1] &&
+        key[2] == m->cyc_key[2])) {
+    if (m->cyc) {
+      TPF_HIP(hipGraphExecDestroy(m->cyc));
+      m->cyc = nullptr;
+    }
+    hipGraph_t graph;
+    TPF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const tpf_status cs = cycle(m, st, 0, m->nlev - 1, gamma, true, nu, nu, coarse_sweeps);
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    if (cs != TPF_OK) return cs;
+    if (ec != hipSuccess) return fail(TPF_ERR_RUNTIME, std::string("capture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&m->cyc, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess)
+      return fail(TPF_ERR_RUNTIME, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    for (int i = 0; i < 3; ++i) m->cyc_key[i] = key[i];
+    m->cyc_stream = st;
+  }
+  TPF_HIP(hipGraphLaunch(m->cyc, st));
+  return TPF_OK;
+}
+}  // namespace tpf
 
 extern "C" tpf_status tpf_mg_level_info(tpf_ppe* s, int level, int64_t* n3, int64_t* pin) {
   MgState* m = mg_of(s);

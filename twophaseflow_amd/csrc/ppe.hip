@@ -427,6 +427,161 @@ __global__ __launch_bounds__(kBlock) void k_fin_true(PcgState* st, const double*
   }
 }
 
+// ---- multigrid-preconditioned flexible CG (FCG(1), Notay) on the CG system
+// Device scalars of one solve; every kernel below is followed by a
+// single-block finish kernel (deterministic reductions).
+struct FcgState {
+  double rr, bb, tol2;
+  double zq;      // <z_k, q_{k-1}>
+  double pq;      // <p_k, q_k>
+  double pq_old;  // <p_{k-1}, q_{k-1}>
+  double pr;      // <p_k, r_k>
+  double alpha, beta;
+  int64_t iter;
+};
+
+// b = -W rhs (b[pin] = 0); r = b - A x (x = 0 when zero_x); r32 = r.
+__global__ __launch_bounds__(kBlock) void k_fcg_init(PpeGeom g, const double* __restrict__ rhs,
+                                                     const double* __restrict__ rho,
+                                                     double* __restrict__ x,
+                                                     double* __restrict__ r,
+                                                     float* __restrict__ r32, int zero_x,
+                                                     double* partials) {
+  __shared__ double sh[2 * kBlock / 64];
+  double v[2] = {0.0, 0.0};
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    const double W = dv(g, 0, i0) * dv(g, 1, i1) * dv(g, 2, i2);
+    const double bi = idx == g.pin ? 0.0 : -W * rhs[idx];
+    double ap = 0.0, dg;
+    if (!zero_x) {  // x[pin] = 0 beforehand
+      stencil<double>(g, x, rho, idx, i0, i1, i2, ap, dg);
+      if (idx == g.pin) ap = x[idx];
+    }
+    const double ri = bi - ap;
+    r[idx] = ri;
+    r32[idx] = (float)ri;
+    v[0] += ri * ri;
+    v[1] += bi * bi;
+  }
+  block_partials<2>(v, partials, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fcg_fin_init(FcgState* st, const double* partials,
+                                                         int nblocks) {
+  __shared__ double sh[2 * kBlock / 64];
+  double v[2];
+  finish_sum<2>(v, partials, nblocks, sh);
+  if (threadIdx.x == 0) {
+    st->rr = v[0];
+    st->bb = v[1];
+    st->iter = 0;
+    st->beta = 0.0;
+  }
+}
+
+// <z, q_old>
+__global__ __launch_bounds__(kBlock) void k_fcg_zq(int64_t n, const float* __restrict__ z,
+                                                   const double* __restrict__ q,
+                                                   double* partials) {
+  __shared__ double sh[kBlock / 64];
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    v[0] += (double)z[i] * q[i];
+  block_partials<1>(v, partials, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fcg_fin_beta(FcgState* st, const double* partials,
+                                                         int nblocks) {
+  __shared__ double sh[kBlock / 64];
+  double v[1];
+  finish_sum<1>(v, partials, nblocks, sh);
+  if (threadIdx.x == 0) {
+    st->zq = v[0];
+    st->beta = st->iter > 0 ? v[0] / st->pq_old : 0.0;
+  }
+}
+
+// p = z - beta p ; q = A p is formed by the next kernel
+__global__ __launch_bounds__(kBlock) void k_fcg_p(int64_t n, const float* __restrict__ z,
+                                                  double* __restrict__ p, const FcgState* st) {
+  const double beta = st->beta;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    p[i] = (double)z[i] - (beta != 0.0 ? beta * p[i] : 0.0);
+}
+
+// q = A p (pin row identity); partials <p,q>, <p,r>
+__global__ __launch_bounds__(kBlock) void k_fcg_matvec(PpeGeom g, const double* __restrict__ p,
+                                                       const double* __restrict__ rho,
+                                                       const double* __restrict__ r,
+                                                       double* __restrict__ q,
+                                                       double* partials) {
+  __shared__ double sh[2 * kBlock / 64];
+  double v[2] = {0.0, 0.0};
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    double ap, dg;
+    stencil<double>(g, p, rho, idx, i0, i1, i2, ap, dg);
+    const double pi = p[idx];
+    if (idx == g.pin) ap = pi;
+    q[idx] = ap;
+    v[0] += pi * ap;
+    v[1] += pi * r[idx];
+  }
+  block_partials<2>(v, partials, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fcg_fin_alpha(FcgState* st, const double* partials,
+                                                          int nblocks) {
+  __shared__ double sh[2 * kBlock / 64];
+  double v[2];
+  finish_sum<2>(v, partials, nblocks, sh);
+  if (threadIdx.x == 0) {
+    st->pq = v[0];
+    st->pr = v[1];
+    st->alpha = v[1] / v[0];
+    st->pq_old = v[0];
+  }
+}
+
+// x += alpha p ; r -= alpha q ; r32 = r ; partial <r,r>
+__global__ __launch_bounds__(kBlock) void k_fcg_update(int64_t n, const double* __restrict__ p,
+                                                       const double* __restrict__ q,
+                                                       double* __restrict__ x,
+                                                       double* __restrict__ r,
+                                                       float* __restrict__ r32,
+                                                       const FcgState* st, double* partials) {
+  __shared__ double sh[kBlock / 64];
+  const double alpha = st->alpha;
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    r32[i] = (float)ri;
+    v[0] += ri * ri;
+  }
+  block_partials<1>(v, partials, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fcg_fin_r(FcgState* st, const double* partials,
+                                                      int nblocks) {
+  __shared__ double sh[kBlock / 64];
+  double v[1];
+  finish_sum<1>(v, partials, nblocks, sh);
+  if (threadIdx.x == 0) {
+    st->rr = v[0];
+    st->iter += 1;
+  }
+}
+
 int grid_for(int64_t total) {
   int64_t blocks = (total + kBlock - 1) / kBlock;
   const int64_t cap = 256 * 8;  // 8 blocks of 256 per CU; grid-stride loops do the rest
@@ -508,6 +663,8 @@ extern "C" tpf_status tpf_ppe_destroy(tpf_ppe* s) {
   if (!s) return TPF_OK;
   tpf_gmres_release(s);
   tpf_mg_release(s);
+  if (s->fcg_st) (void)hipFree(s->fcg_st);
+  if (s->fcg_host) (void)hipHostFree(s->fcg_host);
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   if (s->graph32_exec) (void)hipGraphExecDestroy(s->graph32_exec);
   float* fb[] = {s->r32, s->p32, s->q32, s->x32, s->dinv32, s->rho32};
@@ -540,6 +697,8 @@ static tpf_status join_user_stream(tpf_ppe* s) {
 extern "C" tpf_status tpf_ppe_set_density(tpf_ppe* s, const double* rho) {
   if (!s || !rho) return fail(TPF_ERR_VALUE, "null argument");
   s->rho = rho;
+  s->rho32_src = nullptr;  // derived FP32 copies and the multigrid hierarchy are rebuilt
+  mg_invalidate(s);
   TPF_TRY(join_user_stream(s));
   hipLaunchKernelGGL(k_diag, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g, rho, s->dinv);
   TPF_HIP(hipGetLastError());
@@ -855,5 +1014,105 @@ extern "C" tpf_status tpf_ppe_system_apply_f32(tpf_ppe* s, const float* p, float
   hipLaunchKernelGGL(k_system_apply<float>, dim3(s->grid), dim3(kBlock), 0, s->user_stream, s->g,
                      p, (const float*)s->rho32, out);
   TPF_HIP(hipGetLastError());
+  return TPF_OK;
+}
+
+// ---- tpf_ppe_solve_mg -------------------------------------------------------
+extern "C" tpf_status tpf_ppe_solve_mg(tpf_ppe* s, const double* rhs, double* x,
+                                       const tpf_mg_opts* o, tpf_solve_report* rep) {
+  if (!s || !rhs || !x || !o) return fail(TPF_ERR_VALUE, "null argument");
+  if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set (call tpf_ppe_set_density)");
+  if (!(o->tol > 0.0) || o->maxiter < 1) return fail(TPF_ERR_VALUE, "bad tol/maxiter");
+  if (o->gamma < 1 || o->gamma > 2 || o->nu < 1 || o->coarse_sweeps < 1)
+    return fail(TPF_ERR_VALUE, "bad multigrid cycle options (gamma 1|2, nu >= 1, sweeps >= 1)");
+  const int max_levels = o->max_levels > 0 ? o->max_levels : 12;
+  TPF_TRY(mg_ensure(s, max_levels));  // on the caller's stream, synchronous
+  TPF_TRY(join_user_stream(s));
+  hipStream_t st = s->stream;
+  // workspace: r in s->r, p in s->p, q in s->q (FP64); FP32 residual and
+  // preconditioned residual live in the hierarchy's level-0 b and x.
+  if (!s->fcg_st) {
+    TPF_HIP(hipMalloc(&s->fcg_st, sizeof(FcgState)));
+    TPF_HIP(hipHostMalloc(&s->fcg_host, sizeof(FcgState)));
+  }
+  FcgState* dst = static_cast<FcgState*>(s->fcg_st);
+  FcgState* hst = static_cast<FcgState*>(s->fcg_host);
+  float* r32 = mg_fine_b(s);
+  const float* z32 = mg_fine_x(s);
+  const int64_t n = s->g.total;
+  const dim3 grid(s->grid), blk(kBlock);
+  const double tol2 = o->tol * o->tol;
+  auto read = [&]() -> tpf_status {
+    TPF_HIP(hipMemcpyAsync(hst, dst, sizeof(FcgState), hipMemcpyDeviceToHost, st));
+    TPF_HIP(hipStreamSynchronize(st));
+    return TPF_OK;
+  };
+  if (o->use_x0) hipLaunchKernelGGL(k_set_one, dim3(1), dim3(64), 0, st, x, s->g.pin, 0.0);
+  int64_t iters = 0;
+  double rr = 0.0, bb = 0.0, tr = 0.0;
+  int restarts = 0;
+  bool conv = false;
+  // Residual replacement: when the recursive ||r|| meets tol but the FP64
+  // true residual does not, FCG restarts from x (at most 4 times).
+  for (;; ++restarts) {
+    const int zero_x = (restarts == 0 && !o->use_x0) ? 1 : 0;
+    if (zero_x) hipLaunchKernelGGL(k_fill_zero, grid, blk, 0, st, n, x);
+    hipLaunchKernelGGL(k_fcg_init, grid, blk, 0, st, s->g, rhs, s->rho, x, s->r, r32, 0,
+                       s->partials);
+    hipLaunchKernelGGL(k_fcg_fin_init, dim3(1), blk, 0, st, dst, (const double*)s->partials,
+                       s->grid);
+    TPF_HIP(hipGetLastError());
+    TPF_TRY(read());
+    rr = hst->rr;
+    bb = hst->bb;
+    if (bb == 0.0) {
+      hipLaunchKernelGGL(k_fill_zero, grid, blk, 0, st, n, x);
+      TPF_HIP(hipStreamSynchronize(st));
+      conv = true;
+      tr = 0.0;
+      break;
+    }
+    bool hit = rr <= tol2 * bb;
+    while (!hit && iters < o->maxiter) {
+      TPF_TRY(mg_cycle(s, st, o->nu, o->gamma, o->coarse_sweeps));
+      hipLaunchKernelGGL(k_fcg_zq, grid, blk, 0, st, n, z32, (const double*)s->q, s->partials);
+      hipLaunchKernelGGL(k_fcg_fin_beta, dim3(1), blk, 0, st, dst, (const double*)s->partials,
+                         s->grid);
+      hipLaunchKernelGGL(k_fcg_p, grid, blk, 0, st, n, z32, s->p, (const FcgState*)dst);
+      hipLaunchKernelGGL(k_fcg_matvec, grid, blk, 0, st, s->g, (const double*)s->p, s->rho,
+                         (const double*)s->r, s->q, s->partials);
+      hipLaunchKernelGGL(k_fcg_fin_alpha, dim3(1), blk, 0, st, dst, (const double*)s->partials,
+                         s->grid);
+      hipLaunchKernelGGL(k_fcg_update, grid, blk, 0, st, n, (const double*)s->p,
+                         (const double*)s->q, x, s->r, r32, (const FcgState*)dst, s->partials);
+      hipLaunchKernelGGL(k_fcg_fin_r, dim3(1), blk, 0, st, dst, (const double*)s->partials,
+                         s->grid);
+      TPF_HIP(hipGetLastError());
+      TPF_TRY(read());
+      ++iters;
+      rr = hst->rr;
+      hit = rr <= tol2 * bb;
+    }
+    // FP64 true residual of the CG system judges the result
+    hipLaunchKernelGGL(k_set_one, dim3(1), dim3(64), 0, st, x, s->g.pin, 0.0);
+    hipLaunchKernelGGL(k_true_res, grid, blk, 0, st, s->g, rhs, s->rho, (const double*)x, s->st,
+                       s->partials);
+    hipLaunchKernelGGL(k_fin_true, dim3(1), blk, 0, st, s->st, (const double*)s->partials,
+                       s->grid);
+    TPF_HIP(hipGetLastError());
+    TPF_TRY(read_state(s));
+    tr = s->st_host->true_bb > 0.0 ? std::sqrt(s->st_host->true_rr / s->st_host->true_bb) : 0.0;
+    conv = tr <= o->tol;
+    if (conv || iters >= o->maxiter || restarts >= 4) break;
+  }
+  if (rep) {
+    rep->iterations = iters;
+    rep->relres = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+    rep->true_relres = tr;
+    rep->converged = conv ? 1 : 0;
+    rep->refinements = restarts;
+  }
+  if (!conv)
+    return fail(TPF_ERR_RUNTIME, "multigrid FCG did not reach tol in maxiter iterations");
   return TPF_OK;
 }

@@ -54,9 +54,23 @@ struct tpf_ppe {
   const double* graph32_rho = nullptr;
   void* gmres_work = nullptr;           // GMRES basis and line-solve scratch
   void* mg = nullptr;                   // multigrid hierarchy (FP32)
+  void* fcg_st = nullptr;               // multigrid-FCG device scalars
+  void* fcg_host = nullptr;             // pinned host copy
 };
 
 void tpf_gmres_release(tpf_ppe* s);
 void tpf_mg_release(tpf_ppe* s);
+
+namespace tpf {
+// Multigrid hooks used by the multigrid-preconditioned CG of ppe.hip
+// (implemented in mg.hip).
+tpf_status mg_ensure(tpf_ppe* s, int max_levels);  // hierarchy for the current rho; synchronous
+void mg_invalidate(tpf_ppe* s);                     // density changed: rebuild on next use
+float* mg_fine_b(tpf_ppe* s);                       // level-0 right-hand side (cycle input)
+float* mg_fine_x(tpf_ppe* s);                       // level-0 iterate (cycle output)
+// x_0 = M(b_0): one cycle on stream `st` (not the null stream), replayed from
+// a captured graph per (nu, gamma, coarse_sweeps, stream).
+tpf_status mg_cycle(tpf_ppe* s, hipStream_t st, int nu, int gamma, int coarse_sweeps);
+}  // namespace tpf
 tpf_status tpf_ppe_solve_mixed(tpf_ppe* s, const double* rhs, double* x,
                                const tpf_solve_opts* opts, tpf_solve_report* rep);

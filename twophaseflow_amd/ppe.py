@@ -9,7 +9,11 @@ Mirrors the reference plugin interface (``ppe/interfaces.py:24-88``,
   restarted GMRES with SciPy's stopping rules, preconditioner "line_pcr"
   (default), "jacobi" or "none";
 * CG: ``(-W L) p = -W rhs``, Jacobi preconditioning, the recursively updated
-  ``||r|| <= tol ||b||`` stop (FP64 or FP32-inner/FP64-refined).
+  ``||r|| <= tol ||b||`` stop (FP64 or FP32-inner/FP64-refined); or, with
+  this build's ``ppe_preconditioner="mg"``, flexible CG preconditioned by one
+  FP32 multigrid cycle per iteration (``ppe_mg_cycle`` "W" | "V",
+  ``ppe_mg_smooth``, ``ppe_mg_coarse_sweeps``), judged by the FP64 true
+  residual.
 
 Both raise ``RuntimeError`` when they do not converge and return
 ``p[pin] = 0``.  The Krylov loops run on the device (libtpf); nothing
@@ -97,15 +101,17 @@ class PPESolverFVMMatrixFree(IPPESolver):
         default_pre = "jacobi" if method == "cg" else "line_pcr"
         self.preconditioner = str(
             getattr(solver_cfg, "ppe_preconditioner", default_pre)).strip().lower()
-        if self.preconditioner not in {"jacobi", "line_pcr", "none"}:
-            raise ValueError("ppe_preconditioner must be 'jacobi', 'line_pcr', or 'none', "
-                             f"got {self.preconditioner!r}")
+        if self.preconditioner not in {"jacobi", "line_pcr", "none", "mg"}:
+            raise ValueError("ppe_preconditioner must be 'jacobi', 'line_pcr', 'none' or "
+                             f"'mg', got {self.preconditioner!r}")
+        if self.preconditioner == "mg" and method != "cg":
+            raise ValueError("ppe_preconditioner='mg' is built for ppe_iteration_method='cg'")
         if method == "cg" and self.preconditioner == "line_pcr":
             raise ValueError(
                 "ppe_iteration_method='cg' supports preconditioner='jacobi' or 'none'; "
                 "line_pcr is a nonsymmetric GMRES preconditioner.")
         if method == "cg" and self.preconditioner == "none":
-            raise ValueError("unpreconditioned CG is not built; use 'jacobi'")
+            raise ValueError("unpreconditioned CG is not built; use 'jacobi' or 'mg'")
         pcr_stages = getattr(solver_cfg, "ppe_pcr_stages", 4)
         self.precond_pcr_stages = 4 if pcr_stages is None else int(pcr_stages)
         self.line_solver = str(getattr(solver_cfg, "ppe_line_solver", "pcr")).lower()
@@ -117,6 +123,12 @@ class PPESolverFVMMatrixFree(IPPESolver):
         if self.precision == "mixed" and method != "cg":
             raise ValueError("ppe_precision='mixed' is built for ppe_iteration_method='cg'")
         self.check_every = int(getattr(solver_cfg, "ppe_check_every", 32))
+        cyc = str(getattr(solver_cfg, "ppe_mg_cycle", "W")).upper()
+        if cyc not in ("V", "W"):
+            raise ValueError("ppe_mg_cycle must be 'V' or 'W'")
+        self.mg_gamma = 1 if cyc == "V" else 2
+        self.mg_smooth = int(getattr(solver_cfg, "ppe_mg_smooth", 2))
+        self.mg_coarse_sweeps = int(getattr(solver_cfg, "ppe_mg_coarse_sweeps", 20))
         self.inner_tol = float(getattr(solver_cfg, "ppe_inner_tol", 0.0))
         spec = bc_spec if bc_spec is not None else BoundarySpec(bc_type, grid.shape, grid.N)
         self._pin_dof = spec.pin_dof
@@ -179,9 +191,8 @@ class PPESolverFVMMatrixFree(IPPESolver):
     def solve_into(self, rhs: torch.Tensor, rho: torch.Tensor | None, p: torch.Tensor,
                    use_x0: bool = False) -> dict:
         """Zero-copy solve on CUDA tensors; p holds p_init when use_x0."""
-        if rho is not None:
-            if self._rho is None or rho.data_ptr() != self._rho.data_ptr():
-                self.prepare_operator(rho)
+        if rho is not None:  # the reference prepares the operator on every solve
+            self.prepare_operator(rho)
         elif self._rho is None:
             raise RuntimeError("no density set")
         rep = nat.SolveReport()
@@ -189,6 +200,12 @@ class PPESolverFVMMatrixFree(IPPESolver):
             gopts = self._gmres_opts(use_x0)
             status = self._lib.tpf_ppe_solve_gmres(self._handle, ptr(rhs), ptr(p),
                                                    C.byref(gopts), C.byref(rep))
+        elif self.preconditioner == "mg":
+            mopts = nat.MgOpts(tol=self.tol, maxiter=self.maxiter, use_x0=1 if use_x0 else 0,
+                               nu=self.mg_smooth, gamma=self.mg_gamma,
+                               coarse_sweeps=self.mg_coarse_sweeps, max_levels=0)
+            status = self._lib.tpf_ppe_solve_mg(self._handle, ptr(rhs), ptr(p), C.byref(mopts),
+                                                C.byref(rep))
         else:
             opts = nat.SolveOpts(tol=self.tol, maxiter=self.maxiter, precond=1,
                                  check_every=self.check_every, use_x0=1 if use_x0 else 0,
@@ -207,6 +224,7 @@ class PPESolverFVMMatrixFree(IPPESolver):
             "true_relres": float(rep.true_relres),
             "refinements": int(rep.refinements),
             "precision": self.precision,
+            "preconditioner": self.preconditioner,
         }
         if status != nat.TPF_OK:
             msg = self._lib.tpf_last_error().decode(errors="replace")
