@@ -154,6 +154,30 @@ def ppe_solve_timed(g, rho, rhs, precision, tol=1e-8, maxiter=200000, inner_tol=
             "true_relres": tr, "converged": bool(tr <= tol)}
 
 
+def mg_native_timed(g, rho, rhs, cycle="W", tol=1e-8, reps=2):
+    """Native multigrid-FCG (tpf_ppe_solve_mg through the PPE plugin with
+    ppe_preconditioner="mg"); time includes operator and hierarchy setup."""
+    import torch
+    from twophaseflow_amd import PPESolverFVMMatrixFree, SimulationConfig, SolverConfig
+    cfg = SimulationConfig(solver=SolverConfig(pseudo_tol=tol, pseudo_maxiter=500))
+    cfg.solver.ppe_iteration_method = "cg"
+    cfg.solver.ppe_preconditioner = "mg"
+    cfg.solver.ppe_mg_cycle = cycle
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    p = torch.empty_like(rhs)
+    out = None
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d = s.solve_into(rhs, rho, p)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        out = {"seconds": t, "iterations": d["iterations"], "true_relres": d["true_relres"],
+               "converged": bool(d["true_relres"] <= tol),
+               "cycle": f"{cycle}(2,2)" + (", W on the top 2 levels" if cycle == "W" else "")}
+    return out
+
+
 def mg_solve_timed(g, rho, rhs, gamma=2, net=None, tol=1e-8, reps=2):
     """Multigrid-FCG (net=None) or learned-deflation FCG on the CG system;
     time includes the hierarchy setup (and the network + Galerkin setup)."""
@@ -278,8 +302,8 @@ def profile_mode(args):
         log(f"profile ppe 129^3: {r}")
     else:
         gp, rho, rhs = ppe_case(N=256)
-        r = mg_solve_timed(gp, rho, rhs, gamma=2, reps=1)
-        log(f"profile mg 257^3 W(2,2)-FCG: {r}")
+        r = mg_native_timed(gp, rho, rhs, "W", reps=1)
+        log(f"profile mg 257^3 native W(2,2)-FCG: {r}")
 
 
 def main():
@@ -386,8 +410,12 @@ def main():
             log(f"PPE solve {key}")
             res[key] = ppe_solve_timed(gp, rho, rhs, prec, inner_tol=itol)
             log(f"  {res[key]}")
-        for key, gam in (("mg_fcg_V22", 1), ("mg_fcg_W22", 2)):
+        for key, cyc in (("mg_fcg_native_W22", "W"), ("mg_fcg_native_V22", "V")):
             log(f"PPE solve {key}")
+            res[key] = mg_native_timed(gp, rho, rhs, cyc)
+            log(f"  {res[key]}")
+        for key, gam in (("mg_fcg_V22", 1), ("mg_fcg_W22", 2)):
+            log(f"PPE solve {key} (host FCG loop)")
             res[key] = mg_solve_timed(gp, rho, rhs, gamma=gam)
             log(f"  {res[key]}")
         import numpy as np
@@ -407,7 +435,10 @@ def main():
             "grid": [args.ppe_n + 1] * 3, "density_ratio": 1000, "bubbles": 64, "tol": 1e-8,
             "plain_pcg_fp64": res["fp64"], "pcg_mixed": res["mixed"],
             "pcg_mixed_inner_1e-3": res["mixed_1e-3"], "pcg_mixed_inner_1e-5": res["mixed_1e-5"],
-            "mg_fcg_V22": res["mg_fcg_V22"], "mg_fcg_W22": res["mg_fcg_W22"],
+            "mg_fcg_native_W22": res["mg_fcg_native_W22"],
+            "mg_fcg_native_V22": res["mg_fcg_native_V22"],
+            "speedup_native_mg_vs_plain_pcg": round(pcg_s / res["mg_fcg_native_W22"]["seconds"], 2),
+            "host_loop_mg_fcg_V22": res["mg_fcg_V22"], "host_loop_mg_fcg_W22": res["mg_fcg_W22"],
             "learned_fcg": dict(res["learned"], backbone="W(2,2) multigrid",
                                 coarse_space="16 U-Net vectors on the 33^3 level",
                                 train_problems=args.train_problems,

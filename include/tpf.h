@@ -152,6 +152,7 @@ typedef struct {
   int gamma;           /* 1: V-cycle, 2: W-cycle                                    */
   int coarse_sweeps;   /* forward+reverse sweeps on the coarsest level (<= 1024 nodes) */
   int max_levels;      /* hierarchy depth cap (0: as deep as the grid allows)       */
+  int w_levels;        /* gamma repeats only on the top w_levels levels (0: all)    */
 } tpf_mg_opts;
 
 /* Solves the reference CG system (-W L) p = -W rhs (fvm_matrixfree.py:163-221)

@@ -45,7 +45,7 @@ class GmresOpts(C.Structure):
 class MgOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int64), ("use_x0", C.c_int),
                 ("nu", C.c_int), ("gamma", C.c_int), ("coarse_sweeps", C.c_int),
-                ("max_levels", C.c_int)]
+                ("max_levels", C.c_int), ("w_levels", C.c_int)]
 
 
 # name -> (restype, argtypes); every symbol of include/tpf.h and tpf_testing.h

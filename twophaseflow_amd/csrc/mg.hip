@@ -44,6 +44,14 @@ struct LevelGeom {
   int64_t pin;  // node held at zero (-1: none)
 };
 
+// Fine-level operator evaluated from an FP32 copy of rho (face conductances
+// formed on the fly: 1 array read per node instead of 3 conductances + 1/diag).
+struct FineGeom {
+  const float* rho;
+  float h[3];    // node spacing per slot
+  float ih2[3];  // 2 / h
+};
+
 struct MgLevel {
   LevelGeom g{};
   float* T[3] = {nullptr, nullptr, nullptr};
@@ -51,6 +59,8 @@ struct MgLevel {
   float* x = nullptr;
   float* b = nullptr;
   float* r = nullptr;
+  float* rho32 = nullptr;  // fine level only
+  FineGeom fg{};
 };
 
 __host__ __device__ inline int64_t lstride(const LevelGeom& g, int a) {
@@ -222,6 +232,79 @@ __global__ __launch_bounds__(kBlock) void k_resid(LevelGeom g, const float* __re
   }
 }
 
+// sum_f t_f x_nbr and diag from rho (fine level)
+__device__ inline float nbr_sum_rho(const LevelGeom& g, const FineGeom& fg,
+                                    const float* __restrict__ x, int64_t idx, const int (&c)[3],
+                                    float& diag) {
+  float dv[3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+    dv[b] = g.n[b] == 1 ? 1.0f
+                        : ((c[b] == 0 || c[b] == g.n[b] - 1) ? 0.5f * fg.h[b] : fg.h[b]);
+  const float ri = fg.rho[idx];
+  float s = 0.0f, d = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (g.n[a] == 1) continue;
+    float w = fg.ih2[a];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      if (b != a && g.n[b] > 1) w *= dv[b];
+    const int64_t st = lstride(g, a);
+    if (c[a] < g.n[a] - 1) {
+      const float t = w * __builtin_amdgcn_rcpf(ri + fg.rho[idx + st]);
+      s += t * x[idx + st];
+      d += t;
+    }
+    if (c[a] > 0) {
+      const float t = w * __builtin_amdgcn_rcpf(ri + fg.rho[idx - st]);
+      s += t * x[idx - st];
+      d += t;
+    }
+  }
+  diag = d;
+  return s;
+}
+
+// Threads cover only the nodes of `color`: row (c0, c1) holds the nodes
+// c2 = ((color + c0 + c1) & 1) + 2k, k < (n2 + 1) / 2.
+__global__ __launch_bounds__(kBlock) void k_rbgs_rho(LevelGeom g, FineGeom fg, int color,
+                                                     const float* __restrict__ b, float* x) {
+  const int half = (g.n[2] + 1) >> 1;
+  const int64_t slots = (int64_t)g.n[0] * g.n[1] * half;
+  GRID_LOOP(u, slots) {
+    const int64_t row = u / half;
+    const int k = (int)(u - row * half);
+    int c[3];
+    c[0] = (int)(row / g.n[1]);
+    c[1] = (int)(row - (int64_t)c[0] * g.n[1]);
+    c[2] = ((color + c[0] + c[1]) & 1) + 2 * k;
+    if (c[2] >= g.n[2]) continue;
+    const int64_t idx = row * g.n[2] + c[2];
+    float d;
+    const float s = nbr_sum_rho(g, fg, x, idx, c, d);
+    x[idx] = idx == g.pin ? 0.0f : (b[idx] + s) * __builtin_amdgcn_rcpf(d);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_resid_rho(LevelGeom g, FineGeom fg,
+                                                      const float* __restrict__ b,
+                                                      const float* __restrict__ x,
+                                                      float* __restrict__ r) {
+  GRID_LOOP(idx, g.total) {
+    int c[3];
+    lcoords(g, idx, c);
+    float d;
+    const float s = nbr_sum_rho(g, fg, x, idx, c, d);
+    r[idx] = idx == g.pin ? 0.0f : b[idx] - (d * x[idx] - s);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rho32(int64_t n, const double* __restrict__ rho,
+                                                  float* __restrict__ r32) {
+  GRID_LOOP(i, n) r32[i] = (float)rho[i];
+}
+
 // y = A x (y[pin] = 0)
 __global__ __launch_bounds__(kBlock) void k_apply(LevelGeom g, const float* __restrict__ T0,
                                                   const float* __restrict__ T1,
@@ -369,7 +452,7 @@ struct MgState {
   int max_levels = 0;
   // captured cycle (tpf::mg_cycle)
   hipGraphExec_t cyc = nullptr;
-  int cyc_key[3] = {0, 0, 0};
+  int cyc_key[4] = {0, 0, 0, 0};
   hipStream_t cyc_stream = nullptr;
 };
 
@@ -384,7 +467,7 @@ void free_mg(MgState* m) {
   if (m->cyc) (void)hipGraphExecDestroy(m->cyc);
   for (int l = 0; l < m->nlev; ++l) {
     MgLevel& L = m->lv[l];
-    float* bufs[] = {L.T[0], L.T[1], L.T[2], L.dinv, L.x, L.b, L.r};
+    float* bufs[] = {L.T[0], L.T[1], L.T[2], L.dinv, L.x, L.b, L.r, L.rho32};
     for (float* p : bufs)
       if (p) (void)hipFree(p);
   }
@@ -402,7 +485,10 @@ tpf_status smooth(MgLevel& L, hipStream_t st, const float* b, float* x, int swee
   for (int k = 0; k < sweeps; ++k)
     for (int h = 0; h < 2; ++h) {
       const int color = reverse ? 1 - h : h;
-      MG_LAUNCH(k_rbgs, L.g.total, L.g, color, L.T[0], L.T[1], L.T[2], L.dinv, b, x);
+      if (L.rho32)
+        MG_LAUNCH(k_rbgs_rho, (L.g.total + 1) / 2, L.g, L.fg, color, b, x);
+      else
+        MG_LAUNCH(k_rbgs, L.g.total, L.g, color, L.T[0], L.T[1], L.T[2], L.dinv, b, x);
     }
   return TPF_OK;
 }
@@ -422,6 +508,12 @@ tpf_status descend(MgState* m, hipStream_t st, int from, int to, int nu) {
     MgLevel& C = m->lv[l + 1];
     MG_LAUNCH(k_zero, L.g.total, L.g.total, L.x);
     TPF_TRY(smooth(L, st, L.b, L.x, nu, false));
+    if (L.rho32)
+      MG_LAUNCH(k_resid_rho, L.g.total, L.g, L.fg, L.b, L.x, L.r);
+    else
+      if (L.rho32)
+    MG_LAUNCH(k_resid_rho, L.g.total, L.g, L.fg, L.b, L.x, L.r);
+  else
     MG_LAUNCH(k_resid, L.g.total, L.g, L.T[0], L.T[1], L.T[2], L.b, L.x, L.r);
     MG_LAUNCH(k_restrict, C.g.total, L.g, C.g, L.r, C.b);
   }
@@ -442,7 +534,7 @@ tpf_status ascend(MgState* m, hipStream_t st, int from, int to, int nu) {
 // One gamma-cycle on level l (b_l set).  A first visit starts from x_l = 0,
 // a repeated visit (W-cycle) continues from the current x_l.
 tpf_status cycle(MgState* m, hipStream_t st, int l, int last, int gamma, bool first, int nu_pre,
-                 int nu_post, int coarse_sweeps) {
+                 int nu_post, int coarse_sweeps, int w_levels = 0) {
   MgLevel& L = m->lv[l];
   if (l == last) {
     if (!first) return TPF_OK;  // same right-hand side, same coarse result
@@ -455,10 +547,15 @@ tpf_status cycle(MgState* m, hipStream_t st, int l, int last, int gamma, bool fi
   MgLevel& C = m->lv[l + 1];
   if (first) MG_LAUNCH(k_zero, L.g.total, L.g.total, L.x);
   TPF_TRY(smooth(L, st, L.b, L.x, nu_pre, false));
-  MG_LAUNCH(k_resid, L.g.total, L.g, L.T[0], L.T[1], L.T[2], L.b, L.x, L.r);
+  if (L.rho32)
+    MG_LAUNCH(k_resid_rho, L.g.total, L.g, L.fg, L.b, L.x, L.r);
+  else
+    MG_LAUNCH(k_resid, L.g.total, L.g, L.T[0], L.T[1], L.T[2], L.b, L.x, L.r);
   MG_LAUNCH(k_restrict, C.g.total, L.g, C.g, L.r, C.b);
-  for (int k = 0; k < gamma; ++k)
-    TPF_TRY(cycle(m, st, l + 1, last, gamma, k == 0, nu_pre, nu_post, coarse_sweeps));
+  // gamma repeats apply on the top w_levels levels (all when w_levels <= 0)
+  const int g_here = (w_levels <= 0 || l < w_levels) ? gamma : 1;
+  for (int k = 0; k < g_here; ++k)
+    TPF_TRY(cycle(m, st, l + 1, last, gamma, k == 0, nu_pre, nu_post, coarse_sweeps, w_levels));
   MG_LAUNCH(k_prolong_add, L.g.total, L.g, C.g, C.x, L.x);
   return smooth(L, st, L.b, L.x, nu_post, true);
 }
@@ -542,6 +639,13 @@ extern "C" tpf_status tpf_mg_setup(tpf_ppe* s, int max_levels, int* levels) {
   hipStream_t st = s->user_stream;
   MgLevel& F = m->lv[0];
   MG_LAUNCH(k_fine_T, F.g.total, s->g, F.g, s->rho, F.T[0], F.T[1], F.T[2]);
+  if (!F.rho32) TPF_HIP(hipMalloc(&F.rho32, sizeof(float) * (size_t)F.g.total));
+  MG_LAUNCH(k_rho32, F.g.total, F.g.total, s->rho, F.rho32);
+  F.fg.rho = F.rho32;
+  for (int a = 0; a < 3; ++a) {
+    F.fg.h[a] = (float)s->g.h[a];
+    F.fg.ih2[a] = (float)(2.0 * s->g.ih[a]);
+  }
   for (int l = 0; l < m->nlev; ++l) {
     MgLevel& L = m->lv[l];
     if (l > 0) {
@@ -572,19 +676,21 @@ void mg_invalidate(tpf_ppe* s) {
 float* mg_fine_b(tpf_ppe* s) { return mg_of(s)->lv[0].b; }
 float* mg_fine_x(tpf_ppe* s) { return mg_of(s)->lv[0].x; }
 
-tpf_status mg_cycle(tpf_ppe* s, hipStream_t st, int nu, int gamma, int coarse_sweeps) {
+tpf_status mg_cycle(tpf_ppe* s, hipStream_t st, int nu, int gamma, int coarse_sweeps,
+                    int w_levels) {
   MgState* m = mg_of(s);
   if (!m || m->nlev == 0) return fail(TPF_ERR_RUNTIME, "multigrid hierarchy not built");
-  const int key[3] = {nu, gamma, coarse_sweeps};
+  const int key[4] = {nu, gamma, coarse_sweeps, w_levels};
   if (!(m->cyc && m->cyc_stream == st && key[0] == m->cyc_key[0] && key[1] == m->cyc_key[1] &&
-        key[2] == m->cyc_key[2])) {
+        key[2] == m->cyc_key[2] && key[3] == m->cyc_key[3])) {
     if (m->cyc) {
       TPF_HIP(hipGraphExecDestroy(m->cyc));
       m->cyc = nullptr;
     }
     hipGraph_t graph;
     TPF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    const tpf_status cs = cycle(m, st, 0, m->nlev - 1, gamma, true, nu, nu, coarse_sweeps);
+    const tpf_status cs = cycle(m, st, 0, m->nlev - 1, gamma, true, nu, nu, coarse_sweeps,
+                                w_levels);
     const hipError_t ec = hipStreamEndCapture(st, &graph);
     if (cs != TPF_OK) return cs;
     if (ec != hipSuccess) return fail(TPF_ERR_RUNTIME, std::string("capture: ") + hipGetErrorString(ec));
@@ -592,7 +698,7 @@ tpf_status mg_cycle(tpf_ppe* s, hipStream_t st, int nu, int gamma, int coarse_sw
     (void)hipGraphDestroy(graph);
     if (ei != hipSuccess)
       return fail(TPF_ERR_RUNTIME, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-    for (int i = 0; i < 3; ++i) m->cyc_key[i] = key[i];
+    for (int i = 0; i < 4; ++i) m->cyc_key[i] = key[i];
     m->cyc_stream = st;
   }
   TPF_HIP(hipGraphLaunch(m->cyc, st));

@@ -1074,7 +1074,7 @@ extern "C" tpf_status tpf_ppe_solve_mg(tpf_ppe* s, const double* rhs, double* x,
     }
     bool hit = rr <= tol2 * bb;
     while (!hit && iters < o->maxiter) {
-      TPF_TRY(mg_cycle(s, st, o->nu, o->gamma, o->coarse_sweeps));
+      TPF_TRY(mg_cycle(s, st, o->nu, o->gamma, o->coarse_sweeps, o->w_levels));
       hipLaunchKernelGGL(k_fcg_zq, grid, blk, 0, st, n, z32, (const double*)s->q, s->partials);
       hipLaunchKernelGGL(k_fcg_fin_beta, dim3(1), blk, 0, st, dst, (const double*)s->partials,
                          s->grid);

@@ -69,8 +69,10 @@ void mg_invalidate(tpf_ppe* s);                     // density changed: rebuild 
 float* mg_fine_b(tpf_ppe* s);                       // level-0 right-hand side (cycle input)
 float* mg_fine_x(tpf_ppe* s);                       // level-0 iterate (cycle output)
 // x_0 = M(b_0): one cycle on stream `st` (not the null stream), replayed from
-// a captured graph per (nu, gamma, coarse_sweeps, stream).
-tpf_status mg_cycle(tpf_ppe* s, hipStream_t st, int nu, int gamma, int coarse_sweeps);
+// a captured graph per (nu, gamma, coarse_sweeps, w_levels, stream); gamma
+// repeats apply on the top w_levels levels (all when w_levels <= 0).
+tpf_status mg_cycle(tpf_ppe* s, hipStream_t st, int nu, int gamma, int coarse_sweeps,
+                    int w_levels);
 }  // namespace tpf
 tpf_status tpf_ppe_solve_mixed(tpf_ppe* s, const double* rhs, double* x,
                                const tpf_solve_opts* opts, tpf_solve_report* rep);

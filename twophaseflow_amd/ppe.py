@@ -12,7 +12,8 @@ Mirrors the reference plugin interface (``ppe/interfaces.py:24-88``,
   ``||r|| <= tol ||b||`` stop (FP64 or FP32-inner/FP64-refined); or, with
   this build's ``ppe_preconditioner="mg"``, flexible CG preconditioned by one
   FP32 multigrid cycle per iteration (``ppe_mg_cycle`` "W" | "V",
-  ``ppe_mg_smooth``, ``ppe_mg_coarse_sweeps``), judged by the FP64 true
+  ``ppe_mg_smooth``, ``ppe_mg_coarse_sweeps``, ``ppe_mg_w_levels``: W
+  recursion on the top two levels by default), judged by the FP64 true
   residual.
 
 Both raise ``RuntimeError`` when they do not converge and return
@@ -129,6 +130,7 @@ class PPESolverFVMMatrixFree(IPPESolver):
         self.mg_gamma = 1 if cyc == "V" else 2
         self.mg_smooth = int(getattr(solver_cfg, "ppe_mg_smooth", 2))
         self.mg_coarse_sweeps = int(getattr(solver_cfg, "ppe_mg_coarse_sweeps", 20))
+        self.mg_w_levels = int(getattr(solver_cfg, "ppe_mg_w_levels", 2))
         self.inner_tol = float(getattr(solver_cfg, "ppe_inner_tol", 0.0))
         spec = bc_spec if bc_spec is not None else BoundarySpec(bc_type, grid.shape, grid.N)
         self._pin_dof = spec.pin_dof
@@ -203,7 +205,8 @@ class PPESolverFVMMatrixFree(IPPESolver):
         elif self.preconditioner == "mg":
             mopts = nat.MgOpts(tol=self.tol, maxiter=self.maxiter, use_x0=1 if use_x0 else 0,
                                nu=self.mg_smooth, gamma=self.mg_gamma,
-                               coarse_sweeps=self.mg_coarse_sweeps, max_levels=0)
+                               coarse_sweeps=self.mg_coarse_sweeps, max_levels=0,
+                               w_levels=self.mg_w_levels)
             status = self._lib.tpf_ppe_solve_mg(self._handle, ptr(rhs), ptr(p), C.byref(mopts),
                                                 C.byref(rep))
         else:
