@@ -35,6 +35,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "256^3 PPE time-to-1e-8: learned-FCG vs plain PCG; CCD Mpts/s & HBM %"
 HBM_PEAK_GBS = 8000.0
+# measured HBM bytes of one all-axes ccd_line_kernel<16> launch on 257^3:
+# FETCH_SIZE 239 MiB x 2 + WRITE_SIZE 887 MiB x 1
+TRAFFIC_BYTES_CCD = int((239_000 * 2 + 887_000) * 1024)
 
 
 def log(msg):
@@ -391,7 +394,11 @@ def main():
                    "grid": [N + 1] * 3, "axes": 3, "lines_per_axis": (N + 1) ** 2},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
+                     # recorded in profiles/ccd_line_kernel_257cubed_periodic_fp64.md
+                     # (scale factors 2 and 1 from a 1 GiB copy in the same runs)
+                     "traffic": TRAFFIC_BYTES_CCD,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/)",
                      "kernel": "ccd_line_kernel<16> (one grid, all 3 axes)",
                      "bytes_per_launch": bytes_per_launch, "launch_ms": round(dom_ms, 4)},
         "hbm_copy_GBps": round(copy_bandwidth_GBps(), 1),
