@@ -248,8 +248,8 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
   }
   const int off = periodic ? 0 : 1;  // unknown r -> node s0 + r + off
 
-  // boundary / image nodes (one per line)
-  if (active) {
+  // boundary / image nodes (one per line); a staged tile takes them in LDS below
+  if (active && !(M == 16 && TILED)) {
     if (periodic && p == 0) {
       if (d1) d1[base + (int64_t)g.N * st] = x[0][0] * g.inv_h;
       if (d2) d2[base + (int64_t)g.N * st] = x[0][1] * g.inv_h2;
@@ -279,6 +279,40 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
           if (d2) __builtin_nontemporal_store(x[r][1] * g.inv_h2, d2 + a);
         }
       }
+    }
+  } else if (M == 16 && TILED) {
+    // The workgroup's 64 lines are one contiguous, 128 B-aligned block of
+    // 64 x nax doubles: assemble each output component as that tile in LDS
+    // (interior rows, closure / image nodes) and store it linearly, so every
+    // store instruction writes whole aligned cache lines.
+    const int nax = (int)ln.inner_stride;
+    const int64_t rem = ln.n_lines - (int64_t)block * 64;
+    const int cnt = (int)(rem < 64 ? rem : 64) * nax;
+    __syncthreads();  // G no longer read; the tile space is reused
+#pragma unroll
+    for (int comp = 0; comp < 2; ++comp) {
+      double* dst = comp == 0 ? d1 : d2;
+      const double scale = comp == 0 ? g.inv_h : g.inv_h2;
+      if (dst == nullptr) continue;  // uniform
+      double* row = lds + lane * nax;
+#pragma unroll
+      for (int r = 0; r < M; ++r)
+        if (r < m) row[s0 + r + off] = x[r][comp] * scale;
+      if (periodic && p == 0) row[g.N] = x[0][comp] * scale;
+      if (wall_first)
+        row[0] = (comp == 0 ? g.Ml[0] * x[0][0] + g.Ml[1] * x[0][1] + bl0
+                            : g.Ml[2] * x[0][0] + g.Ml[3] * x[0][1] + bl1) * scale;
+      if (wall_last)
+        row[g.N] = (comp == 0 ? g.Mr[0] * xl0 + g.Mr[1] * xl1 + br0
+                              : g.Mr[2] * xl0 + g.Mr[3] * xl1 + br1) * scale;
+      __syncthreads();
+      double* out = dst + (int64_t)block * 64 * nax;
+#pragma unroll
+      for (int i = 0; i < kTilePer; ++i) {
+        const int e = threadIdx.x + i * (int)blockDim.x;
+        if (e < cnt) __builtin_nontemporal_store(lds[e], out + e);
+      }
+      __syncthreads();
     }
   } else {
     // Re-stage [64 lines][16 rows] per wavefront so a store instruction covers
