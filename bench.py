@@ -212,6 +212,50 @@ def mg_solve_timed(g, rho, rhs, gamma=2, net=None, tol=1e-8, reps=2):
     return out
 
 
+def ccd_batch_bench(reps=20):
+    """48 independent 2-D wall problems, 16 each at 129^2, 257^2 and 513^2
+    nodes (the uneven-batch sizes), all axes: one batched call (one grid per
+    launch shape) against one tpf_ccd_apply per problem."""
+    import numpy as np
+    import torch
+    from twophaseflow_amd import CCDBatch, CCDSolver, Grid, GridConfig
+    rng = np.random.default_rng(5)
+    solvers, fs, d1s, d2s = [], [], [], []
+    for N in (128, 256, 512):
+        for _ in range(16):
+            g = Grid(GridConfig(ndim=2, N=(N, N), L=(1.0, 1.0)))
+            solvers.append(CCDSolver(g, bc_type="wall"))
+            f = torch.from_numpy(rng.standard_normal((N + 1, N + 1))).cuda()
+            fs.append(f)
+            d1s.append(torch.empty((2,) + tuple(f.shape), dtype=torch.float64, device="cuda"))
+            d2s.append(torch.empty_like(d1s[-1]))
+    batch = CCDBatch(solvers)
+    pts = sum(2 * f.numel() for f in fs)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def loop():
+        for s_, f, a, b in zip(solvers, fs, d1s, d2s):
+            s_.apply_into(f, a, b, 0b11)
+
+    t_b = timed(lambda: batch.apply_into(fs, d1s, d2s))
+    t_l = timed(loop)
+    return {"problems": 48, "sizes": "16 x 129^2, 16 x 257^2, 16 x 513^2, wall, both axes",
+            "batched_ms": round(t_b, 4), "per_problem_launches_ms": round(t_l, 4),
+            "batched_Mpts_per_s": round(pts / (t_b * 1e-3) / 1e6, 1),
+            "algorithmic_GBps": round(40 * (pts / 2) / (t_b * 1e-3) / 1e9, 1),
+            "launches": {"batched": 3, "per_problem": 48}}
+
+
 def copy_bandwidth_GBps(reps=10):
     """Device-to-device copy of 1 GiB (read + write counted), event-timed."""
     import torch
@@ -406,6 +450,7 @@ def main():
         "ccd_axes": [{"axis": a["axis"], "ms": round(a["ms"], 4), "GBps": round(a["GBps"], 1)}
                      for a in axes],
         "ccd_2d_1025": {"Mpts_per_s": round(mpts2, 2), "ms_per_step": round(sec2 * 1e3, 4)},
+        "ccd_batch_uneven_2d": ccd_batch_bench(),
     }
 
     if not args.no_ppe:

@@ -72,6 +72,30 @@ TPF_API tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double* d1
 TPF_API tpf_status tpf_ccd_axis_info(tpf_ccd_plan* plan, int axis, int* partitions,
                              int* max_rows, int* window, double* trunc_coeff);
 
+/* Launch shape of one axis (no counterpart in the reference: the knobs the
+ * autotuner in twophaseflow_amd/autotune.py sets).  mode 0: automatic (a
+ * wavefront per line for < 4096 lines of > 512 unknowns, else a lane per
+ * line); 1: a lane per line (<= 16 partitions, rows cap 8/16/32/64);
+ * 2: a wavefront per line (<= 64 partitions, rows cap 8/16).  rows_cap 0
+ * picks the default for the mode.  Validated on the host; the axis' tables
+ * are rebuilt on next use. */
+TPF_API tpf_status tpf_ccd_plan_set_launch(tpf_ccd_plan* plan, int axis, int mode,
+                                   int rows_cap);
+
+/* Batches of independent CCD problems (no counterpart in the reference; the
+ * batching extension).  A batch fixes `count` plans and the axes to
+ * differentiate per plan; every (problem, axis) pair with the same launch
+ * shape (rows cap, partitions) goes into one grid whose workgroups look up
+ * their problem in a device table.  apply() takes per-problem device arrays
+ * f[j], d1[j], d2[j] (outputs stacked per selected axis as tpf_ccd_apply);
+ * asynchronous on `stream`; calls on one batch must share one stream. */
+typedef struct tpf_ccd_batch tpf_ccd_batch;
+TPF_API tpf_status tpf_ccd_batch_create(tpf_ccd_batch** batch, tpf_ccd_plan* const* plans,
+                                const uint32_t* axis_masks, int count);
+TPF_API tpf_status tpf_ccd_batch_apply(tpf_ccd_batch* batch, const double* const* f,
+                               double* const* d1, double* const* d2, void* stream);
+TPF_API tpf_status tpf_ccd_batch_destroy(tpf_ccd_batch* batch);
+
 /* ---------------------------------------------------------------- PPE --- */
 typedef struct tpf_ppe tpf_ppe;
 

@@ -91,3 +91,21 @@ def test_reduced_system_window_truncation_is_negligible():
     assert P.value == 16 and M.value == 16
     assert W.value < 16 and t.value < 1e-18
     nat.lib().tpf_ccd_plan_destroy(plan)
+
+
+def test_launch_mode_selection():
+    """Few long lines (2-D 1024^2) take the wavefront-per-line kernel (<= 64
+    partitions), many or shorter lines the lane-per-line kernel (<= 16)."""
+    plan = _plan((1025, 1025), (1, 0))
+    P, M, W, t = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    nat.check(nat.lib().tpf_ccd_axis_info(plan, 0, C.byref(P), C.byref(M), C.byref(W), C.byref(t)))
+    assert (P.value, M.value) == (64, 16)
+    nat.check(nat.lib().tpf_ccd_plan_set_launch(plan, 0, 1, 0))
+    nat.check(nat.lib().tpf_ccd_axis_info(plan, 0, C.byref(P), C.byref(M), C.byref(W), C.byref(t)))
+    assert (P.value, M.value) == (16, 64)
+    assert nat.lib().tpf_ccd_plan_set_launch(plan, 0, 2, 32) == nat.TPF_ERR_VALUE
+    nat.lib().tpf_ccd_plan_destroy(plan)
+    plan = _plan((257, 257, 257), (1, 1, 1))
+    nat.check(nat.lib().tpf_ccd_axis_info(plan, 2, C.byref(P), C.byref(M), C.byref(W), C.byref(t)))
+    assert (P.value, M.value) == (16, 16)
+    nat.lib().tpf_ccd_plan_destroy(plan)

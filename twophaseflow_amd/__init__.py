@@ -11,13 +11,13 @@ the stream only.  There is no CPU fallback.
 from .config import GridConfig, NumericsConfig, SimulationConfig, SolverConfig
 from .grid import Grid
 from .boundary import BoundarySpec, axis_kinds, pin_dof
-from .ccd import CCDSolver
+from .ccd import CCDBatch, CCDSolver
 from .ppe import (IPPESolver, MatrixAssemblyUnavailable, PPESolverFVMMatrixFree,
                   create_ppe_solver, register_ppe_solver)
 
 __all__ = [
     "GridConfig", "NumericsConfig", "SimulationConfig", "SolverConfig", "Grid",
-    "BoundarySpec", "axis_kinds", "pin_dof", "CCDSolver", "IPPESolver",
+    "BoundarySpec", "axis_kinds", "pin_dof", "CCDSolver", "CCDBatch", "IPPESolver",
     "MatrixAssemblyUnavailable", "PPESolverFVMMatrixFree", "create_ppe_solver",
     "register_ppe_solver",
 ]

@@ -67,6 +67,18 @@ class CCDSolver:
                 "trunc_coeff": t.value}
 
     # -- device-level entry points ---------------------------------------------
+    MODES = {"auto": 0, "lane": 1, "wave": 2}
+
+    def set_launch(self, axis: int, mode: str = "auto", rows_cap: int = 0) -> None:
+        """Launch shape of one axis: kernel mapping ("auto", "lane": a lane
+        per line, "wave": a wavefront per line) and rows-per-partition cap
+        (0 = default); the knobs the autotuner sets."""
+        self._check_axis(axis)
+        if mode not in self.MODES:
+            raise ValueError(f"mode must be one of {sorted(self.MODES)}")
+        nat.check(self._lib.tpf_ccd_plan_set_launch(self._plan, int(axis), self.MODES[mode],
+                                                     int(rows_cap)))
+
     def differentiate_into(self, f: torch.Tensor, d1, d2, axis: int) -> None:
         """Zero-copy form: f, d1, d2 are contiguous CUDA float64 tensors of
         grid.shape (d1 or d2 may be None).  Asynchronous on torch's stream."""
@@ -150,3 +162,53 @@ class CCDSolver:
         d2 = torch.empty_like(f) if want2 else None
         self.differentiate_into(f, d1, d2, axis)
         return (io.back(d1) if want1 else None, io.back(d2) if want2 else None)
+
+
+class CCDBatch:
+    """Many independent CCD problems (uneven grids allowed) differentiated in
+    one grid per launch shape (libtpf ``tpf_ccd_batch_*``).  No counterpart
+    in the reference: the batching extension.
+
+    ``solvers``: CCDSolver instances; ``axes``: per solver the axes to
+    differentiate (default all).  ``apply_into(fs, d1s, d2s)`` takes CUDA
+    FP64 tensors, with d1s[j] / d2s[j] of shape (len(axes[j]), *shape_j)."""
+
+    def __init__(self, solvers, axes=None):
+        self._lib = nat.lib()
+        self.solvers = list(solvers)
+        n = len(self.solvers)
+        if n == 0:
+            raise ValueError("CCDBatch needs at least one solver")
+        axes = axes if axes is not None else [tuple(range(s.ndim)) for s in self.solvers]
+        self.axes = [tuple(a) for a in axes]
+        masks = (C.c_uint32 * n)(*[sum(1 << a for a in ax) for ax in self.axes])
+        plans = (C.c_void_p * n)(*[s._plan.value for s in self.solvers])
+        h = C.c_void_p()
+        nat.check(self._lib.tpf_ccd_batch_create(C.byref(h), plans, masks, n))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.tpf_ccd_batch_destroy(h)
+            self._h = None
+
+    def apply_into(self, fs, d1s, d2s) -> None:
+        n = len(self.solvers)
+        if len(fs) != n or (d1s is not None and len(d1s) != n) or (d2s is not None and len(d2s) != n):
+            raise ValueError("CCDBatch.apply_into: one field and output pair per problem")
+        for j, (s, f) in enumerate(zip(self.solvers, fs)):
+            shape = tuple(int(x) + 1 for x in s.grid.N)
+            want = (len(self.axes[j]),) + shape
+            if (not f.is_cuda or f.dtype != torch.float64 or not f.is_contiguous()
+                    or tuple(f.shape) != shape):
+                raise ValueError(f"CCDBatch: field {j} must be a contiguous CUDA FP64 {shape}")
+            for o in ((d1s[j] if d1s is not None else None), (d2s[j] if d2s is not None else None)):
+                if o is not None and (not o.is_cuda or o.dtype != torch.float64
+                                      or not o.is_contiguous() or tuple(o.shape) != want):
+                    raise ValueError(f"CCDBatch: outputs of problem {j} must be CUDA FP64 {want}")
+        arr = C.c_void_p * n
+        f_p = arr(*[f.data_ptr() for f in fs])
+        d1_p = arr(*[(o.data_ptr() if o is not None else None) for o in d1s]) if d1s is not None else None
+        d2_p = arr(*[(o.data_ptr() if o is not None else None) for o in d2s]) if d2s is not None else None
+        nat.check(self._lib.tpf_ccd_batch_apply(self._h, f_p, d1_p, d2_p, stream_handle()))

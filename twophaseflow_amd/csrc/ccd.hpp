@@ -4,7 +4,8 @@
 // (ccd/ccd_solver_helpers.py:227-314 wall, :406-423 periodic), written in
 // the scaled unknowns (u, v) = (h f', h^2 f''), in which every block is
 // independent of h.  A line of n unknowns is cut into P partitions (one
-// wavefront each, P <= 16); each partition is solved locally with a
+// wavefront each and P <= 16 when a lane holds a line; one lane each and
+// P <= 64 when a wavefront holds a line); each partition is solved locally with a
 // precomputed unpivoted block LU, and the partitions are coupled through a
 // precomputed reduced (SPIKE) system whose inverse rows are stored per
 // partition.
@@ -14,7 +15,8 @@
 
 namespace tpf {
 
-constexpr int kMaxParts = 16;   // wavefronts per workgroup
+constexpr int kMaxParts = 16;      // partitions of the lane-per-line kernel (wavefronts per workgroup)
+constexpr int kMaxWaveParts = 64;  // partitions of the wavefront-per-line kernel (lanes)
 constexpr int kRowTab = 16;     // doubles per row: Lf(4) iD(4) V(4) W(4)
 
 // CCD coefficients, Table 1 of the paper (ccd/ccd_solver.py:70-76).
@@ -34,7 +36,7 @@ struct CcdAxisGeom {
   int P, M, NQ, d0;      // partitions, rows cap, coupling window, window start
   int rel;               // 1: window partition = p + d0 + t; 0: partition = t
   int periodic;
-  int s[kMaxParts + 1];  // partition starts (unknown index), s[P] = n
+  int s[kMaxWaveParts + 1];  // partition starts (unknown index), s[P] = n
   double inv_h, inv_h2;
   // wall closure in scaled unknowns (ccd/ccd_solver.py:408-456)
   double Lb[4], Ub[4], Ml[4], Mr[4];

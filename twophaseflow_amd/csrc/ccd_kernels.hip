@@ -346,6 +346,232 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
   }
 }
 
+// Wavefront-per-line mapping (few, long lines: 2-D grids): lane = partition
+// (P <= 64), wavefront = line, 16 lines per workgroup.  The workgroup's lines
+// are staged through LDS as one [16][nax] tile, loaded and stored with
+// coalesced 128 B accesses on either axis (16 consecutive lines share every
+// node's cache line on a strided axis; they are one contiguous block on the
+// contiguous axis).  Tile rows are padded by one double per 16 nodes so the
+// per-lane partition reads (16 nodes apart) spread over the LDS banks.  The
+// per-lane work is the lane-per-line solve above; tables come from L2; the
+// partition end values are exchanged through the (then dead) tile space.
+constexpr int kWaveLines = 16;
+
+__host__ __device__ inline int wave_pitch(int nax) { return nax + (nax >> 4) + 1; }
+__device__ inline int tix(int pitch, int l, int j) { return l * pitch + j + (j >> 4); }
+
+// Cooperative tile transfer between global lines [line0, line0 + nl) and LDS.
+template <bool LOAD>
+__device__ __forceinline__ void wave_tile_io(const CcdLines& ln, int nax, int pitch,
+                                             int64_t line0, int nl, const double* src,
+                                             double* dst, double* tile) {
+  const int cnt = nl * nax;
+  if (ln.stride == 1) {  // contiguous axis: one block of nl*nax doubles
+    const int64_t g0 = line0 * ln.inner_stride;
+    for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
+      const int l = e / nax, j = e - l * nax;
+      if (LOAD) tile[tix(pitch, l, j)] = src[g0 + e];
+      else __builtin_nontemporal_store(tile[tix(pitch, l, j)], dst + g0 + e);
+    }
+  } else {  // strided axis: consecutive threads take consecutive lines of one node
+    for (int e = threadIdx.x; e < kWaveLines * nax; e += blockDim.x) {
+      const int j = e / kWaveLines, l = e - j * kWaveLines;
+      if (l >= nl) continue;
+      const uint32_t bi = (uint32_t)(line0 + l), in = (uint32_t)ln.inner;
+      const int64_t a = (int64_t)(bi / in) * ln.outer + (int64_t)(bi % in) * ln.inner_stride +
+                        (int64_t)j * ln.stride;
+      if (LOAD) tile[tix(pitch, l, j)] = src[a];
+      else __builtin_nontemporal_store(tile[tix(pitch, l, j)], dst + a);
+    }
+  }
+}
+
+template <int M>
+__device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLines& ln,
+                                              const double* __restrict__ f,
+                                              double* __restrict__ d1, double* __restrict__ d2,
+                                              const unsigned block) {
+  extern __shared__ double lds[];
+  const int p = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int64_t line0 = (int64_t)block * kWaveLines;
+  const int64_t remain = ln.n_lines - line0;
+  const int nl = (int)(remain < kWaveLines ? remain : kWaveLines);
+  const int nax = g.N + 1;
+  const int pitch = wave_pitch(nax);
+  const bool active = w < nl && p < g.P;
+  const int pp = p < g.P ? p : g.P - 1;  // idle lanes mirror the last partition (no stores)
+  const int s0 = (int)((int64_t)g.n * pp / g.P);
+  const int m = (int)((int64_t)g.n * (pp + 1) / g.P) - s0;
+  const bool periodic = g.periodic != 0;
+  const bool wall_first = active && !periodic && p == 0;
+  const bool wall_last = active && !periodic && p == g.P - 1;
+
+  wave_tile_io<true>(ln, nax, pitch, line0, nl, f, nullptr, lds);
+  __syncthreads();
+  const int lw = w < nl ? w : 0;  // rows of inactive waves read line 0 (never stored)
+  double fv[M + 2];
+  const int j0 = periodic ? s0 - 1 : s0;
+#pragma unroll
+  for (int k = 0; k < M + 2; ++k) {
+    fv[k] = 0.0;
+    if (k < m + 2) {
+      int j = j0 + k;
+      if (periodic) j = j < 0 ? j + g.N : (j >= g.N ? j - g.N : j);
+      fv[k] = lds[tix(pitch, lw, j)];
+    }
+  }
+  double cl0 = 0.0, cl1 = 0.0, bl0 = 0.0, bl1 = 0.0;
+  if (wall_first) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) bl0 += g.cIl[k] * fv[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bl1 += g.cIIl[k] * fv[k];
+    cl0 = kAlpha1 * bl0 + kB1 * bl1;
+    cl1 = kB2 * bl0 + kBeta2 * bl1;
+  }
+  double cr0 = 0.0, cr1 = 0.0, br0 = 0.0, br1 = 0.0;
+  if (wall_last) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double v = lds[tix(pitch, lw, g.N - k)];
+      if (k < 4) br0 += g.cIr[k] * v;
+      br1 += g.cIIr[k] * v;
+    }
+    cr0 = kAlpha1 * br0 - kB1 * br1;
+    cr1 = -kB2 * br0 + kBeta2 * br1;
+  }
+  __syncthreads();  // tile consumed; its space holds the exchange buffer below
+  double* Gw = lds + w * 4 * 64;
+  // reduced-system rows of all partitions -> LDS after the exchange buffer,
+  // when they fit the tile space (the host sized the tile for it)
+  const int nq = g.P * 16 * g.NQ;
+  const bool qlds = kWaveLines * 4 * 64 + nq <= kWaveLines * wave_pitch(nax);
+  const double* qsrc = g.qtab;
+  if (qlds) {
+    double* qd = lds + kWaveLines * 4 * 64;
+    for (int i = threadIdx.x; i < nq; i += blockDim.x) qd[i] = g.qtab[i];
+    qsrc = qd;  // visible after the barrier that precedes the coupling
+  }
+  const double* tab = g.rowtab + (size_t)pp * M * kRowTab;
+  double x[M][2];
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    if (r < m) {
+      double r0 = kA1 * (fv[r + 2] - fv[r]);
+      double r1 = kA2 * ((fv[r] - 2.0 * fv[r + 1]) + fv[r + 2]);
+      if (r == 0) {
+        r0 -= cl0;
+        r1 -= cl1;
+      }
+      if (r == m - 1) {
+        r0 -= cr0;
+        r1 -= cr1;
+      }
+      if (r > 0) {
+        const double* t = tab + r * kRowTab;
+        r0 -= t[0] * x[r - 1][0] + t[1] * x[r - 1][1];
+        r1 -= t[2] * x[r - 1][0] + t[3] * x[r - 1][1];
+      }
+      x[r][0] = r0;
+      x[r][1] = r1;
+    }
+  }
+#pragma unroll
+  for (int r = M - 1; r >= 0; --r) {
+    if (r < m) {
+      double a = x[r][0], c = x[r][1];
+      if (r + 1 < M && r < m - 1) {
+        a -= kAlpha1 * x[r + 1][0] - kB1 * x[r + 1][1];
+        c -= -kB2 * x[r + 1][0] + kBeta2 * x[r + 1][1];
+      }
+      const double* t = tab + r * kRowTab + 4;
+      x[r][0] = t[0] * a + t[1] * c;
+      x[r][1] = t[2] * a + t[3] * c;
+      if (r == m - 1) {
+        Gw[2 * 64 + p] = x[r][0];
+        Gw[3 * 64 + p] = x[r][1];
+      }
+    }
+  }
+  Gw[0 * 64 + p] = x[0][0];
+  Gw[1 * 64 + p] = x[0][1];
+  __syncthreads();
+  double lr0 = 0.0, lr1 = 0.0, lr2 = 0.0, lr3 = 0.0;
+  {
+    const int NQ = g.NQ;
+    const double* q = qsrc + (size_t)pp * 4 * NQ * 4;
+    for (int t = 0; t < NQ; ++t) {
+      int qq = g.rel ? pp + g.d0 + t : t;
+      if (periodic) qq = qq < 0 ? qq + g.P : (qq >= g.P ? qq - g.P : qq);
+      else qq = qq < 0 ? 0 : (qq >= g.P ? g.P - 1 : qq);
+      const double G0 = Gw[0 * 64 + qq], G1 = Gw[1 * 64 + qq];
+      const double G2 = Gw[2 * 64 + qq], G3 = Gw[3 * 64 + qq];
+      const double* q0 = q + (0 * NQ + t) * 4;
+      const double* q1 = q + (1 * NQ + t) * 4;
+      const double* q2 = q + (2 * NQ + t) * 4;
+      const double* q3 = q + (3 * NQ + t) * 4;
+      lr0 += q0[0] * G0 + q0[1] * G1 + q0[2] * G2 + q0[3] * G3;
+      lr1 += q1[0] * G0 + q1[1] * G1 + q1[2] * G2 + q1[3] * G3;
+      lr2 += q2[0] * G0 + q2[1] * G1 + q2[2] * G2 + q2[3] * G3;
+      lr3 += q3[0] * G0 + q3[1] * G1 + q3[2] * G2 + q3[3] * G3;
+    }
+  }
+  double xl0 = 0.0, xl1 = 0.0;
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    if (r < m) {
+      const double* t = tab + r * kRowTab + 8;
+      const double u = x[r][0] - (t[0] * lr0 + t[1] * lr1 + t[4] * lr2 + t[5] * lr3);
+      const double v = x[r][1] - (t[2] * lr0 + t[3] * lr1 + t[6] * lr2 + t[7] * lr3);
+      x[r][0] = u;
+      x[r][1] = v;
+      if (r == m - 1) {
+        xl0 = u;
+        xl1 = v;
+      }
+    }
+  }
+  const int off = periodic ? 0 : 1;
+  __syncthreads();  // exchange buffer dead; the tile space takes the outputs
+#pragma unroll
+  for (int comp = 0; comp < 2; ++comp) {
+    double* dst = comp == 0 ? d1 : d2;
+    const double scale = comp == 0 ? g.inv_h : g.inv_h2;
+    if (dst == nullptr) continue;  // uniform
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < M; ++r)
+        if (r < m) lds[tix(pitch, w, s0 + r + off)] = x[r][comp] * scale;
+      if (periodic && p == 0) lds[tix(pitch, w, g.N)] = x[0][comp] * scale;
+      if (wall_first)
+        lds[tix(pitch, w, 0)] = (comp == 0 ? g.Ml[0] * x[0][0] + g.Ml[1] * x[0][1] + bl0
+                                           : g.Ml[2] * x[0][0] + g.Ml[3] * x[0][1] + bl1) * scale;
+      if (wall_last)
+        lds[tix(pitch, w, g.N)] = (comp == 0 ? g.Mr[0] * xl0 + g.Mr[1] * xl1 + br0
+                                             : g.Mr[2] * xl0 + g.Mr[3] * xl1 + br1) * scale;
+    }
+    __syncthreads();
+    wave_tile_io<false>(ln, nax, pitch, line0, nl, nullptr, dst, lds);
+    __syncthreads();
+  }
+}
+
+template <int M>
+__global__ __launch_bounds__(64 * kWaveLines) void ccd_wave_kernel(CcdLaunch L) {
+  const unsigned blk = blockIdx.x;
+  const int sl = (L.nslots > 1 && blk >= L.block_begin[1]) + (L.nslots > 2 && blk >= L.block_begin[2]);
+  const CcdSlot& S = L.slot[sl];
+  ccd_wave_tile<M>(S.g, S.ln, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
+}
+
+// LDS of the wavefront kernel: the [16][pitch] tile (it also holds the 16 KB
+// exchange buffer once the inputs are read).
+size_t wave_lds_bytes(const CcdAxisGeom& g) {
+  const size_t tile = sizeof(double) * kWaveLines * (size_t)wave_pitch(g.N + 1);
+  return std::max(tile, sizeof(double) * kWaveLines * 4 * 64);
+}
+
 template <int M>
 __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdLaunch L) {
   // Block order = slot order (axis 0 lines first).  An XCD-contiguous
@@ -356,7 +582,68 @@ __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdLaunch L) {
   ccd_lines<M>(S.g, S.ln, S.contig != 0, S.tiled != 0, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
 }
 
+// Batched launch over many (problem, axis) slots kept in device memory:
+// workgroups [begin[s], begin[s+1]) belong to slot s.  All slots share the
+// rows cap M and the partition count (one block size per launch).
+template <int M>
+__global__ __launch_bounds__(1024) void ccd_batch_kernel(const CcdSlot* __restrict__ slots,
+                                                         const unsigned* __restrict__ begin,
+                                                         int nslots) {
+  const unsigned blk = blockIdx.x;
+  int lo = 0, hi = nslots - 1;  // largest s with begin[s] <= blk (uniform)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (begin[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  const CcdSlot& S = slots[lo];
+  ccd_lines<M>(S.g, S.ln, S.contig != 0, S.tiled != 0, S.f, S.d1, S.d2, blk - begin[lo]);
+}
+
+template <int M>
+__global__ __launch_bounds__(64 * kWaveLines) void ccd_wave_batch_kernel(
+    const CcdSlot* __restrict__ slots, const unsigned* __restrict__ begin, int nslots) {
+  const unsigned blk = blockIdx.x;
+  int lo = 0, hi = nslots - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (begin[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  const CcdSlot& S = slots[lo];
+  ccd_wave_tile<M>(S.g, S.ln, S.f, S.d1, S.d2, blk - begin[lo]);
+}
+
+using BatchFn = void (*)(const CcdSlot*, const unsigned*, int);
+using WaveFn = void (*)(CcdLaunch);
+
+BatchFn pick_wave_batch_kernel(int M) {
+  switch (M) {
+    case 8: return ccd_wave_batch_kernel<8>;
+    case 16: return ccd_wave_batch_kernel<16>;
+  }
+  return nullptr;
+}
+
+BatchFn pick_batch_kernel(int M) {
+  switch (M) {
+    case 8: return ccd_batch_kernel<8>;
+    case 16: return ccd_batch_kernel<16>;
+    case 32: return ccd_batch_kernel<32>;
+    case 64: return ccd_batch_kernel<64>;
+  }
+  return nullptr;
+}
+
 using KernelFn = void (*)(CcdLaunch);
+
+WaveFn pick_wave_kernel(int M) {
+  switch (M) {
+    case 8: return ccd_wave_kernel<8>;
+    case 16: return ccd_wave_kernel<16>;
+  }
+  return nullptr;
+}
 
 KernelFn pick_kernel(int M) {
   switch (M) {
@@ -447,27 +734,40 @@ tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const dou
   size_t shmem = 0;
   unsigned blocks = 0;
   int M = -1;
+  const bool wave = plan->axes[axes[0]].wave != 0;
   for (int i = 0; i < naxes; ++i) {
     CcdAxisHost& ax = plan->axes[axes[i]];
     CcdSlot& S = L.slot[i];
     S.g = ax.geom;
     S.ln = lines_for(plan, axes[i]);
     S.contig = S.ln.stride == 1;
-    S.tiled = S.contig && tile_fits(ax.geom, plan->n_nodes[axes[i]]);
+    S.tiled = !wave && S.contig && tile_fits(ax.geom, plan->n_nodes[axes[i]]);
     S.f = f;
     S.d1 = d1[i];
     S.d2 = d2[i];
     if (M < 0) M = ax.geom.M;
-    if (ax.geom.M != M || ax.geom.P != plan->axes[axes[0]].geom.P)
-      return fail(TPF_ERR_VALUE, "fused CCD launch needs one rows cap and partition count");
-    const size_t sh = lds_bytes(ax.geom, S.contig, plan->n_nodes[axes[i]]);
+    if (ax.geom.M != M || ax.geom.P != plan->axes[axes[0]].geom.P ||
+        ax.wave != plan->axes[axes[0]].wave)
+      return fail(TPF_ERR_VALUE, "fused CCD launch needs one kernel, rows cap and partition count");
+    const size_t sh = wave ? 0 : lds_bytes(ax.geom, S.contig, plan->n_nodes[axes[i]]);
     shmem = sh > shmem ? sh : shmem;
     if (S.ln.n_lines > (int64_t)0x3fffffff)
       return fail(TPF_ERR_VALUE, "too many CCD lines for one launch");
     L.block_begin[i] = blocks;
-    blocks += (unsigned)((S.ln.n_lines + 63) / 64);
+    const int64_t per = wave ? kWaveLines : 64;
+    blocks += (unsigned)((S.ln.n_lines + per - 1) / per);
   }
   L.block_begin[naxes] = blocks;
+  if (wave) {
+    WaveFn k = pick_wave_kernel(M);
+    if (!k) return fail(TPF_ERR_VALUE, "unsupported CCD rows cap " + std::to_string(M));
+    size_t sh = 0;
+    for (int i = 0; i < naxes; ++i) sh = std::max(sh, wave_lds_bytes(L.slot[i].g));
+    TPF_TRY(raise_lds_limit((const void*)k, sh));
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * kWaveLines), sh, stream, L);
+    TPF_HIP(hipGetLastError());
+    return TPF_OK;
+  }
   KernelFn k = pick_kernel(M);
   if (!k) return fail(TPF_ERR_VALUE, "unsupported CCD rows cap " + std::to_string(M));
   TPF_TRY(raise_lds_limit((const void*)k, shmem));
@@ -509,11 +809,47 @@ extern "C" tpf_status tpf_ccd_plan_create(tpf_ccd_plan** out, int ndim, const in
 namespace tpf {
 // Lazily builds and uploads the tables of one axis (the reference builds
 // wall solvers lazily per used axis, ccd_solver.py:108-111, 294-299).
+// Kernel of one axis: a wavefront per line when the lines are few and long
+// (auto: < 4096 lines of > 512 unknowns, e.g. 2-D 1024^2 grids, where the
+// lane-per-line kernel would need 64 rows per partition), else a lane per
+// line.  The wavefront kernel uses 8 rows per partition when the line fits
+// 64 partitions, else 16.
+tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
+  const int n_ax = (int)plan->n_nodes[axis];
+  const bool periodic = plan->bc[axis] == TPF_BC_PERIODIC;
+  const int n = periodic ? n_ax - 1 : n_ax - 2;
+  int64_t total = 1;
+  for (int a = 0; a < plan->ndim; ++a) total *= plan->n_nodes[a];
+  const int64_t lines = total / n_ax;
+  const int mode = plan->mode[axis];
+  const int cap = plan->rows_cap[axis];
+  bool wave = mode == 2 || (mode == 0 && lines < 4096 && n > 512);
+  if (wave && cap != 0 && cap != 8 && cap != 16)
+    return fail(TPF_ERR_VALUE, "the wavefront-per-line CCD kernel takes rows cap 8 or 16");
+  if (wave && n >= 1) {
+    const int M = cap > 0 ? cap : ((n + 7) / 8 <= kMaxWaveParts ? 8 : 16);
+    const int P = (n + M - 1) / M;
+    if (P > kMaxWaveParts) {
+      if (mode == 2) return fail(TPF_ERR_VALUE, "line too long for 64 partitions of the rows cap");
+      wave = false;
+    } else {
+      TPF_TRY(build_axis_tables(ax, n_ax, plan->h[axis], periodic, M, P));
+    }
+  }
+  if (!wave) {
+    TPF_TRY(build_axis_tables(ax, n_ax, plan->h[axis], periodic, cap, 0));
+    if (ax.geom.P > kMaxParts)
+      return fail(TPF_ERR_VALUE, "rows cap " + std::to_string(cap) +
+                                     " needs more than 16 partitions for the lane-per-line kernel");
+  }
+  ax.wave = wave ? 1 : 0;
+  return TPF_OK;
+}
+
 tpf_status ensure_axis(tpf_ccd_plan* plan, int axis) {
   CcdAxisHost& ax = plan->axes[axis];
   if (ax.d_rowtab) return TPF_OK;
-  TPF_TRY(build_axis_tables(ax, (int)plan->n_nodes[axis], plan->h[axis],
-                            plan->bc[axis] == TPF_BC_PERIODIC, 0, 0));
+  TPF_TRY(build_axis(plan, axis, ax));
   TPF_HIP(hipMalloc(&ax.d_rowtab, ax.rowtab.size() * sizeof(double)));
   TPF_HIP(hipMalloc(&ax.d_qtab, ax.qtab.size() * sizeof(double)));
   TPF_HIP(hipMemcpy(ax.d_rowtab, ax.rowtab.data(), ax.rowtab.size() * sizeof(double),
@@ -568,7 +904,8 @@ extern "C" tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double*
   bool same = true;  // one grid when every axis has the same launch shape
   for (int i = 1; i < n; ++i)
     same = same && plan->axes[axes[i]].geom.M == plan->axes[axes[0]].geom.M &&
-           plan->axes[axes[i]].geom.P == plan->axes[axes[0]].geom.P;
+           plan->axes[axes[i]].geom.P == plan->axes[axes[0]].geom.P &&
+           plan->axes[axes[i]].wave == plan->axes[axes[0]].wave;
   if (same) return launch_axes(plan, axes, n, f, o1, o2, (hipStream_t)stream);
   for (int i = 0; i < n; ++i)
     TPF_TRY(launch_axes(plan, &axes[i], 1, f, &o1[i], &o2[i], (hipStream_t)stream));
@@ -579,9 +916,7 @@ extern "C" tpf_status tpf_ccd_axis_info(tpf_ccd_plan* plan, int axis, int* parti
                                         int* max_rows, int* window, double* trunc_coeff) {
   if (!plan || axis < 0 || axis >= plan->ndim) return fail(TPF_ERR_VALUE, "bad plan/axis");
   CcdAxisHost& ax = plan->axes[axis];
-  if (ax.rowtab.empty())
-    TPF_TRY(build_axis_tables(ax, (int)plan->n_nodes[axis], plan->h[axis],
-                              plan->bc[axis] == TPF_BC_PERIODIC, 0, 0));
+  if (ax.rowtab.empty()) TPF_TRY(build_axis(plan, axis, ax));
   if (partitions) *partitions = ax.geom.P;
   if (max_rows) *max_rows = ax.geom.M;
   if (window) *window = ax.geom.NQ;
@@ -595,13 +930,199 @@ extern "C" tpf_status tpf_test_ccd_host_model(tpf_ccd_plan* plan, const double* 
                                               double* d2, int axis) {
   if (!plan || !f || axis < 0 || axis >= plan->ndim) return fail(TPF_ERR_VALUE, "bad args");
   CcdAxisHost& ax = plan->axes[axis];
-  if (ax.rowtab.empty())
-    TPF_TRY(build_axis_tables(ax, (int)plan->n_nodes[axis], plan->h[axis],
-                              plan->bc[axis] == TPF_BC_PERIODIC, 0, 0));
+  if (ax.rowtab.empty()) TPF_TRY(build_axis(plan, axis, ax));
   const CcdLines ln = lines_for(plan, axis);
   for (int64_t b = 0; b < ln.n_lines; ++b) {
     const int64_t base = (b / ln.inner) * ln.outer + (b % ln.inner) * ln.inner_stride;
     host_model_line(ax, f + base, ln.stride, d1 ? d1 + base : nullptr, d2 ? d2 + base : nullptr);
+  }
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ccd_plan_set_launch(tpf_ccd_plan* plan, int axis, int mode,
+                                              int rows_cap) {
+  if (!plan || axis < 0 || axis >= plan->ndim) return fail(TPF_ERR_VALUE, "bad plan/axis");
+  if (mode < 0 || mode > 2) return fail(TPF_ERR_VALUE, "mode must be 0 (auto), 1 or 2");
+  if (rows_cap != 0 && rows_cap != 8 && rows_cap != 16 && rows_cap != 32 && rows_cap != 64)
+    return fail(TPF_ERR_VALUE, "rows cap must be 0 (default), 8, 16, 32 or 64");
+  // validate on the host before dropping the current tables
+  const int old_mode = plan->mode[axis], old_cap = plan->rows_cap[axis];
+  plan->mode[axis] = mode;
+  plan->rows_cap[axis] = rows_cap;
+  CcdAxisHost trial;
+  const tpf_status st = build_axis(plan, axis, trial);
+  if (st != TPF_OK) {
+    plan->mode[axis] = old_mode;
+    plan->rows_cap[axis] = old_cap;
+    return st;
+  }
+  CcdAxisHost& ax = plan->axes[axis];
+  if (ax.d_rowtab) TPF_HIP(hipFree(ax.d_rowtab));
+  if (ax.d_qtab) TPF_HIP(hipFree(ax.d_qtab));
+  ax = CcdAxisHost();
+  return TPF_OK;
+}
+
+// ---- batches of independent problems ---------------------------------------
+struct tpf_ccd_batch {
+  struct Group {
+    int M, P, wave;
+    int first, count;  // slots [first, first + count)
+    unsigned blocks;
+    size_t shmem;
+  };
+  int nproblems = 0;
+  std::vector<tpf::CcdSlot> slots;      // geometry; pointers filled per call
+  std::vector<int> slot_problem, slot_k;
+  std::vector<int64_t> totals;          // nodes per problem
+  std::vector<unsigned> begin;          // per group: count + 1 entries
+  std::vector<int> begin_off;           // group -> offset into begin
+  std::vector<Group> groups;
+  tpf::CcdSlot* d_slots = nullptr;
+  unsigned* d_begin = nullptr;
+  tpf::CcdSlot* h_slots = nullptr;      // pinned staging
+  hipEvent_t staged = nullptr;          // the last upload of h_slots
+};
+
+extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* const* plans,
+                                           const uint32_t* axis_masks, int count) {
+  if (!out || !plans || !axis_masks || count < 1) return fail(TPF_ERR_VALUE, "bad batch arguments");
+  *out = nullptr;
+  auto* b = new tpf_ccd_batch();
+  b->nproblems = count;
+  struct Item {
+    int problem, k, axis;
+  };
+  std::vector<Item> items;
+  for (int j = 0; j < count; ++j) {
+    tpf_ccd_plan* pl = plans[j];
+    if (!pl || (axis_masks[j] >> pl->ndim)) {
+      delete b;
+      return fail(TPF_ERR_VALUE, "batch entry " + std::to_string(j) + ": bad plan or axis mask");
+    }
+    int64_t total = 1;
+    for (int a = 0; a < pl->ndim; ++a) total *= pl->n_nodes[a];
+    b->totals.push_back(total);
+    int k = 0;
+    for (int a = 0; a < pl->ndim; ++a) {
+      if (!(axis_masks[j] & (1u << a))) continue;
+      const tpf_status st = ensure_axis(pl, a);
+      if (st != TPF_OK) {
+        delete b;
+        return st;
+      }
+      items.push_back({j, k++, a});
+    }
+  }
+  // group by launch shape (M, P), keeping the input order inside a group
+  struct Shape {
+    int wave, M, P;
+    bool operator==(const Shape& o) const { return wave == o.wave && M == o.M && P == o.P; }
+  };
+  std::vector<Shape> shapes;
+  for (const Item& it : items) {
+    const CcdAxisHost& ax = plans[it.problem]->axes[it.axis];
+    const Shape sh{ax.wave, ax.geom.M, ax.geom.P};
+    if (std::find(shapes.begin(), shapes.end(), sh) == shapes.end()) shapes.push_back(sh);
+  }
+  for (const auto& sh : shapes) {
+    tpf_ccd_batch::Group G{sh.M, sh.P, sh.wave, (int)b->slots.size(), 0, 0, 0};
+    b->begin_off.push_back((int)b->begin.size());
+    unsigned blocks = 0;
+    for (const Item& it : items) {
+      tpf_ccd_plan* pl = plans[it.problem];
+      const CcdAxisHost& ax = pl->axes[it.axis];
+      const CcdAxisGeom& g = ax.geom;
+      if (!(Shape{ax.wave, g.M, g.P} == sh)) continue;
+      CcdSlot S{};
+      S.g = g;
+      S.ln = lines_for(pl, it.axis);
+      S.contig = S.ln.stride == 1;
+      S.tiled = !sh.wave && S.contig && tile_fits(g, pl->n_nodes[it.axis]);
+      b->slots.push_back(S);
+      b->slot_problem.push_back(it.problem);
+      b->slot_k.push_back(it.k);
+      b->begin.push_back(blocks);
+      const int64_t per = sh.wave ? kWaveLines : 64;
+      blocks += (unsigned)((S.ln.n_lines + per - 1) / per);
+      if (!sh.wave) G.shmem = std::max(G.shmem, lds_bytes(g, S.contig, pl->n_nodes[it.axis]));
+      else G.shmem = std::max(G.shmem, wave_lds_bytes(g));
+      G.count += 1;
+    }
+    b->begin.push_back(blocks);
+    G.blocks = blocks;
+    b->groups.push_back(G);
+  }
+  const size_t ns = b->slots.size();
+  hipError_t e = hipMalloc(&b->d_slots, sizeof(CcdSlot) * ns);
+  if (e == hipSuccess) e = hipMalloc(&b->d_begin, sizeof(unsigned) * b->begin.size());
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_slots, sizeof(CcdSlot) * ns);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_begin, b->begin.data(), sizeof(unsigned) * b->begin.size(),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    tpf_ccd_batch_destroy(b);
+    return fail(TPF_ERR_RUNTIME, std::string("batch allocation: ") + hipGetErrorString(e));
+  }
+  std::copy(b->slots.begin(), b->slots.end(), b->h_slots);
+  for (const auto& G : b->groups) {
+    const void* k = G.wave ? (const void*)pick_wave_batch_kernel(G.M)
+                           : (const void*)pick_batch_kernel(G.M);
+    if (G.shmem == 0) continue;
+    const tpf_status st = raise_lds_limit(k, G.shmem);
+    if (st != TPF_OK) {
+      tpf_ccd_batch_destroy(b);
+      return st;
+    }
+  }
+  *out = b;
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ccd_batch_destroy(tpf_ccd_batch* b) {
+  if (!b) return TPF_OK;
+  if (b->staged) {
+    (void)hipEventSynchronize(b->staged);
+    (void)hipEventDestroy(b->staged);
+  }
+  if (b->d_slots) (void)hipFree(b->d_slots);
+  if (b->d_begin) (void)hipFree(b->d_begin);
+  if (b->h_slots) (void)hipHostFree(b->h_slots);
+  delete b;
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ccd_batch_apply(tpf_ccd_batch* b, const double* const* f,
+                                          double* const* d1, double* const* d2, void* stream) {
+  if (!b || !f || (!d1 && !d2)) return fail(TPF_ERR_VALUE, "null batch argument");
+  hipStream_t st = (hipStream_t)stream;
+  TPF_HIP(hipEventSynchronize(b->staged));  // the previous upload has left h_slots
+  for (size_t i = 0; i < b->slots.size(); ++i) {
+    const int j = b->slot_problem[i], k = b->slot_k[i];
+    if (!f[j]) return fail(TPF_ERR_VALUE, "batch entry " + std::to_string(j) + ": null input");
+    double* o1 = (d1 && d1[j]) ? d1[j] + (int64_t)k * b->totals[j] : nullptr;
+    double* o2 = (d2 && d2[j]) ? d2[j] + (int64_t)k * b->totals[j] : nullptr;
+    if ((const double*)o1 == f[j] || (const double*)o2 == f[j])
+      return fail(TPF_ERR_VALUE, "CCD outputs must not alias the input");
+    b->h_slots[i].f = f[j];
+    b->h_slots[i].d1 = o1;
+    b->h_slots[i].d2 = o2;
+  }
+  TPF_HIP(hipMemcpyAsync(b->d_slots, b->h_slots, sizeof(CcdSlot) * b->slots.size(),
+                         hipMemcpyHostToDevice, st));
+  TPF_HIP(hipEventRecord(b->staged, st));
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    const auto& G = b->groups[gi];
+    if (G.wave)
+      hipLaunchKernelGGL(pick_wave_batch_kernel(G.M), dim3(G.blocks), dim3(64 * kWaveLines),
+                         G.shmem, st, (const CcdSlot*)(b->d_slots + G.first),
+                         (const unsigned*)(b->d_begin + b->begin_off[gi]), G.count);
+    else
+      hipLaunchKernelGGL(pick_batch_kernel(G.M), dim3(G.blocks), dim3(64 * G.P), G.shmem, st,
+                         (const CcdSlot*)(b->d_slots + G.first),
+                         (const unsigned*)(b->d_begin + b->begin_off[gi]), G.count);
+    TPF_HIP(hipGetLastError());
   }
   return TPF_OK;
 }

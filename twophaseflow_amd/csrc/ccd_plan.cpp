@@ -107,7 +107,7 @@ tpf_status build_axis_tables(CcdAxisHost& ax, int n_nodes, double h, int periodi
   int M = rows_cap > 0 ? rows_cap : choose_rows_cap(n);
   if (M <= 0) return fail(TPF_ERR_VALUE, "CCD line too long for one workgroup: n=" + std::to_string(n));
   int P = parts_override > 0 ? parts_override : (n + M - 1) / M;
-  if (P > kMaxParts || (n + P - 1) / P > M)
+  if (P > kMaxWaveParts || (n + P - 1) / P > M)
     return fail(TPF_ERR_VALUE, "CCD partitioning does not fit: n=" + std::to_string(n) +
                                    " P=" + std::to_string(P) + " M=" + std::to_string(M));
   const int min_rows = periodic ? 2 : 4;

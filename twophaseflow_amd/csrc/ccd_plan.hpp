@@ -9,6 +9,7 @@
 namespace tpf {
 
 struct CcdAxisHost {
+  int wave = 0;                 // 1: one wavefront per line (lane = partition)
   CcdAxisGeom geom{};           // device pointers filled after upload
   std::vector<double> rowtab;   // [P][M][kRowTab]
   std::vector<double> qtab;     // [P][4][NQ*4]
@@ -38,5 +39,7 @@ struct tpf_ccd_plan {
   double h[3] = {1, 1, 1};
   int bc[3] = {0, 0, 0};
   int precision = 0;
+  int rows_cap[3] = {0, 0, 0};  // per-axis rows-per-partition override (0: default)
+  int mode[3] = {0, 0, 0};      // per-axis kernel: 0 auto, 1 lane per line, 2 wavefront per line
   tpf::CcdAxisHost axes[3];
 };
