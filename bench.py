@@ -407,7 +407,17 @@ def main():
     bytes_per_launch = 8 * (1 + 2 * 3) * npts
     achieved = bytes_per_launch / (dom_ms * 1e-3) / 1e9
     all_axes_gbs = 8.0 * 7 * npts / sec / 1e9
-    del d1, d2
+    log("timing 3-D CCD, FP32 solves + FP64 refinement")
+    ccd_r = CCDSolver(g, bc_type="periodic", precision="fp32_refined")
+    r1 = torch.empty_like(d1)
+    r2 = torch.empty_like(d2)
+    sec_r = time_ccd(ccd_r, f, r1, r2, 0b111, args.steps, args.warmup, 3)
+    refined = {"Mpts_per_s": round(3 * npts / sec_r / 1e6, 2),
+               "ms_per_step": round(sec_r * 1e3, 4),
+               "max_rel_err_vs_fp64": {
+                   "d1": float((r1 - d1).abs().max() / d1.abs().max()),
+                   "d2": float((r2 - d2).abs().max() / d2.abs().max())}}
+    del d1, d2, r1, r2
 
     # ---- CCD 2-D 1025^2 periodic, all axes
     log("timing 2-D CCD")
@@ -450,6 +460,7 @@ def main():
         "ccd_axes": [{"axis": a["axis"], "ms": round(a["ms"], 4), "GBps": round(a["GBps"], 1)}
                      for a in axes],
         "ccd_2d_1025": {"Mpts_per_s": round(mpts2, 2), "ms_per_step": round(sec2 * 1e3, 4)},
+        "ccd_fp32_refined_257": refined,
         "ccd_batch_uneven_2d": ccd_batch_bench(),
     }
 

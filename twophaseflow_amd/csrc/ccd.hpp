@@ -31,6 +31,8 @@ constexpr double kB2 = -9.0 / 8.0;
 struct CcdAxisGeom {
   const double* rowtab;  // [P][M][kRowTab]
   const double* qtab;    // [P][4][NQ*4]: rows (left0,left1,right0,right1)
+  const float* rowtab32; // FP32 copies (FP32-refined plans only)
+  const float* qtab32;
   int n;                 // unknowns per line (N periodic, N-1 wall)
   int N;                 // cells along the axis
   int P, M, NQ, d0;      // partitions, rows cap, coupling window, window start

@@ -346,6 +346,351 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
   }
 }
 
+// ---- FP32 solve + FP64 refinement (lane-per-line mapping, M <= 16) --------
+// One partition solve in FP32 on the RHS held in v: block-LU sweeps with the
+// FP32 tables, partition ends -> G ([4P][64]), barrier, coupling through
+// the reduced-system rows, spike correction.  Every thread of the block
+// calls it (it contains a barrier).
+template <int M>
+__device__ __forceinline__ void solve32(const CcdAxisGeom& g, const float* __restrict__ tab,
+                                        const float* __restrict__ qt, float (&v)[M][2], int m,
+                                        int p, int lane, bool periodic, float* G) {
+#pragma unroll
+  for (int r = 1; r < M; ++r) {
+    if (r < m) {
+      const float* t = tab + r * kRowTab;
+      v[r][0] -= t[0] * v[r - 1][0] + t[1] * v[r - 1][1];
+      v[r][1] -= t[2] * v[r - 1][0] + t[3] * v[r - 1][1];
+    }
+  }
+#pragma unroll
+  for (int r = M - 1; r >= 0; --r) {
+    if (r < m) {
+      float a = v[r][0], c = v[r][1];
+      if (r + 1 < M && r < m - 1) {
+        a -= (float)kAlpha1 * v[r + 1][0] - (float)kB1 * v[r + 1][1];
+        c -= -(float)kB2 * v[r + 1][0] + (float)kBeta2 * v[r + 1][1];
+      }
+      const float* t = tab + r * kRowTab + 4;
+      v[r][0] = t[0] * a + t[1] * c;
+      v[r][1] = t[2] * a + t[3] * c;
+      if (r == m - 1) {
+        G[(p * 4 + 2) * 64 + lane] = v[r][0];
+        G[(p * 4 + 3) * 64 + lane] = v[r][1];
+      }
+    }
+  }
+  G[(p * 4 + 0) * 64 + lane] = v[0][0];
+  G[(p * 4 + 1) * 64 + lane] = v[0][1];
+  __syncthreads();
+  float lr0 = 0.f, lr1 = 0.f, lr2 = 0.f, lr3 = 0.f;
+  const int NQ = g.NQ;
+  const float* q = qt + (size_t)p * 4 * NQ * 4;
+  for (int t = 0; t < NQ; ++t) {
+    int qq = g.rel ? p + g.d0 + t : t;
+    if (periodic) qq = qq < 0 ? qq + g.P : (qq >= g.P ? qq - g.P : qq);
+    else qq = qq < 0 ? 0 : (qq >= g.P ? g.P - 1 : qq);
+    const float G0 = G[(qq * 4 + 0) * 64 + lane], G1 = G[(qq * 4 + 1) * 64 + lane];
+    const float G2 = G[(qq * 4 + 2) * 64 + lane], G3 = G[(qq * 4 + 3) * 64 + lane];
+    const float* q0 = q + (0 * NQ + t) * 4;
+    const float* q1 = q + (1 * NQ + t) * 4;
+    const float* q2 = q + (2 * NQ + t) * 4;
+    const float* q3 = q + (3 * NQ + t) * 4;
+    lr0 += q0[0] * G0 + q0[1] * G1 + q0[2] * G2 + q0[3] * G3;
+    lr1 += q1[0] * G0 + q1[1] * G1 + q1[2] * G2 + q1[3] * G3;
+    lr2 += q2[0] * G0 + q2[1] * G1 + q2[2] * G2 + q2[3] * G3;
+    lr3 += q3[0] * G0 + q3[1] * G1 + q3[2] * G2 + q3[3] * G3;
+  }
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    if (r < m) {
+      const float* t = tab + r * kRowTab + 8;
+      v[r][0] -= t[0] * lr0 + t[1] * lr1 + t[4] * lr2 + t[5] * lr3;
+      v[r][1] -= t[2] * lr0 + t[3] * lr1 + t[6] * lr2 + t[7] * lr3;
+    }
+  }
+}
+
+// FP32-refined variant of ccd_lines: the FP64 right-hand side is solved in
+// FP32 (x1), the residual of the exact block system (interior rows
+// Lb x_{i-1} + x_i + Ub x_{i+1}, wall closure rows with diagonal I + Lb Ml /
+// I + Ub Mr) is formed in FP64 from x1 and the neighbours' end rows, a second
+// FP32 solve gives the correction d, and x = x1 + d is scaled and stored in
+// FP64.  HBM traffic is that of the FP64 kernel.
+template <int M>
+__device__ __forceinline__ void ccd_lines_ref(const CcdAxisGeom& g, const CcdLines& ln,
+                                              const bool CONTIG, const bool TILED,
+                                              const double* __restrict__ f,
+                                              double* __restrict__ d1, double* __restrict__ d2,
+                                              const unsigned block) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x & 63;
+  const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t b = (int64_t)block * 64 + lane;
+  const bool active = b < ln.n_lines;
+  int64_t base = 0;
+  if (active) {
+    if (CONTIG) {
+      base = b * ln.inner_stride;
+    } else {
+      const uint32_t bi = (uint32_t)b, in = (uint32_t)ln.inner;
+      base = (int64_t)(bi / in) * ln.outer + (int64_t)(bi % in) * ln.inner_stride;
+    }
+  }
+  const int64_t st = ln.stride;
+  const int s0 = g.s[p];
+  const int m = g.s[p + 1] - s0;
+  const bool periodic = g.periodic != 0;
+  const bool wall_first = !periodic && p == 0;
+  const bool wall_last = !periodic && p == g.P - 1;
+
+  double fv[M + 2];
+  const int j0 = periodic ? s0 - 1 : s0;
+  if (M == 16 && TILED) {
+    const int nax = (int)ln.inner_stride;
+    const int64_t rem = ln.n_lines - (int64_t)block * 64;
+    const int cnt = (int)(rem < 64 ? rem : 64) * nax;
+    const double* src = f + (int64_t)block * 64 * nax;
+    double tmp[kTilePer];
+#pragma unroll
+    for (int i = 0; i < kTilePer; ++i) {
+      const int e = threadIdx.x + i * (int)blockDim.x;
+      tmp[i] = e < cnt ? src[e] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < kTilePer; ++i) {
+      const int e = threadIdx.x + i * (int)blockDim.x;
+      if (e < cnt) lds[e] = tmp[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < M + 2; ++k) {
+      fv[k] = 0.0;
+      if (k < m + 2) {
+        int j = j0 + k;
+        if (periodic) j = j < 0 ? j + g.N : (j >= g.N ? j - g.N : j);
+        if (active) fv[k] = lds[lane * nax + j];
+      }
+    }
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int k = 0; k < M + 2; ++k) {
+      fv[k] = 0.0;
+      if (k < m + 2) {
+        int j = j0 + k;
+        if (periodic) j = j < 0 ? j + g.N : (j >= g.N ? j - g.N : j);
+        if (active) fv[k] = f[base + (int64_t)j * st];
+      }
+    }
+  }
+  // FP32 tables, the partition-end exchange G and the x1 end rows E in LDS
+  float* ltab = reinterpret_cast<float*>(lds);
+  float* lq = ltab + g.P * M * kRowTab;
+  float* G = lq + g.P * 16 * g.NQ;
+  float* E = G + 4 * g.P * 64;
+  {
+    const int nt = g.P * M * kRowTab, nq = g.P * 16 * g.NQ;
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) ltab[i] = g.rowtab32[i];
+    for (int i = threadIdx.x; i < nq; i += blockDim.x) lq[i] = g.qtab32[i];
+    __syncthreads();
+  }
+  double cl0 = 0.0, cl1 = 0.0, bl0 = 0.0, bl1 = 0.0;
+  if (wall_first) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) bl0 += g.cIl[k] * fv[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bl1 += g.cIIl[k] * fv[k];
+    cl0 = kAlpha1 * bl0 + kB1 * bl1;
+    cl1 = kB2 * bl0 + kBeta2 * bl1;
+  }
+  double cr0 = 0.0, cr1 = 0.0, br0 = 0.0, br1 = 0.0;
+  if (wall_last) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double v = active ? f[base + (int64_t)(g.N - k) * st] : 0.0;
+      if (k < 4) br0 += g.cIr[k] * v;
+      br1 += g.cIIr[k] * v;
+    }
+    cr0 = kAlpha1 * br0 - kB1 * br1;
+    cr1 = -kB2 * br0 + kBeta2 * br1;
+  }
+  // FP64 right-hand side of row r (closure terms on the end rows)
+  auto rhs = [&](int r, double& r0, double& r1) {
+    r0 = kA1 * (fv[r + 2] - fv[r]);
+    r1 = kA2 * ((fv[r] - 2.0 * fv[r + 1]) + fv[r + 2]);
+    if (r == 0) {
+      r0 -= cl0;
+      r1 -= cl1;
+    }
+    if (r == m - 1) {
+      r0 -= cr0;
+      r1 -= cr1;
+    }
+  };
+  const float* tab = ltab + (size_t)p * M * kRowTab;
+  float x1[M][2], v[M][2];
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    x1[r][0] = x1[r][1] = 0.f;
+    if (r < m) {
+      double r0, r1;
+      rhs(r, r0, r1);
+      x1[r][0] = (float)r0;
+      x1[r][1] = (float)r1;
+    }
+  }
+  solve32<M>(g, tab, lq, x1, m, p, lane, periodic, G);
+  // end rows of x1 for the neighbours' residual rows (static indices only)
+  E[(p * 4 + 0) * 64 + lane] = x1[0][0];
+  E[(p * 4 + 1) * 64 + lane] = x1[0][1];
+#pragma unroll
+  for (int r = 0; r < M; ++r)
+    if (r == m - 1) {
+      E[(p * 4 + 2) * 64 + lane] = x1[r][0];
+      E[(p * 4 + 3) * 64 + lane] = x1[r][1];
+    }
+  __syncthreads();
+  const bool has_left = periodic || p > 0, has_right = periodic || p < g.P - 1;
+  const int pl = p > 0 ? p - 1 : g.P - 1, pr = p < g.P - 1 ? p + 1 : 0;
+  double lx0 = 0.0, lx1 = 0.0, rx0 = 0.0, rx1 = 0.0;  // neighbour rows
+  if (has_left) {
+    lx0 = E[(pl * 4 + 2) * 64 + lane];
+    lx1 = E[(pl * 4 + 3) * 64 + lane];
+  }
+  if (has_right) {
+    rx0 = E[(pr * 4 + 0) * 64 + lane];
+    rx1 = E[(pr * 4 + 1) * 64 + lane];
+  }
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    v[r][0] = v[r][1] = 0.f;
+    if (r < m) {
+      double r0, r1;
+      rhs(r, r0, r1);
+      const double u = x1[r][0], w = x1[r][1];
+      // diagonal block: identity, or the closure-modified end rows
+      double a0 = u, a1 = w;
+      if (wall_first && r == 0) {
+        const double mu = g.Ml[0] * u + g.Ml[1] * w, mv = g.Ml[2] * u + g.Ml[3] * w;
+        a0 += kAlpha1 * mu + kB1 * mv;
+        a1 += kB2 * mu + kBeta2 * mv;
+      }
+      if (wall_last && r == m - 1) {
+        const double mu = g.Mr[0] * u + g.Mr[1] * w, mv = g.Mr[2] * u + g.Mr[3] * w;
+        a0 += kAlpha1 * mu - kB1 * mv;
+        a1 += -kB2 * mu + kBeta2 * mv;
+      }
+      // lower neighbour (Lb) and upper neighbour (Ub)
+      const bool lo = r > 0 || has_left, hi = r < m - 1 || has_right;
+      const double l0 = r > 0 ? (double)x1[r - 1][0] : lx0, l1 = r > 0 ? (double)x1[r - 1][1] : lx1;
+      const double h0 = r < m - 1 ? (double)x1[r + 1][0] : rx0;
+      const double h1 = r < m - 1 ? (double)x1[r + 1][1] : rx1;
+      if (lo) {
+        a0 += kAlpha1 * l0 + kB1 * l1;
+        a1 += kB2 * l0 + kBeta2 * l1;
+      }
+      if (hi) {
+        a0 += kAlpha1 * h0 - kB1 * h1;
+        a1 += -kB2 * h0 + kBeta2 * h1;
+      }
+      v[r][0] = (float)(r0 - a0);
+      v[r][1] = (float)(r1 - a1);
+    }
+  }
+  solve32<M>(g, tab, lq, v, m, p, lane, periodic, G);
+  auto xo = [&](int r, int c) { return (double)x1[r][c] + (double)v[r][c]; };
+  const double x00 = xo(0, 0), x01 = xo(0, 1);
+  double xl0 = 0.0, xl1 = 0.0;  // last row (wall node N closure)
+#pragma unroll
+  for (int r = 0; r < M; ++r)
+    if (r == m - 1) {
+      xl0 = xo(r, 0);
+      xl1 = xo(r, 1);
+    }
+  const int off = periodic ? 0 : 1;
+  if (active && !(M == 16 && TILED)) {
+    if (periodic && p == 0) {
+      if (d1) d1[base + (int64_t)g.N * st] = x00 * g.inv_h;
+      if (d2) d2[base + (int64_t)g.N * st] = x01 * g.inv_h2;
+    }
+    if (wall_first) {
+      if (d1) d1[base] = (g.Ml[0] * x00 + g.Ml[1] * x01 + bl0) * g.inv_h;
+      if (d2) d2[base] = (g.Ml[2] * x00 + g.Ml[3] * x01 + bl1) * g.inv_h2;
+    }
+    if (wall_last) {
+      if (d1) d1[base + (int64_t)g.N * st] = (g.Mr[0] * xl0 + g.Mr[1] * xl1 + br0) * g.inv_h;
+      if (d2) d2[base + (int64_t)g.N * st] = (g.Mr[2] * xl0 + g.Mr[3] * xl1 + br1) * g.inv_h2;
+    }
+  }
+  if (!CONTIG) {
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        if (r < m) {
+          const int64_t a = base + (int64_t)(s0 + r + off) * st;
+          if (d1) __builtin_nontemporal_store(xo(r, 0) * g.inv_h, d1 + a);
+          if (d2) __builtin_nontemporal_store(xo(r, 1) * g.inv_h2, d2 + a);
+        }
+      }
+    }
+  } else if (M == 16 && TILED) {
+    const int nax = (int)ln.inner_stride;
+    const int64_t rem = ln.n_lines - (int64_t)block * 64;
+    const int cnt = (int)(rem < 64 ? rem : 64) * nax;
+    __syncthreads();  // tables and exchange buffers dead; the tile space is reused
+#pragma unroll
+    for (int comp = 0; comp < 2; ++comp) {
+      double* dst = comp == 0 ? d1 : d2;
+      const double scale = comp == 0 ? g.inv_h : g.inv_h2;
+      if (dst == nullptr) continue;
+      double* row = lds + lane * nax;
+#pragma unroll
+      for (int r = 0; r < M; ++r)
+        if (r < m) row[s0 + r + off] = xo(r, comp) * scale;
+      if (periodic && p == 0) row[g.N] = (comp == 0 ? x00 : x01) * scale;
+      if (wall_first)
+        row[0] = (comp == 0 ? g.Ml[0] * x00 + g.Ml[1] * x01 + bl0
+                            : g.Ml[2] * x00 + g.Ml[3] * x01 + bl1) * scale;
+      if (wall_last)
+        row[g.N] = (comp == 0 ? g.Mr[0] * xl0 + g.Mr[1] * xl1 + br0
+                              : g.Mr[2] * xl0 + g.Mr[3] * xl1 + br1) * scale;
+      __syncthreads();
+      double* out = dst + (int64_t)block * 64 * nax;
+#pragma unroll
+      for (int i = 0; i < kTilePer; ++i) {
+        const int e = threadIdx.x + i * (int)blockDim.x;
+        if (e < cnt) __builtin_nontemporal_store(lds[e], out + e);
+      }
+      __syncthreads();
+    }
+  } else {
+    __syncthreads();  // exchange buffers dead; the staging space is reused
+    double* S = lds + (size_t)p * 64 * kStagePitch;
+    const int64_t line0 = (int64_t)block * 64;
+#pragma unroll
+    for (int comp = 0; comp < 2; ++comp) {
+      double* dst = comp == 0 ? d1 : d2;
+      const double scale = comp == 0 ? g.inv_h : g.inv_h2;
+      if (dst == nullptr) continue;
+#pragma unroll
+      for (int r = 0; r < kStageRows; ++r)
+        if (r < M) S[lane * kStagePitch + r] = xo(r, comp) * scale;
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < kStageRows; ++it) {
+        const int e = lane + 64 * it;
+        const int l = e >> 4, r = e & 15;
+        const int64_t bl = line0 + l;
+        if (r < m && bl < ln.n_lines)
+          __builtin_nontemporal_store(S[l * kStagePitch + r],
+                                      dst + bl * ln.inner_stride + (int64_t)(s0 + r + off));
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // Wavefront-per-line mapping (few, long lines: 2-D grids): lane = partition
 // (P <= 64), wavefront = line, 16 lines per workgroup.  The workgroup's lines
 // are staged through LDS as one [16][nax] tile, loaded and stored with
@@ -572,14 +917,18 @@ size_t wave_lds_bytes(const CcdAxisGeom& g) {
   return std::max(tile, sizeof(double) * kWaveLines * 4 * 64);
 }
 
-template <int M>
+template <int M, bool REF>
 __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdLaunch L) {
   // Block order = slot order (axis 0 lines first).  An XCD-contiguous
   // remapping of blocks was measured slower for the fused 3-axis grid.
   const unsigned blk = blockIdx.x;
   const int sl = (L.nslots > 1 && blk >= L.block_begin[1]) + (L.nslots > 2 && blk >= L.block_begin[2]);
   const CcdSlot& S = L.slot[sl];
-  ccd_lines<M>(S.g, S.ln, S.contig != 0, S.tiled != 0, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
+  if constexpr (REF)
+    ccd_lines_ref<M>(S.g, S.ln, S.contig != 0, S.tiled != 0, S.f, S.d1, S.d2,
+                     blk - L.block_begin[sl]);
+  else
+    ccd_lines<M>(S.g, S.ln, S.contig != 0, S.tiled != 0, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
 }
 
 // Batched launch over many (problem, axis) slots kept in device memory:
@@ -645,12 +994,19 @@ WaveFn pick_wave_kernel(int M) {
   return nullptr;
 }
 
-KernelFn pick_kernel(int M) {
+KernelFn pick_kernel(int M, bool ref = false) {
+  if (ref) {
+    switch (M) {
+      case 8: return ccd_line_kernel<8, true>;
+      case 16: return ccd_line_kernel<16, true>;
+    }
+    return nullptr;
+  }
   switch (M) {
-    case 8: return ccd_line_kernel<8>;
-    case 16: return ccd_line_kernel<16>;
-    case 32: return ccd_line_kernel<32>;
-    case 64: return ccd_line_kernel<64>;
+    case 8: return ccd_line_kernel<8, false>;
+    case 16: return ccd_line_kernel<16, false>;
+    case 32: return ccd_line_kernel<32, false>;
+    case 64: return ccd_line_kernel<64, false>;
   }
   return nullptr;
 }
@@ -768,7 +1124,8 @@ tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const dou
     TPF_HIP(hipGetLastError());
     return TPF_OK;
   }
-  KernelFn k = pick_kernel(M);
+  const bool ref = plan->precision == TPF_PREC_FP32_REFINED;
+  KernelFn k = pick_kernel(M, ref);
   if (!k) return fail(TPF_ERR_VALUE, "unsupported CCD rows cap " + std::to_string(M));
   TPF_TRY(raise_lds_limit((const void*)k, shmem));
   hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * L.slot[0].g.P), shmem, stream, L);
@@ -787,9 +1144,8 @@ extern "C" tpf_status tpf_ccd_plan_create(tpf_ccd_plan** out, int ndim, const in
   if (!out || !n_nodes || !h || !bc) return fail(TPF_ERR_VALUE, "null argument");
   *out = nullptr;
   if (ndim < 1 || ndim > 3) return fail(TPF_ERR_VALUE, "ndim must be 1, 2 or 3");
-  if (precision == TPF_PREC_FP32_REFINED)
-    return fail(TPF_ERR_VALUE, "FP32-refined CCD is not built; use TPF_PREC_FP64");
-  if (precision != TPF_PREC_FP64) return fail(TPF_ERR_VALUE, "unknown precision mode");
+  if (precision != TPF_PREC_FP64 && precision != TPF_PREC_FP32_REFINED)
+    return fail(TPF_ERR_VALUE, "unknown precision mode");
   auto* plan = new tpf_ccd_plan();
   plan->ndim = ndim;
   plan->precision = precision;
@@ -823,7 +1179,10 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
   const int64_t lines = total / n_ax;
   const int mode = plan->mode[axis];
   const int cap = plan->rows_cap[axis];
-  bool wave = mode == 2 || (mode == 0 && lines < 4096 && n > 512);
+  const bool ref = plan->precision == TPF_PREC_FP32_REFINED;
+  if (ref && (mode == 2 || cap > 16))
+    return fail(TPF_ERR_VALUE, "FP32-refined CCD runs the lane-per-line kernel with rows cap <= 16");
+  bool wave = !ref && (mode == 2 || (mode == 0 && lines < 4096 && n > 512));
   if (wave && cap != 0 && cap != 8 && cap != 16)
     return fail(TPF_ERR_VALUE, "the wavefront-per-line CCD kernel takes rows cap 8 or 16");
   if (wave && n >= 1) {
@@ -841,6 +1200,9 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
     if (ax.geom.P > kMaxParts)
       return fail(TPF_ERR_VALUE, "rows cap " + std::to_string(cap) +
                                      " needs more than 16 partitions for the lane-per-line kernel");
+    if (ref && ax.geom.M > 16)
+      return fail(TPF_ERR_VALUE, "FP32-refined CCD supports lines of <= 256 unknowns (rows cap "
+                                 "16 x 16 partitions); this axis has " + std::to_string(n));
   }
   ax.wave = wave ? 1 : 0;
   return TPF_OK;
@@ -858,6 +1220,15 @@ tpf_status ensure_axis(tpf_ccd_plan* plan, int axis) {
                     hipMemcpyHostToDevice));
   ax.geom.rowtab = ax.d_rowtab;
   ax.geom.qtab = ax.d_qtab;
+  if (plan->precision == TPF_PREC_FP32_REFINED) {
+    std::vector<float> r32(ax.rowtab.begin(), ax.rowtab.end()), q32(ax.qtab.begin(), ax.qtab.end());
+    TPF_HIP(hipMalloc(&ax.d_rowtab32, r32.size() * sizeof(float)));
+    TPF_HIP(hipMalloc(&ax.d_qtab32, q32.size() * sizeof(float)));
+    TPF_HIP(hipMemcpy(ax.d_rowtab32, r32.data(), r32.size() * sizeof(float), hipMemcpyHostToDevice));
+    TPF_HIP(hipMemcpy(ax.d_qtab32, q32.data(), q32.size() * sizeof(float), hipMemcpyHostToDevice));
+    ax.geom.rowtab32 = ax.d_rowtab32;
+    ax.geom.qtab32 = ax.d_qtab32;
+  }
   return TPF_OK;
 }
 }  // namespace tpf
@@ -867,6 +1238,8 @@ extern "C" tpf_status tpf_ccd_plan_destroy(tpf_ccd_plan* plan) {
   for (auto& ax : plan->axes) {
     if (ax.d_rowtab) (void)hipFree(ax.d_rowtab);
     if (ax.d_qtab) (void)hipFree(ax.d_qtab);
+    if (ax.d_rowtab32) (void)hipFree(ax.d_rowtab32);
+    if (ax.d_qtab32) (void)hipFree(ax.d_qtab32);
   }
   delete plan;
   return TPF_OK;
@@ -959,6 +1332,8 @@ extern "C" tpf_status tpf_ccd_plan_set_launch(tpf_ccd_plan* plan, int axis, int 
   CcdAxisHost& ax = plan->axes[axis];
   if (ax.d_rowtab) TPF_HIP(hipFree(ax.d_rowtab));
   if (ax.d_qtab) TPF_HIP(hipFree(ax.d_qtab));
+  if (ax.d_rowtab32) TPF_HIP(hipFree(ax.d_rowtab32));
+  if (ax.d_qtab32) TPF_HIP(hipFree(ax.d_qtab32));
   ax = CcdAxisHost();
   return TPF_OK;
 }
@@ -999,6 +1374,10 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
     if (!pl || (axis_masks[j] >> pl->ndim)) {
       delete b;
       return fail(TPF_ERR_VALUE, "batch entry " + std::to_string(j) + ": bad plan or axis mask");
+    }
+    if (pl->precision != TPF_PREC_FP64) {
+      delete b;
+      return fail(TPF_ERR_VALUE, "batches take FP64 plans (entry " + std::to_string(j) + ")");
     }
     int64_t total = 1;
     for (int a = 0; a < pl->ndim; ++a) total *= pl->n_nodes[a];

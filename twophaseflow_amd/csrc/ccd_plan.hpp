@@ -16,6 +16,8 @@ struct CcdAxisHost {
   double trunc_coeff = 0.0;     // largest dropped reduced-system coefficient (relative)
   double* d_rowtab = nullptr;
   double* d_qtab = nullptr;
+  float* d_rowtab32 = nullptr;  // FP32-refined plans
+  float* d_qtab32 = nullptr;
 };
 
 // Smallest rows-per-partition cap in {8,16,32,64} that fits n unknowns into
