@@ -46,7 +46,11 @@ typedef struct tpf_ccd_plan tpf_ccd_plan;
 /* Replaces CCDSolver.__init__ + the per-axis factorisations
  * (ccd/ccd_solver.py:99-120, ccd/ccd_solver_helpers.py:227-314, 406-423).
  * h[a] = L[a]/N[a]; bc[a] in {TPF_BC_WALL, TPF_BC_PERIODIC}.
- * Wall axes need n_nodes >= 6 (TPF_ERR_VALUE otherwise); periodic >= 4. */
+ * Wall axes need n_nodes >= 6 (TPF_ERR_VALUE otherwise); periodic >= 4.
+ * precision: TPF_PREC_FP64, or TPF_PREC_FP32_REFINED (FP32 solves plus one
+ * FP64 residual refinement; FP64 inputs and outputs; lines of <= 256
+ * unknowns, lane-per-line mapping, rows cap <= 16, TPF_ERR_VALUE beyond;
+ * relative error <= 1e-12, no counterpart in the reference). */
 TPF_API tpf_status tpf_ccd_plan_create(tpf_ccd_plan** plan, int ndim,
                                const int64_t* n_nodes, const double* h,
                                const int* bc, int precision);
