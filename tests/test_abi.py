@@ -84,6 +84,23 @@ def test_host_model_matches_dense_lu_oracle(shape, bcs):
     nat.lib().tpf_ccd_plan_destroy(plan)
 
 
+@pytest.mark.parametrize("counts", [(1033, 1033, 1033), (5, 17, 2), (9,), (0, 4, 4), (64, 1)])
+def test_fused_launch_block_map_is_a_bijection(counts):
+    """Every (slot, slot-local workgroup) of a fused launch is visited once."""
+    cnt = np.asarray(counts, dtype=np.uint32)
+    total = int(cnt.sum())
+    slot = np.full(max(total, 1), -1, dtype=np.int32)
+    local = np.zeros(max(total, 1), dtype=np.uint32)
+    nat.check(nat.lib().tpf_test_ccd_block_map(len(counts), cnt.ctypes.data, slot.ctypes.data,
+                                               local.ctypes.data))
+    got = sorted(zip(slot[:total].tolist(), local[:total].tolist()))
+    want = [(s, l) for s, c in enumerate(counts) for l in range(c)]
+    assert got == want
+    if min(counts) > 0:  # interleaved head: slot = b % nslots
+        head = len(counts) * min(counts)
+        assert (slot[:head] == np.arange(head) % len(counts)).all()
+
+
 def test_reduced_system_window_truncation_is_negligible():
     plan = _plan((257, 257, 257), (1, 1, 1))
     P, M, W, t = C.c_int(), C.c_int(), C.c_int(), C.c_double()
