@@ -78,11 +78,16 @@ TPF_API tpf_status tpf_ccd_axis_info(tpf_ccd_plan* plan, int axis, int* partitio
 
 /* Launch shape of one axis (no counterpart in the reference: the knobs the
  * autotuner in twophaseflow_amd/autotune.py sets).  mode 0: automatic (a
- * wavefront per line for < 4096 lines of > 512 unknowns, else a lane per
- * line); 1: a lane per line (<= 16 partitions, rows cap 8/16/32/64);
- * 2: a wavefront per line (<= 64 partitions, rows cap 8/16).  rows_cap 0
- * picks the default for the mode.  Validated on the host; the axis' tables
- * are rebuilt on next use. */
+ * wavefront per line for < 4096 lines of > 512 unknowns; else a lane per
+ * line, with the explicit partition inverse on strided FP64 axes of
+ * <= 16 x 16 rows); 1: a lane per line, block recurrences (<= 16
+ * partitions, rows cap 8/16/32/64); 2: a wavefront per line (<= 64
+ * partitions, rows cap 8/16); 3: a lane per line, explicit partition
+ * inverse (strided axes, <= 16 partitions, rows cap 8/16).  An
+ * explicit-inverse axis fused in one tpf_ccd_apply launch with a
+ * recurrence axis runs the recurrence kernel.  rows_cap 0 picks the default
+ * for the mode.  Validated on the host; the axis' tables are rebuilt on
+ * next use. */
 TPF_API tpf_status tpf_ccd_plan_set_launch(tpf_ccd_plan* plan, int axis, int mode,
                                    int rows_cap);
 

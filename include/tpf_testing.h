@@ -19,10 +19,11 @@ TPF_API tpf_status tpf_test_ccd_host_model(tpf_ccd_plan* plan, const double* f,
                                            double* d1, double* d2, int axis);
 
 /* Workgroup order of a fused multi-axis CCD launch: for the nslots (<= 3)
- * axis slots with counts[i] workgroups each, writes slot[b] and the
- * slot-local workgroup local[b] of every launch workgroup b. */
-TPF_API tpf_status tpf_test_ccd_block_map(int nslots, const unsigned* counts, int* slot,
-                                          unsigned* local);
+ * axis slots with counts[i] workgroups each (bit i of strided_mask: slot i
+ * is a strided axis, whose workgroups are renumbered XCD by XCD), writes
+ * slot[b] and the slot-local workgroup local[b] of every launch workgroup b. */
+TPF_API tpf_status tpf_test_ccd_block_map(int nslots, const unsigned* counts,
+                                          unsigned strided_mask, int* slot, unsigned* local);
 
 #ifdef __cplusplus
 }

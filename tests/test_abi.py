@@ -84,21 +84,61 @@ def test_host_model_matches_dense_lu_oracle(shape, bcs):
     nat.lib().tpf_ccd_plan_destroy(plan)
 
 
-@pytest.mark.parametrize("counts", [(1033, 1033, 1033), (5, 17, 2), (9,), (0, 4, 4), (64, 1)])
-def test_fused_launch_block_map_is_a_bijection(counts):
-    """Every (slot, slot-local workgroup) of a fused launch is visited once."""
+@pytest.mark.parametrize("counts", [(1033, 1033, 1033), (5, 17, 2), (9,), (0, 4, 4), (64, 1),
+                                    (1033, 1033)])
+@pytest.mark.parametrize("strided_mask", [0, 0b011])
+def test_fused_launch_block_map_is_a_bijection(counts, strided_mask):
+    """Every (slot, slot-local workgroup) of a fused launch is visited once,
+    with and without the XCD renumbering of strided slots."""
     cnt = np.asarray(counts, dtype=np.uint32)
     total = int(cnt.sum())
     slot = np.full(max(total, 1), -1, dtype=np.int32)
     local = np.zeros(max(total, 1), dtype=np.uint32)
-    nat.check(nat.lib().tpf_test_ccd_block_map(len(counts), cnt.ctypes.data, slot.ctypes.data,
-                                               local.ctypes.data))
+    nat.check(nat.lib().tpf_test_ccd_block_map(len(counts), cnt.ctypes.data, strided_mask,
+                                               slot.ctypes.data, local.ctypes.data))
     got = sorted(zip(slot[:total].tolist(), local[:total].tolist()))
     want = [(s, l) for s, c in enumerate(counts) for l in range(c)]
     assert got == want
     if min(counts) > 0:  # interleaved head: slot = b % nslots
         head = len(counts) * min(counts)
         assert (slot[:head] == np.arange(head) % len(counts)).all()
+
+
+EXPLICIT_CASES = [((257, 9), (1, 0), 16), ((256, 7, 4), (0, 0, 0), 16), ((40, 33), (0, 1), 8),
+                  ((65, 30, 7), (1, 0, 1), 8), ((7, 6), (0, 0), 8), ((100, 8), (1, 1), 0)]
+
+
+@pytest.mark.parametrize("shape,bcs,cap", EXPLICIT_CASES)
+def test_explicit_inverse_tables_match_dense_lu_oracle(shape, bcs, cap):
+    """The explicit partition-inverse tables (mode 3, strided axes) through
+    the scalar model of that kernel, against the dense-LU oracle; the
+    contiguous axis refuses the mode."""
+    L = [1.0 + 0.2 * a for a in range(len(shape))]
+    plan = _plan(shape, bcs, L)
+    f = np.random.default_rng(5).standard_normal(shape)
+    kinds = tuple("periodic" if b else "wall" for b in bcs)
+    o = occd.OracleCCD([s - 1 for s in shape], L, kinds)
+    for axis in range(len(shape) - 1):  # strided axes
+        if shape[axis] < (4 if kinds[axis] == "periodic" else 6):
+            continue
+        nat.check(nat.lib().tpf_ccd_plan_set_launch(plan, axis, 3, cap))
+        d1 = np.full(shape, np.nan)
+        d2 = np.full(shape, np.nan)
+        nat.check(nat.lib().tpf_test_ccd_host_model(plan, f.ctypes.data, d1.ctypes.data,
+                                                    d2.ctypes.data, axis))
+        r1, r2 = o.differentiate(f, axis)
+        assert np.max(np.abs(d1 - r1)) <= 1e-13 * np.max(np.abs(r1))
+        assert np.max(np.abs(d2 - r2)) <= 1e-13 * np.max(np.abs(r2))
+    with pytest.raises(ValueError, match="strided axes only"):
+        nat.check(nat.lib().tpf_ccd_plan_set_launch(plan, len(shape) - 1, 3, cap))
+    nat.lib().tpf_ccd_plan_destroy(plan)
+
+
+def test_explicit_inverse_rejects_long_lines():
+    plan = _plan((1025, 9), (1, 0))
+    with pytest.raises(ValueError, match="256 unknowns"):
+        nat.check(nat.lib().tpf_ccd_plan_set_launch(plan, 0, 3, 0))
+    nat.lib().tpf_ccd_plan_destroy(plan)
 
 
 def test_reduced_system_window_truncation_is_negligible():

@@ -88,20 +88,32 @@ def test_torch_and_numpy_inputs_agree():
     np.testing.assert_array_equal(a2, b2.cpu().numpy())
 
 
-def test_paper_table_U1a_through_gpu():
-    """paper/sections/12u1_ccd_operator.tex:46-50."""
-    _announce("ccd U1-a")
+@pytest.mark.parametrize("mode", ["lane", "explicit"])
+def test_paper_table_U1a_through_gpu(mode):
+    """paper/sections/12u1_ccd_operator.tex:46-50: errors to 1 % of the table.
+
+    The block recurrences ("lane", the reference's elimination order)
+    reproduce every entry to 1 %.  The explicit partition inverse sums other
+    terms in another order; at N = 128 the table's f'' error (1.5e-10) is
+    within a factor 4000 of FP64 rounding of f'' = O(40), and that kernel's
+    rounding moves it by 3.7e-12 (2.5 %).  Its bound adds a rounding
+    allowance of 1e-13 x max|f^(k)| (a tenth of the 1e-12 parity bound)."""
+    _announce(f"ccd U1-a ({mode})")
     table = {16: (2.57e-6, 3.68e-5), 32: (3.86e-8, 5.70e-7), 64: (5.97e-10, 8.89e-9),
              128: (9.36e-12, 1.49e-10)}
+    slack = (0.0, 0.0) if mode == "lane" else (1e-13 * 2 * np.pi, 1e-13 * (2 * np.pi) ** 2)
     for N, (r1, r2) in table.items():
         g = _grid((N, N))
         x = np.linspace(0, 1, N + 1)
         X, Y = np.meshgrid(x, x, indexing="ij")
         f = np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y)
-        d1, d2 = CCDSolver(g, bc_type="periodic").differentiate(f, 0)
+        ccd = CCDSolver(g, bc_type="periodic")
+        ccd.set_launch(0, mode)
+        d1, d2 = ccd.differentiate(f, 0)
         e1 = np.max(np.abs(d1 - 2 * np.pi * np.cos(2 * np.pi * X) * np.sin(2 * np.pi * Y)))
         e2 = np.max(np.abs(d2 + (2 * np.pi) ** 2 * f))
-        assert abs(e1 - r1) <= 0.01 * r1 and abs(e2 - r2) <= 0.01 * r2, (N, e1, e2)
+        assert abs(e1 - r1) <= 0.01 * r1 + slack[0], (N, e1)
+        assert abs(e2 - r2) <= 0.01 * r2 + slack[1], (N, e2)
 
 
 def test_wall_cubic_exact_and_nullspace():
@@ -188,3 +200,47 @@ def test_full_size_256_cubed_properties():
         s1, s2 = occd.periodic_spectral(sub, axis, 1.0 / N)
         assert _rel(d1[tuple(sl)].cpu(), s1) <= 1e-12
         assert _rel(d2[tuple(sl)].cpu(), s2) <= 1e-12
+
+
+@pytest.mark.parametrize("N,bc", [((40, 33, 17), "wall"), ((48, 64, 9), "periodic"),
+                                  ((255, 31, 6), ("wall", "periodic", "wall")),
+                                  ((128, 100), "wall_periodic")])
+def test_explicit_inverse_kernel_paths(N, bc):
+    """Strided axes through the explicit partition inverse: alone, fused with
+    each other (one explicit grid), and fused with the contiguous axis (then
+    run by the recurrence kernel); lane mode forced for comparison."""
+    _announce(f"ccd explicit inverse N={N} bc={bc}")
+    g = _grid(N)
+    o = occd.OracleCCD(N, [1.0] * len(N), bc)
+    f_np = np.random.default_rng(8).standard_normal(g.shape)
+    f = torch.from_numpy(f_np).cuda()
+    ccd = CCDSolver(g, bc_type=bc)
+    lane = CCDSolver(g, bc_type=bc)
+    nd = len(N)
+    for a in range(nd - 1):
+        lane.set_launch(a, "lane")
+    refs = [o.differentiate(f_np, a) for a in range(nd)]
+    strided = (1 << (nd - 1)) - 1
+    for mask in (strided, (1 << nd) - 1):
+        k = bin(mask).count("1")
+        d1 = torch.empty((k,) + g.shape, dtype=torch.float64, device="cuda")
+        d2 = torch.empty_like(d1)
+        e1 = torch.empty_like(d1)
+        e2 = torch.empty_like(d1)
+        ccd.apply_into(f, d1, d2, mask)
+        lane.apply_into(f, e1, e2, mask)
+        torch.cuda.synchronize()
+        for i, a in enumerate([a for a in range(nd) if mask >> a & 1]):
+            assert _rel(d1[i].cpu(), refs[a][0]) <= 1e-12, (mask, a)
+            assert _rel(d2[i].cpu(), refs[a][1]) <= 1e-12, (mask, a)
+            assert _rel(e1[i].cpu(), refs[a][0]) <= 1e-12, (mask, a)
+            assert _rel(e2[i].cpu(), refs[a][1]) <= 1e-12, (mask, a)
+    for a in range(nd - 1):
+        n = N[a] if o.kinds[a] == "periodic" else N[a] - 1
+        if -(-n // 8) > 16:
+            continue
+        ccd.set_launch(a, "explicit", 8)
+        d1, d2 = ccd.differentiate(f, a)
+        assert _rel(d1.cpu(), refs[a][0]) <= 1e-12 and _rel(d2.cpu(), refs[a][1]) <= 1e-12
+    with pytest.raises(ValueError, match="strided axes only"):
+        ccd.set_launch(nd - 1, "explicit")

@@ -78,7 +78,7 @@ SIGNATURES = {
     "tpf_fields_correct_velocity": (C.c_int, [C.c_int, _i64p, _ip, C.c_int, C.c_double,
                                               _vp, _vp, _vp, _vp]),
     "tpf_test_ccd_host_model": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
-    "tpf_test_ccd_block_map": (C.c_int, [C.c_int, _vp, _vp, _vp]),
+    "tpf_test_ccd_block_map": (C.c_int, [C.c_int, _vp, C.c_uint, _vp, _vp]),
     "tpf_conv3d_fwd": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                  C.c_int, C.c_int, C.c_int, _vp]),
     "tpf_conv3d_wgrad": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,

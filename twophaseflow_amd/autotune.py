@@ -46,12 +46,16 @@ def _save(table: dict) -> None:
         pass  # a read-only home only loses the cache
 
 
-def admissible_shapes(n_nodes: int, periodic: bool):
-    """(mode, rows_cap) pairs the plan accepts for one axis."""
+def admissible_shapes(n_nodes: int, periodic: bool, strided: bool = True):
+    """(mode, rows_cap) pairs the plan accepts for one axis ("explicit" only
+    on strided axes)."""
     n = n_nodes - 1 if periodic else n_nodes - 2
     min_rows = 2 if periodic else 4
     out = []
-    for mode, caps, max_parts in (("lane", CANDIDATES, MAX_PARTS), ("wave", (8, 16), 64)):
+    shapes = [("lane", CANDIDATES, MAX_PARTS), ("wave", (8, 16), 64)]
+    if strided:
+        shapes.append(("explicit", (8, 16), MAX_PARTS))
+    for mode, caps, max_parts in shapes:
         for m in caps:
             parts = -(-n // m)
             if parts <= max_parts and (parts == 1 or n // parts >= min_rows):
@@ -79,7 +83,7 @@ def tune_ccd(solver, reps: int = 10, use_cache: bool = True) -> dict:
             result[axis] = dict(table[key], cached=True)
             continue
         times = {}
-        for mode, m in admissible_shapes(shape[axis], periodic):
+        for mode, m in admissible_shapes(shape[axis], periodic, strided=axis < len(shape) - 1):
             solver.set_launch(axis, mode, m)
             solver.differentiate_into(f, d1, d2, axis)  # builds the tables
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

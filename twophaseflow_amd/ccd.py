@@ -67,7 +67,7 @@ class CCDSolver:
                 "trunc_coeff": t.value}
 
     # -- device-level entry points ---------------------------------------------
-    MODES = {"auto": 0, "lane": 1, "wave": 2}
+    MODES = {"auto": 0, "lane": 1, "wave": 2, "explicit": 3}
 
     def set_launch(self, axis: int, mode: str = "auto", rows_cap: int = 0) -> None:
         """Launch shape of one axis: kernel mapping ("auto", "lane": a lane

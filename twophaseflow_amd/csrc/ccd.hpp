@@ -17,6 +17,7 @@ namespace tpf {
 
 constexpr int kMaxParts = 16;      // partitions of the lane-per-line kernel (wavefronts per workgroup)
 constexpr int kMaxWaveParts = 64;  // partitions of the wavefront-per-line kernel (lanes)
+constexpr int kMaxExplicitRows = 16;  // rows cap of the explicit-inverse kernel
 constexpr int kRowTab = 16;     // doubles per row: Lf(4) iD(4) V(4) W(4)
 
 // CCD coefficients, Table 1 of the paper (ccd/ccd_solver.py:70-76).
@@ -33,12 +34,14 @@ struct CcdAxisGeom {
   const double* qtab;    // [P][4][NQ*4]: rows (left0,left1,right0,right1)
   const float* rowtab32; // FP32 copies (FP32-refined plans only)
   const float* qtab32;
+  const double* ctab;    // [types][M][M+2][2]: partition inverse applied to f (explicit mode)
   int n;                 // unknowns per line (N periodic, N-1 wall)
   int N;                 // cells along the axis
   int P, M, NQ, d0;      // partitions, rows cap, coupling window, window start
   int rel;               // 1: window partition = p + d0 + t; 0: partition = t
   int periodic;
   int s[kMaxWaveParts + 1];  // partition starts (unknown index), s[P] = n
+  int ctype[kMaxParts];      // partition -> ctab type (explicit mode)
   double inv_h, inv_h2;
   // wall closure in scaled unknowns (ccd/ccd_solver.py:408-456)
   double Lb[4], Ub[4], Ml[4], Mr[4];

@@ -947,17 +947,200 @@ __host__ __device__ inline void slot_of(const CcdLaunch& L, unsigned blk, int& s
   local = 0;
 }
 
+// Strided axes: renumber a slot's workgroups so that the ones an XCD runs
+// together (slot-local indices congruent mod 8 share an XCD under the
+// round-robin dispatch of the interleaved order) take adjacent line groups.
+// Their 512 B rows are not 128 B aligned on odd node counts, and the cache
+// lines two neighbouring groups share are then fetched and written back once,
+// through one L2, instead of by two XCDs.
+__host__ __device__ inline unsigned xcd_local(unsigned local, unsigned count) {
+  const unsigned per = count / 8;
+  return local < per * 8 ? (local % 8) * per + local / 8 : local;
+}
+
 template <int M, bool REF>
 __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdLaunch L) {
   int sl;
   unsigned local;
   slot_of(L, blockIdx.x, sl, local);
   const CcdSlot& S = L.slot[sl];
+  if (!S.contig) local = xcd_local(local, L.block_begin[sl + 1] - L.block_begin[sl]);
   if constexpr (REF)
     ccd_lines_ref<M>(S.g, S.ln, S.contig != 0, S.tiled != 0, S.f, S.d1, S.d2, local);
   else
     ccd_lines<M>(S.g, S.ln, S.contig != 0, S.tiled != 0, S.f, S.d1, S.d2, local);
 }
+
+
+// ---- explicit partition inverse (lane per line, strided axes) --------------
+// Same mapping as ccd_lines (lane = line, wavefront = partition), but the
+// local solve is the precomputed partition inverse applied to the first
+// differences of the line values: y_r = S_r . (fv[k+1] - fv[k]), 2 x (M+1)
+// per row, wall closure folded in (differences keep f'' free of the
+// cancellation of a sum over raw values).  No
+// recurrence state is kept: a lane holds only fv, so the kernel fits 64
+// VGPRs and two workgroups share a CU, one loading or storing while the
+// other computes.  Tables are wavefront-uniform and read through the scalar
+// cache (constant address space).  Rows are produced in order and stored
+// directly (strided axes: every store instruction writes 64 consecutive
+// lines).
+typedef const __attribute__((address_space(4))) double* kdptr;
+
+template <int M>
+__device__ __forceinline__ void ccd_lines_explicit(const CcdAxisGeom& g, const CcdLines& ln,
+                                                   const double* __restrict__ f,
+                                                   double* __restrict__ d1,
+                                                   double* __restrict__ d2,
+                                                   const unsigned block) {
+  extern __shared__ double lds[];
+  constexpr int K = M + 2;
+  const int lane = threadIdx.x & 63;
+  const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t b = (int64_t)block * 64 + lane;
+  const bool active = b < ln.n_lines;
+  int64_t base = 0;
+  if (active) {
+    const uint32_t bi = (uint32_t)b, in = (uint32_t)ln.inner;
+    base = (int64_t)(bi / in) * ln.outer + (int64_t)(bi % in) * ln.inner_stride;
+  }
+  const int64_t st = ln.stride;
+  const int s0 = g.s[p];
+  const int m = g.s[p + 1] - s0;
+  const bool periodic = g.periodic != 0;
+  const bool wall_first = !periodic && p == 0;
+  const bool wall_last = !periodic && p == g.P - 1;
+
+  double fv[K];
+  const int j0 = periodic ? s0 - 1 : s0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    fv[k] = 0.0;
+    if (k < m + 2) {
+      int j = j0 + k;
+      if (periodic) j = j < 0 ? j + g.N : (j >= g.N ? j - g.N : j);
+      if (active) fv[k] = f[base + (int64_t)j * st];
+    }
+  }
+  // first differences in place (fv[K-1] is unused; the table column is 0)
+#pragma unroll
+  for (int k = 0; k + 1 < K; ++k) fv[k] = fv[k + 1] - fv[k];
+  const kdptr C = (kdptr)(g.ctab) + (size_t)g.ctype[p] * M * K * 2;
+  const kdptr T = (kdptr)(g.rowtab) + (size_t)p * M * kRowTab;
+
+  // ---- partition end values of the local solution -> LDS
+  double* G = lds;  // [4P][64]
+  {
+    double a0 = 0.0, a1 = 0.0, z0 = 0.0, z1 = 0.0;
+    const kdptr cl = C + (size_t)(m - 1) * K * 2;
+#pragma unroll
+    for (int k = 0; k + 1 < K; ++k) {
+      a0 += C[2 * k] * fv[k];
+      a1 += C[2 * k + 1] * fv[k];
+      z0 += cl[2 * k] * fv[k];
+      z1 += cl[2 * k + 1] * fv[k];
+    }
+    G[(p * 4 + 0) * 64 + lane] = a0;
+    G[(p * 4 + 1) * 64 + lane] = a1;
+    G[(p * 4 + 2) * 64 + lane] = z0;
+    G[(p * 4 + 3) * 64 + lane] = z1;
+  }
+  __syncthreads();
+
+  // ---- coupling: (left, right) end values of the neighbours = Q_p . G
+  double lr0 = 0.0, lr1 = 0.0, lr2 = 0.0, lr3 = 0.0;
+  {
+    const int NQ = g.NQ;
+    const kdptr q = (kdptr)(g.qtab) + (size_t)p * 4 * NQ * 4;
+    for (int t = 0; t < NQ; ++t) {
+      int qq = g.rel ? p + g.d0 + t : t;
+      if (periodic) qq = qq < 0 ? qq + g.P : (qq >= g.P ? qq - g.P : qq);
+      else qq = qq < 0 ? 0 : (qq >= g.P ? g.P - 1 : qq);
+      const double G0 = G[(qq * 4 + 0) * 64 + lane];
+      const double G1 = G[(qq * 4 + 1) * 64 + lane];
+      const double G2 = G[(qq * 4 + 2) * 64 + lane];
+      const double G3 = G[(qq * 4 + 3) * 64 + lane];
+      const kdptr q0 = q + (0 * NQ + t) * 4;
+      const kdptr q1 = q + (1 * NQ + t) * 4;
+      const kdptr q2 = q + (2 * NQ + t) * 4;
+      const kdptr q3 = q + (3 * NQ + t) * 4;
+      lr0 += q0[0] * G0 + q0[1] * G1 + q0[2] * G2 + q0[3] * G3;
+      lr1 += q1[0] * G0 + q1[1] * G1 + q1[2] * G2 + q1[3] * G3;
+      lr2 += q2[0] * G0 + q2[1] * G1 + q2[2] * G2 + q2[3] * G3;
+      lr3 += q3[0] * G0 + q3[1] * G1 + q3[2] * G2 + q3[3] * G3;
+    }
+  }
+
+  // ---- rows: x_r = C_r . fv - V_r (lr0, lr1) - W_r (lr2, lr3), stored at once
+  const int off = periodic ? 0 : 1;
+  double x00 = 0.0, x01 = 0.0, xl0 = 0.0, xl1 = 0.0;
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    if (r < m) {
+      const kdptr cr = C + r * K * 2;
+      double u = 0.0, v = 0.0;
+#pragma unroll
+      for (int k = 0; k + 1 < K; ++k) {
+        u += cr[2 * k] * fv[k];
+        v += cr[2 * k + 1] * fv[k];
+      }
+      const kdptr t = T + r * kRowTab + 8;
+      u -= t[0] * lr0 + t[1] * lr1 + t[4] * lr2 + t[5] * lr3;
+      v -= t[2] * lr0 + t[3] * lr1 + t[6] * lr2 + t[7] * lr3;
+      if (r == 0) {
+        x00 = u;
+        x01 = v;
+      }
+      if (r == m - 1) {
+        xl0 = u;
+        xl1 = v;
+      }
+      if (active) {
+        const int64_t a = base + (int64_t)(s0 + r + off) * st;
+        if (d1) __builtin_nontemporal_store(u * g.inv_h, d1 + a);
+        if (d2) __builtin_nontemporal_store(v * g.inv_h2, d2 + a);
+      }
+    }
+  }
+  if (!active) return;
+  if (periodic && p == 0) {
+    if (d1) d1[base + (int64_t)g.N * st] = x00 * g.inv_h;
+    if (d2) d2[base + (int64_t)g.N * st] = x01 * g.inv_h2;
+  }
+  if (wall_first) {  // fv holds differences now: the closure reads nodes 0..5 again
+    double bl0 = 0.0, bl1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double v = f[base + (int64_t)k * st];
+      if (k < 4) bl0 += g.cIl[k] * v;
+      bl1 += g.cIIl[k] * v;
+    }
+    if (d1) d1[base] = (g.Ml[0] * x00 + g.Ml[1] * x01 + bl0) * g.inv_h;
+    if (d2) d2[base] = (g.Ml[2] * x00 + g.Ml[3] * x01 + bl1) * g.inv_h2;
+  }
+  if (wall_last) {
+    double br0 = 0.0, br1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double v = f[base + (int64_t)(g.N - k) * st];
+      if (k < 4) br0 += g.cIr[k] * v;
+      br1 += g.cIIr[k] * v;
+    }
+    if (d1) d1[base + (int64_t)g.N * st] = (g.Mr[0] * xl0 + g.Mr[1] * xl1 + br0) * g.inv_h;
+    if (d2) d2[base + (int64_t)g.N * st] = (g.Mr[2] * xl0 + g.Mr[3] * xl1 + br1) * g.inv_h2;
+  }
+}
+
+template <int M>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
+void ccd_explicit_kernel(CcdLaunch L) {
+  int sl;
+  unsigned local;
+  slot_of(L, blockIdx.x, sl, local);
+  const CcdSlot& S = L.slot[sl];
+  local = xcd_local(local, L.block_begin[sl + 1] - L.block_begin[sl]);
+  ccd_lines_explicit<M>(S.g, S.ln, S.f, S.d1, S.d2, local);
+}
+
 
 // Batched launch over many (problem, axis) slots kept in device memory:
 // workgroups [begin[s], begin[s+1]) belong to slot s.  All slots share the
@@ -991,6 +1174,22 @@ __global__ __launch_bounds__(64 * kWaveLines) void ccd_wave_batch_kernel(
   ccd_wave_tile<M>(S.g, S.ln, S.f, S.d1, S.d2, blk - begin[lo]);
 }
 
+template <int M>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
+void ccd_explicit_batch_kernel(const CcdSlot* __restrict__ slots, const unsigned* __restrict__ begin,
+                               int nslots) {
+  const unsigned blk = blockIdx.x;
+  int lo = 0, hi = nslots - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (begin[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  const CcdSlot& S = slots[lo];
+  ccd_lines_explicit<M>(S.g, S.ln, S.f, S.d1, S.d2,
+                        xcd_local(blk - begin[lo], begin[lo + 1] - begin[lo]));
+}
+
 using BatchFn = void (*)(const CcdSlot*, const unsigned*, int);
 using WaveFn = void (*)(CcdLaunch);
 
@@ -998,6 +1197,14 @@ BatchFn pick_wave_batch_kernel(int M) {
   switch (M) {
     case 8: return ccd_wave_batch_kernel<8>;
     case 16: return ccd_wave_batch_kernel<16>;
+  }
+  return nullptr;
+}
+
+BatchFn pick_explicit_batch_kernel(int M) {
+  switch (M) {
+    case 8: return ccd_explicit_batch_kernel<8>;
+    case 16: return ccd_explicit_batch_kernel<16>;
   }
   return nullptr;
 }
@@ -1118,7 +1325,7 @@ tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const dou
   size_t shmem = 0;
   unsigned blocks = 0;
   int M = -1;
-  const bool wave = plan->axes[axes[0]].wave != 0;
+  const bool wave = plan->axes[axes[0]].wave == 1;
   for (int i = 0; i < naxes; ++i) {
     CcdAxisHost& ax = plan->axes[axes[i]];
     CcdSlot& S = L.slot[i];
@@ -1131,7 +1338,7 @@ tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const dou
     S.d2 = d2[i];
     if (M < 0) M = ax.geom.M;
     if (ax.geom.M != M || ax.geom.P != plan->axes[axes[0]].geom.P ||
-        ax.wave != plan->axes[axes[0]].wave)
+        (ax.wave == 1) != (plan->axes[axes[0]].wave == 1))
       return fail(TPF_ERR_VALUE, "fused CCD launch needs one kernel, rows cap and partition count");
     const size_t sh = wave ? 0 : lds_bytes(ax.geom, S.contig, plan->n_nodes[axes[i]]);
     shmem = sh > shmem ? sh : shmem;
@@ -1149,6 +1356,17 @@ tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const dou
     for (int i = 0; i < naxes; ++i) sh = std::max(sh, wave_lds_bytes(L.slot[i].g));
     TPF_TRY(raise_lds_limit((const void*)k, sh));
     hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * kWaveLines), sh, stream, L);
+    TPF_HIP(hipGetLastError());
+    return TPF_OK;
+  }
+  bool all_explicit = true;  // explicit-mode axes fused with lane-mode ones run as lane
+  for (int i = 0; i < naxes; ++i) all_explicit = all_explicit && plan->axes[axes[i]].wave == 2;
+  if (all_explicit) {  // strided axes, explicit partition inverse
+    if (M != 8 && M != 16) return fail(TPF_ERR_VALUE, "explicit CCD kernel takes rows cap 8 or 16");
+    KernelFn k = M == 16 ? ccd_explicit_kernel<16> : ccd_explicit_kernel<8>;
+    const size_t sh = sizeof(double) * 4 * L.slot[0].g.P * 64;
+    TPF_TRY(raise_lds_limit((const void*)k, sh));
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * L.slot[0].g.P), sh, stream, L);
     TPF_HIP(hipGetLastError());
     return TPF_OK;
   }
@@ -1208,8 +1426,21 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
   const int mode = plan->mode[axis];
   const int cap = plan->rows_cap[axis];
   const bool ref = plan->precision == TPF_PREC_FP32_REFINED;
-  if (ref && (mode == 2 || cap > 16))
+  if (ref && (mode >= 2 || cap > 16))
     return fail(TPF_ERR_VALUE, "FP32-refined CCD runs the lane-per-line kernel with rows cap <= 16");
+  const bool strided = lines_for(plan, axis).stride != 1;
+  if (mode == 3) {  // lane per line, explicit partition inverse (strided axes)
+    if (!strided)
+      return fail(TPF_ERR_VALUE, "the explicit-inverse CCD kernel runs strided axes only");
+    if (cap != 0 && cap != 8 && cap != 16)
+      return fail(TPF_ERR_VALUE, "the explicit-inverse CCD kernel takes rows cap 8 or 16");
+    TPF_TRY(build_axis_tables(ax, n_ax, plan->h[axis], periodic, cap, 0));
+    if (ax.geom.M > kMaxExplicitRows || ax.geom.P > kMaxParts)
+      return fail(TPF_ERR_VALUE, "the explicit-inverse CCD kernel supports lines of <= 256 unknowns "
+                                 "(16 partitions x 16 rows); this axis has " + std::to_string(n));
+    ax.wave = 2;
+    return TPF_OK;
+  }
   bool wave = !ref && (mode == 2 || (mode == 0 && lines < 4096 && n > 512));
   if (wave && cap != 0 && cap != 8 && cap != 16)
     return fail(TPF_ERR_VALUE, "the wavefront-per-line CCD kernel takes rows cap 8 or 16");
@@ -1232,7 +1463,11 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
       return fail(TPF_ERR_VALUE, "FP32-refined CCD supports lines of <= 256 unknowns (rows cap "
                                  "16 x 16 partitions); this axis has " + std::to_string(n));
   }
-  ax.wave = wave ? 1 : 0;
+  // Auto: strided FP64 axes of <= 16 partitions x 16 rows take the explicit
+  // partition inverse (no recurrence state: two workgroups per CU).
+  const bool expl = !wave && mode == 0 && !ref && strided && ax.geom.M <= kMaxExplicitRows &&
+                    ax.geom.P <= kMaxParts && !ax.ctab.empty();
+  ax.wave = wave ? 1 : (expl ? 2 : 0);
   return TPF_OK;
 }
 
@@ -1248,6 +1483,12 @@ tpf_status ensure_axis(tpf_ccd_plan* plan, int axis) {
                     hipMemcpyHostToDevice));
   ax.geom.rowtab = ax.d_rowtab;
   ax.geom.qtab = ax.d_qtab;
+  if (ax.wave == 2) {
+    TPF_HIP(hipMalloc(&ax.d_ctab, ax.ctab.size() * sizeof(double)));
+    TPF_HIP(hipMemcpy(ax.d_ctab, ax.ctab.data(), ax.ctab.size() * sizeof(double),
+                      hipMemcpyHostToDevice));
+    ax.geom.ctab = ax.d_ctab;
+  }
   if (plan->precision == TPF_PREC_FP32_REFINED) {
     std::vector<float> r32(ax.rowtab.begin(), ax.rowtab.end()), q32(ax.qtab.begin(), ax.qtab.end());
     TPF_HIP(hipMalloc(&ax.d_rowtab32, r32.size() * sizeof(float)));
@@ -1268,6 +1509,7 @@ extern "C" tpf_status tpf_ccd_plan_destroy(tpf_ccd_plan* plan) {
     if (ax.d_qtab) (void)hipFree(ax.d_qtab);
     if (ax.d_rowtab32) (void)hipFree(ax.d_rowtab32);
     if (ax.d_qtab32) (void)hipFree(ax.d_qtab32);
+    if (ax.d_ctab) (void)hipFree(ax.d_ctab);
   }
   delete plan;
   return TPF_OK;
@@ -1306,7 +1548,7 @@ extern "C" tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double*
   for (int i = 1; i < n; ++i)
     same = same && plan->axes[axes[i]].geom.M == plan->axes[axes[0]].geom.M &&
            plan->axes[axes[i]].geom.P == plan->axes[axes[0]].geom.P &&
-           plan->axes[axes[i]].wave == plan->axes[axes[0]].wave;
+           (plan->axes[axes[i]].wave == 1) == (plan->axes[axes[0]].wave == 1);
   if (same) return launch_axes(plan, axes, n, f, o1, o2, (hipStream_t)stream);
   for (int i = 0; i < n; ++i)
     TPF_TRY(launch_axes(plan, &axes[i], 1, f, &o1[i], &o2[i], (hipStream_t)stream));
@@ -1335,15 +1577,16 @@ extern "C" tpf_status tpf_test_ccd_host_model(tpf_ccd_plan* plan, const double* 
   const CcdLines ln = lines_for(plan, axis);
   for (int64_t b = 0; b < ln.n_lines; ++b) {
     const int64_t base = (b / ln.inner) * ln.outer + (b % ln.inner) * ln.inner_stride;
-    host_model_line(ax, f + base, ln.stride, d1 ? d1 + base : nullptr, d2 ? d2 + base : nullptr);
+    (ax.wave == 2 ? host_model_line_explicit : host_model_line)(
+        ax, f + base, ln.stride, d1 ? d1 + base : nullptr, d2 ? d2 + base : nullptr);
   }
   return TPF_OK;
 }
 
 // Test hook (host only): the workgroup -> (slot, slot-local block) map of a
 // fused launch whose slots have counts[0..nslots) workgroups.
-extern "C" tpf_status tpf_test_ccd_block_map(int nslots, const unsigned* counts, int* slot,
-                                             unsigned* local) {
+extern "C" tpf_status tpf_test_ccd_block_map(int nslots, const unsigned* counts,
+                                             unsigned strided_mask, int* slot, unsigned* local) {
   if (nslots < 1 || nslots > 3 || !counts || !slot || !local) return fail(TPF_ERR_VALUE, "bad args");
   CcdLaunch L{};
   L.nslots = nslots;
@@ -1353,14 +1596,17 @@ extern "C" tpf_status tpf_test_ccd_block_map(int nslots, const unsigned* counts,
     total += counts[i];
   }
   L.block_begin[nslots] = total;
-  for (unsigned b = 0; b < total; ++b) slot_of(L, b, slot[b], local[b]);
+  for (unsigned b = 0; b < total; ++b) {
+    slot_of(L, b, slot[b], local[b]);
+    if (strided_mask >> slot[b] & 1u) local[b] = xcd_local(local[b], counts[slot[b]]);
+  }
   return TPF_OK;
 }
 
 extern "C" tpf_status tpf_ccd_plan_set_launch(tpf_ccd_plan* plan, int axis, int mode,
                                               int rows_cap) {
   if (!plan || axis < 0 || axis >= plan->ndim) return fail(TPF_ERR_VALUE, "bad plan/axis");
-  if (mode < 0 || mode > 2) return fail(TPF_ERR_VALUE, "mode must be 0 (auto), 1 or 2");
+  if (mode < 0 || mode > 3) return fail(TPF_ERR_VALUE, "mode must be 0 (auto), 1, 2 or 3");
   if (rows_cap != 0 && rows_cap != 8 && rows_cap != 16 && rows_cap != 32 && rows_cap != 64)
     return fail(TPF_ERR_VALUE, "rows cap must be 0 (default), 8, 16, 32 or 64");
   // validate on the host before dropping the current tables
@@ -1379,6 +1625,7 @@ extern "C" tpf_status tpf_ccd_plan_set_launch(tpf_ccd_plan* plan, int axis, int 
   if (ax.d_qtab) TPF_HIP(hipFree(ax.d_qtab));
   if (ax.d_rowtab32) TPF_HIP(hipFree(ax.d_rowtab32));
   if (ax.d_qtab32) TPF_HIP(hipFree(ax.d_qtab32));
+  if (ax.d_ctab) TPF_HIP(hipFree(ax.d_ctab));
   ax = CcdAxisHost();
   return TPF_OK;
 }
@@ -1386,7 +1633,7 @@ extern "C" tpf_status tpf_ccd_plan_set_launch(tpf_ccd_plan* plan, int axis, int 
 // ---- batches of independent problems ---------------------------------------
 struct tpf_ccd_batch {
   struct Group {
-    int M, P, wave;
+    int M, P, wave;  // wave: 0 lane recurrences, 1 wavefront per line, 2 explicit inverse
     int first, count;  // slots [first, first + count)
     unsigned blocks;
     size_t shmem;
@@ -1438,7 +1685,7 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
       items.push_back({j, k++, a});
     }
   }
-  // group by launch shape (M, P), keeping the input order inside a group
+  // group by kernel and launch shape (M, P), keeping the input order inside a group
   struct Shape {
     int wave, M, P;
     bool operator==(const Shape& o) const { return wave == o.wave && M == o.M && P == o.P; }
@@ -1462,15 +1709,16 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
       S.g = g;
       S.ln = lines_for(pl, it.axis);
       S.contig = S.ln.stride == 1;
-      S.tiled = !sh.wave && S.contig && tile_fits(g, pl->n_nodes[it.axis]);
+      S.tiled = sh.wave == 0 && S.contig && tile_fits(g, pl->n_nodes[it.axis]);
       b->slots.push_back(S);
       b->slot_problem.push_back(it.problem);
       b->slot_k.push_back(it.k);
       b->begin.push_back(blocks);
-      const int64_t per = sh.wave ? kWaveLines : 64;
+      const int64_t per = sh.wave == 1 ? kWaveLines : 64;
       blocks += (unsigned)((S.ln.n_lines + per - 1) / per);
-      if (!sh.wave) G.shmem = std::max(G.shmem, lds_bytes(g, S.contig, pl->n_nodes[it.axis]));
-      else G.shmem = std::max(G.shmem, wave_lds_bytes(g));
+      if (sh.wave == 0) G.shmem = std::max(G.shmem, lds_bytes(g, S.contig, pl->n_nodes[it.axis]));
+      else if (sh.wave == 1) G.shmem = std::max(G.shmem, wave_lds_bytes(g));
+      else G.shmem = std::max(G.shmem, sizeof(double) * 4 * (size_t)g.P * 64);
       G.count += 1;
     }
     b->begin.push_back(blocks);
@@ -1491,8 +1739,9 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
   }
   std::copy(b->slots.begin(), b->slots.end(), b->h_slots);
   for (const auto& G : b->groups) {
-    const void* k = G.wave ? (const void*)pick_wave_batch_kernel(G.M)
-                           : (const void*)pick_batch_kernel(G.M);
+    const void* k = G.wave == 1   ? (const void*)pick_wave_batch_kernel(G.M)
+                    : G.wave == 2 ? (const void*)pick_explicit_batch_kernel(G.M)
+                                  : (const void*)pick_batch_kernel(G.M);
     if (G.shmem == 0) continue;
     const tpf_status st = raise_lds_limit(k, G.shmem);
     if (st != TPF_OK) {
@@ -1538,12 +1787,13 @@ extern "C" tpf_status tpf_ccd_batch_apply(tpf_ccd_batch* b, const double* const*
   TPF_HIP(hipEventRecord(b->staged, st));
   for (size_t gi = 0; gi < b->groups.size(); ++gi) {
     const auto& G = b->groups[gi];
-    if (G.wave)
+    if (G.wave == 1)
       hipLaunchKernelGGL(pick_wave_batch_kernel(G.M), dim3(G.blocks), dim3(64 * kWaveLines),
                          G.shmem, st, (const CcdSlot*)(b->d_slots + G.first),
                          (const unsigned*)(b->d_begin + b->begin_off[gi]), G.count);
     else
-      hipLaunchKernelGGL(pick_batch_kernel(G.M), dim3(G.blocks), dim3(64 * G.P), G.shmem, st,
+      hipLaunchKernelGGL(G.wave == 2 ? pick_explicit_batch_kernel(G.M) : pick_batch_kernel(G.M),
+                         dim3(G.blocks), dim3(64 * G.P), G.shmem, st,
                          (const CcdSlot*)(b->d_slots + G.first),
                          (const unsigned*)(b->d_begin + b->begin_off[gi]), G.count);
     TPF_HIP(hipGetLastError());

@@ -139,6 +139,7 @@ tpf_status build_axis_tables(CcdAxisHost& ax, int n_nodes, double h, int periodi
 
   // Local block LU + spikes per partition.
   ax.rowtab.assign(size_t(P) * M * kRowTab, 0.0);
+  ax.ctab.clear();
   std::vector<M2> Vfirst(P), Vlast(P), Wfirst(P), Wlast(P);
   for (int p = 0; p < P; ++p) {
     const int s0 = g.s[p], m = g.s[p + 1] - g.s[p];
@@ -189,6 +190,77 @@ tpf_status build_axis_tables(CcdAxisHost& ax, int n_nodes, double h, int periodi
       put(row + 4, iD[r]);
       put(row + 8, V[r]);
       put(row + 12, W[r]);
+    }
+    if (M <= kMaxExplicitRows && P <= kMaxParts) {
+      // Partition inverse applied to the line values, in first differences:
+      // y_r = sum_k C[r][k] fv[k] with C[r][.] the local solution (no
+      // neighbour coupling) for fv = e_k, wall closures included.  Every row
+      // of C sums to 0 (constants have no derivative), so
+      // y_r = sum_{k <= m} S[r][k] (fv[k+1] - fv[k]) with S[r][k] =
+      // -sum_{j <= k} C[r][j]; the differences of smooth data are small and
+      // exact, which keeps f'' free of the cancellation a sum over raw values
+      // would carry.  Identical partitions share a type.
+      const int K = M + 2;
+      std::vector<ld> Cl(size_t(M) * K * 2, 0.0L);
+      std::vector<double> Cp(size_t(M) * K * 2, 0.0);
+      for (int k = 0; k < m + 2; ++k) {
+        std::vector<ld> fv(m + 2, 0.0L), y(2 * m), x(2 * m);
+        fv[k] = 1.0L;
+        for (int r = 0; r < m; ++r) {
+          ld r0 = (ld)kA1 * (fv[r + 2] - fv[r]);
+          ld r1 = (ld)kA2 * ((fv[r] - 2.0L * fv[r + 1]) + fv[r + 2]);
+          if (!periodic && s0 + r == 0) {
+            ld b0 = 0, b1 = 0;
+            for (int q = 0; q < 4; ++q) b0 += (ld)g.cIl[q] * fv[q];
+            for (int q = 0; q < 6; ++q) b1 += (ld)g.cIIl[q] * fv[q];
+            r0 -= kLb.a * b0 + kLb.b * b1;
+            r1 -= kLb.c * b0 + kLb.d * b1;
+          }
+          if (!periodic && s0 + r == n - 1) {
+            ld b0 = 0, b1 = 0;
+            for (int q = 0; q < 4; ++q) b0 += (ld)g.cIr[q] * fv[m + 1 - q];
+            for (int q = 0; q < 6; ++q) b1 += (ld)g.cIIr[q] * fv[m + 1 - q];
+            r0 -= kUb.a * b0 + kUb.b * b1;
+            r1 -= kUb.c * b0 + kUb.d * b1;
+          }
+          if (r > 0) {
+            r0 -= Lf[r].a * y[2 * r - 2] + Lf[r].b * y[2 * r - 1];
+            r1 -= Lf[r].c * y[2 * r - 2] + Lf[r].d * y[2 * r - 1];
+          }
+          y[2 * r] = r0;
+          y[2 * r + 1] = r1;
+        }
+        for (int r = m - 1; r >= 0; --r) {
+          ld a = y[2 * r], c = y[2 * r + 1];
+          if (r < m - 1) {
+            a -= kUb.a * x[2 * r + 2] + kUb.b * x[2 * r + 3];
+            c -= kUb.c * x[2 * r + 2] + kUb.d * x[2 * r + 3];
+          }
+          x[2 * r] = iD[r].a * a + iD[r].b * c;
+          x[2 * r + 1] = iD[r].c * a + iD[r].d * c;
+        }
+        for (int r = 0; r < m; ++r) {
+          Cl[(size_t(r) * K + k) * 2 + 0] = x[2 * r];
+          Cl[(size_t(r) * K + k) * 2 + 1] = x[2 * r + 1];
+        }
+      }
+      for (int r = 0; r < m; ++r)
+        for (int c = 0; c < 2; ++c) {
+          ld acc = 0.0L;
+          for (int k = 0; k <= m; ++k) {
+            acc += Cl[(size_t(r) * K + k) * 2 + c];
+            Cp[(size_t(r) * K + k) * 2 + c] = (double)(-acc);
+          }
+        }
+      const size_t tsz = Cp.size();
+      int type = -1;
+      for (size_t t = 0; t * tsz < ax.ctab.size() && type < 0; ++t)
+        if (std::memcmp(&ax.ctab[t * tsz], Cp.data(), tsz * sizeof(double)) == 0) type = (int)t;
+      if (type < 0) {
+        type = (int)(ax.ctab.size() / tsz);
+        ax.ctab.insert(ax.ctab.end(), Cp.begin(), Cp.end());
+      }
+      g.ctype[p] = type;
     }
   }
 
@@ -388,6 +460,77 @@ void host_model_line(const CcdAxisHost& ax, const double* f, int64_t stride, dou
       d2[0] = v0 * g.inv_h2;
       d2[int64_t(g.N) * stride] = vN * g.inv_h2;
     }
+  }
+}
+
+// Scalar model of the explicit-inverse kernel on one line (tests of ctab).
+void host_model_line_explicit(const CcdAxisHost& ax, const double* f, int64_t stride, double* d1,
+                              double* d2) {
+  const CcdAxisGeom& g = ax.geom;
+  const int P = g.P, M = g.M, NQ = g.NQ, K = M + 2;
+  std::vector<double> G(size_t(4) * P), fvs(size_t(P) * K, 0.0);
+  auto node = [&](int j) { return f[int64_t(j) * stride]; };
+  auto row = [&](int p, int r, int c) {  // sum_k S[r][k] (fv[k+1] - fv[k])
+    const double* S = &ax.ctab[size_t(g.ctype[p]) * M * K * 2];
+    const double* fv = &fvs[size_t(p) * K];
+    double acc = 0.0;
+    for (int k = 0; k + 1 < K; ++k) acc += S[(size_t(r) * K + k) * 2 + c] * (fv[k + 1] - fv[k]);
+    return acc;
+  };
+  for (int p = 0; p < P; ++p) {
+    const int s0 = g.s[p], m = g.s[p + 1] - s0;
+    for (int k = 0; k < m + 2; ++k)
+      fvs[size_t(p) * K + k] = node(g.periodic ? ((s0 - 1 + k) % g.N + g.N) % g.N : s0 + k);
+    G[4 * p + 0] = row(p, 0, 0);
+    G[4 * p + 1] = row(p, 0, 1);
+    G[4 * p + 2] = row(p, m - 1, 0);
+    G[4 * p + 3] = row(p, m - 1, 1);
+  }
+  std::vector<double> X(size_t(2) * g.n);
+  for (int p = 0; p < P; ++p) {
+    const int s0 = g.s[p], m = g.s[p + 1] - s0;
+    double lr[4] = {0, 0, 0, 0};
+    for (int o = 0; o < 4; ++o)
+      for (int t = 0; t < NQ; ++t) {
+        int q = g.rel ? p + g.d0 + t : t;
+        q = g.periodic ? ((q % P) + P) % P : std::min(std::max(q, 0), P - 1);
+        for (int c = 0; c < 4; ++c)
+          lr[o] += ax.qtab[((size_t(p) * 4 + o) * NQ + t) * 4 + c] * G[4 * q + c];
+      }
+    for (int r = 0; r < m; ++r) {
+      const double* t = &ax.rowtab[(size_t(p) * M + r) * kRowTab + 8];
+      X[2 * (s0 + r)] = row(p, r, 0) - (t[0] * lr[0] + t[1] * lr[1] + t[4] * lr[2] + t[5] * lr[3]);
+      X[2 * (s0 + r) + 1] =
+          row(p, r, 1) - (t[2] * lr[0] + t[3] * lr[1] + t[6] * lr[2] + t[7] * lr[3]);
+    }
+  }
+  const int off = g.periodic ? 0 : 1;
+  for (int j = 0; j < g.n; ++j) {
+    if (d1) d1[int64_t(j + off) * stride] = X[2 * j] * g.inv_h;
+    if (d2) d2[int64_t(j + off) * stride] = X[2 * j + 1] * g.inv_h2;
+  }
+  if (g.periodic) {
+    if (d1) d1[int64_t(g.N) * stride] = X[0] * g.inv_h;
+    if (d2) d2[int64_t(g.N) * stride] = X[1] * g.inv_h2;
+    return;
+  }
+  double b0 = 0, b1 = 0, c0 = 0, c1 = 0;
+  for (int k = 0; k < 4; ++k) {
+    b0 += g.cIl[k] * node(k);
+    c0 += g.cIr[k] * node(g.N - k);
+  }
+  for (int k = 0; k < 6; ++k) {
+    b1 += g.cIIl[k] * node(k);
+    c1 += g.cIIr[k] * node(g.N - k);
+  }
+  const double *x1 = &X[0], *xl = &X[2 * (g.n - 1)];
+  if (d1) {
+    d1[0] = (g.Ml[0] * x1[0] + g.Ml[1] * x1[1] + b0) * g.inv_h;
+    d1[int64_t(g.N) * stride] = (g.Mr[0] * xl[0] + g.Mr[1] * xl[1] + c0) * g.inv_h;
+  }
+  if (d2) {
+    d2[0] = (g.Ml[2] * x1[0] + g.Ml[3] * x1[1] + b1) * g.inv_h2;
+    d2[int64_t(g.N) * stride] = (g.Mr[2] * xl[0] + g.Mr[3] * xl[1] + c1) * g.inv_h2;
   }
 }
 

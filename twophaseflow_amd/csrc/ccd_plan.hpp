@@ -9,15 +9,18 @@
 namespace tpf {
 
 struct CcdAxisHost {
-  int wave = 0;                 // 1: one wavefront per line (lane = partition)
+  int wave = 0;                 // 1: one wavefront per line (lane = partition);
+                                // 2: lane per line, explicit partition inverse
   CcdAxisGeom geom{};           // device pointers filled after upload
   std::vector<double> rowtab;   // [P][M][kRowTab]
   std::vector<double> qtab;     // [P][4][NQ*4]
+  std::vector<double> ctab;     // [types][M][M+2][2] (M <= 16), geom.ctype maps partitions
   double trunc_coeff = 0.0;     // largest dropped reduced-system coefficient (relative)
   double* d_rowtab = nullptr;
   double* d_qtab = nullptr;
   float* d_rowtab32 = nullptr;  // FP32-refined plans
   float* d_qtab32 = nullptr;
+  double* d_ctab = nullptr;
 };
 
 // Smallest rows-per-partition cap in {8,16,32,64} that fits n unknowns into
@@ -32,6 +35,9 @@ tpf_status build_axis_tables(CcdAxisHost& ax, int n_nodes, double h, int periodi
 // Scalar model of the kernel on one line with node stride `stride`.
 void host_model_line(const CcdAxisHost& ax, const double* f, int64_t stride, double* d1,
                      double* d2);
+// The same for the explicit partition-inverse kernel (tables of mode 3).
+void host_model_line_explicit(const CcdAxisHost& ax, const double* f, int64_t stride, double* d1,
+                              double* d2);
 
 }  // namespace tpf
 
