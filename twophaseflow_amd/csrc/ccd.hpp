@@ -34,7 +34,7 @@ struct CcdAxisGeom {
   const double* qtab;    // [P][4][NQ*4]: rows (left0,left1,right0,right1)
   const float* rowtab32; // FP32 copies (FP32-refined plans only)
   const float* qtab32;
-  const double* ctab;    // [types][M][M+2][2]: partition inverse applied to f (explicit mode)
+  const double* ctab;    // [types][2][M][M+2]: partition inverse on first differences (explicit mode)
   int n;                 // unknowns per line (N periodic, N-1 wall)
   int N;                 // cells along the axis
   int P, M, NQ, d0;      // partitions, rows cap, coupling window, window start

@@ -1031,13 +1031,14 @@ __device__ __forceinline__ void ccd_lines_explicit(const CcdAxisGeom& g, const C
   double* G = lds;  // [4P][64]
   {
     double a0 = 0.0, a1 = 0.0, z0 = 0.0, z1 = 0.0;
-    const kdptr cl = C + (size_t)(m - 1) * K * 2;
+    const kdptr c0 = C, c1 = C + M * K;  // row 0, components 0 and 1
+    const kdptr l0 = C + (m - 1) * K, l1 = C + (M + m - 1) * K;  // row m-1
 #pragma unroll
     for (int k = 0; k + 1 < K; ++k) {
-      a0 += C[2 * k] * fv[k];
-      a1 += C[2 * k + 1] * fv[k];
-      z0 += cl[2 * k] * fv[k];
-      z1 += cl[2 * k + 1] * fv[k];
+      a0 += c0[k] * fv[k];
+      a1 += c1[k] * fv[k];
+      z0 += l0[k] * fv[k];
+      z1 += l1[k] * fv[k];
     }
     G[(p * 4 + 0) * 64 + lane] = a0;
     G[(p * 4 + 1) * 64 + lane] = a1;
@@ -1076,12 +1077,12 @@ __device__ __forceinline__ void ccd_lines_explicit(const CcdAxisGeom& g, const C
 #pragma unroll
   for (int r = 0; r < M; ++r) {
     if (r < m) {
-      const kdptr cr = C + r * K * 2;
+      const kdptr cu = C + r * K, cv = C + (M + r) * K;
       double u = 0.0, v = 0.0;
 #pragma unroll
       for (int k = 0; k + 1 < K; ++k) {
-        u += cr[2 * k] * fv[k];
-        v += cr[2 * k + 1] * fv[k];
+        u += cu[k] * fv[k];
+        v += cv[k] * fv[k];
       }
       const kdptr t = T + r * kRowTab + 8;
       u -= t[0] * lr0 + t[1] * lr1 + t[4] * lr2 + t[5] * lr3;

@@ -244,12 +244,12 @@ tpf_status build_axis_tables(CcdAxisHost& ax, int n_nodes, double h, int periodi
           Cl[(size_t(r) * K + k) * 2 + 1] = x[2 * r + 1];
         }
       }
-      for (int r = 0; r < m; ++r)
+      for (int r = 0; r < m; ++r)  // component-major: [c][r][k]
         for (int c = 0; c < 2; ++c) {
           ld acc = 0.0L;
           for (int k = 0; k <= m; ++k) {
             acc += Cl[(size_t(r) * K + k) * 2 + c];
-            Cp[(size_t(r) * K + k) * 2 + c] = (double)(-acc);
+            Cp[(size_t(c) * M + r) * K + k] = (double)(-acc);
           }
         }
       const size_t tsz = Cp.size();
@@ -474,7 +474,7 @@ void host_model_line_explicit(const CcdAxisHost& ax, const double* f, int64_t st
     const double* S = &ax.ctab[size_t(g.ctype[p]) * M * K * 2];
     const double* fv = &fvs[size_t(p) * K];
     double acc = 0.0;
-    for (int k = 0; k + 1 < K; ++k) acc += S[(size_t(r) * K + k) * 2 + c] * (fv[k + 1] - fv[k]);
+    for (int k = 0; k + 1 < K; ++k) acc += S[(size_t(c) * M + r) * K + k] * (fv[k + 1] - fv[k]);
     return acc;
   };
   for (int p = 0; p < P; ++p) {

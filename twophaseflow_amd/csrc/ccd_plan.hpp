@@ -14,7 +14,7 @@ struct CcdAxisHost {
   CcdAxisGeom geom{};           // device pointers filled after upload
   std::vector<double> rowtab;   // [P][M][kRowTab]
   std::vector<double> qtab;     // [P][4][NQ*4]
-  std::vector<double> ctab;     // [types][M][M+2][2] (M <= 16), geom.ctype maps partitions
+  std::vector<double> ctab;     // [types][2][M][M+2] (M <= 16), geom.ctype maps partitions
   double trunc_coeff = 0.0;     // largest dropped reduced-system coefficient (relative)
   double* d_rowtab = nullptr;
   double* d_qtab = nullptr;
