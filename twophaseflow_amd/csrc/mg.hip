@@ -80,6 +80,7 @@ __device__ inline void lcoords(const LevelGeom& g, int64_t idx, int (&c)[3]) {
   for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < (total); \
        idx += (int64_t)gridDim.x * kBlock)
 
+
 // Fine conductances from rho (FP64 in, FP32 out).
 __global__ __launch_bounds__(kBlock) void k_fine_T(PpeGeom pg, LevelGeom g,
                                                    const double* __restrict__ rho,
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_rbgs(LevelGeom g, int color, const f
                                                  const float* __restrict__ dinv,
                                                  const float* __restrict__ b, float* x) {
   const float* const T[3] = {T0, T1, T2};
-  GRID_LOOP(idx, g.total) {
+  CHUNK_LOOP(idx, g.total) {
     int c[3];
     lcoords(g, idx, c);
     if (((c[0] + c[1] + c[2]) & 1) != color) continue;
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_resid(LevelGeom g, const float* __re
                                                   const float* __restrict__ x,
                                                   float* __restrict__ r) {
   const float* const T[3] = {T0, T1, T2};
-  GRID_LOOP(idx, g.total) {
+  CHUNK_LOOP(idx, g.total) {
     int c[3];
     lcoords(g, idx, c);
     float d;
@@ -272,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_rbgs_rho(LevelGeom g, FineGeom fg, i
                                                      const float* __restrict__ b, float* x) {
   const int half = (g.n[2] + 1) >> 1;
   const int64_t slots = (int64_t)g.n[0] * g.n[1] * half;
-  GRID_LOOP(u, slots) {
+  CHUNK_LOOP(u, slots) {
     const int64_t row = u / half;
     const int k = (int)(u - row * half);
     int c[3];
@@ -291,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_resid_rho(LevelGeom g, FineGeom fg,
                                                       const float* __restrict__ b,
                                                       const float* __restrict__ x,
                                                       float* __restrict__ r) {
-  GRID_LOOP(idx, g.total) {
+  CHUNK_LOOP(idx, g.total) {
     int c[3];
     lcoords(g, idx, c);
     float d;
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(LevelGeom g, const float* __re
 __global__ __launch_bounds__(kBlock) void k_restrict(LevelGeom f, LevelGeom c,
                                                      const float* __restrict__ rf,
                                                      float* __restrict__ bc) {
-  GRID_LOOP(idx, c.total) {
+  CHUNK_LOOP(idx, c.total) {
     int I[3];
     lcoords(c, idx, I);
     float acc = 0.0f;
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LevelGeom f, LevelGeom c,
 __global__ __launch_bounds__(kBlock) void k_prolong_add(LevelGeom f, LevelGeom c,
                                                         const float* __restrict__ xc,
                                                         float* __restrict__ xf) {
-  GRID_LOOP(idx, f.total) {
+  CHUNK_LOOP(idx, f.total) {
     if (idx == f.pin) continue;
     int j[3];
     lcoords(f, idx, j);
@@ -511,10 +512,7 @@ tpf_status descend(MgState* m, hipStream_t st, int from, int to, int nu) {
     if (L.rho32)
       MG_LAUNCH(k_resid_rho, L.g.total, L.g, L.fg, L.b, L.x, L.r);
     else
-      if (L.rho32)
-    MG_LAUNCH(k_resid_rho, L.g.total, L.g, L.fg, L.b, L.x, L.r);
-  else
-    MG_LAUNCH(k_resid, L.g.total, L.g, L.T[0], L.T[1], L.T[2], L.b, L.x, L.r);
+      MG_LAUNCH(k_resid, L.g.total, L.g, L.T[0], L.T[1], L.T[2], L.b, L.x, L.r);
     MG_LAUNCH(k_restrict, C.g.total, L.g, C.g, L.r, C.b);
   }
   return TPF_OK;

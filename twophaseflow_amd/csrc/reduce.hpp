@@ -7,6 +7,25 @@ namespace tpf {
 
 constexpr int kBlock = 256;
 
+// FP32 stencil sweeps: each workgroup takes one contiguous chunk of the index
+// range, and the chunks are numbered XCD by XCD (workgroups are dispatched
+// to the 8 XCDs round-robin), so an XCD sweeps one contiguous slab and the
+// +-1 row / plane neighbours a workgroup reads were fetched into the same
+// L2 by its neighbouring workgroups.  Measured on the multigrid levels
+// (257^3 W-cycle FCG 32.7 -> 28.0 ms); the FP64 CG kernels stay on
+// grid-stride loops, which measured faster there (one wide sweep front).
+__device__ inline void xcd_chunk(int64_t total, int64_t& begin, int64_t& end) {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned lb = nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
+  const int64_t chunk = (total + nb - 1) / nb;
+  begin = (int64_t)lb * chunk;
+  end = begin + chunk < total ? begin + chunk : total;
+}
+#define CHUNK_LOOP(idx, total)                                   \
+  int64_t idx##_b, idx##_e;                                      \
+  xcd_chunk((total), idx##_b, idx##_e);                          \
+  for (int64_t idx = idx##_b + threadIdx.x; idx < idx##_e; idx += kBlock)
+
 // Sums K doubles over a block of kBlock threads; thread 0 gets the totals.
 template <int K>
 __device__ inline void block_sum(double (&v)[K], double* sh) {
