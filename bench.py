@@ -35,9 +35,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "256^3 PPE time-to-1e-8: learned-FCG vs plain PCG; CCD Mpts/s & HBM %"
 HBM_PEAK_GBS = 8000.0
-# measured HBM bytes of one all-axes ccd_line_kernel<16> launch on 257^3:
-# FETCH_SIZE 239 MiB x 2 + WRITE_SIZE 887 MiB x 1
-TRAFFIC_BYTES_CCD = int((239_000 * 2 + 887_000) * 1024)
+# measured HBM bytes of one all-axes ccd_line_kernel<16> launch on 257^3
+# (KiB per dispatch, mean of 3): FETCH_SIZE 199969 x 2 + WRITE_SIZE 862913 x 1
+TRAFFIC_BYTES_CCD = int((199_969 * 2 + 862_913) * 1024)
 
 
 def log(msg):
@@ -338,11 +338,15 @@ def profile_mode(args):
         d2 = torch.empty_like(d1)
         for _ in range(args.steps):
             ccd.apply_into(f, d1, d2, 0b111)
+        for _ in range(args.steps):  # single-axis calls: explicit kernel on axes 0, 1
+            for axis in range(3):
+                ccd.differentiate_into(f, d1[axis], d2[axis], axis)
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        log(f"profile ccd: {args.steps} steps of 3 axis launches on 257^3, then a 1 GiB copy")
+        log(f"profile ccd: {args.steps} fused 3-axis launches and {args.steps} x 3 single-axis "
+            f"launches on 257^3, then a 1 GiB copy")
     elif args.profile == "ppe":
         gp, rho, rhs = ppe_case(N=128)
         r = ppe_solve_timed(gp, rho, rhs, "fp64")
@@ -407,6 +411,18 @@ def main():
     bytes_per_launch = 8 * (1 + 2 * 3) * npts
     achieved = bytes_per_launch / (dom_ms * 1e-3) / 1e9
     all_axes_gbs = 8.0 * 7 * npts / sec / 1e9
+    # axes 0 and 1 alone: one grid of the explicit partition-inverse kernel
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(max(args.steps, 5))]
+    for a, b in evs:
+        a.record()
+        ccd.apply_into(f, d1[:2], d2[:2], 0b011)
+        b.record()
+    torch.cuda.synchronize()
+    s_ms = sum(a.elapsed_time(b) for a, b in evs) / len(evs)
+    strided = {"axes": [0, 1], "kernel": "ccd_explicit_kernel<16> (one grid)",
+               "ms": round(s_ms, 4),
+               "algorithmic_GBps": round(8 * (1 + 2 * 2) * npts / (s_ms * 1e-3) / 1e9, 1)}
     log("timing 3-D CCD, FP32 solves + FP64 refinement")
     ccd_r = CCDSolver(g, bc_type="periodic", precision="fp32_refined")
     r1 = torch.empty_like(d1)
@@ -457,8 +473,9 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "launch_ms": round(dom_ms, 4)},
         "hbm_copy_GBps": round(copy_bandwidth_GBps(), 1),
         "ccd_all_axes_GBps": round(all_axes_gbs, 1),
-        "ccd_axes": [{"axis": a["axis"], "ms": round(a["ms"], 4), "GBps": round(a["GBps"], 1)}
-                     for a in axes],
+        "ccd_axes": [{"axis": a["axis"], "ms": round(a["ms"], 4), "GBps": round(a["GBps"], 1),
+                      "kernel": a["info"]["kernel"]} for a in axes],
+        "ccd_strided_axes_fused": strided,
         "ccd_2d_1025": {"Mpts_per_s": round(mpts2, 2), "ms_per_step": round(sec2 * 1e3, 4)},
         "ccd_fp32_refined_257": refined,
         "ccd_batch_uneven_2d": ccd_batch_bench(),

@@ -76,6 +76,12 @@ TPF_API tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double* d1
 TPF_API tpf_status tpf_ccd_axis_info(tpf_ccd_plan* plan, int axis, int* partitions,
                              int* max_rows, int* window, double* trunc_coeff);
 
+/* Kernel a single-axis call on `axis` runs (no counterpart in the
+ * reference): 1 lane per line with block recurrences, 2 wavefront per
+ * line, 3 lane per line with the explicit partition inverse (the values of
+ * tpf_ccd_plan_set_launch's mode). */
+TPF_API tpf_status tpf_ccd_axis_kernel(tpf_ccd_plan* plan, int axis, int* kernel);
+
 /* Launch shape of one axis (no counterpart in the reference: the knobs the
  * autotuner in twophaseflow_amd/autotune.py sets).  mode 0: automatic (a
  * wavefront per line for < 4096 lines of > 512 unknowns; else a lane per

@@ -1,7 +1,8 @@
 """GPU checks of the batched CCD launch and the launch-shape autotuner.
 
-A batch runs the same per-line arithmetic as tpf_ccd_apply, so every
-problem's f' and f'' must equal the single-problem results bit for bit;
+A batch runs the same per-line arithmetic (and kernel choice) as
+tpf_ccd_apply on each problem's axes, so every problem's f' and f'' must
+equal the single-problem results bit for bit;
 other rows caps change the partitioning and agree with the dense-LU oracle
 to 1e-12 (the CCD parity bound).
 """
@@ -43,20 +44,29 @@ def test_batch_equals_single_problem_launches():
     d2s = [torch.empty_like(d) for d in d1s]
     batch.apply_into(fs, d1s, d2s)
     for j, (s, f, a) in enumerate(zip(solvers, fs, axes)):
+        r1 = torch.empty_like(d1s[j])
+        r2 = torch.empty_like(d2s[j])
+        s.apply_into(f, r1, r2, sum(1 << ax for ax in a))
         for k, ax in enumerate(a):
-            r1 = torch.empty_like(f)
-            r2 = torch.empty_like(f)
-            s.differentiate_into(f, r1, r2, ax)
-            assert torch.equal(d1s[j][k], r1) and torch.equal(d2s[j][k], r2), (j, ax)
+            assert torch.equal(d1s[j][k], r1[k]) and torch.equal(d2s[j][k], r2[k]), (j, ax)
     # repeated calls reuse the device tables (pointers restaged per call)
     fs2 = [f * 2.0 for f in fs]
     batch.apply_into(fs2, d1s, d2s)
     torch.cuda.synchronize()
     s0 = solvers[0]
-    r1 = torch.empty_like(fs2[0])
-    r2 = torch.empty_like(fs2[0])
-    s0.differentiate_into(fs2[0], r1, r2, 1)
-    assert torch.equal(d1s[0][1], r1)
+    r1 = torch.empty_like(d1s[0])
+    r2 = torch.empty_like(d2s[0])
+    s0.apply_into(fs2[0], r1, r2, 0b11)
+    assert torch.equal(d1s[0][1], r1[1])
+    # a strided axis alone runs the explicit inverse in the batch as in a single call
+    sb = _solver((128, 128), "wall")
+    bb = CCDBatch([sb], [(0,)])
+    o1 = torch.empty((1, 129, 129), dtype=torch.float64, device="cuda")
+    o2 = torch.empty_like(o1)
+    bb.apply_into([fs[0]], [o1], [o2])
+    e1, e2 = sb.differentiate(fs[0], 0)
+    assert sb.axis_info(0)["kernel"] == "explicit"
+    assert torch.equal(o1[0], e1) and torch.equal(o2[0], e2)
 
 
 def test_batch_rejects_bad_shapes():

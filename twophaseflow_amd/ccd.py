@@ -63,8 +63,11 @@ class CCDSolver:
         t = C.c_double()
         nat.check(self._lib.tpf_ccd_axis_info(self._plan, axis, C.byref(P), C.byref(M),
                                                C.byref(W), C.byref(t)))
+        k = C.c_int()
+        nat.check(self._lib.tpf_ccd_axis_kernel(self._plan, axis, C.byref(k)))
+        kernel = {v: name for name, v in self.MODES.items()}[k.value]
         return {"partitions": P.value, "max_rows": M.value, "window": W.value,
-                "trunc_coeff": t.value}
+                "trunc_coeff": t.value, "kernel": kernel}
 
     # -- device-level entry points ---------------------------------------------
     MODES = {"auto": 0, "lane": 1, "wave": 2, "explicit": 3}

@@ -1568,6 +1568,14 @@ extern "C" tpf_status tpf_ccd_axis_info(tpf_ccd_plan* plan, int axis, int* parti
   return TPF_OK;
 }
 
+extern "C" tpf_status tpf_ccd_axis_kernel(tpf_ccd_plan* plan, int axis, int* kernel) {
+  if (!plan || !kernel || axis < 0 || axis >= plan->ndim) return fail(TPF_ERR_VALUE, "bad plan/axis");
+  CcdAxisHost& ax = plan->axes[axis];
+  if (ax.rowtab.empty()) TPF_TRY(build_axis(plan, axis, ax));
+  *kernel = ax.wave == 1 ? 2 : (ax.wave == 2 ? 3 : 1);
+  return TPF_OK;
+}
+
 // Test hook (host only, no device): runs the scalar model of the kernel on
 // host arrays so the SPIKE tables can be checked without a GPU.
 extern "C" tpf_status tpf_test_ccd_host_model(tpf_ccd_plan* plan, const double* f, double* d1,
@@ -1659,7 +1667,7 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
   auto* b = new tpf_ccd_batch();
   b->nproblems = count;
   struct Item {
-    int problem, k, axis;
+    int problem, k, axis, kind;  // kind: the kernel tpf_ccd_apply would run for this axis
   };
   std::vector<Item> items;
   for (int j = 0; j < count; ++j) {
@@ -1676,6 +1684,7 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
     for (int a = 0; a < pl->ndim; ++a) total *= pl->n_nodes[a];
     b->totals.push_back(total);
     int k = 0;
+    const size_t first = items.size();
     for (int a = 0; a < pl->ndim; ++a) {
       if (!(axis_masks[j] & (1u << a))) continue;
       const tpf_status st = ensure_axis(pl, a);
@@ -1683,8 +1692,21 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
         delete b;
         return st;
       }
-      items.push_back({j, k++, a});
+      items.push_back({j, k++, a, pl->axes[a].wave});
     }
+    // tpf_ccd_apply's rule: axes of one launch shape go in one grid, and an
+    // explicit-inverse axis grouped with a recurrence axis runs as recurrence
+    bool same = true, all_explicit = true;
+    for (size_t i = first; i < items.size(); ++i) {
+      const CcdAxisHost& x = pl->axes[items[i].axis];
+      const CcdAxisHost& x0 = pl->axes[items[first].axis];
+      same = same && x.geom.M == x0.geom.M && x.geom.P == x0.geom.P &&
+             (x.wave == 1) == (x0.wave == 1);
+      all_explicit = all_explicit && x.wave == 2;
+    }
+    if (same && !all_explicit)
+      for (size_t i = first; i < items.size(); ++i)
+        if (items[i].kind == 2) items[i].kind = 0;
   }
   // group by kernel and launch shape (M, P), keeping the input order inside a group
   struct Shape {
@@ -1694,7 +1716,7 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
   std::vector<Shape> shapes;
   for (const Item& it : items) {
     const CcdAxisHost& ax = plans[it.problem]->axes[it.axis];
-    const Shape sh{ax.wave, ax.geom.M, ax.geom.P};
+    const Shape sh{it.kind, ax.geom.M, ax.geom.P};
     if (std::find(shapes.begin(), shapes.end(), sh) == shapes.end()) shapes.push_back(sh);
   }
   for (const auto& sh : shapes) {
@@ -1705,7 +1727,7 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
       tpf_ccd_plan* pl = plans[it.problem];
       const CcdAxisHost& ax = pl->axes[it.axis];
       const CcdAxisGeom& g = ax.geom;
-      if (!(Shape{ax.wave, g.M, g.P} == sh)) continue;
+      if (!(Shape{it.kind, g.M, g.P} == sh)) continue;
       CcdSlot S{};
       S.g = g;
       S.ln = lines_for(pl, it.axis);
