@@ -216,6 +216,17 @@ __global__ __launch_bounds__(kBlock) void k_rbgs(LevelGeom g, int color, const f
   }
 }
 
+// First forward half-sweep from x = 0 on a coarse level (fused zeroing).
+__global__ __launch_bounds__(kBlock) void k_rbgs_first(LevelGeom g, const float* __restrict__ dinv,
+                                                       const float* __restrict__ b,
+                                                       float* __restrict__ x) {
+  CHUNK_LOOP(idx, g.total) {
+    int c[3];
+    lcoords(g, idx, c);
+    x[idx] = ((c[0] + c[1] + c[2]) & 1) == 0 ? b[idx] * dinv[idx] : 0.0f;
+  }
+}
+
 // r = b - A x (r[pin] = 0)
 __global__ __launch_bounds__(kBlock) void k_resid(LevelGeom g, const float* __restrict__ T0,
                                                   const float* __restrict__ T1,
@@ -288,6 +299,40 @@ __global__ __launch_bounds__(kBlock) void k_rbgs_rho(LevelGeom g, FineGeom fg, i
   }
 }
 
+// First forward half-sweep from x = 0 (fused with the zeroing): colour-0
+// nodes take b / diag (their neighbours are all 0), colour-1 nodes 0.
+__global__ __launch_bounds__(kBlock) void k_rbgs_rho_first(LevelGeom g, FineGeom fg,
+                                                           const float* __restrict__ b,
+                                                           float* __restrict__ x) {
+  CHUNK_LOOP(idx, g.total) {
+    int c[3];
+    lcoords(g, idx, c);
+    float v = 0.0f;
+    if (((c[0] + c[1] + c[2]) & 1) == 0 && idx != g.pin) {
+      float dv[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        dv[a] = g.n[a] == 1 ? 1.0f
+                            : ((c[a] == 0 || c[a] == g.n[a] - 1) ? 0.5f * fg.h[a] : fg.h[a]);
+      const float ri = fg.rho[idx];
+      float d = 0.0f;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {  // the diagonal exactly as nbr_sum_rho forms it
+        if (g.n[a] == 1) continue;
+        float w = fg.ih2[a];
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+          if (e != a && g.n[e] > 1) w *= dv[e];
+        const int64_t st = lstride(g, a);
+        if (c[a] < g.n[a] - 1) d += w * __builtin_amdgcn_rcpf(ri + fg.rho[idx + st]);
+        if (c[a] > 0) d += w * __builtin_amdgcn_rcpf(ri + fg.rho[idx - st]);
+      }
+      v = b[idx] * __builtin_amdgcn_rcpf(d);
+    }
+    x[idx] = v;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_resid_rho(LevelGeom g, FineGeom fg,
                                                       const float* __restrict__ b,
                                                       const float* __restrict__ x,
@@ -323,68 +368,76 @@ __global__ __launch_bounds__(kBlock) void k_apply(LevelGeom g, const float* __re
 }
 
 // b_c = R r_f, full weighting (1/2 per off-centre offset and axis); b_c[pin] = 0.
+__device__ inline float restrict_node(const LevelGeom& f, const LevelGeom& c, const float* rf,
+                                      int64_t idx) {
+  int I[3];
+  lcoords(c, idx, I);
+  float acc = 0.0f;
+  for (int d0 = -1; d0 <= 1; ++d0) {
+    const int j0 = 2 * I[0] + d0;
+    if (c.n[0] == 1 ? d0 != 0 : (j0 < 0 || j0 >= f.n[0])) continue;
+    const float w0 = d0 == 0 ? 1.0f : 0.5f;
+    for (int d1 = -1; d1 <= 1; ++d1) {
+      const int j1 = 2 * I[1] + d1;
+      if (c.n[1] == 1 ? d1 != 0 : (j1 < 0 || j1 >= f.n[1])) continue;
+      const float w1 = w0 * (d1 == 0 ? 1.0f : 0.5f);
+      const int64_t row = ((int64_t)(c.n[0] == 1 ? 0 : j0) * f.n[1] + (c.n[1] == 1 ? 0 : j1)) *
+                          f.n[2];
+      for (int d2 = -1; d2 <= 1; ++d2) {
+        const int j2 = 2 * I[2] + d2;
+        if (j2 < 0 || j2 >= f.n[2]) continue;
+        acc += w1 * (d2 == 0 ? 1.0f : 0.5f) * rf[row + j2];
+      }
+    }
+  }
+  return idx == c.pin ? 0.0f : acc;
+}
+
 __global__ __launch_bounds__(kBlock) void k_restrict(LevelGeom f, LevelGeom c,
                                                      const float* __restrict__ rf,
                                                      float* __restrict__ bc) {
-  CHUNK_LOOP(idx, c.total) {
-    int I[3];
-    lcoords(c, idx, I);
-    float acc = 0.0f;
-    for (int d0 = -1; d0 <= 1; ++d0) {
-      const int j0 = 2 * I[0] + d0;
-      if (c.n[0] == 1 ? d0 != 0 : (j0 < 0 || j0 >= f.n[0])) continue;
-      const float w0 = d0 == 0 ? 1.0f : 0.5f;
-      for (int d1 = -1; d1 <= 1; ++d1) {
-        const int j1 = 2 * I[1] + d1;
-        if (c.n[1] == 1 ? d1 != 0 : (j1 < 0 || j1 >= f.n[1])) continue;
-        const float w1 = w0 * (d1 == 0 ? 1.0f : 0.5f);
-        const int64_t row = ((int64_t)(c.n[0] == 1 ? 0 : j0) * f.n[1] + (c.n[1] == 1 ? 0 : j1)) *
-                            f.n[2];
-        for (int d2 = -1; d2 <= 1; ++d2) {
-          const int j2 = 2 * I[2] + d2;
-          if (j2 < 0 || j2 >= f.n[2]) continue;
-          acc += w1 * (d2 == 0 ? 1.0f : 0.5f) * rf[row + j2];
-        }
-      }
-    }
-    bc[idx] = idx == c.pin ? 0.0f : acc;
-  }
+  CHUNK_LOOP(idx, c.total) bc[idx] = restrict_node(f, c, rf, idx);
 }
 
 // x_f += P x_c (trilinear interpolation); the fine pin stays 0.
+__device__ inline float prolong_node(const LevelGeom& f, const LevelGeom& c, const float* xc,
+                                     int64_t idx) {
+  int j[3];
+  lcoords(f, idx, j);
+  int lo[3], hi[3];
+  float wl[3], wh[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = j[a] >> 1;
+    if (j[a] & 1) {
+      hi[a] = lo[a] + 1;
+      wl[a] = 0.5f;
+      wh[a] = 0.5f;
+    } else {
+      hi[a] = lo[a];
+      wl[a] = 1.0f;
+      wh[a] = 0.0f;
+    }
+  }
+  float acc = 0.0f;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int i0 = (s & 4) ? hi[0] : lo[0];
+    const int i1 = (s & 2) ? hi[1] : lo[1];
+    const int i2 = (s & 1) ? hi[2] : lo[2];
+    const float w = ((s & 4) ? wh[0] : wl[0]) * ((s & 2) ? wh[1] : wl[1]) *
+                    ((s & 1) ? wh[2] : wl[2]);
+    if (w != 0.0f) acc += w * xc[((int64_t)i0 * c.n[1] + i1) * c.n[2] + i2];
+  }
+  return acc;
+}
+
 __global__ __launch_bounds__(kBlock) void k_prolong_add(LevelGeom f, LevelGeom c,
                                                         const float* __restrict__ xc,
                                                         float* __restrict__ xf) {
   CHUNK_LOOP(idx, f.total) {
     if (idx == f.pin) continue;
-    int j[3];
-    lcoords(f, idx, j);
-    int lo[3], hi[3];
-    float wl[3], wh[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = j[a] >> 1;
-      if (j[a] & 1) {
-        hi[a] = lo[a] + 1;
-        wl[a] = 0.5f;
-        wh[a] = 0.5f;
-      } else {
-        hi[a] = lo[a];
-        wl[a] = 1.0f;
-        wh[a] = 0.0f;
-      }
-    }
-    float acc = 0.0f;
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const int i0 = (s & 4) ? hi[0] : lo[0];
-      const int i1 = (s & 2) ? hi[1] : lo[1];
-      const int i2 = (s & 1) ? hi[2] : lo[2];
-      const float w = ((s & 4) ? wh[0] : wl[0]) * ((s & 2) ? wh[1] : wl[1]) *
-                      ((s & 1) ? wh[2] : wl[2]);
-      if (w != 0.0f) acc += w * xc[((int64_t)i0 * c.n[1] + i1) * c.n[2] + i2];
-    }
-    xf[idx] += acc;
+    xf[idx] += prolong_node(f, c, xc, idx);
   }
 }
 
@@ -445,6 +498,160 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(LevelGeom g, const float*
   if (live) x[idx] = xs[idx];
 }
 
+// A whole V-visit of the last two levels in one workgroup: level f
+// (<= kTwoLevelMax nodes, kTwoLevelPer per thread, coefficients held in
+// registers, iterate and residual in LDS) and the coarsest level c
+// (<= kCoarseMax nodes, one per thread, in LDS).  Pre-smoothing from x_f = 0
+// (or from the current x_f), residual, restriction, `sweeps` symmetric
+// coarse sweeps, prolongation, reverse post-smoothing; the per-node
+// arithmetic is that of k_rbgs, k_resid, k_restrict, k_coarse_solve and
+// k_prolong_add.  It replaces ~12 latency-bound launches by one.
+constexpr int kTwoLevelPer = 5;
+constexpr int kTwoLevelMax = 1024 * kTwoLevelPer;
+
+__global__ __launch_bounds__(1024) void k_two_level(
+    LevelGeom gf, const float* __restrict__ Tf0, const float* __restrict__ Tf1,
+    const float* __restrict__ Tf2, const float* __restrict__ dinvf, const float* __restrict__ bf,
+    float* __restrict__ xf, float* __restrict__ rf, LevelGeom gc, const float* __restrict__ Tc0,
+    const float* __restrict__ Tc1, const float* __restrict__ Tc2, const float* __restrict__ dinvc,
+    float* __restrict__ bc, float* __restrict__ xc, int nu, int sweeps, int from_zero) {
+  __shared__ float xs[kTwoLevelMax];
+  __shared__ float rs[kTwoLevelMax];
+  __shared__ float xcs[kCoarseMax];
+  __shared__ float bcs[kCoarseMax];
+  const int tid = threadIdx.x;
+  // fine-level coefficients of this thread's nodes
+  float thi[kTwoLevelPer][3], tlo[kTwoLevelPer][3], di[kTwoLevelPer], bi[kTwoLevelPer];
+  int cf[kTwoLevelPer][3], colf[kTwoLevelPer];
+  const float* const Tf[3] = {Tf0, Tf1, Tf2};
+#pragma unroll
+  for (int k = 0; k < kTwoLevelPer; ++k) {
+    const int64_t idx = tid + (int64_t)k * 1024;
+    colf[k] = -1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      thi[k][a] = tlo[k][a] = 0.0f;
+      cf[k][a] = 0;
+    }
+    di[k] = bi[k] = 0.0f;
+    if (idx < gf.total) {
+      lcoords(gf, idx, cf[k]);
+      colf[k] = (cf[k][0] + cf[k][1] + cf[k][2]) & 1;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (gf.n[a] == 1) continue;
+        if (cf[k][a] < gf.n[a] - 1) thi[k][a] = Tf[a][idx];
+        if (cf[k][a] > 0) tlo[k][a] = Tf[a][idx - lstride(gf, a)];
+      }
+      di[k] = dinvf[idx];
+      bi[k] = bf[idx];
+      xs[idx] = from_zero ? 0.0f : xf[idx];
+    }
+  }
+  __syncthreads();
+  // neighbour sum in nbr_sum's order
+  auto nsum = [&](int k, int64_t idx, float& diag) {
+    float s = 0.0f;
+    diag = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (gf.n[a] == 1) continue;
+      const int64_t st = lstride(gf, a);
+      if (cf[k][a] < gf.n[a] - 1) {
+        s += thi[k][a] * xs[idx + st];
+        diag += thi[k][a];
+      }
+      if (cf[k][a] > 0) {
+        s += tlo[k][a] * xs[idx - st];
+        diag += tlo[k][a];
+      }
+    }
+    return s;
+  };
+  auto smooth_f = [&](bool reverse) {
+    for (int sw = 0; sw < nu; ++sw)
+      for (int h = 0; h < 2; ++h) {
+        const int color = reverse ? 1 - h : h;
+#pragma unroll
+        for (int k = 0; k < kTwoLevelPer; ++k) {
+          const int64_t idx = tid + (int64_t)k * 1024;
+          if (colf[k] == color) {
+            float d;
+            const float s = nsum(k, idx, d);
+            xs[idx] = (bi[k] + s) * di[k];
+          }
+        }
+        __syncthreads();
+      }
+  };
+  smooth_f(false);
+  // residual (pin row 0)
+#pragma unroll
+  for (int k = 0; k < kTwoLevelPer; ++k) {
+    const int64_t idx = tid + (int64_t)k * 1024;
+    if (colf[k] >= 0) {
+      float d;
+      const float s = nsum(k, idx, d);
+      const float r = idx == gf.pin ? 0.0f : bi[k] - (d * xs[idx] - s);
+      rs[idx] = r;
+      rf[idx] = r;
+    }
+  }
+  __syncthreads();
+  // restriction and the coarsest level's symmetric sweeps from x_c = 0
+  const bool live = tid < gc.total;
+  int c[3] = {0, 0, 0};
+  float t_hi[3] = {0, 0, 0}, t_lo[3] = {0, 0, 0};
+  float bci = 0.0f, dci = 0.0f;
+  int color = 0;
+  if (live) {
+    bci = restrict_node(gf, gc, rs, tid);
+    bcs[tid] = bci;
+    bc[tid] = bci;
+    lcoords(gc, tid, c);
+    color = (c[0] + c[1] + c[2]) & 1;
+    const float* const Tc[3] = {Tc0, Tc1, Tc2};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (gc.n[a] == 1) continue;
+      if (c[a] < gc.n[a] - 1) t_hi[a] = Tc[a][tid];
+      if (c[a] > 0) t_lo[a] = Tc[a][tid - lstride(gc, a)];
+    }
+    dci = dinvc[tid];
+    xcs[tid] = 0.0f;
+  }
+  __syncthreads();
+  for (int s = 0; s < 4 * sweeps; ++s) {
+    const int col = (s < 2 * sweeps) ? (s & 1) : 1 - (s & 1);
+    if (live && color == col) {
+      float acc = bci;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (gc.n[a] == 1) continue;
+        const int64_t st = lstride(gc, a);
+        if (c[a] < gc.n[a] - 1) acc += t_hi[a] * xcs[tid + st];
+        if (c[a] > 0) acc += t_lo[a] * xcs[tid - st];
+      }
+      xcs[tid] = acc * dci;
+    }
+    __syncthreads();
+  }
+  if (live) xc[tid] = xcs[tid];
+  // prolongation (fine pin stays 0) and reverse post-smoothing
+#pragma unroll
+  for (int k = 0; k < kTwoLevelPer; ++k) {
+    const int64_t idx = tid + (int64_t)k * 1024;
+    if (colf[k] >= 0 && idx != gf.pin) xs[idx] += prolong_node(gf, gc, xcs, idx);
+  }
+  __syncthreads();
+  smooth_f(true);
+#pragma unroll
+  for (int k = 0; k < kTwoLevelPer; ++k) {
+    const int64_t idx = tid + (int64_t)k * 1024;
+    if (colf[k] >= 0) xf[idx] = xs[idx];
+  }
+}
+
 // ---- host ------------------------------------------------------------------
 struct MgState {
   int nlev = 0;
@@ -481,11 +688,21 @@ void free_mg(MgState* m) {
     TPF_HIP(hipGetLastError());                                                           \
   } while (0)
 
+// `from_zero`: x is not read and starts at 0 (the zeroing is fused into
+// the first forward half-sweep).
 tpf_status smooth(MgLevel& L, hipStream_t st, const float* b, float* x, int sweeps,
-                  bool reverse) {
+                  bool reverse, bool from_zero = false) {
   for (int k = 0; k < sweeps; ++k)
     for (int h = 0; h < 2; ++h) {
       const int color = reverse ? 1 - h : h;
+      if (from_zero && k == 0 && h == 0) {
+        if (reverse) return fail(TPF_ERR_VALUE, "zero start needs a forward sweep");
+        if (L.rho32)
+          MG_LAUNCH(k_rbgs_rho_first, L.g.total, L.g, L.fg, b, x);
+        else
+          MG_LAUNCH(k_rbgs_first, L.g.total, L.g, L.dinv, b, x);
+        continue;
+      }
       if (L.rho32)
         MG_LAUNCH(k_rbgs_rho, (L.g.total + 1) / 2, L.g, L.fg, color, b, x);
       else
@@ -507,8 +724,8 @@ tpf_status descend(MgState* m, hipStream_t st, int from, int to, int nu) {
   for (int l = from; l < to; ++l) {
     MgLevel& L = m->lv[l];
     MgLevel& C = m->lv[l + 1];
-    MG_LAUNCH(k_zero, L.g.total, L.g.total, L.x);
-    TPF_TRY(smooth(L, st, L.b, L.x, nu, false));
+    if (nu == 0) MG_LAUNCH(k_zero, L.g.total, L.g.total, L.x);
+    TPF_TRY(smooth(L, st, L.b, L.x, nu, false, true));
     if (L.rho32)
       MG_LAUNCH(k_resid_rho, L.g.total, L.g, L.fg, L.b, L.x, L.r);
     else
@@ -538,20 +755,28 @@ tpf_status cycle(MgState* m, hipStream_t st, int l, int last, int gamma, bool fi
     if (!first) return TPF_OK;  // same right-hand side, same coarse result
     if (L.g.total <= kCoarseMax) return coarse_solve(L, st, coarse_sweeps);
     // truncated hierarchy: symmetric smoothing only on the stop level
-    MG_LAUNCH(k_zero, L.g.total, L.g.total, L.x);
-    TPF_TRY(smooth(L, st, L.b, L.x, coarse_sweeps, false));
+    if (coarse_sweeps == 0) MG_LAUNCH(k_zero, L.g.total, L.g.total, L.x);
+    TPF_TRY(smooth(L, st, L.b, L.x, coarse_sweeps, false, true));
     return smooth(L, st, L.b, L.x, coarse_sweeps, true);
   }
   MgLevel& C = m->lv[l + 1];
-  if (first) MG_LAUNCH(k_zero, L.g.total, L.g.total, L.x);
-  TPF_TRY(smooth(L, st, L.b, L.x, nu_pre, false));
+  const int g_here = (w_levels <= 0 || l < w_levels) ? gamma : 1;
+  if (l + 1 == last && g_here == 1 && !L.rho32 && nu_pre == nu_post &&
+      L.g.total <= kTwoLevelMax && C.g.total <= kCoarseMax) {
+    hipLaunchKernelGGL(k_two_level, dim3(1), dim3(1024), 0, st, L.g, L.T[0], L.T[1], L.T[2],
+                       L.dinv, L.b, L.x, L.r, C.g, C.T[0], C.T[1], C.T[2], C.dinv, C.b, C.x,
+                       nu_pre, coarse_sweeps, first ? 1 : 0);
+    TPF_HIP(hipGetLastError());
+    return TPF_OK;
+  }
+  if (first && nu_pre == 0) MG_LAUNCH(k_zero, L.g.total, L.g.total, L.x);
+  TPF_TRY(smooth(L, st, L.b, L.x, nu_pre, false, first));
   if (L.rho32)
     MG_LAUNCH(k_resid_rho, L.g.total, L.g, L.fg, L.b, L.x, L.r);
   else
     MG_LAUNCH(k_resid, L.g.total, L.g, L.T[0], L.T[1], L.T[2], L.b, L.x, L.r);
   MG_LAUNCH(k_restrict, C.g.total, L.g, C.g, L.r, C.b);
   // gamma repeats apply on the top w_levels levels (all when w_levels <= 0)
-  const int g_here = (w_levels <= 0 || l < w_levels) ? gamma : 1;
   for (int k = 0; k < g_here; ++k)
     TPF_TRY(cycle(m, st, l + 1, last, gamma, k == 0, nu_pre, nu_post, coarse_sweeps, w_levels));
   MG_LAUNCH(k_prolong_add, L.g.total, L.g, C.g, C.x, L.x);
