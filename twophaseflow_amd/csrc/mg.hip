@@ -200,16 +200,24 @@ __device__ inline float nbr_sum(const LevelGeom& g, const float* const (&T)[3],
 }
 
 // One red-black Gauss-Seidel half sweep: nodes with (c0+c1+c2) % 2 == color.
+// Threads cover only the nodes of `color` (as k_rbgs_rho).
 __global__ __launch_bounds__(kBlock) void k_rbgs(LevelGeom g, int color, const float* __restrict__ T0,
                                                  const float* __restrict__ T1,
                                                  const float* __restrict__ T2,
                                                  const float* __restrict__ dinv,
                                                  const float* __restrict__ b, float* x) {
   const float* const T[3] = {T0, T1, T2};
-  CHUNK_LOOP(idx, g.total) {
+  const int half = (g.n[2] + 1) >> 1;
+  const int64_t slots = (int64_t)g.n[0] * g.n[1] * half;
+  CHUNK_LOOP(u, slots) {
+    const uint32_t row = (uint32_t)u / (uint32_t)half;
+    const int k = (int)((uint32_t)u - row * (uint32_t)half);
     int c[3];
-    lcoords(g, idx, c);
-    if (((c[0] + c[1] + c[2]) & 1) != color) continue;
+    c[0] = (int)(row / (uint32_t)g.n[1]);
+    c[1] = (int)(row - (uint32_t)c[0] * (uint32_t)g.n[1]);
+    c[2] = ((color + c[0] + c[1]) & 1) + 2 * k;
+    if (c[2] >= g.n[2]) continue;
+    const int64_t idx = (int64_t)row * g.n[2] + c[2];
     float d;
     const float s = nbr_sum(g, T, x, idx, c, d);
     x[idx] = (b[idx] + s) * dinv[idx];
@@ -284,15 +292,15 @@ __global__ __launch_bounds__(kBlock) void k_rbgs_rho(LevelGeom g, FineGeom fg, i
                                                      const float* __restrict__ b, float* x) {
   const int half = (g.n[2] + 1) >> 1;
   const int64_t slots = (int64_t)g.n[0] * g.n[1] * half;
-  CHUNK_LOOP(u, slots) {
-    const int64_t row = u / half;
-    const int k = (int)(u - row * half);
+  CHUNK_LOOP(u, slots) {  // 32-bit index arithmetic (levels hold < 2^32 nodes, like lcoords)
+    const uint32_t row = (uint32_t)u / (uint32_t)half;
+    const int k = (int)((uint32_t)u - row * (uint32_t)half);
     int c[3];
-    c[0] = (int)(row / g.n[1]);
-    c[1] = (int)(row - (int64_t)c[0] * g.n[1]);
+    c[0] = (int)(row / (uint32_t)g.n[1]);
+    c[1] = (int)(row - (uint32_t)c[0] * (uint32_t)g.n[1]);
     c[2] = ((color + c[0] + c[1]) & 1) + 2 * k;
     if (c[2] >= g.n[2]) continue;
-    const int64_t idx = row * g.n[2] + c[2];
+    const int64_t idx = (int64_t)row * g.n[2] + c[2];
     float d;
     const float s = nbr_sum_rho(g, fg, x, idx, c, d);
     x[idx] = idx == g.pin ? 0.0f : (b[idx] + s) * __builtin_amdgcn_rcpf(d);
@@ -706,7 +714,7 @@ tpf_status smooth(MgLevel& L, hipStream_t st, const float* b, float* x, int swee
       if (L.rho32)
         MG_LAUNCH(k_rbgs_rho, (L.g.total + 1) / 2, L.g, L.fg, color, b, x);
       else
-        MG_LAUNCH(k_rbgs, L.g.total, L.g, color, L.T[0], L.T[1], L.T[2], L.dinv, b, x);
+        MG_LAUNCH(k_rbgs, (L.g.total + 1) / 2, L.g, color, L.T[0], L.T[1], L.T[2], L.dinv, b, x);
     }
   return TPF_OK;
 }
@@ -806,6 +814,8 @@ extern "C" tpf_status tpf_mg_setup(tpf_ppe* s, int max_levels, int* levels) {
   if (!s || !levels) return fail(TPF_ERR_VALUE, "null argument");
   if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set (call tpf_ppe_set_density)");
   if (max_levels < 1) return fail(TPF_ERR_VALUE, "max_levels must be >= 1");
+  if (s->g.total >= ((int64_t)1 << 32))
+    return fail(TPF_ERR_VALUE, "multigrid levels index nodes in 32 bits (< 2^32 nodes)");
   max_levels = std::min(max_levels, kMaxLevels);
   // level sizes
   std::vector<LevelGeom> geo;
