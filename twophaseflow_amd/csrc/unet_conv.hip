@@ -11,7 +11,8 @@
 //
 // MFMA v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)+j],
 // B[4(l/16)+j][l%16] and D[4(l/16)+i][l%16].  The forward GEMM is
-// M = 16 voxels along x, N = 16 output channels per tile, K = 27 x Cin.
+// M = 16 voxels along x, N = 16 output channels per tile, K = 27 x Cin,
+// operands staged in LDS.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -35,9 +36,11 @@ __device__ inline float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) 
 
 constexpr int kWaves = 4;  // wavefronts per block
 
-// One wavefront: output voxels (z, y, x0..x0+15) x COT tiles of 16 channels.
+// Direct form for small channel counts (few MFMAs per staged operand): one
+// wavefront computes output voxels (z, y, x0..x0+15) x COT tiles of 16
+// channels, reading its operands straight from memory.
 template <int COT>
-__global__ __launch_bounds__(64 * kWaves) void conv_fwd(const uint16_t* __restrict__ X,
+__global__ __launch_bounds__(64 * kWaves) void conv_fwd_direct(const uint16_t* __restrict__ X,
                                                        const uint16_t* __restrict__ Wt,
                                                        const float* __restrict__ bias,
                                                        void* __restrict__ Y, int D, int H, int W,
@@ -71,6 +74,91 @@ __global__ __launch_bounds__(64 * kWaves) void conv_fwd(const uint16_t* __restri
       }
     }
   }
+  // epilogue: lane holds voxels x0 + 4*kq + i, channel t*16 + row
+#pragma unroll
+  for (int t = 0; t < COT; ++t) {
+    const int co = t * 16 + row;
+    const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int x = x0 + 4 * kq + i;
+      if (x >= W) continue;
+      float v = acc[t][i] + bb;
+      if (relu) v = v > 0.f ? v : 0.f;
+      const int64_t o = (((int64_t)z * H + y) * W + x) * Cout + co;
+      if (out_f32) reinterpret_cast<float*>(Y)[o] = v;
+      else reinterpret_cast<uint16_t*>(Y)[o] = f2bf(v);
+    }
+  }
+}
+
+// Forward convolution as an implicit GEMM with LDS staging.  A workgroup
+// of 4 wavefronts computes 4 consecutive y rows (one per wavefront) x 16
+// voxels along x x all Cout channels (COT tiles of 16).  Per (dz, dy) the
+// four input rows' 18 voxels (x0-1 .. x0+16, zero outside the grid) are
+// staged in LDS, and per tap the Cout x Cin weight block; every wavefront
+// then reads its MFMA operands from LDS, so a weight tile is fetched once
+// per 64 output voxels and an input voxel once per (dz, dy) instead of
+// once per tap and channel tile.  Row pitches are padded by 4 BF16 so the
+// 16 rows of an operand read spread over the LDS banks.
+constexpr int kRowsPerBlock = kWaves;  // y rows per workgroup (one per wavefront)
+
+template <int COT>
+__global__ __launch_bounds__(64 * kWaves) void conv_fwd(const uint16_t* __restrict__ X,
+                                                       const uint16_t* __restrict__ Wt,
+                                                       const float* __restrict__ bias,
+                                                       void* __restrict__ Y, int D, int H, int W,
+                                                       int Cin, int Cout, int relu, int out_f32) {
+  extern __shared__ uint16_t sm[];
+  const int pitch = Cin + 4;
+  uint16_t* As = sm;                                  // [4 rows][18 voxels][pitch]
+  uint16_t* Bs = sm + kRowsPerBlock * 18 * pitch;     // [Cout][pitch]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int xt = (W + 15) >> 4, yb = (H + kRowsPerBlock - 1) / kRowsPerBlock;
+  const int blk = blockIdx.x;
+  const int z = blk / (yb * xt);
+  const int rem = blk - z * yb * xt;
+  const int y0 = (rem / xt) * kRowsPerBlock;
+  const int x0 = (rem - (rem / xt) * xt) * 16;
+  const int y = y0 + wv;
+  const int row = lane & 15, kq = lane >> 4;
+  const int c4 = Cin >> 2;  // 8-byte chunks per voxel
+  v4f acc[COT];
+#pragma unroll
+  for (int t = 0; t < COT; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int dzy = 0; dzy < 9; ++dzy) {
+    const int dz = dzy / 3 - 1, dy = dzy % 3 - 1;
+    const int zz = z + dz;
+    __syncthreads();  // previous operands consumed
+    for (int e = tid; e < kRowsPerBlock * 18 * c4; e += 64 * kWaves) {
+      const int r = e / (18 * c4), rr = e - r * 18 * c4, v = rr / c4, c = (rr - v * c4) * 4;
+      const int yy = y0 + r + dy, xx = x0 - 1 + v;
+      uint64_t val = 0;
+      if (zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W)
+        val = *reinterpret_cast<const uint64_t*>(X + (((int64_t)zz * H + yy) * W + xx) * Cin + c);
+      *reinterpret_cast<uint64_t*>(As + (r * 18 + v) * pitch + c) = val;
+    }
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+      if (dx > -1) __syncthreads();  // previous weight block consumed
+      for (int e = tid; e < Cout * c4; e += 64 * kWaves) {
+        const int co = e / c4, c = (e - co * c4) * 4;
+        *reinterpret_cast<uint64_t*>(Bs + co * pitch + c) =
+            *reinterpret_cast<const uint64_t*>(Wt + ((int64_t)tap * Cout + co) * Cin + c);
+      }
+      __syncthreads();
+      const uint16_t* ap = As + (wv * 18 + row + 1 + dx) * pitch + 4 * kq;
+      for (int c0 = 0; c0 < Cin; c0 += 16) {
+        const v4s a = *reinterpret_cast<const v4s*>(ap + c0);
+#pragma unroll
+        for (int t = 0; t < COT; ++t) {
+          const v4s b = *reinterpret_cast<const v4s*>(Bs + (t * 16 + row) * pitch + c0 + 4 * kq);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  if (y >= H) return;
   // epilogue: lane holds voxels x0 + 4*kq + i, channel t*16 + row
 #pragma unroll
   for (int t = 0; t < COT; ++t) {
@@ -161,22 +249,35 @@ extern "C" tpf_status tpf_conv3d_fwd(const void* X, const void* Wt, const float*
   if (!X || !Wt || !Y) return fail(TPF_ERR_VALUE, "null argument");
   if (D < 1 || H < 1 || W < 1 || Cin < 16 || Cin % 16 || Cout < 16 || Cout % 16 || Cout > 128)
     return fail(TPF_ERR_VALUE, "conv3d: Cin, Cout must be multiples of 16 (Cout <= 128)");
-  const int64_t waves = (int64_t)D * H * ((W + 15) / 16);
-  const int64_t blocks = (waves + kWaves - 1) / kWaves;
+  if (Cin > 128) return fail(TPF_ERR_VALUE, "conv3d: Cin <= 128");
+  const int64_t blocks = (int64_t)D * ((H + kRowsPerBlock - 1) / kRowsPerBlock) * ((W + 15) / 16);
   if (blocks > 0x7fffffffLL) return fail(TPF_ERR_VALUE, "conv3d grid too large");
   const uint16_t* x = (const uint16_t*)X;
   const uint16_t* w = (const uint16_t*)Wt;
   hipStream_t s = (hipStream_t)stream;
-  switch (Cout / 16) {
-    case 1: hipLaunchKernelGGL(conv_fwd<1>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
-    case 2: hipLaunchKernelGGL(conv_fwd<2>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
-    case 3: hipLaunchKernelGGL(conv_fwd<3>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
-    case 4: hipLaunchKernelGGL(conv_fwd<4>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
-    case 5: hipLaunchKernelGGL(conv_fwd<5>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
-    case 6: hipLaunchKernelGGL(conv_fwd<6>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
-    case 7: hipLaunchKernelGGL(conv_fwd<7>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
-    case 8: hipLaunchKernelGGL(conv_fwd<8>, dim3(blocks), dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32); break;
+  const size_t sh = sizeof(uint16_t) * (size_t)(Cin + 4) * (kRowsPerBlock * 18 + Cout);
+  using Fn = void (*)(const uint16_t*, const uint16_t*, const float*, void*, int, int, int, int,
+                      int, int, int);
+  static const Fn kFns[8] = {conv_fwd<1>, conv_fwd<2>, conv_fwd<3>, conv_fwd<4>,
+                             conv_fwd<5>, conv_fwd<6>, conv_fwd<7>, conv_fwd<8>};
+  static const Fn kDirect[8] = {conv_fwd_direct<1>, conv_fwd_direct<2>, conv_fwd_direct<3>,
+                                conv_fwd_direct<4>, conv_fwd_direct<5>, conv_fwd_direct<6>,
+                                conv_fwd_direct<7>, conv_fwd_direct<8>};
+  // LDS staging pays once a staged operand feeds enough MFMAs: measured
+  // 1.8-4x faster from 32 channels in and out (65^3 x 64: 0.33 ms against
+  // 1.36 ms), 10-20 % slower at 16 channels.
+  if (Cin < 32 || Cout < 32) {
+    const int64_t waves = (int64_t)D * H * ((W + 15) / 16);
+    hipLaunchKernelGGL(kDirect[Cout / 16 - 1], dim3((waves + kWaves - 1) / kWaves),
+                       dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32);
+    TPF_HIP(hipGetLastError());
+    return TPF_OK;
   }
+  const Fn k = kFns[Cout / 16 - 1];
+  if (sh > 64 * 1024)
+    TPF_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * kWaves), sh, s, x, w, bias, Y, D, H, W, Cin, Cout,
+                     relu, out_f32);
   TPF_HIP(hipGetLastError());
   return TPF_OK;
 }
