@@ -42,6 +42,8 @@ struct CcdAxisGeom {
   int periodic;
   int s[kMaxWaveParts + 1];  // partition starts (unknown index), s[P] = n
   int ctype[kMaxParts];      // partition -> ctab type (explicit mode)
+  int uniform;               // 1: every partition has the same rows and relative coupling
+                             //    rows (periodic, equal partitions): partition 0's tables serve all
   double inv_h, inv_h2;
   // wall closure in scaled unknowns (ccd/ccd_solver.py:408-456)
   double Lb[4], Ub[4], Ml[4], Mr[4];

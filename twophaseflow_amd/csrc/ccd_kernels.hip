@@ -18,6 +18,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 #include "ccd_plan.hpp"
 #include "../../include/tpf_testing.h"
@@ -25,6 +26,9 @@
 namespace tpf {
 
 namespace {
+
+// Wavefront-uniform table reads through the scalar cache.
+typedef const __attribute__((address_space(4))) double* kdptr;
 
 constexpr int kStageRows = 16;              // rows per LDS re-staging chunk
 constexpr int kStagePitch = kStageRows + 1; // padded line pitch (doubles)
@@ -731,7 +735,7 @@ __device__ __forceinline__ void wave_tile_io(const CcdLines& ln, int nax, int pi
   }
 }
 
-template <int M>
+template <int M, bool UNI>
 __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLines& ln,
                                               const double* __restrict__ f,
                                               double* __restrict__ d1, double* __restrict__ d2,
@@ -791,14 +795,17 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
   // reduced-system rows of all partitions -> LDS after the exchange buffer,
   // when they fit the tile space (the host sized the tile for it)
   const int nq = g.P * 16 * g.NQ;
-  const bool qlds = kWaveLines * 4 * 64 + nq <= kWaveLines * wave_pitch(nax);
+  const bool qlds = !UNI && kWaveLines * 4 * 64 + nq <= kWaveLines * wave_pitch(nax);
   const double* qsrc = g.qtab;
   if (qlds) {
     double* qd = lds + kWaveLines * 4 * 64;
     for (int i = threadIdx.x; i < nq; i += blockDim.x) qd[i] = g.qtab[i];
     qsrc = qd;  // visible after the barrier that precedes the coupling
   }
-  const double* tab = g.rowtab + (size_t)pp * M * kRowTab;
+  // UNI: every partition's rows are partition 0's, read wavefront-uniform
+  // through the scalar cache instead of per lane on the dependent sweeps
+  using TabPtr = typename std::conditional<UNI, kdptr, const double*>::type;
+  const TabPtr tab = UNI ? (TabPtr)g.rowtab : (TabPtr)(g.rowtab + (size_t)pp * M * kRowTab);
   double x[M][2];
 #pragma unroll
   for (int r = 0; r < M; ++r) {
@@ -814,7 +821,7 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
         r1 -= cr1;
       }
       if (r > 0) {
-        const double* t = tab + r * kRowTab;
+        const TabPtr t = tab + r * kRowTab;
         r0 -= t[0] * x[r - 1][0] + t[1] * x[r - 1][1];
         r1 -= t[2] * x[r - 1][0] + t[3] * x[r - 1][1];
       }
@@ -830,7 +837,7 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
         a -= kAlpha1 * x[r + 1][0] - kB1 * x[r + 1][1];
         c -= -kB2 * x[r + 1][0] + kBeta2 * x[r + 1][1];
       }
-      const double* t = tab + r * kRowTab + 4;
+      const TabPtr t = tab + r * kRowTab + 4;
       x[r][0] = t[0] * a + t[1] * c;
       x[r][1] = t[2] * a + t[3] * c;
       if (r == m - 1) {
@@ -845,17 +852,17 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
   double lr0 = 0.0, lr1 = 0.0, lr2 = 0.0, lr3 = 0.0;
   {
     const int NQ = g.NQ;
-    const double* q = qsrc + (size_t)pp * 4 * NQ * 4;
+    const TabPtr q = UNI ? (TabPtr)g.qtab : (TabPtr)(qsrc + (size_t)pp * 4 * NQ * 4);
     for (int t = 0; t < NQ; ++t) {
       int qq = g.rel ? pp + g.d0 + t : t;
       if (periodic) qq = qq < 0 ? qq + g.P : (qq >= g.P ? qq - g.P : qq);
       else qq = qq < 0 ? 0 : (qq >= g.P ? g.P - 1 : qq);
       const double G0 = Gw[0 * 64 + qq], G1 = Gw[1 * 64 + qq];
       const double G2 = Gw[2 * 64 + qq], G3 = Gw[3 * 64 + qq];
-      const double* q0 = q + (0 * NQ + t) * 4;
-      const double* q1 = q + (1 * NQ + t) * 4;
-      const double* q2 = q + (2 * NQ + t) * 4;
-      const double* q3 = q + (3 * NQ + t) * 4;
+      const TabPtr q0 = q + (0 * NQ + t) * 4;
+      const TabPtr q1 = q + (1 * NQ + t) * 4;
+      const TabPtr q2 = q + (2 * NQ + t) * 4;
+      const TabPtr q3 = q + (3 * NQ + t) * 4;
       lr0 += q0[0] * G0 + q0[1] * G1 + q0[2] * G2 + q0[3] * G3;
       lr1 += q1[0] * G0 + q1[1] * G1 + q1[2] * G2 + q1[3] * G3;
       lr2 += q2[0] * G0 + q2[1] * G1 + q2[2] * G2 + q2[3] * G3;
@@ -866,7 +873,7 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
 #pragma unroll
   for (int r = 0; r < M; ++r) {
     if (r < m) {
-      const double* t = tab + r * kRowTab + 8;
+      const TabPtr t = tab + r * kRowTab + 8;
       const double u = x[r][0] - (t[0] * lr0 + t[1] * lr1 + t[4] * lr2 + t[5] * lr3);
       const double v = x[r][1] - (t[2] * lr0 + t[3] * lr1 + t[6] * lr2 + t[7] * lr3);
       x[r][0] = u;
@@ -907,7 +914,10 @@ __global__ __launch_bounds__(64 * kWaveLines) void ccd_wave_kernel(CcdLaunch L) 
   const unsigned blk = blockIdx.x;
   const int sl = (L.nslots > 1 && blk >= L.block_begin[1]) + (L.nslots > 2 && blk >= L.block_begin[2]);
   const CcdSlot& S = L.slot[sl];
-  ccd_wave_tile<M>(S.g, S.ln, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
+  if (S.g.uniform)
+    ccd_wave_tile<M, true>(S.g, S.ln, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
+  else
+    ccd_wave_tile<M, false>(S.g, S.ln, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
 }
 
 // LDS of the wavefront kernel: the [16][pitch] tile (it also holds the 16 KB
@@ -984,7 +994,6 @@ __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdLaunch L) {
 // cache (constant address space).  Rows are produced in order and stored
 // directly (strided axes: every store instruction writes 64 consecutive
 // lines).
-typedef const __attribute__((address_space(4))) double* kdptr;
 
 template <int M>
 __device__ __forceinline__ void ccd_lines_explicit(const CcdAxisGeom& g, const CcdLines& ln,
@@ -1172,7 +1181,10 @@ __global__ __launch_bounds__(64 * kWaveLines) void ccd_wave_batch_kernel(
     else hi = mid - 1;
   }
   const CcdSlot& S = slots[lo];
-  ccd_wave_tile<M>(S.g, S.ln, S.f, S.d1, S.d2, blk - begin[lo]);
+  if (S.g.uniform)
+    ccd_wave_tile<M, true>(S.g, S.ln, S.f, S.d1, S.d2, blk - begin[lo]);
+  else
+    ccd_wave_tile<M, false>(S.g, S.ln, S.f, S.d1, S.d2, blk - begin[lo]);
 }
 
 template <int M>

@@ -334,6 +334,11 @@ tpf_status build_axis_tables(CcdAxisHost& ax, int n_nodes, double h, int periodi
     g.rel = 1;
   }
   ax.trunc_coeff = dropped;
+  // Periodic lines cut into equal partitions: the block rows are identical
+  // and the reduced system is circulant, so with a relative coupling window
+  // every partition's tables equal partition 0's (the reduced inverse's rows
+  // agree to its rounding).
+  g.uniform = periodic && g.rel && n % P == 0 ? 1 : 0;
   ax.qtab.assign(size_t(P) * 4 * g.NQ * 4, 0.0);
   for (int p = 0; p < P; ++p) {
     const bool has_left = periodic || p > 0;
