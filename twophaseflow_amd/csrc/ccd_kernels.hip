@@ -129,13 +129,17 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
   // ---- per-row LU/spike tables and coupling rows -> LDS (all WGs share them;
   // reading them from LDS keeps scalar-cache misses off the dependent sweeps).
   // A staged contiguous-axis tile has been consumed by now; its space is reused.
+  // Uniform partitions (periodic, equal sizes) share partition 0's rows:
+  // only those are staged.
   const bool kLdsTab = M <= 32;
   const int NQ = g.NQ;
+  const int np = g.uniform ? 1 : g.P;  // partitions with their own tables
+  const int pt = g.uniform ? 0 : p;    // this wavefront's table partition
   double* ltab = lds;
-  double* lq = lds + (kLdsTab ? g.P * M * kRowTab : 0);
-  double* G = lq + (kLdsTab ? g.P * 16 * NQ : 0);  // [4P][64]
+  double* lq = lds + (kLdsTab ? np * M * kRowTab : 0);
+  double* G = lq + (kLdsTab ? np * 16 * NQ : 0);  // [4P][64]
   if (kLdsTab) {
-    const int nt = g.P * M * kRowTab, nq = g.P * 16 * NQ;
+    const int nt = np * M * kRowTab, nq = np * 16 * NQ;
     for (int i = threadIdx.x; i < nt; i += blockDim.x) ltab[i] = g.rowtab[i];
     for (int i = threadIdx.x; i < nq; i += blockDim.x) lq[i] = g.qtab[i];
     __syncthreads();
@@ -165,7 +169,7 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
   }
 
   // ---- RHS + forward sweep: y_r = R_r - Lf_r y_{r-1}
-  const double* tab = (kLdsTab ? ltab : g.rowtab) + (size_t)p * M * kRowTab;
+  const double* tab = (kLdsTab ? ltab : g.rowtab) + (size_t)pt * M * kRowTab;
   double x[M][2];
 #pragma unroll
   for (int r = 0; r < M; ++r) {
@@ -214,7 +218,7 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
   // ---- coupling: (left, right) end values of the neighbours = Q_p . G
   double lr0 = 0.0, lr1 = 0.0, lr2 = 0.0, lr3 = 0.0;
   {
-    const double* q = (kLdsTab ? lq : g.qtab) + (size_t)p * 4 * NQ * 4;
+    const double* q = (kLdsTab ? lq : g.qtab) + (size_t)pt * 4 * NQ * 4;
     for (int t = 0; t < NQ; ++t) {
       int qq = g.rel ? p + g.d0 + t : t;
       if (periodic) qq = qq < 0 ? qq + g.P : (qq >= g.P ? qq - g.P : qq);
