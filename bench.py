@@ -212,7 +212,7 @@ def mg_solve_timed(g, rho, rhs, gamma=2, net=None, tol=1e-8, reps=2):
     return out
 
 
-def ccd_batch_bench(reps=20):
+def ccd_batch_bench(reps=30):
     """48 independent 2-D wall problems, 16 each at 129^2, 257^2 and 513^2
     nodes (the uneven-batch sizes), all axes: one batched call (one grid per
     launch shape) against one tpf_ccd_apply per problem."""
@@ -232,25 +232,32 @@ def ccd_batch_bench(reps=20):
     batch = CCDBatch(solvers)
     pts = sum(2 * f.numel() for f in fs)
 
-    def timed(fn):
-        fn()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
+    def timed(fn, warm=5, n=reps):
+        """Median and mean of per-call HIP-event times after `warm` calls
+        (single calls here have shown rare multi-ms stalls of the stream)."""
+        for _ in range(warm):
             fn()
-        e1.record()
         torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(n)]
+        for a, b in ev:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        ts = sorted(a.elapsed_time(b) for a, b in ev)
+        return ts[len(ts) // 2], sum(ts) / len(ts)
 
     def loop():
         for s_, f, a, b in zip(solvers, fs, d1s, d2s):
             s_.apply_into(f, a, b, 0b11)
 
-    t_b = timed(lambda: batch.apply_into(fs, d1s, d2s))
-    t_l = timed(loop)
+    t_b, m_b = timed(lambda: batch.apply_into(fs, d1s, d2s))
+    t_l, m_l = timed(loop)
     return {"problems": 48, "sizes": "16 x 129^2, 16 x 257^2, 16 x 513^2, wall, both axes",
             "batched_ms": round(t_b, 4), "per_problem_launches_ms": round(t_l, 4),
+            "timing": "median of per-call HIP-event times after 5 warm-up calls",
+            "mean_ms": {"batched": round(m_b, 4), "per_problem_launches": round(m_l, 4)},
             "batched_Mpts_per_s": round(pts / (t_b * 1e-3) / 1e6, 1),
             "algorithmic_GBps": round(40 * (pts / 2) / (t_b * 1e-3) / 1e9, 1),
             "launches": {"batched": 3, "per_problem": 48}}

@@ -1158,56 +1158,69 @@ void ccd_explicit_kernel(CcdLaunch L) {
 
 // Batched launch over many (problem, axis) slots kept in device memory:
 // workgroups [begin[s], begin[s+1]) belong to slot s.  All slots share the
-// rows cap M and the partition count (one block size per launch).
-template <int M>
-__global__ __launch_bounds__(1024) void ccd_batch_kernel(const CcdSlot* __restrict__ slots,
-                                                         const unsigned* __restrict__ begin,
-                                                         int nslots) {
-  const unsigned blk = blockIdx.x;
+// rows cap M and the partition count (one block size per launch).  The
+// per-call array pointers of a group of <= kArgSlots slots travel in the
+// kernel arguments (BatchPtrs), so a call copies nothing to the device;
+// larger groups read them from the device slot table.
+constexpr int kArgSlots = 128;
+struct BatchPtrs {
+  int n;  // 0: pointers from the slot table
+  const double* f[kArgSlots];
+  double* d1[kArgSlots];
+  double* d2[kArgSlots];
+};
+
+__device__ __forceinline__ int batch_slot(const unsigned* __restrict__ begin, int nslots,
+                                          unsigned blk) {
   int lo = 0, hi = nslots - 1;  // largest s with begin[s] <= blk (uniform)
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (begin[mid] <= blk) lo = mid;
     else hi = mid - 1;
   }
+  return lo;
+}
+
+template <int M>
+__global__ __launch_bounds__(1024) void ccd_batch_kernel(const CcdSlot* __restrict__ slots,
+                                                         const unsigned* __restrict__ begin,
+                                                         int nslots, BatchPtrs P) {
+  const unsigned blk = blockIdx.x;
+  const int lo = batch_slot(begin, nslots, blk);
   const CcdSlot& S = slots[lo];
-  ccd_lines<M>(S.g, S.ln, S.contig != 0, S.tiled != 0, S.f, S.d1, S.d2, blk - begin[lo]);
+  ccd_lines<M>(S.g, S.ln, S.contig != 0, S.tiled != 0, P.n ? P.f[lo] : S.f,
+               P.n ? P.d1[lo] : S.d1, P.n ? P.d2[lo] : S.d2, blk - begin[lo]);
 }
 
 template <int M>
 __global__ __launch_bounds__(64 * kWaveLines) void ccd_wave_batch_kernel(
-    const CcdSlot* __restrict__ slots, const unsigned* __restrict__ begin, int nslots) {
+    const CcdSlot* __restrict__ slots, const unsigned* __restrict__ begin, int nslots,
+    BatchPtrs P) {
   const unsigned blk = blockIdx.x;
-  int lo = 0, hi = nslots - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (begin[mid] <= blk) lo = mid;
-    else hi = mid - 1;
-  }
+  const int lo = batch_slot(begin, nslots, blk);
   const CcdSlot& S = slots[lo];
+  const double* f = P.n ? P.f[lo] : S.f;
+  double* d1 = P.n ? P.d1[lo] : S.d1;
+  double* d2 = P.n ? P.d2[lo] : S.d2;
   if (S.g.uniform)
-    ccd_wave_tile<M, true>(S.g, S.ln, S.f, S.d1, S.d2, blk - begin[lo]);
+    ccd_wave_tile<M, true>(S.g, S.ln, f, d1, d2, blk - begin[lo]);
   else
-    ccd_wave_tile<M, false>(S.g, S.ln, S.f, S.d1, S.d2, blk - begin[lo]);
+    ccd_wave_tile<M, false>(S.g, S.ln, f, d1, d2, blk - begin[lo]);
 }
 
 template <int M>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
 void ccd_explicit_batch_kernel(const CcdSlot* __restrict__ slots, const unsigned* __restrict__ begin,
-                               int nslots) {
+                               int nslots, BatchPtrs P) {
   const unsigned blk = blockIdx.x;
-  int lo = 0, hi = nslots - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (begin[mid] <= blk) lo = mid;
-    else hi = mid - 1;
-  }
+  const int lo = batch_slot(begin, nslots, blk);
   const CcdSlot& S = slots[lo];
-  ccd_lines_explicit<M>(S.g, S.ln, S.f, S.d1, S.d2,
+  ccd_lines_explicit<M>(S.g, S.ln, P.n ? P.f[lo] : S.f, P.n ? P.d1[lo] : S.d1,
+                        P.n ? P.d2[lo] : S.d2,
                         xcd_local(blk - begin[lo], begin[lo + 1] - begin[lo]));
 }
 
-using BatchFn = void (*)(const CcdSlot*, const unsigned*, int);
+using BatchFn = void (*)(const CcdSlot*, const unsigned*, int, BatchPtrs);
 using WaveFn = void (*)(CcdLaunch);
 
 BatchFn pick_wave_batch_kernel(int M) {
@@ -1772,6 +1785,10 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
   if (e == hipSuccess)
     e = hipMemcpy(b->d_begin, b->begin.data(), sizeof(unsigned) * b->begin.size(),
                   hipMemcpyHostToDevice);
+  // geometry of every slot; the array pointers come per call (kernel
+  // arguments, or a restaged table for groups above kArgSlots)
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_slots, b->slots.data(), sizeof(CcdSlot) * ns, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     tpf_ccd_batch_destroy(b);
     return fail(TPF_ERR_RUNTIME, std::string("batch allocation: ") + hipGetErrorString(e));
@@ -1809,32 +1826,52 @@ extern "C" tpf_status tpf_ccd_batch_apply(tpf_ccd_batch* b, const double* const*
                                           double* const* d1, double* const* d2, void* stream) {
   if (!b || !f || (!d1 && !d2)) return fail(TPF_ERR_VALUE, "null batch argument");
   hipStream_t st = (hipStream_t)stream;
-  TPF_HIP(hipEventSynchronize(b->staged));  // the previous upload has left h_slots
-  for (size_t i = 0; i < b->slots.size(); ++i) {
+  const size_t ns = b->slots.size();
+  std::vector<const double*> sf(ns);
+  std::vector<double*> s1(ns), s2(ns);
+  bool table = false;  // some group reads its pointers from the device slot table
+  for (const auto& G : b->groups) table = table || G.count > kArgSlots;
+  for (size_t i = 0; i < ns; ++i) {
     const int j = b->slot_problem[i], k = b->slot_k[i];
     if (!f[j]) return fail(TPF_ERR_VALUE, "batch entry " + std::to_string(j) + ": null input");
     double* o1 = (d1 && d1[j]) ? d1[j] + (int64_t)k * b->totals[j] : nullptr;
     double* o2 = (d2 && d2[j]) ? d2[j] + (int64_t)k * b->totals[j] : nullptr;
     if ((const double*)o1 == f[j] || (const double*)o2 == f[j])
       return fail(TPF_ERR_VALUE, "CCD outputs must not alias the input");
-    b->h_slots[i].f = f[j];
-    b->h_slots[i].d1 = o1;
-    b->h_slots[i].d2 = o2;
+    sf[i] = f[j];
+    s1[i] = o1;
+    s2[i] = o2;
   }
-  TPF_HIP(hipMemcpyAsync(b->d_slots, b->h_slots, sizeof(CcdSlot) * b->slots.size(),
-                         hipMemcpyHostToDevice, st));
-  TPF_HIP(hipEventRecord(b->staged, st));
+  if (table) {
+    TPF_HIP(hipEventSynchronize(b->staged));  // the previous upload has left h_slots
+    for (size_t i = 0; i < ns; ++i) {
+      b->h_slots[i].f = sf[i];
+      b->h_slots[i].d1 = s1[i];
+      b->h_slots[i].d2 = s2[i];
+    }
+    TPF_HIP(hipMemcpyAsync(b->d_slots, b->h_slots, sizeof(CcdSlot) * ns, hipMemcpyHostToDevice, st));
+    TPF_HIP(hipEventRecord(b->staged, st));
+  }
   for (size_t gi = 0; gi < b->groups.size(); ++gi) {
     const auto& G = b->groups[gi];
+    BatchPtrs P;
+    P.n = 0;
+    if (G.count <= kArgSlots) {
+      P.n = G.count;
+      for (int i = 0; i < G.count; ++i) {
+        P.f[i] = sf[G.first + i];
+        P.d1[i] = s1[G.first + i];
+        P.d2[i] = s2[G.first + i];
+      }
+    }
+    const CcdSlot* slots = b->d_slots + G.first;
+    const unsigned* begin = b->d_begin + b->begin_off[gi];
     if (G.wave == 1)
       hipLaunchKernelGGL(pick_wave_batch_kernel(G.M), dim3(G.blocks), dim3(64 * kWaveLines),
-                         G.shmem, st, (const CcdSlot*)(b->d_slots + G.first),
-                         (const unsigned*)(b->d_begin + b->begin_off[gi]), G.count);
+                         G.shmem, st, slots, begin, G.count, P);
     else
       hipLaunchKernelGGL(G.wave == 2 ? pick_explicit_batch_kernel(G.M) : pick_batch_kernel(G.M),
-                         dim3(G.blocks), dim3(64 * G.P), G.shmem, st,
-                         (const CcdSlot*)(b->d_slots + G.first),
-                         (const unsigned*)(b->d_begin + b->begin_off[gi]), G.count);
+                         dim3(G.blocks), dim3(64 * G.P), G.shmem, st, slots, begin, G.count, P);
     TPF_HIP(hipGetLastError());
   }
   return TPF_OK;
