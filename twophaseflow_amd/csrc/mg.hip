@@ -308,15 +308,28 @@ __global__ __launch_bounds__(kBlock) void k_rbgs_rho(LevelGeom g, FineGeom fg, i
 }
 
 // First forward half-sweep from x = 0 (fused with the zeroing): colour-0
-// nodes take b / diag (their neighbours are all 0), colour-1 nodes 0.
+// nodes take b / diag (their neighbours are all 0), colour-1 nodes 0.  A
+// thread takes the node pair (2k, 2k+1) of a row, so the diagonal is formed
+// for one node per thread without a checkerboard branch in the wavefront.
 __global__ __launch_bounds__(kBlock) void k_rbgs_rho_first(LevelGeom g, FineGeom fg,
                                                            const float* __restrict__ b,
                                                            float* __restrict__ x) {
-  CHUNK_LOOP(idx, g.total) {
+  const int half = (g.n[2] + 1) >> 1;
+  const int64_t slots = (int64_t)g.n[0] * g.n[1] * half;
+  CHUNK_LOOP(u, slots) {
+    const uint32_t row = (uint32_t)u / (uint32_t)half;
+    const int k = (int)((uint32_t)u - row * (uint32_t)half);
     int c[3];
-    lcoords(g, idx, c);
+    c[0] = (int)(row / (uint32_t)g.n[1]);
+    c[1] = (int)(row - (uint32_t)c[0] * (uint32_t)g.n[1]);
+    const int odd = (c[0] + c[1]) & 1;  // colour-0 node of the pair: 2k + odd
+    c[2] = 2 * k + odd;
+    const int64_t r0 = (int64_t)row * g.n[2];
+    if (2 * k + (1 - odd) < g.n[2]) x[r0 + 2 * k + (1 - odd)] = 0.0f;
+    if (c[2] >= g.n[2]) continue;
+    const int64_t idx = r0 + c[2];
     float v = 0.0f;
-    if (((c[0] + c[1] + c[2]) & 1) == 0 && idx != g.pin) {
+    if (idx != g.pin) {
       float dv[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a)
@@ -706,7 +719,7 @@ tpf_status smooth(MgLevel& L, hipStream_t st, const float* b, float* x, int swee
       if (from_zero && k == 0 && h == 0) {
         if (reverse) return fail(TPF_ERR_VALUE, "zero start needs a forward sweep");
         if (L.rho32)
-          MG_LAUNCH(k_rbgs_rho_first, L.g.total, L.g, L.fg, b, x);
+          MG_LAUNCH(k_rbgs_rho_first, (L.g.total + 1) / 2, L.g, L.fg, b, x);
         else
           MG_LAUNCH(k_rbgs_first, L.g.total, L.g, L.dinv, b, x);
         continue;
