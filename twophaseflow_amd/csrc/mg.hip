@@ -683,6 +683,7 @@ struct MgState {
   hipGraphExec_t cyc = nullptr;
   int cyc_key[4] = {0, 0, 0, 0};
   hipStream_t cyc_stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;  // tpf_mg_vcycle's stream ordering
 };
 
 MgState* mg_of(tpf_ppe* s) { return static_cast<MgState*>(s->mg); }
@@ -694,6 +695,8 @@ unsigned grid_of(int64_t total) {
 
 void free_mg(MgState* m) {
   if (m->cyc) (void)hipGraphExecDestroy(m->cyc);
+  if (m->ev_in) (void)hipEventDestroy(m->ev_in);
+  if (m->ev_out) (void)hipEventDestroy(m->ev_out);
   for (int l = 0; l < m->nlev; ++l) {
     MgLevel& L = m->lv[l];
     float* bufs[] = {L.T[0], L.T[1], L.T[2], L.dinv, L.x, L.b, L.r, L.rho32};
@@ -967,8 +970,25 @@ extern "C" tpf_status tpf_mg_vcycle(tpf_ppe* s, const float* r, float* z, int nu
   TPF_TRY(check_level(s, 0));
   MgState* m = mg_of(s);
   const int last = (stop_level < 0 || stop_level >= m->nlev) ? m->nlev - 1 : stop_level;
-  hipStream_t st = s->user_stream;
   MgLevel& F = m->lv[0];
+  if (last == m->nlev - 1 && nu_pre == nu_post && s->stream) {
+    // full cycle: replay the captured graph on the handle's stream, ordered
+    // after and before the caller's stream (~150 launches become one)
+    if (!m->ev_in) {
+      TPF_HIP(hipEventCreateWithFlags(&m->ev_in, hipEventDisableTiming));
+      TPF_HIP(hipEventCreateWithFlags(&m->ev_out, hipEventDisableTiming));
+    }
+    hipStream_t st = s->stream;
+    TPF_HIP(hipEventRecord(m->ev_in, s->user_stream));
+    TPF_HIP(hipStreamWaitEvent(st, m->ev_in, 0));
+    MG_LAUNCH(k_copy, F.g.total, F.g.total, r, F.b);
+    TPF_TRY(mg_cycle(s, st, nu_pre, gamma, coarse_sweeps, 0));
+    MG_LAUNCH(k_copy, F.g.total, F.g.total, F.x, z);
+    TPF_HIP(hipEventRecord(m->ev_out, st));
+    TPF_HIP(hipStreamWaitEvent(s->user_stream, m->ev_out, 0));
+    return TPF_OK;
+  }
+  hipStream_t st = s->user_stream;
   MG_LAUNCH(k_copy, F.g.total, F.g.total, r, F.b);
   TPF_TRY(cycle(m, st, 0, last, gamma, true, nu_pre, nu_post, coarse_sweeps));
   MG_LAUNCH(k_copy, F.g.total, F.g.total, F.x, z);
