@@ -69,6 +69,27 @@ def test_batch_equals_single_problem_launches():
     assert torch.equal(o1[0], e1) and torch.equal(o2[0], e2)
 
 
+def test_batch_groups_above_the_kernel_argument_limit():
+    """A group of more than 128 slots (here 140) takes its array pointers from
+    the restaged device slot table instead of the kernel arguments."""
+    _announce("ccd batch 70 x 2-D 24x20 (140 slots in one group)")
+    solvers = [_solver((24, 20), "wall") for _ in range(70)]
+    batch = CCDBatch(solvers)
+    rng = np.random.default_rng(7)
+    fs = [torch.from_numpy(rng.standard_normal((25, 21))).cuda() for _ in solvers]
+    d1s = [torch.empty((2, 25, 21), dtype=torch.float64, device="cuda") for _ in solvers]
+    d2s = [torch.empty_like(d) for d in d1s]
+    for rep in range(2):  # the second call restages the table behind the first
+        scale = 1.0 + rep
+        batch.apply_into([f * scale for f in fs], d1s, d2s)
+        torch.cuda.synchronize()
+        for s, f, a, b in zip(solvers, fs, d1s, d2s):
+            r1 = torch.empty_like(a)
+            r2 = torch.empty_like(b)
+            s.apply_into(f * scale, r1, r2, 0b11)
+            assert torch.equal(a, r1) and torch.equal(b, r2)
+
+
 def test_batch_rejects_bad_shapes():
     s = _solver((32, 32), "wall")
     batch = CCDBatch([s])
