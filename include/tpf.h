@@ -204,6 +204,17 @@ typedef struct {
 TPF_API tpf_status tpf_ppe_solve_mg(tpf_ppe* ppe, const double* rhs, double* p,
                             const tpf_mg_opts* opts, tpf_solve_report* rep);
 
+/* tpf_ppe_solve_mg with a learned coarse correction after every cycle (no
+ * counterpart in the reference; the learned preconditioner's device path):
+ * z = V(r); z += P W E^-1 W^T R (r - A z), R / P the restriction and
+ * prolongation chains between level 0 and `level`, W [k][nodes(level)] FP32
+ * device vectors (1 <= k <= 32), einv the k x k FP64 host matrix E^-1.
+ * Synchronous; TPF_ERR_RUNTIME when maxiter is reached above tol. */
+TPF_API tpf_status tpf_ppe_solve_mg_deflated(tpf_ppe* ppe, const double* rhs, double* p,
+                                             const tpf_mg_opts* opts, const float* W, int k,
+                                             int level, const double* einv,
+                                             tpf_solve_report* rep);
+
 /* FP64 ||b - A p||_2 / ||b||_2 on the CG system (the judge of every solve). */
 TPF_API tpf_status tpf_ppe_true_relres(tpf_ppe* ppe, const double* rhs,
                                const double* p, double* relres);

@@ -158,6 +158,28 @@ def test_learned_deflation_is_linear_and_converges():
     assert info["fcg_exit"] == "converged" and info["true_relres"] <= 1e-8
 
 
+def test_native_learned_deflation_matches_host_loop():
+    """tpf_ppe_solve_mg_deflated (device FCG(1) + the learned coarse
+    correction after each cycle) against the host FCG(1) with the same V(2,2)
+    cycle and the same coarse space: both reach tol on the FP64 judge, in
+    iteration counts within one (the correction's dot products and E^-1 solve
+    round differently)."""
+    _announce("native learned deflation vs host loop, 64^3")
+    s, H, O, rho = _setup((64, 64, 64))
+    net = L.UNet(seed=3).cuda()
+    level = L.level_of(H, 33)
+    M = L.LearnedDeflationPreconditioner(H, net, level, gamma=1)
+    rhs = torch.from_numpy(np.random.default_rng(10).standard_normal(s.shape)).cuda()
+    p_h, info_h = L.fcg_solve(s, rhs, M, tol=1e-8, maxiter=100, m=1, fallback=False)
+    p_n, info_n = L.fcg_solve_native(s, rhs, M, tol=1e-8, maxiter=100, gamma=1, w_levels=0,
+                                     fallback=False)
+    assert info_h["fcg_exit"] == "converged" and info_n["fcg_exit"] == "converged"
+    assert info_n["true_relres"] <= 1e-8
+    assert abs(info_n["fcg_iterations"] - info_h["fcg_iterations"]) <= 1, (info_n, info_h)
+    with pytest.raises(ValueError):
+        L.fcg_solve_native(s, rhs, L.LearnedDeflationPreconditioner(H, None, level))
+
+
 def test_galerkin_operator_is_symmetric():
     _announce("galerkin symmetry")
     s, H, O, rho = _setup((64, 64, 64))

@@ -73,6 +73,8 @@ SIGNATURES = {
                                       C.POINTER(SolveReport)]),
     "tpf_ppe_gmres_precond": (C.c_int, [_vp, _vp, _vp, C.POINTER(GmresOpts)]),
     "tpf_ppe_solve_mg": (C.c_int, [_vp, _vp, _vp, C.POINTER(MgOpts), C.POINTER(SolveReport)]),
+    "tpf_ppe_solve_mg_deflated": (C.c_int, [_vp, _vp, _vp, C.POINTER(MgOpts), _vp, C.c_int,
+                                            C.c_int, _dp, C.POINTER(SolveReport)]),
     "tpf_fields_bubbles": (C.c_int, [C.c_int, _i64p, _dp, C.c_int, _dp, _dp, C.c_double,
                                      C.c_double, C.c_double, _vp, _vp, _vp]),
     "tpf_fields_mass": (C.c_int, [C.c_int, _i64p, _dp, _ip, _vp, _dp, _vp]),

@@ -37,6 +37,8 @@ namespace {
 
 constexpr int kMaxLevels = 12;
 constexpr int kCoarseMax = 1024;  // nodes of the level solved in one block
+constexpr int kMaxDefl = 32;      // learned coarse vectors
+constexpr int kDeflBlocks = 1024; // dot-product blocks per vector
 
 struct LevelGeom {
   int n[3];
@@ -673,6 +675,51 @@ __global__ __launch_bounds__(1024) void k_two_level(
   }
 }
 
+// Learned coarse correction on level L (A-DEF2): c = W t_L (k dot products,
+// block partials per vector), y = E^-1 c, u = sum_j y_j W_j.
+__global__ __launch_bounds__(kBlock) void k_defl_dot(int64_t n, const float* __restrict__ W,
+                                                     const float* __restrict__ t,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[kBlock / 64];
+  const int j = blockIdx.y;
+  const float* w = W + (int64_t)j * n;
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    v[0] += (double)w[i] * (double)t[i];
+  block_sum<1>(v, sh);
+  if (threadIdx.x == 0) part[(int64_t)j * gridDim.x + blockIdx.x] = v[0];
+}
+
+// One workgroup: c_j = sum_b part[j][b] (fixed order), y = Einv c (FP64).
+__global__ __launch_bounds__(kBlock) void k_defl_solve(int k, int nb, const double* __restrict__ part,
+                                                       const double* __restrict__ Einv,
+                                                       float* __restrict__ y) {
+  __shared__ double c[64];
+  const int j = threadIdx.x;
+  if (j < k) {
+    double a = 0.0;
+    for (int b = 0; b < nb; ++b) a += part[(int64_t)j * nb + b];
+    c[j] = a;
+  }
+  __syncthreads();
+  if (j < k) {
+    double a = 0.0;
+    for (int i = 0; i < k; ++i) a += Einv[j * k + i] * c[i];
+    y[j] = (float)a;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_defl_comb(int64_t n, int k, const float* __restrict__ W,
+                                                      const float* __restrict__ y,
+                                                      float* __restrict__ u) {
+  GRID_LOOP(i, n) {
+    float a = 0.0f;
+    for (int j = 0; j < k; ++j) a += y[j] * W[(int64_t)j * n + i];
+    u[i] = a;
+  }
+}
+
 // ---- host ------------------------------------------------------------------
 struct MgState {
   int nlev = 0;
@@ -684,6 +731,9 @@ struct MgState {
   int cyc_key[4] = {0, 0, 0, 0};
   hipStream_t cyc_stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;  // tpf_mg_vcycle's stream ordering
+  double* defl_part = nullptr;  // [kMaxDefl][blocks] dot partials
+  double* defl_einv = nullptr;  // [kMaxDefl^2]
+  float* defl_y = nullptr;      // [kMaxDefl]
 };
 
 MgState* mg_of(tpf_ppe* s) { return static_cast<MgState*>(s->mg); }
@@ -697,6 +747,9 @@ void free_mg(MgState* m) {
   if (m->cyc) (void)hipGraphExecDestroy(m->cyc);
   if (m->ev_in) (void)hipEventDestroy(m->ev_in);
   if (m->ev_out) (void)hipEventDestroy(m->ev_out);
+  if (m->defl_part) (void)hipFree(m->defl_part);
+  if (m->defl_einv) (void)hipFree(m->defl_einv);
+  if (m->defl_y) (void)hipFree(m->defl_y);
   for (int l = 0; l < m->nlev; ++l) {
     MgLevel& L = m->lv[l];
     float* bufs[] = {L.T[0], L.T[1], L.T[2], L.dinv, L.x, L.b, L.r, L.rho32};
@@ -951,6 +1004,56 @@ tpf_status mg_cycle(tpf_ppe* s, hipStream_t st, int nu, int gamma, int coarse_sw
     m->cyc_stream = st;
   }
   TPF_HIP(hipGraphLaunch(m->cyc, st));
+  return TPF_OK;
+}
+
+tpf_status mg_deflate_setup(tpf_ppe* s, int k, int level, const double* einv_host) {
+  MgState* m = mg_of(s);
+  if (!m || m->nlev == 0) return fail(TPF_ERR_RUNTIME, "multigrid hierarchy not built");
+  if (k < 1 || k > kMaxDefl) return fail(TPF_ERR_VALUE, "deflation: 1 <= k <= 32 vectors");
+  if (level < 1 || level >= m->nlev) return fail(TPF_ERR_VALUE, "deflation level out of range");
+  if (!m->defl_part) {
+    TPF_HIP(hipMalloc(&m->defl_part, sizeof(double) * kMaxDefl * kDeflBlocks));
+    TPF_HIP(hipMalloc(&m->defl_einv, sizeof(double) * kMaxDefl * kMaxDefl));
+    TPF_HIP(hipMalloc(&m->defl_y, sizeof(float) * kMaxDefl));
+  }
+  TPF_HIP(hipMemcpy(m->defl_einv, einv_host, sizeof(double) * k * k, hipMemcpyHostToDevice));
+  return TPF_OK;
+}
+
+tpf_status mg_deflate(tpf_ppe* s, hipStream_t st, const float* W, int k, int level) {
+  MgState* m = mg_of(s);
+  MgLevel& F = m->lv[0];
+  // t = r - A z on the fine level (level-0 b holds r, x holds z = V(r))
+  if (F.rho32)
+    MG_LAUNCH(k_resid_rho, F.g.total, F.g, F.fg, (const float*)F.b, (const float*)F.x, F.r);
+  else
+    MG_LAUNCH(k_resid, F.g.total, F.g, F.T[0], F.T[1], F.T[2], (const float*)F.b,
+              (const float*)F.x, F.r);
+  for (int l = 0; l < level; ++l)
+    MG_LAUNCH(k_restrict, m->lv[l + 1].g.total, m->lv[l].g, m->lv[l + 1].g,
+              (const float*)m->lv[l].r, m->lv[l + 1].r);
+  MgLevel& C = m->lv[level];
+  const int64_t nc = C.g.total;
+  const int nb = (int)std::min<int64_t>((nc + kBlock - 1) / kBlock, kDeflBlocks);
+  hipLaunchKernelGGL(k_defl_dot, dim3(nb, k), dim3(kBlock), 0, st, nc, W, (const float*)C.r,
+                     m->defl_part);
+  hipLaunchKernelGGL(k_defl_solve, dim3(1), dim3(kBlock), 0, st, k, nb,
+                     (const double*)m->defl_part, (const double*)m->defl_einv, m->defl_y);
+  MG_LAUNCH(k_defl_comb, nc, nc, k, W, (const float*)m->defl_y, C.x);
+  // z += P ... P u
+  for (int l = level - 1; l >= 0; --l) {
+    MgLevel& L = m->lv[l];
+    if (l > 0) MG_LAUNCH(k_zero, L.g.total, L.g.total, L.x);
+    MG_LAUNCH(k_prolong_add, L.g.total, L.g, m->lv[l + 1].g, (const float*)m->lv[l + 1].x, L.x);
+  }
+  return TPF_OK;
+}
+
+tpf_status mg_level_total(tpf_ppe* s, int level, int64_t* total) {
+  MgState* m = mg_of(s);
+  if (!m || level < 0 || level >= m->nlev) return fail(TPF_ERR_VALUE, "level out of range");
+  *total = m->lv[level].g.total;
   return TPF_OK;
 }
 }  // namespace tpf

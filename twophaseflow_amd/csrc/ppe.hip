@@ -1018,8 +1018,32 @@ extern "C" tpf_status tpf_ppe_system_apply_f32(tpf_ppe* s, const float* p, float
 }
 
 // ---- tpf_ppe_solve_mg -------------------------------------------------------
+namespace tpf {
+struct Deflation {  // learned coarse correction after each cycle (k = 0: none)
+  const float* W;
+  int k, level;
+};
+static tpf_status solve_mg(tpf_ppe* s, const double* rhs, double* x, const tpf_mg_opts* o,
+                           tpf_solve_report* rep, const Deflation& defl,
+                           const double* einv = nullptr);
+}  // namespace tpf
+
 extern "C" tpf_status tpf_ppe_solve_mg(tpf_ppe* s, const double* rhs, double* x,
                                        const tpf_mg_opts* o, tpf_solve_report* rep) {
+  return tpf::solve_mg(s, rhs, x, o, rep, tpf::Deflation{nullptr, 0, 0});
+}
+
+extern "C" tpf_status tpf_ppe_solve_mg_deflated(tpf_ppe* s, const double* rhs, double* x,
+                                                const tpf_mg_opts* o, const float* W, int k,
+                                                int level, const double* einv,
+                                                tpf_solve_report* rep) {
+  if (!W || !einv) return fail(TPF_ERR_VALUE, "null deflation argument");
+  return tpf::solve_mg(s, rhs, x, o, rep, tpf::Deflation{W, k, level}, einv);
+}
+
+namespace tpf {
+static tpf_status solve_mg(tpf_ppe* s, const double* rhs, double* x, const tpf_mg_opts* o,
+                           tpf_solve_report* rep, const Deflation& defl, const double* einv) {
   if (!s || !rhs || !x || !o) return fail(TPF_ERR_VALUE, "null argument");
   if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set (call tpf_ppe_set_density)");
   if (!(o->tol > 0.0) || o->maxiter < 1) return fail(TPF_ERR_VALUE, "bad tol/maxiter");
@@ -1027,6 +1051,7 @@ extern "C" tpf_status tpf_ppe_solve_mg(tpf_ppe* s, const double* rhs, double* x,
     return fail(TPF_ERR_VALUE, "bad multigrid cycle options (gamma 1|2, nu >= 1, sweeps >= 1)");
   const int max_levels = o->max_levels > 0 ? o->max_levels : 12;
   TPF_TRY(mg_ensure(s, max_levels));  // on the caller's stream, synchronous
+  if (defl.k > 0) TPF_TRY(mg_deflate_setup(s, defl.k, defl.level, einv));
   TPF_TRY(join_user_stream(s));
   hipStream_t st = s->stream;
   // workspace: r in s->r, p in s->p, q in s->q (FP64); FP32 residual and
@@ -1075,6 +1100,7 @@ extern "C" tpf_status tpf_ppe_solve_mg(tpf_ppe* s, const double* rhs, double* x,
     bool hit = rr <= tol2 * bb;
     while (!hit && iters < o->maxiter) {
       TPF_TRY(mg_cycle(s, st, o->nu, o->gamma, o->coarse_sweeps, o->w_levels));
+      if (defl.k > 0) TPF_TRY(mg_deflate(s, st, defl.W, defl.k, defl.level));
       hipLaunchKernelGGL(k_fcg_zq, grid, blk, 0, st, n, z32, (const double*)s->q, s->partials);
       hipLaunchKernelGGL(k_fcg_fin_beta, dim3(1), blk, 0, st, dst, (const double*)s->partials,
                          s->grid);
@@ -1116,3 +1142,4 @@ extern "C" tpf_status tpf_ppe_solve_mg(tpf_ppe* s, const double* rhs, double* x,
     return fail(TPF_ERR_RUNTIME, "multigrid FCG did not reach tol in maxiter iterations");
   return TPF_OK;
 }
+}  // namespace tpf

@@ -73,6 +73,11 @@ float* mg_fine_x(tpf_ppe* s);                       // level-0 iterate (cycle ou
 // repeats apply on the top w_levels levels (all when w_levels <= 0).
 tpf_status mg_cycle(tpf_ppe* s, hipStream_t st, int nu, int gamma, int coarse_sweeps,
                     int w_levels);
+// Learned coarse correction (A-DEF2) after a cycle: z += P^L (W^T E^-1 W (R^L (r - A z)))
+// with W [k][n_level] on level `level`; E^-1 is uploaded by mg_deflate_setup.
+tpf_status mg_deflate_setup(tpf_ppe* s, int k, int level, const double* einv_host);
+tpf_status mg_deflate(tpf_ppe* s, hipStream_t st, const float* W, int k, int level);
+tpf_status mg_level_total(tpf_ppe* s, int level, int64_t* total);
 }  // namespace tpf
 tpf_status tpf_ppe_solve_mixed(tpf_ppe* s, const double* rhs, double* x,
                                const tpf_solve_opts* opts, tpf_solve_report* rep);

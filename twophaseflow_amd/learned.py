@@ -451,6 +451,43 @@ def fcg_solve(ppe_solver, rhs, precond, tol=1e-8, maxiter=2000, m=8, stall_windo
     return x, info
 
 
+def fcg_solve_native(ppe_solver, rhs, precond, tol=1e-8, maxiter=500, nu=2, gamma=2,
+                     coarse_sweeps=20, w_levels=2, fallback=True):
+    """The device FCG(1) of ``tpf_ppe_solve_mg`` (graph-replayed multigrid
+    cycle, FP64 vectors and reductions, FP64 true-residual judge) with the
+    learned coarse correction of ``precond`` (a LearnedDeflationPreconditioner)
+    applied after every cycle on the device (``tpf_ppe_solve_mg_deflated``):
+    z = V(r); z += P W E^-1 W^T R (r - A z).  Plain Jacobi-PCG from the FCG
+    iterate is the fallback when FCG stops above tol."""
+    if precond.net is None:
+        raise ValueError("fcg_solve_native needs a learned coarse space (net is None)")
+    lib = _lib()
+    W = precond.W.float().contiguous()
+    k = int(W.shape[0])
+    einv = torch.cholesky_inverse(precond.L).double().cpu().numpy()
+    einv = np.ascontiguousarray(0.5 * (einv + einv.T))
+    p = torch.zeros_like(rhs)
+    opts = nat.MgOpts(tol=tol, maxiter=maxiter, use_x0=0, nu=nu, gamma=gamma,
+                      coarse_sweeps=coarse_sweeps, max_levels=0, w_levels=w_levels)
+    rep = nat.SolveReport()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    status = lib.tpf_ppe_solve_mg_deflated(ppe_solver._handle, _p(rhs), _p(p), C.byref(opts),
+                                            _p(W), k, int(precond.level),
+                                            einv.ctypes.data_as(C.POINTER(C.c_double)),
+                                            C.byref(rep))
+    info = {"fcg_iterations": int(rep.iterations), "fcg_restarts": int(rep.refinements),
+            "fcg_exit": "converged" if status == nat.TPF_OK else "maxiter",
+            "fcg_seconds": time.perf_counter() - t0, "fallback_iterations": 0}
+    if status not in (nat.TPF_OK, nat.TPF_ERR_RUNTIME):
+        nat.check(status)
+    if status != nat.TPF_OK and fallback:
+        ppe_solver.solve_into(rhs, None, p, use_x0=True)
+        info["fallback_iterations"] = ppe_solver.last_diagnostics["iterations"]
+    info["true_relres"] = ppe_solver.true_relres(rhs, p)
+    return p, info
+
+
 # ------------------------------------------------------------- training ---
 def make_problem(N, seed, nb_range=(8, 128), ratio_range=(10.0, 1000.0)):
     """Random bubble density and a divergence right-hand side on an N^3 wall
