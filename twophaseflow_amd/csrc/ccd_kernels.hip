@@ -1,17 +1,21 @@
 // CCD line-solve kernels for gfx950 and the C-ABI entry points that launch
 // them (replace ccd/ccd_solver_helpers.py:38-77 and :426-465).
 //
-// Mapping: one workgroup = 64 lines x P partitions.  Lane = line, wavefront
-// = partition, so every table value a wavefront needs is uniform and comes
-// through the scalar cache.  A lane keeps its partition (<= M rows, 2 FP64
-// unknowns per row) in registers: RHS -> local block LU forward/back sweep
-// -> partition end values to LDS -> coupling (precomputed reduced-system
-// rows) -> spike correction -> store.
-//
-// Strided axes: lanes read/write consecutive addresses (coalesced).
-// Contiguous (last) axis: loads hit the same cache lines per lane row and
-// are absorbed by L1; stores are re-staged through LDS per wavefront so
-// each store instruction writes whole 128 B segments.
+// Kernels (one line = the 2x2-block tridiagonal system of one grid line, cut
+// into P partitions of <= M rows and coupled through precomputed reduced
+// (SPIKE) rows):
+//  * ccd_line_kernel<M>: 64 lines x P partitions per workgroup, lane = line,
+//    wavefront = partition (tables wavefront-uniform, staged in LDS).  A lane
+//    keeps its partition (<= M rows, 2 FP64 unknowns per row) in registers:
+//    RHS -> local block LU forward/back sweep -> partition ends to LDS ->
+//    coupling -> spike correction -> store.  The contiguous (last) axis is
+//    staged through LDS as one coalesced [64][nax] tile.
+//  * ccd_explicit_kernel<M>: strided axes, same mapping, the local solve as
+//    the precomputed partition inverse applied to first differences (no
+//    recurrence state: 64 VGPRs, two workgroups per CU).
+//  * ccd_wave_kernel<M>: few long lines; lane = partition, wavefront = line.
+//  * ccd_line_kernel<M, true>: FP32 solves with one FP64 refinement.
+//  * batch kernels: the same per-line code over a device table of slots.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
