@@ -219,6 +219,45 @@ TPF_API tpf_status tpf_ppe_solve_mg_deflated(tpf_ppe* ppe, const double* rhs, do
 TPF_API tpf_status tpf_ppe_true_relres(tpf_ppe* ppe, const double* rhs,
                                const double* p, double* relres);
 
+/* ----------------------------------------------------------- FCCD PPE --- */
+/* The FCCD matrix-free PPE, L p = D_f[(1/rho)_f G_f p] with the FCCD face
+ * gradient G_f p = (p_hi - p_lo)/H - (H/24)(q_hi - q_lo), q the nodal CCD
+ * second derivative: the reference's default configuration (phase_density
+ * coefficients, no interface coupling, full_face boundary space, legacy
+ * operator pairing) on uniform grids.  Calls are ordered on the stream given
+ * to tpf_fccd_ppe_create. */
+typedef struct tpf_fccd_ppe tpf_fccd_ppe;
+
+/* Replaces PPESolverFCCDMatrixFree.__init__ with its FCCDSolver / CCDSolver
+ * (ppe/fccd_matrixfree.py:108-213, ccd/fccd.py:165-195): bc[a] in
+ * {TPF_BC_WALL, TPF_BC_PERIODIC}, L[a] domain length (H = L/N), pin_dof =
+ * the gauge node (centre node by default, fccd_matrixfree.py:187-192). */
+TPF_API tpf_status tpf_fccd_ppe_create(tpf_fccd_ppe** ppe, int ndim, const int64_t* n_nodes,
+                                       const double* L, const int* bc, int64_t pin_dof,
+                                       void* stream);
+TPF_API tpf_status tpf_fccd_ppe_destroy(tpf_fccd_ppe* ppe);
+
+/* Replaces prepare_operator (ppe/fccd_matrixfree_lifecycle.py:44-96): keeps
+ * rho (face coefficients 2/(rho_lo + rho_hi) are formed inside each matvec)
+ * and builds the Jacobi inverse (fccd_matrixfree_helpers.py:251-264).  rho
+ * must stay valid until the next call. */
+TPF_API tpf_status tpf_fccd_ppe_set_density(tpf_fccd_ppe* ppe, const double* rho);
+
+/* Replaces PPESolverFCCDMatrixFree.apply (fccd_matrixfree.py:378-397):
+ * out = L p, out[pin] = p[pin], periodic image rows out = p_img - p_src.
+ * Asynchronous; p and out distinct. */
+TPF_API tpf_status tpf_fccd_ppe_apply(tpf_fccd_ppe* ppe, const double* p, double* out);
+
+/* Replaces PPESolverFCCDMatrixFree.solve (fccd_matrixfree.py:505-600): SciPy
+ * 1.15 GMRES (gmres_helpers.py:16-42) on L p = b, b = rhs with b[pin] = 0 and
+ * zero image rows; opts->precond 0 none, 1 Jacobi; opts->restart as SciPy's
+ * (the reference passes ppe_restart, None = 20); opts->use_x0: p holds p_init
+ * (images synced, pin 0).  On exit p[pin] = 0 and image nodes equal their
+ * sources; rep->true_relres = ||L p - b|| / ||b|| on that field.
+ * Synchronous; TPF_ERR_RUNTIME when GMRES ends above tol (report filled). */
+TPF_API tpf_status tpf_fccd_ppe_solve_gmres(tpf_fccd_ppe* ppe, const double* rhs, double* p,
+                                            const tpf_gmres_opts* opts, tpf_solve_report* rep);
+
 /* ------------------------------------------------------------- fields --- */
 /* Replaces InitialConditionBuilder.build (gas spheres in liquid) +
  * update_properties (simulation/initial_conditions/builder.py:100-172,

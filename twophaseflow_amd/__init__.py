@@ -1,8 +1,9 @@
 """twophaseflow_amd -- MI355X-native CCD operator and variable-density PPE.
 
 Drop-in replacements for the two hot paths of TwoPhaseFlow behind its own
-Python API: ``CCDSolver`` (ccd/ccd_solver.py) and the ``IPPESolver`` plugin
-``PPESolverFVMMatrixFree`` (ppe/fvm_matrixfree.py).  The compute runs in
+Python API: ``CCDSolver`` (ccd/ccd_solver.py) and the ``IPPESolver`` plugins
+``PPESolverFVMMatrixFree`` (ppe/fvm_matrixfree.py) and
+``PPESolverFCCDMatrixFree`` (ppe/fccd_matrixfree.py, default configuration).  The compute runs in
 hand-written HIP kernels for gfx950 in ``libtpf.so`` (C ABI in
 include/tpf.h), reached through ctypes; PyTorch provides device memory and
 the stream only.  There is no CPU fallback.
@@ -12,12 +13,13 @@ from .config import GridConfig, NumericsConfig, SimulationConfig, SolverConfig
 from .grid import Grid
 from .boundary import BoundarySpec, axis_kinds, pin_dof
 from .ccd import CCDBatch, CCDSolver
-from .ppe import (IPPESolver, MatrixAssemblyUnavailable, PPESolverFVMMatrixFree,
-                  create_ppe_solver, register_ppe_solver)
+from .ppe import (IPPESolver, MatrixAssemblyUnavailable, PPESolverFCCDMatrixFree,
+                  PPESolverFVMMatrixFree, create_ppe_solver, register_ppe_solver)
 
 __all__ = [
     "GridConfig", "NumericsConfig", "SimulationConfig", "SolverConfig", "Grid",
     "BoundarySpec", "axis_kinds", "pin_dof", "CCDSolver", "CCDBatch", "IPPESolver",
-    "MatrixAssemblyUnavailable", "PPESolverFVMMatrixFree", "create_ppe_solver",
+    "MatrixAssemblyUnavailable", "PPESolverFVMMatrixFree", "PPESolverFCCDMatrixFree",
+    "create_ppe_solver",
     "register_ppe_solver",
 ]

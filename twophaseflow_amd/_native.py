@@ -75,6 +75,13 @@ SIGNATURES = {
     "tpf_ppe_solve_mg": (C.c_int, [_vp, _vp, _vp, C.POINTER(MgOpts), C.POINTER(SolveReport)]),
     "tpf_ppe_solve_mg_deflated": (C.c_int, [_vp, _vp, _vp, C.POINTER(MgOpts), _vp, C.c_int,
                                             C.c_int, _dp, C.POINTER(SolveReport)]),
+    "tpf_fccd_ppe_create": (C.c_int, [C.POINTER(_vp), C.c_int, _i64p, _dp, _ip, C.c_int64,
+                                      _vp]),
+    "tpf_fccd_ppe_destroy": (C.c_int, [_vp]),
+    "tpf_fccd_ppe_set_density": (C.c_int, [_vp, _vp]),
+    "tpf_fccd_ppe_apply": (C.c_int, [_vp, _vp, _vp]),
+    "tpf_fccd_ppe_solve_gmres": (C.c_int, [_vp, _vp, _vp, C.POINTER(GmresOpts),
+                                           C.POINTER(SolveReport)]),
     "tpf_fields_bubbles": (C.c_int, [C.c_int, _i64p, _dp, C.c_int, _dp, _dp, C.c_double,
                                      C.c_double, C.c_double, _vp, _vp, _vp]),
     "tpf_fields_mass": (C.c_int, [C.c_int, _i64p, _dp, _ip, _vp, _dp, _vp]),

@@ -1,0 +1,519 @@
+// FCCD matrix-free PPE: L p = D_f[(1/rho)_f G_f p] with the FCCD face
+// gradient G_f p = (p_hi - p_lo)/H - (H/24)(q_hi - q_lo), q the nodal CCD
+// second derivative along the face's axis, solved by restarted GMRES.
+//
+// Reference (paths relative to src/twophase): ppe/fccd_matrixfree.py
+// (PPESolverFCCDMatrixFree: apply :378-397, operator core :399-417, weighted
+// divergence :769-818, solve :505-600, pin and periodic image rows
+// :300-376), ccd/fccd.py:243-277 (face_gradient), :464-502 (periodic face
+// divergence), ccd/fccd_helpers.py:17-35 (uniform weights),
+// ppe/fccd_matrixfree_helpers.py:35-51 (node widths), :207-248 (face
+// coefficients 2/(rho_lo + rho_hi)), :251-264 (Jacobi inverse).  Built for
+// the reference's default configuration: phase_density coefficients, no
+// interface coupling, full_face boundary space, legacy operator pairing,
+// uniform grids.
+//
+// One matvec = one all-axes CCD launch writing q only (d1 NULL), then one
+// streaming kernel that forms both face gradients of every axis at a node
+// from p, q and rho (neighbours come through L1/L2) and accumulates the
+// weighted divergence; the face arrays the reference materialises per axis
+// never reach HBM.  Image nodes of periodic axes never feed a non-image row
+// (the CCD and the face stencils use the unique nodes 0..N-1 along a
+// periodic axis, and the other axes' lines through a non-image node hold
+// only non-image nodes), so the reference's image synchronisation of p
+// before the core is not needed: the image rows are overwritten by
+// p_img - p_src afterwards.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "gmres.hpp"
+#include "tpf_common.hpp"
+
+namespace tpf {
+namespace {
+
+constexpr int kBlock = 256;
+
+// Node grid right-aligned into 3 slots (leading slots n = 1); slot 2 is the
+// contiguous axis.
+struct FccdGeom {
+  int n[3];
+  int periodic[3];
+  int qk[3];          // index of the slot's axis in the q stack
+  int64_t total;
+  int64_t pin;
+  double inv_H[3];    // 1 / H, H = L / N (face width)
+  double H24[3];      // H / 24
+  double inv_w[3];    // 1 / node width, interior
+  double inv_w0[3];   // 1 / node width at a wall node (H / 2)
+  double h_min[3];    // Jacobi: smallest face width of the axis
+};
+
+__device__ __forceinline__ void coords(const FccdGeom& g, int64_t idx, int (&c)[3]) {
+  const uint32_t plane = (uint32_t)(g.n[1] * g.n[2]);
+  const uint32_t u = (uint32_t)idx;
+  c[0] = (int)(u / plane);
+  const uint32_t rem = u - (uint32_t)c[0] * plane;
+  c[1] = (int)(rem / (uint32_t)g.n[2]);
+  c[2] = (int)(rem - (uint32_t)c[1] * (uint32_t)g.n[2]);
+}
+
+__device__ __forceinline__ int64_t stride_of(const FccdGeom& g, int a) {
+  return a == 0 ? (int64_t)g.n[1] * g.n[2] : (a == 1 ? (int64_t)g.n[2] : 1);
+}
+
+// Source of an image node: the first periodic axis (in axis order) on which
+// the node sits at index N sets that index to 0 (fccd_matrixfree.py:300-345).
+// Returns -1 for a non-image node.
+__device__ __forceinline__ int64_t image_source(const FccdGeom& g, int64_t idx,
+                                                const int (&c)[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (g.periodic[a] && g.n[a] > 1 && c[a] == g.n[a] - 1)
+      return idx - (int64_t)c[a] * stride_of(g, a);
+  return -1;
+}
+
+// Values one axis-0 plane contributes to a node's stencil.
+struct Plane0 {
+  double p, q, rho;
+};
+
+// Contribution of one axis to node idx (value c along it, stride st): the
+// weighted divergence of the FCCD face gradients on its two faces
+// (fccd_matrixfree.py:769-818; periodic: ccd/fccd.py:490-502).  lo / hi hold
+// the neighbours' p and q (wrapped on periodic axes); rho_lo is the lower
+// neighbour's rho, rho_hi the upper face's partner rho (never wrapped), and
+// rho_lo_partner the lower face's upper rho (the node itself, or node N for
+// face N-1 of a periodic axis).
+__device__ __forceinline__ double axis_term(const FccdGeom& g, int a, int i, double pc, double qc,
+                                            double rc, const Plane0& lo, const Plane0& hi,
+                                            double rho_lo_partner) {
+  const int N = g.n[a] - 1;
+  const double iH = g.inv_H[a], H24 = g.H24[a];
+  if (g.periodic[a]) {
+    const double g_hi = (hi.p - pc) * iH - H24 * (hi.q - qc);
+    const double F_hi = (2.0 / (rc + hi.rho)) * g_hi;
+    const double g_lo = (pc - lo.p) * iH - H24 * (qc - lo.q);
+    const double F_lo = (2.0 / (lo.rho + rho_lo_partner)) * g_lo;
+    return (F_hi - F_lo) * iH;
+  }
+  if (i == 0) {
+    const double g_hi = (hi.p - pc) * iH - H24 * (hi.q - qc);
+    return (2.0 / (rc + hi.rho)) * g_hi * g.inv_w0[a];
+  }
+  if (i == N) {
+    const double g_lo = (pc - lo.p) * iH - H24 * (qc - lo.q);
+    return -((2.0 / (lo.rho + rc)) * g_lo * g.inv_w0[a]);
+  }
+  const double g_hi = (hi.p - pc) * iH - H24 * (hi.q - qc);
+  const double g_lo = (pc - lo.p) * iH - H24 * (qc - lo.q);
+  const double c_hi = 2.0 / (rc + hi.rho);
+  const double c_lo = 2.0 / (lo.rho + rc);
+  return (c_hi * g_hi - c_lo * g_lo) * g.inv_w[a];
+}
+
+// Neighbours of node idx along slot a (1 or 2) straight from memory (the
+// rows involved are being read by this and the adjacent workgroups).
+__device__ __forceinline__ void neighbours(const FccdGeom& g, int a, int i, int64_t idx,
+                                           const double* __restrict__ p,
+                                           const double* __restrict__ qa,
+                                           const double* __restrict__ rho, Plane0& lo,
+                                           Plane0& hi, double& rho_lo_partner, double rc) {
+  const int N = g.n[a] - 1;
+  const int64_t st = stride_of(g, a);
+  const bool per = g.periodic[a];
+  rho_lo_partner = rc;
+  if (i < N) {
+    const int64_t hv = (per && i + 1 == N) ? idx - (int64_t)i * st : idx + st;
+    hi = Plane0{p[hv], qa[hv], rho[idx + st]};
+  }
+  if (i > 0) {
+    lo = Plane0{p[idx - st], qa[idx - st], rho[idx - st]};
+  } else if (per) {  // face N-1: nodes N-1 and 0, rho at N-1 and N
+    const int64_t lv = idx + (int64_t)(N - 1) * st;
+    lo = Plane0{p[lv], qa[lv], rho[lv]};
+    rho_lo_partner = rho[lv + st];
+  }
+}
+
+// out = L p with the pin row p[pin] and image rows p_img - p_src.
+// Thread = one node of an axis-0 plane (coalesced over the plane); the
+// workgroup marches kSlab planes along axis 0 keeping the previous, current
+// and next planes' p, q_0 and rho in registers, so each of them is read from
+// HBM once per slab (plus one halo plane on each side).
+constexpr int kSlab = 32;
+
+__global__ __launch_bounds__(kBlock) void k_fccd_apply(FccdGeom g, const double* __restrict__ p,
+                                                       const double* __restrict__ q,
+                                                       const double* __restrict__ rho,
+                                                       double* __restrict__ out) {
+  const int plane = g.n[1] * g.n[2];
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= plane) return;
+  const int c1 = j / g.n[2], c2 = j - c1 * g.n[2];
+  const int n0 = g.n[0], N0 = n0 - 1;
+  const bool has0 = n0 > 1, per0 = g.periodic[0] != 0;
+  const int s0 = blockIdx.y * kSlab;
+  const int s1 = min(n0, s0 + kSlab);
+  const double* q0 = q + (int64_t)g.qk[0] * g.total;
+  const double* q1 = q + (int64_t)g.qk[1] * g.total;
+  const double* q2 = q + (int64_t)g.qk[2] * g.total;
+  // image along slot 1 or 2 (the first periodic slot wins; slot 0 is checked
+  // per plane below and comes first)
+  int64_t img12 = -1;
+  if (g.periodic[1] && g.n[1] > 1 && c1 == g.n[1] - 1) img12 = (int64_t)c1 * g.n[2];
+  else if (g.periodic[2] && g.n[2] > 1 && c2 == g.n[2] - 1) img12 = c2;
+  auto ld = [&](int vplane, int rplane) {
+    const int64_t v = (int64_t)vplane * plane + j, r = (int64_t)rplane * plane + j;
+    return Plane0{p[v], q0[v], rho[r]};
+  };
+  Plane0 prev{0.0, 0.0, 0.0}, cur = ld(s0, s0), next{0.0, 0.0, 0.0};
+  if (has0 && s0 > 0) prev = ld(s0 - 1, s0 - 1);
+  else if (has0 && per0) prev = ld(N0 - 1, N0 - 1);
+  for (int i0 = s0; i0 < s1; ++i0) {
+    if (has0 && i0 < N0) {
+      const int vn = (per0 && i0 + 1 == N0) ? 0 : i0 + 1;
+      next = ld(vn, i0 + 1);
+    }
+    const int64_t idx = (int64_t)i0 * plane + j;
+    double res;
+    if (has0 && per0 && i0 == N0) {
+      res = p[idx] - p[j];  // image along axis 0 (cur holds plane 0's wrapped values)
+    } else if (img12 >= 0) {
+      res = cur.p - p[idx - img12];
+    } else if (idx == g.pin) {
+      res = cur.p;
+    } else {
+      const double pc = cur.p, rc = cur.rho;
+      double acc = 0.0;
+      if (has0) {
+        const double rlp = (per0 && i0 == 0) ? rho[(int64_t)N0 * plane + j] : rc;
+        acc += axis_term(g, 0, i0, pc, cur.q, rc, prev, next, rlp);
+      }
+      Plane0 lo{0.0, 0.0, 0.0}, hi{0.0, 0.0, 0.0};
+      double rlp = rc;
+      if (g.n[1] > 1) {
+        neighbours(g, 1, c1, idx, p, q1, rho, lo, hi, rlp, rc);
+        acc += axis_term(g, 1, c1, pc, q1[idx], rc, lo, hi, rlp);
+      }
+      if (g.n[2] > 1) {
+        neighbours(g, 2, c2, idx, p, q2, rho, lo, hi, rlp, rc);
+        acc += axis_term(g, 2, c2, pc, q2[idx], rc, lo, hi, rlp);
+      }
+      res = acc;
+    }
+    __builtin_nontemporal_store(res, &out[idx]);
+    prev = cur;
+    cur = next;
+  }
+}
+
+// Jacobi inverse (fccd_matrixfree_helpers.py:251-264): diag = -sum_axes
+// 2/(rho h_min^2), diag[pin] = 1, |diag| <= 1e-30 -> 1.
+__global__ __launch_bounds__(kBlock) void k_fccd_jacobi(FccdGeom g, const double* __restrict__ rho,
+                                                        double* __restrict__ dinv) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    const double r = rho[idx];
+    double diag = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (g.n[a] > 1) diag -= 2.0 / (r * g.h_min[a] * g.h_min[a]);
+    if (idx == g.pin) diag = 1.0;
+    dinv[idx] = 1.0 / (fabs(diag) > 1e-30 ? diag : 1.0);
+  }
+}
+
+// z = r * dinv, z[pin] = 0 (fccd_matrixfree.py:559-566).
+__global__ __launch_bounds__(kBlock) void k_fccd_precond(int64_t n, int64_t pin,
+                                                         const double* __restrict__ dinv,
+                                                         const double* __restrict__ r,
+                                                         double* __restrict__ z) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    z[i] = i == pin ? 0.0 : r[i] * dinv[i];
+}
+
+// b = rhs with b[pin] = 0 and zero image rows (fccd_matrixfree.py:514-516, 370-376).
+__global__ __launch_bounds__(kBlock) void k_fccd_rhs(FccdGeom g, const double* __restrict__ rhs,
+                                                     double* __restrict__ b) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int c[3];
+    coords(g, idx, c);
+    b[idx] = (idx == g.pin || image_source(g, idx, c) >= 0) ? 0.0 : rhs[idx];
+  }
+}
+
+// x = gauge(src): x[pin] = 0, image nodes take their sources' values from
+// `src` (a copy of x: the reference's fancy assignment reads every source
+// before writing).
+__global__ __launch_bounds__(kBlock) void k_fccd_gauge(FccdGeom g, const double* __restrict__ src,
+                                                       double* __restrict__ x) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int c[3];
+    coords(g, idx, c);
+    const int64_t s = image_source(g, idx, c);
+    x[idx] = idx == g.pin ? 0.0 : (s >= 0 ? src[s] : src[idx]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fccd_fill(int64_t n, double v, double* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    x[i] = v;
+}
+
+// y = x - y
+__global__ __launch_bounds__(kBlock) void k_fccd_xmy(int64_t n, const double* __restrict__ x,
+                                                     double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    y[i] = x[i] - y[i];
+}
+
+// sum of squares, per block then one block (deterministic order)
+__global__ __launch_bounds__(kBlock) void k_fccd_sumsq(int64_t n, const double* __restrict__ x,
+                                                       double* __restrict__ part) {
+  __shared__ double sh[kBlock];
+  double v = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    v += x[i] * x[i];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ __launch_bounds__(kBlock) void k_fccd_sumsq_fin(int nb, double* __restrict__ part) {
+  __shared__ double sh[kBlock];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kBlock) v += part[i];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[nb] = sh[0];
+}
+
+int grid_for(int64_t total) {
+  const int64_t blocks = (total + kBlock - 1) / kBlock;
+  const int64_t cap = 256 * 8;
+  return (int)std::max<int64_t>(1, std::min(blocks, cap));
+}
+
+}  // namespace
+}  // namespace tpf
+
+struct tpf_fccd_ppe {
+  tpf::FccdGeom g{};
+  int ndim = 0;
+  hipStream_t stream = nullptr;
+  int grid = 1;
+  tpf_ccd_plan* ccd = nullptr;
+  const double* rho = nullptr;
+  double* q = nullptr;       // ndim x total: CCD second derivative per axis
+  double* dinv = nullptr;    // Jacobi inverse (built with the density)
+  double* part = nullptr;    // norm partials (+1 result)
+  double* host = nullptr;    // pinned result
+  void* gmres_work = nullptr;
+};
+
+using namespace tpf;
+
+#define FCCD_LAUNCH(kern, ...)                                                       \
+  do {                                                                               \
+    hipLaunchKernelGGL(kern, dim3(s->grid), dim3(kBlock), 0, s->stream, __VA_ARGS__); \
+    TPF_HIP(hipGetLastError());                                                      \
+  } while (0)
+
+extern "C" tpf_status tpf_fccd_ppe_destroy(tpf_fccd_ppe* s) {
+  if (!s) return TPF_OK;
+  gmres_work_free(&s->gmres_work);
+  if (s->ccd) tpf_ccd_plan_destroy(s->ccd);
+  double* bufs[] = {s->q, s->dinv, s->part};
+  for (double* b : bufs)
+    if (b) (void)hipFree(b);
+  if (s->host) (void)hipHostFree(s->host);
+  delete s;
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_fccd_ppe_create(tpf_fccd_ppe** out, int ndim, const int64_t* n_nodes,
+                                          const double* L, const int* bc, int64_t pin_dof,
+                                          void* stream) {
+  if (!out || !n_nodes || !L || !bc) return fail(TPF_ERR_VALUE, "null argument");
+  *out = nullptr;
+  if (ndim < 1 || ndim > 3) return fail(TPF_ERR_VALUE, "ndim must be 1, 2 or 3");
+  auto* s = new tpf_fccd_ppe();
+  FccdGeom& g = s->g;
+  s->ndim = ndim;
+  double h[3] = {1.0, 1.0, 1.0};
+  for (int a = 0; a < 3; ++a) {
+    g.n[a] = 1;
+    g.periodic[a] = 0;
+    g.qk[a] = 0;
+  }
+  for (int a = 0; a < ndim; ++a) {
+    const int slot = 3 - ndim + a;
+    if (n_nodes[a] < 3 || !(L[a] > 0.0) || (bc[a] != TPF_BC_WALL && bc[a] != TPF_BC_PERIODIC)) {
+      delete s;
+      return fail(TPF_ERR_VALUE, "FCCD PPE axis needs >= 3 nodes, L > 0 and a wall/periodic bc");
+    }
+    g.n[slot] = (int)n_nodes[a];
+    g.periodic[slot] = bc[a] == TPF_BC_PERIODIC;
+    g.qk[slot] = a;
+    h[slot] = L[a] / (double)(n_nodes[a] - 1);
+  }
+  g.total = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  if (g.total >= (int64_t)0xffffffffLL) {
+    delete s;
+    return fail(TPF_ERR_VALUE, "FCCD PPE grid too large (2^32 nodes)");
+  }
+  if (pin_dof < 0 || pin_dof >= g.total) {
+    delete s;
+    return fail(TPF_ERR_VALUE, "pin_dof out of range");
+  }
+  g.pin = pin_dof;
+  for (int a = 0; a < 3; ++a) {
+    g.inv_H[a] = 1.0 / h[a];
+    g.H24[a] = h[a] / 24.0;
+    g.inv_w[a] = 1.0 / h[a];
+    g.inv_w0[a] = 1.0 / (0.5 * h[a]);
+    g.h_min[a] = h[a];
+  }
+  s->stream = (hipStream_t)stream;
+  s->grid = grid_for(g.total);
+  double hh[3];
+  int bcs[3];
+  for (int a = 0; a < ndim; ++a) {
+    hh[a] = h[3 - ndim + a];
+    bcs[a] = bc[a];
+  }
+  tpf_status st = tpf_ccd_plan_create(&s->ccd, ndim, n_nodes, hh, bcs, TPF_PREC_FP64);
+  if (st != TPF_OK) {
+    s->ccd = nullptr;
+    tpf_fccd_ppe_destroy(s);
+    return st;  // message from the CCD plan
+  }
+  const size_t bytes = sizeof(double) * (size_t)g.total;
+  if (hipMalloc(&s->q, bytes * ndim) != hipSuccess || hipMalloc(&s->dinv, bytes) != hipSuccess ||
+      hipMalloc(&s->part, sizeof(double) * (s->grid + 1)) != hipSuccess ||
+      hipHostMalloc(&s->host, sizeof(double)) != hipSuccess) {
+    tpf_fccd_ppe_destroy(s);
+    return fail(TPF_ERR_RUNTIME, "FCCD PPE workspace allocation failed");
+  }
+  *out = s;
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_fccd_ppe_set_density(tpf_fccd_ppe* s, const double* rho) {
+  if (!s || !rho) return fail(TPF_ERR_VALUE, "null argument");
+  s->rho = rho;
+  FCCD_LAUNCH(k_fccd_jacobi, s->g, rho, s->dinv);
+  return TPF_OK;
+}
+
+namespace {
+tpf_status fccd_apply(tpf_fccd_ppe* s, const double* p, double* out) {
+  const uint32_t mask = (1u << s->ndim) - 1u;
+  TPF_TRY(tpf_ccd_apply(s->ccd, p, nullptr, s->q, mask, s->stream));
+  const int plane = s->g.n[1] * s->g.n[2];
+  const dim3 grid((unsigned)((plane + kBlock - 1) / kBlock),
+                  (unsigned)((s->g.n[0] + kSlab - 1) / kSlab));
+  hipLaunchKernelGGL(k_fccd_apply, grid, dim3(kBlock), 0, s->stream, s->g, p,
+                     (const double*)s->q, s->rho, out);
+  TPF_HIP(hipGetLastError());
+  return TPF_OK;
+}
+
+tpf_status fccd_norm2(tpf_fccd_ppe* s, const double* x, double& out) {
+  FCCD_LAUNCH(k_fccd_sumsq, s->g.total, x, s->part);
+  hipLaunchKernelGGL(k_fccd_sumsq_fin, dim3(1), dim3(kBlock), 0, s->stream, s->grid, s->part);
+  TPF_HIP(hipGetLastError());
+  TPF_HIP(hipMemcpyAsync(s->host, s->part + s->grid, sizeof(double), hipMemcpyDeviceToHost,
+                         s->stream));
+  TPF_HIP(hipStreamSynchronize(s->stream));
+  out = std::sqrt(*s->host);
+  return TPF_OK;
+}
+}  // namespace
+
+extern "C" tpf_status tpf_fccd_ppe_apply(tpf_fccd_ppe* s, const double* p, double* out) {
+  if (!s || !p || !out) return fail(TPF_ERR_VALUE, "null argument");
+  if (p == out) return fail(TPF_ERR_VALUE, "p and out must be distinct arrays");
+  if (!s->rho) return fail(TPF_ERR_RUNTIME, "prepare_operator(rho) must be called before apply(p)");
+  return fccd_apply(s, p, out);
+}
+
+extern "C" tpf_status tpf_fccd_ppe_solve_gmres(tpf_fccd_ppe* s, const double* rhs, double* x,
+                                               const tpf_gmres_opts* o, tpf_solve_report* rep) {
+  if (!s || !rhs || !x || !o) return fail(TPF_ERR_VALUE, "null argument");
+  if (!s->rho) return fail(TPF_ERR_RUNTIME, "FCCD PPE density not set");
+  if (o->precond != 0 && o->precond != 1)
+    return fail(TPF_ERR_VALUE, "FCCD PPE supports preconditioner 0 (none) or 1 (Jacobi)");
+  if (!(o->tol > 0.0) || o->maxiter < 1 || o->restart < 1)
+    return fail(TPF_ERR_VALUE, "GMRES needs tol > 0, maxiter >= 1, restart >= 1");
+  const int64_t n = s->g.total;
+  const int m = (int)std::min<int64_t>(o->restart, n);  // SciPy: min(restart, n)
+  TPF_TRY(gmres_work(&s->gmres_work, n, m, s->grid, 1));
+  double* b = gmres_rhs(s->gmres_work);
+  double* tmp = gmres_scratch(s->gmres_work);
+  FCCD_LAUNCH(k_fccd_rhs, s->g, rhs, b);
+  if (o->use_x0) {  // sync images, pin 0 (fccd_matrixfree.py:537-543)
+    TPF_HIP(hipMemcpyAsync(tmp, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+    FCCD_LAUNCH(k_fccd_gauge, s->g, (const double*)tmp, x);
+  } else {
+    FCCD_LAUNCH(k_fccd_fill, n, 0.0, x);
+  }
+  const bool jac = o->precond == 1;
+  const dim3 grid(s->grid);
+  tpf_solve_report r{};
+  tpf_status st = gmres_core(
+      &s->gmres_work, s->stream, (unsigned)s->grid, n, m,
+      [&](const double* in, double* out) { return fccd_apply(s, in, out); },
+      [&](const double* in, double* out) -> tpf_status {
+        if (jac) {
+          hipLaunchKernelGGL(k_fccd_precond, grid, dim3(kBlock), 0, s->stream, n, s->g.pin,
+                             (const double*)s->dinv, in, out);
+          TPF_HIP(hipGetLastError());
+        } else {
+          TPF_HIP(hipMemcpyAsync(out, in, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                 s->stream));
+        }
+        return TPF_OK;
+      },
+      b, x, o->tol, o->maxiter, r);
+  if (st != TPF_OK) return st;
+  // sol[pin] = 0, images from sources; then the true residual on that field
+  // (fccd_matrixfree.py:580-589)
+  TPF_HIP(hipMemcpyAsync(tmp, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+  FCCD_LAUNCH(k_fccd_gauge, s->g, (const double*)tmp, x);
+  st = fccd_apply(s, x, tmp);
+  if (st == TPF_OK) {
+    FCCD_LAUNCH(k_fccd_xmy, n, (const double*)b, tmp);
+    double rn = 0.0, bn = 0.0;
+    st = fccd_norm2(s, tmp, rn);
+    if (st == TPF_OK) st = fccd_norm2(s, b, bn);
+    r.true_relres = bn > 0.0 ? rn / bn : rn;
+  }
+  TPF_HIP(hipStreamSynchronize(s->stream));
+  if (st != TPF_OK) return st;
+  if (rep) *rep = r;
+  if (!r.converged)
+    return fail(TPF_ERR_RUNTIME, "PPESolverFCCDMatrixFree did not converge: info=" +
+                                     std::to_string(o->maxiter) + ", relative_l2=" +
+                                     std::to_string(r.true_relres) +
+                                     ", maxiter=" + std::to_string(o->maxiter));
+  return TPF_OK;
+}

@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "gmres.hpp"
 #include "ppe_impl.hpp"
 #include "reduce.hpp"
 
@@ -321,11 +322,13 @@ __global__ __launch_bounds__(kBlock) void k_multiaxpy(int64_t n, const double* _
 struct GmresWork {
   int m = 0;             // restart the basis was sized for
   int chunks = 0;        // ceil((m + 1) / 16)
+  int64_t n = 0;         // vector length
+  int nscratch = 0;      // extra n-vectors in s8
   double* V = nullptr;   // (m + 1) x n basis, row-major
   double* w = nullptr;   // residual
-  double* t = nullptr;   // L v
-  double* b = nullptr;   // rhs with rhs[pin] = 0
-  double* s8 = nullptr;  // 8 n: Thomas factors or PCR ping-pong arrays
+  double* t = nullptr;   // A v
+  double* b = nullptr;   // right-hand side as solved
+  double* s8 = nullptr;  // nscratch n: Thomas factors or PCR ping-pong arrays
   double* coef = nullptr;
   double* part = nullptr;
   double* dots = nullptr;
@@ -342,27 +345,37 @@ void free_work(GmresWork* gw) {
   delete gw;
 }
 
-tpf_status ensure_work(tpf_ppe* s, int m) {
-  GmresWork* gw = static_cast<GmresWork*>(s->gmres_work);
-  if (gw && gw->m >= m) return TPF_OK;
+// The basis and scratch of one handle, grown on demand (kept between solves).
+tpf_status ensure_work(void** slot, int64_t n, int m, int grid_blocks, int nscratch) {
+  GmresWork* gw = static_cast<GmresWork*>(*slot);
+  if (gw && gw->m >= m && gw->n == n && gw->nscratch >= nscratch) return TPF_OK;
   if (gw) free_work(gw);
   gw = new GmresWork();
-  s->gmres_work = gw;
-  const size_t n = (size_t)s->g.total;
+  *slot = gw;
   gw->m = m;
+  gw->n = n;
+  gw->nscratch = nscratch;
   gw->chunks = (m + 1 + kChunk - 1) / kChunk;
   TPF_HIP(hipMalloc(&gw->V, sizeof(double) * n * (size_t)(m + 1)));
   TPF_HIP(hipMalloc(&gw->w, sizeof(double) * n));
   TPF_HIP(hipMalloc(&gw->t, sizeof(double) * n));
   TPF_HIP(hipMalloc(&gw->b, sizeof(double) * n));
-  TPF_HIP(hipMalloc(&gw->s8, sizeof(double) * n * 8));
+  if (nscratch > 0) TPF_HIP(hipMalloc(&gw->s8, sizeof(double) * n * nscratch));
   TPF_HIP(hipMalloc(&gw->coef, sizeof(double) * (size_t)(m + 1)));
-  TPF_HIP(hipMalloc(&gw->part, sizeof(double) * kDot * (size_t)s->grid * gw->chunks));
+  TPF_HIP(hipMalloc(&gw->part, sizeof(double) * kDot * (size_t)grid_blocks * gw->chunks));
   TPF_HIP(hipMalloc(&gw->dots, sizeof(double) * kDot * (size_t)gw->chunks));
   TPF_HIP(hipHostMalloc(&gw->host_dn, sizeof(double) * kDot * (size_t)gw->chunks));
   TPF_HIP(hipHostMalloc(&gw->host_up, sizeof(double) * (size_t)(m + 1)));
   return TPF_OK;
 }
+
+// Stream, grid and work of the Krylov vector kernels.
+struct Krylov {
+  GmresWork* gw;
+  hipStream_t st;
+  dim3 grid;
+  int64_t n;
+};
 
 struct Ctx {
   tpf_ppe* s;
@@ -370,6 +383,7 @@ struct Ctx {
   hipStream_t st;
   dim3 grid;
   int64_t n;
+  Krylov kr() const { return Krylov{gw, st, grid, n}; }
   int precond;
   int stages;
   double c_tau, h_min2;
@@ -432,7 +446,8 @@ tpf_status psolve(Ctx& c, const double* r, double* z) {
 // out[j*17 + k] = <V_{16j+k}, w> for 16j+k < kcount, out[j*17 + 16] = <w,w>.
 // Synchronises the stream: every upload of host_up issued before this call
 // has completed when it returns, so host_up may be rewritten afterwards.
-tpf_status multidot(Ctx& c, const double* V, int kcount, const double* w, double* out) {
+tpf_status multidot(const Krylov& c, const double* V, int kcount, const double* w,
+                    double* out) {
   const int chunks = std::max(1, (kcount + kChunk - 1) / kChunk);
   hipLaunchKernelGGL(k_multidot, dim3(c.grid.x, chunks), dim3(kBlock), 0, c.st, c.n, V, kcount,
                      w, c.gw->part);
@@ -447,7 +462,7 @@ tpf_status multidot(Ctx& c, const double* V, int kcount, const double* w, double
   return TPF_OK;
 }
 
-tpf_status norm2(Ctx& c, const double* w, double& out) {
+tpf_status norm2(const Krylov& c, const double* w, double& out) {
   double d[kDot];
   TPF_TRY(multidot(c, w, 0, w, d));
   out = std::sqrt(d[kChunk]);
@@ -457,8 +472,8 @@ tpf_status norm2(Ctx& c, const double* w, double& out) {
 // w += sign * sum_k coef[k] V_k.  Asynchronous: the stream synchronisation
 // in the next multidot / norm2 (every caller issues one before the next
 // multiaxpy) completes the upload before host_up is written again.
-tpf_status multiaxpy(Ctx& c, const double* V, int kc, const double* coef, double sign,
-                     double* w) {
+tpf_status multiaxpy(const Krylov& c, const double* V, int kc, const double* coef,
+                     double sign, double* w) {
   std::copy(coef, coef + kc, c.gw->host_up);
   TPF_HIP(hipMemcpyAsync(c.gw->coef, c.gw->host_up, sizeof(double) * kc, hipMemcpyHostToDevice,
                          c.st));
@@ -496,7 +511,7 @@ tpf_status check_opts(tpf_ppe* s, const tpf_gmres_opts* o) {
 }
 
 tpf_status make_ctx(tpf_ppe* s, const tpf_gmres_opts* o, int m, Ctx& c) {
-  TPF_TRY(ensure_work(s, m));
+  TPF_TRY(ensure_work(&s->gmres_work, s->g.total, m, s->grid, 8));
   hipEvent_t ev;
   TPF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   TPF_HIP(hipEventRecord(ev, s->user_stream));
@@ -512,66 +527,37 @@ tpf_status make_ctx(tpf_ppe* s, const tpf_gmres_opts* o, int m, Ctx& c) {
 }
 
 }  // namespace
-}  // namespace tpf
 
-using namespace tpf;
-
-void tpf_gmres_release(tpf_ppe* s) {
-  if (s->gmres_work) free_work(static_cast<GmresWork*>(s->gmres_work));
-  s->gmres_work = nullptr;
-}
-
-extern "C" tpf_status tpf_ppe_gmres_precond(tpf_ppe* s, const double* r, double* z,
-                                            const tpf_gmres_opts* o) {
-  TPF_TRY(check_opts(s, o));
-  if (!r || !z || r == z) return fail(TPF_ERR_VALUE, "r and z must be distinct device arrays");
-  Ctx c;
-  TPF_TRY(make_ctx(s, o, 1, c));
-  TPF_TRY(psolve(c, r, z));
-  TPF_HIP(hipStreamSynchronize(c.st));
-  return TPF_OK;
-}
-
-extern "C" tpf_status tpf_ppe_solve_gmres(tpf_ppe* s, const double* rhs, double* x,
-                                          const tpf_gmres_opts* o, tpf_solve_report* rep) {
-  TPF_TRY(check_opts(s, o));
-  if (!rhs || !x) return fail(TPF_ERR_VALUE, "null argument");
-  if (!(o->tol > 0.0) || o->maxiter < 1 || o->restart < 1)
-    return fail(TPF_ERR_VALUE, "GMRES needs tol > 0, maxiter >= 1, restart >= 1");
-  const int m = (int)std::min<int64_t>(o->restart, s->g.total);  // SciPy: min(restart, n)
-  Ctx c;
-  TPF_TRY(make_ctx(s, o, m, c));
-  GmresWork* gw = c.gw;
-  const int64_t n = c.n;
-  const int64_t pin = s->g.pin;
-  double* b = gw->b;
+// Restarted left-preconditioned GMRES(m) on `b` (SciPy 1.15's loop, see the
+// file header); x holds the initial guess on entry.  Fills r; r.converged = 0
+// when the cycle budget ends above tol (the caller reports the error).
+tpf_status gmres_core(void** work, hipStream_t st, unsigned grid, int64_t n, int m,
+                      const GmresOp& apply, const GmresOp& psolve, const double* b, double* x,
+                      double tol, int64_t maxiter, tpf_solve_report& r) {
+  GmresWork* gw = static_cast<GmresWork*>(*work);
+  if (!gw || gw->m < m || gw->n != n) return fail(TPF_ERR_RUNTIME, "GMRES work not sized");
+  const Krylov c{gw, st, dim3(grid), n};
   double* V = gw->V;
-  TPF_HIP(hipMemcpyAsync(b, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, c.st));
-  TPF_LAUNCH(k_set_pin, dim3(1), b, pin);
-  if (o->use_x0) TPF_LAUNCH(k_set_pin, dim3(1), x, pin);
-  else TPF_LAUNCH(k_fill, c.grid, n, 0.0, x);
-  tpf_solve_report r{};
+  r = tpf_solve_report{};
   double bnrm2 = 0.0;
   TPF_TRY(norm2(c, b, bnrm2));
   if (bnrm2 == 0.0) {  // SciPy returns b itself (zero)
     TPF_LAUNCH(k_fill, c.grid, n, 0.0, x);
-    TPF_HIP(hipStreamSynchronize(c.st));
     r.converged = 1;
-    if (rep) *rep = r;
     return TPF_OK;
   }
-  const double atol = o->tol * bnrm2;
+  const double atol = tol * bnrm2;
   const double eps = 2.220446049250313e-16;
   // Mb_nrm2 = ||M b|| seeds the inner tolerance
   double Mb = 0.0;
-  TPF_TRY(psolve(c, b, gw->w));
+  TPF_TRY(psolve(b, gw->w));
   TPF_TRY(norm2(c, gw->w, Mb));
   double ptol_max = 1.0;
   double ptol = Mb * std::min(ptol_max, atol / bnrm2);
   double presid = 0.0;
-  // r = b - L x
-  TPF_TRY(apply_L(c, x, gw->w));
-  TPF_LAUNCH(k_xmy, c.grid, n, (const double*)b, gw->w);
+  // r = b - A x
+  TPF_TRY(apply(x, gw->w));
+  TPF_LAUNCH(k_xmy, c.grid, n, b, gw->w);
   double rnorm = 0.0;
   TPF_TRY(norm2(c, gw->w, rnorm));
   int64_t inner = 0, cycles = 0;
@@ -579,9 +565,9 @@ extern "C" tpf_status tpf_ppe_solve_gmres(tpf_ppe* s, const double* rhs, double*
   std::vector<double> h((size_t)m * ld), S(ld), gc(m), gs(m), y(ld), hc(ld);
   std::vector<double> dots((size_t)kDot * gw->chunks);
   const bool early = rnorm < atol;  // SciPy's first-cycle check
-  while (!early && cycles < o->maxiter) {
+  while (!early && cycles < maxiter) {
     ++cycles;
-    TPF_TRY(psolve(c, gw->w, V));  // v0 = M r / ||M r||
+    TPF_TRY(psolve(gw->w, V));  // v0 = M r / ||M r||
     double t0 = 0.0;
     TPF_TRY(norm2(c, V, t0));
     TPF_LAUNCH(k_axpby, c.grid, n, 0.0, (const double*)V, 1.0 / t0, V);
@@ -592,8 +578,8 @@ extern "C" tpf_status tpf_ppe_solve_gmres(tpf_ppe* s, const double* rhs, double*
     for (col = 0; col < m; ++col) {
       double* vc = V + (int64_t)col * n;
       double* vn = V + (int64_t)(col + 1) * n;
-      TPF_TRY(apply_L(c, vc, gw->t));
-      TPF_TRY(psolve(c, gw->t, vn));  // w, built in place in v_{col+1}
+      TPF_TRY(apply(vc, gw->t));
+      TPF_TRY(psolve(gw->t, vn));  // w, built in place in v_{col+1}
       std::fill(hc.begin(), hc.begin() + col + 1, 0.0);
       double h0 = 0.0;
       for (int pass = 0; pass < 2; ++pass) {
@@ -645,21 +631,75 @@ extern "C" tpf_status tpf_ppe_solve_gmres(tpf_ppe* s, const double* rhs, double*
     }
     if (y[0] != 0.0) y[0] /= h[0];
     TPF_TRY(multiaxpy(c, V, col + 1, y.data(), 1.0, x));
-    TPF_TRY(apply_L(c, x, gw->w));
-    TPF_LAUNCH(k_xmy, c.grid, n, (const double*)b, gw->w);
+    TPF_TRY(apply(x, gw->w));
+    TPF_LAUNCH(k_xmy, c.grid, n, b, gw->w);
     TPF_TRY(norm2(c, gw->w, rnorm));
     if (rnorm <= atol || breakdown) break;
     if (presid <= ptol) ptol_max = std::max(eps, 0.25 * ptol_max);
     else ptol_max = std::min(1.0, 1.5 * ptol_max);
     ptol = presid * std::min(ptol_max, atol / rnorm);
   }
-  TPF_LAUNCH(k_set_pin, dim3(1), x, pin);  // sol[pin] = 0
-  TPF_HIP(hipStreamSynchronize(c.st));
   r.iterations = inner;
   r.relres = rnorm / bnrm2;
   r.true_relres = rnorm / bnrm2;
   r.converged = (early || rnorm <= atol) ? 1 : 0;
   r.refinements = (int)cycles;
+  return TPF_OK;
+}
+
+tpf_status gmres_work(void** slot, int64_t n, int m, int grid_blocks, int nscratch) {
+  return ensure_work(slot, n, m, grid_blocks, nscratch);
+}
+
+void gmres_work_free(void** slot) {
+  if (*slot) free_work(static_cast<GmresWork*>(*slot));
+  *slot = nullptr;
+}
+
+double* gmres_rhs(void* work) { return static_cast<GmresWork*>(work)->b; }
+
+double* gmres_scratch(void* work) { return static_cast<GmresWork*>(work)->s8; }
+
+}  // namespace tpf
+
+using namespace tpf;
+
+void tpf_gmres_release(tpf_ppe* s) { gmres_work_free(&s->gmres_work); }
+
+extern "C" tpf_status tpf_ppe_gmres_precond(tpf_ppe* s, const double* r, double* z,
+                                            const tpf_gmres_opts* o) {
+  TPF_TRY(check_opts(s, o));
+  if (!r || !z || r == z) return fail(TPF_ERR_VALUE, "r and z must be distinct device arrays");
+  Ctx c;
+  TPF_TRY(make_ctx(s, o, 1, c));
+  TPF_TRY(psolve(c, r, z));
+  TPF_HIP(hipStreamSynchronize(c.st));
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ppe_solve_gmres(tpf_ppe* s, const double* rhs, double* x,
+                                          const tpf_gmres_opts* o, tpf_solve_report* rep) {
+  TPF_TRY(check_opts(s, o));
+  if (!rhs || !x) return fail(TPF_ERR_VALUE, "null argument");
+  if (!(o->tol > 0.0) || o->maxiter < 1 || o->restart < 1)
+    return fail(TPF_ERR_VALUE, "GMRES needs tol > 0, maxiter >= 1, restart >= 1");
+  const int m = (int)std::min<int64_t>(o->restart, s->g.total);  // SciPy: min(restart, n)
+  Ctx c;
+  TPF_TRY(make_ctx(s, o, m, c));
+  const int64_t n = c.n;
+  const int64_t pin = s->g.pin;
+  double* b = c.gw->b;
+  TPF_HIP(hipMemcpyAsync(b, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, c.st));
+  TPF_LAUNCH(k_set_pin, dim3(1), b, pin);
+  if (o->use_x0) TPF_LAUNCH(k_set_pin, dim3(1), x, pin);
+  else TPF_LAUNCH(k_fill, c.grid, n, 0.0, x);
+  tpf_solve_report r{};
+  TPF_TRY(gmres_core(&s->gmres_work, c.st, c.grid.x, n, m,
+                     [&](const double* in, double* out) { return apply_L(c, in, out); },
+                     [&](const double* in, double* out) { return psolve(c, in, out); }, b, x,
+                     o->tol, o->maxiter, r));
+  TPF_LAUNCH(k_set_pin, dim3(1), x, pin);  // sol[pin] = 0
+  TPF_HIP(hipStreamSynchronize(c.st));
   if (rep) *rep = r;
   if (!r.converged)
     return fail(TPF_ERR_RUNTIME, "PPE GMRES did not converge cleanly (info=" +

@@ -16,7 +16,12 @@ Mirrors the reference plugin interface (``ppe/interfaces.py:24-88``,
   recursion on the top two levels by default), judged by the FP64 true
   residual.
 
-Both raise ``RuntimeError`` when they do not converge and return
+``PPESolverFCCDMatrixFree`` mirrors the FCCD matrix-free PPE
+(``ppe/fccd_matrixfree.py``) in its default configuration: the operator is
+the CCD second derivative per axis plus one fused face-gradient / weighted
+divergence kernel, solved with the same device GMRES.
+
+All raise ``RuntimeError`` when they do not converge and return
 ``p[pin] = 0``.  The Krylov loops run on the device (libtpf); nothing
 iterates in Python.
 
@@ -284,6 +289,226 @@ class PPESolverFVMMatrixFree(IPPESolver):
         return out.value
 
 
+class PPESolverFCCDMatrixFree(IPPESolver):
+    """Matrix-free FCCD PPE on the GPU: L p = D_f[(1/rho)_f G_f p] with the
+    FCCD face gradient (ppe/fccd_matrixfree.py:108-600, ccd/fccd.py:243-277),
+    solved by SciPy-semantics GMRES on the device.
+
+    Built for the reference's default configuration: ``ppe_coefficient_scheme
+    = "phase_density"``, ``ppe_interface_coupling_scheme = "none"``,
+    ``boundary_face_space = "full_face"``, legacy ``scalar_operator_pairing``,
+    uniform grids, wall or periodic axes.  The other schemes raise
+    ``NotImplementedError``; invalid names raise ``ValueError`` as in the
+    reference (fccd_matrixfree.py:155-186)."""
+
+    scheme_names = ("fccd_iterative",)
+    _scheme_aliases = {"fccd_matrixfree": "fccd_iterative", "fccd": "fccd_iterative"}
+
+    def __init__(self, backend, config, grid, fccd=None, *, bc_spec=None, bc_type=None):
+        self.backend = backend
+        self.xp = np
+        self.grid = grid
+        self.ndim = grid.ndim
+        self.fccd = fccd
+        if bc_type is None:
+            bc_type = getattr(fccd, "bc_type", None)
+        if bc_type is None:
+            bc_type = getattr(getattr(config, "numerics", None), "bc_type", "wall")
+        self.bc_type = bc_type
+        self._kinds = axis_kinds(bc_type, self.ndim)
+        solver_cfg = getattr(config, "solver", config)
+        self.tol = float(getattr(solver_cfg, "pseudo_tol", 1.0e-8))
+        self.maxiter = int(getattr(solver_cfg, "pseudo_maxiter", 500))
+        restart = getattr(solver_cfg, "ppe_restart", None)
+        self.restart = int(restart) if restart is not None else None
+        self.preconditioner = str(getattr(solver_cfg, "ppe_preconditioner", "none")).strip().lower()
+        self.coefficient_scheme = str(
+            getattr(solver_cfg, "ppe_coefficient_scheme", "phase_density")).strip().lower()
+        self.interface_coupling_scheme = str(
+            getattr(solver_cfg, "ppe_interface_coupling_scheme", "none")).strip().lower()
+        contract = str(getattr(solver_cfg, "pressure_force_contract", None)
+                       or "raw_compact_gradient").strip().lower()
+        pairing = str(getattr(solver_cfg, "scalar_operator_pairing", None)
+                      or "legacy").strip().lower()
+        pairing = {"raw": "legacy", "old": "legacy", "certified": "require_certified",
+                   "require": "require_certified", "variational": "variational_operator",
+                   "l_var": "variational_operator"}.get(pairing, pairing)
+        face_space = str(getattr(solver_cfg, "boundary_face_space", None)
+                         or "full_face").strip().lower().replace("-", "_")
+        face_space = {"full": "full_face", "unconstrained": "full_face"}.get(face_space,
+                                                                            face_space)
+        if pairing == "require_certified" and contract in ("variational_adjoint", "variational",
+                                                           "adjoint", "pressure_adjoint"):
+            raise RuntimeError(
+                "scalar_operator_pairing='require_certified' is fail-closed for FCCD until "
+                "PPE.apply(p)==D(Gp) certification is implemented; use "
+                "scalar_operator_pairing='variational_operator' to select L_var=D(Gp).")
+        if self.coefficient_scheme not in {"phase_density", "phase_separated"}:
+            raise ValueError("FCCD PPE supports ppe_coefficient_scheme="
+                             "'phase_density'|'phase_separated'")
+        if self.interface_coupling_scheme not in {"none", "jump_decomposition", "affine_jump"}:
+            raise ValueError("FCCD PPE supports ppe_interface_coupling_scheme="
+                             "'none'|'jump_decomposition'|'affine_jump'")
+        if self.coefficient_scheme == "phase_density" and self.interface_coupling_scheme != "none":
+            raise ValueError("phase_density PPE requires interface_coupling='none'")
+        if self.preconditioner not in {"jacobi", "none"}:
+            raise ValueError("FCCD PPE supports preconditioner='jacobi'|'none'")
+        if self.coefficient_scheme != "phase_density":
+            raise NotImplementedError(
+                "this build's FCCD PPE covers ppe_coefficient_scheme='phase_density'")
+        if pairing == "variational_operator":
+            raise NotImplementedError(
+                "this build's FCCD PPE covers the legacy scalar_operator_pairing")
+        if face_space != "full_face":
+            raise NotImplementedError("this build's FCCD PPE covers boundary_face_space="
+                                      "'full_face'")
+        if not getattr(grid, "uniform", True):
+            raise ValueError("this build's FCCD PPE needs a uniform grid")
+        self.shape = tuple(int(n) + 1 for n in grid.N)
+        if bc_spec is not None:
+            self._pin_dof = int(bc_spec.pin_dof)
+        else:  # fccd_matrixfree.py:187-192
+            self._pin_dof = int(np.ravel_multi_index(tuple(n // 2 for n in grid.N), self.shape))
+        self._pin_dofs = (self._pin_dof,)
+        self.last_base_pressure = None
+        self.last_diagnostics: dict = {}
+        self._lib = nat.lib()
+        self._handle = None
+        self._rho = None
+        self._create()
+
+    def _create(self):
+        require_gpu()
+        h = C.c_void_p()
+        bc = [nat.BC_PERIODIC if k == "periodic" else nat.BC_WALL for k in self._kinds]
+        nat.check(self._lib.tpf_fccd_ppe_create(C.byref(h), self.ndim, nat.i64arr(self.shape),
+                                                nat.darr(self.grid.L), nat.iarr(bc),
+                                                int(self._pin_dof), stream_handle()))
+        self._handle = h
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            self._lib.tpf_fccd_ppe_destroy(h)
+            self._handle = None
+
+    def update_grid(self, grid=None) -> None:
+        if grid is not None:
+            self.grid = grid
+            self.ndim = grid.ndim
+            self._kinds = axis_kinds(self.bc_type, self.ndim)
+            self.shape = tuple(int(n) + 1 for n in grid.N)
+            self._pin_dof = int(np.ravel_multi_index(tuple(n // 2 for n in grid.N), self.shape))
+            self._pin_dofs = (self._pin_dof,)
+        self.__del__()
+        self._rho = None
+        self._create()
+
+    def invalidate_cache(self) -> None:
+        self._rho = None
+
+    # -- operator --------------------------------------------------------------
+    def prepare_operator(self, rho) -> None:
+        """Keep rho on the device and build the Jacobi inverse
+        (fccd_matrixfree_lifecycle.py:44-96)."""
+        io = Handoff(rho)
+        t = io.to_device(rho)
+        if tuple(t.shape) != self.shape:
+            raise ValueError(f"rho shape {tuple(t.shape)} != grid.shape {self.shape}")
+        self._rho = t
+        nat.check(self._lib.tpf_fccd_ppe_set_density(self._handle, ptr(t)))
+
+    def apply_into(self, p: torch.Tensor, out: torch.Tensor) -> None:
+        """out = L p on CUDA tensors (asynchronous)."""
+        if self._rho is None:
+            raise RuntimeError("prepare_operator(rho) must be called before apply(p)")
+        nat.check(self._lib.tpf_fccd_ppe_apply(self._handle, ptr(p), ptr(out)))
+
+    def apply(self, p):
+        """L p with the pin row and the periodic image rows
+        (fccd_matrixfree.py:378-397)."""
+        io = Handoff(p)
+        t = io.to_device(p)
+        out = torch.empty_like(t)
+        self.apply_into(t, out)
+        return io.back(out)
+
+    # -- solve -----------------------------------------------------------------
+    def _gmres_opts(self, use_x0: bool) -> "nat.GmresOpts":
+        # SciPy's gmres takes restart=None as 20 (gmres_helpers.py:16-42)
+        restart = 20 if self.restart is None else self.restart
+        return nat.GmresOpts(tol=self.tol, maxiter=self.maxiter, restart=restart,
+                             precond=1 if self.preconditioner == "jacobi" else 0,
+                             pcr_stages=0, use_x0=1 if use_x0 else 0, c_tau=0.0)
+
+    def system_rhs(self, rhs: torch.Tensor) -> torch.Tensor:
+        """rhs with rhs[pin] = 0 and zero periodic image rows
+        (fccd_matrixfree.py:514-516, 370-376; fccd_matrixfree_helpers.py:337-451)."""
+        b = rhs.clone()
+        b.view(-1)[self._pin_dof] = 0.0
+        for a, kind in enumerate(self._kinds):
+            if kind == "periodic":
+                b.select(a, self.shape[a] - 1).zero_()
+        return b
+
+    def solve_into(self, rhs: torch.Tensor, rho, p: torch.Tensor, use_x0: bool = False) -> dict:
+        """Zero-copy solve on CUDA tensors; p holds p_init when use_x0."""
+        if rho is not None:  # the reference prepares the operator on every solve
+            self.prepare_operator(rho)
+        elif self._rho is None:
+            raise RuntimeError("no density set")
+        rep = nat.SolveReport()
+        status = self._lib.tpf_fccd_ppe_solve_gmres(self._handle, ptr(rhs), ptr(p),
+                                                    C.byref(self._gmres_opts(use_x0)),
+                                                    C.byref(rep))
+        if status not in (nat.TPF_OK, nat.TPF_ERR_RUNTIME):
+            nat.check(status)
+        msg = self._lib.tpf_last_error().decode(errors="replace") if status else ""
+        b = self.system_rhs(rhs)
+        res = torch.empty_like(p)
+        self.apply_into(p, res)
+        res -= b
+        rhs_l2 = float(torch.linalg.vector_norm(b))
+        res_l2 = float(torch.linalg.vector_norm(res))
+        self.last_diagnostics = {
+            "ppe_phase_count": 1.0, "ppe_pin_count": 1.0, "ppe_mean_gauge": 0.0,
+            "ppe_rhs_phase_mean_before_max": 0.0, "ppe_rhs_phase_mean_after_max": 0.0,
+            "ppe_interface_coupling_jump": 0.0, "ppe_interface_coupling_affine_jump": 0.0,
+            "ppe_linear_info": float(0 if status == nat.TPF_OK else self.maxiter),
+            "ppe_linear_converged": float(status == nat.TPF_OK),
+            "ppe_linear_rhs_l2": rhs_l2,
+            "ppe_linear_residual_l2": res_l2,
+            "ppe_linear_relative_l2": res_l2 / rhs_l2 if rhs_l2 > 0 else res_l2,
+            "ppe_linear_residual_linf": float(res.abs().max()) if res.numel() else 0.0,
+            "iterations": int(rep.iterations),
+            "restart_cycles": int(rep.refinements),
+            "true_relres": float(rep.true_relres),
+        }
+        if status != nat.TPF_OK:
+            d = self.last_diagnostics
+            raise RuntimeError(
+                "PPESolverFCCDMatrixFree did not converge: "
+                f"info={self.maxiter}, relative_l2={d['ppe_linear_relative_l2']:.3e}, "
+                f"linf={d['ppe_linear_residual_linf']:.3e}, maxiter={self.maxiter} ({msg})")
+        self.last_base_pressure = p.clone()
+        return self.last_diagnostics
+
+    def solve(self, rhs, rho, dt: float = 0.0, p_init=None):
+        io = Handoff(rhs)
+        rhs_t = io.to_device(rhs)
+        if tuple(rhs_t.shape) != self.shape:
+            raise ValueError(f"rhs shape {tuple(rhs_t.shape)} != grid.shape {self.shape}")
+        rho_t = io.to_device(rho)
+        if p_init is None:
+            p = torch.empty_like(rhs_t)
+            use_x0 = False
+        else:
+            p = io.to_device(p_init).clone()
+            use_x0 = True
+        self.solve_into(rhs_t, rho_t, p, use_x0=use_x0)
+        return io.back(p)
+
+
 # -- factory (ppe/factory.py:31-120) ------------------------------------------
 _SOLVER_REGISTRY: Dict[str, Callable] = {}
 
@@ -297,8 +522,15 @@ def _create_fvm_matrixfree(config, backend, grid, ccd, bc_spec):
                                   bc_type=config.numerics.bc_type, bc_spec=bc_spec)
 
 
+def _create_fccd_matrixfree(config, backend, grid, ccd, bc_spec):
+    return PPESolverFCCDMatrixFree(backend, config, grid, None, bc_spec=bc_spec,
+                                   bc_type=config.numerics.bc_type)
+
+
 register_ppe_solver("fvm_iterative", _create_fvm_matrixfree)
 register_ppe_solver("fvm_matrixfree", _create_fvm_matrixfree)
+for _name in ("fccd_iterative", "fccd_matrixfree", "fccd"):
+    register_ppe_solver(_name, _create_fccd_matrixfree)
 
 
 def create_ppe_solver(config, backend, grid, ccd=None, bc_spec=None) -> IPPESolver:
