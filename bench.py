@@ -361,6 +361,98 @@ def unet_mfma(reps=10):
     return out
 
 
+def fccd_ppe_bench(reps=20, cpu=True):
+    """FCCD matrix-free PPE (PPESolverFCCDMatrixFree, default configuration).
+
+    Matvec at 257^3 on the C3 density (wall, 64 bubbles, ratio 1000): the CCD
+    second derivative of every axis (d1 not written) followed by the fused
+    face-gradient / divergence kernel; algorithmic bytes 32 + 48 = 80 B per
+    node (CCD: read p, write 3 q; fused: read p, rho, q0..q2, write L p).
+    Solves to 1e-8: GMRES(20) preconditioned by one FVM multigrid cycle per
+    step at 257^3 and 65^3; Jacobi-GMRES(80) at 65^3 (restart 20, the
+    reference's default, stalls there).  CPU: the oracle's apply on the 65^3
+    case (one thread of NumPy/SciPy)."""
+    import numpy as np
+    import torch
+    from twophaseflow_amd import (CCDSolver, PPESolverFCCDMatrixFree, SimulationConfig,
+                                  SolverConfig)
+
+    def solver(g, tol=1e-8, maxiter=1000, restart=80, pre="jacobi"):
+        cfg = SimulationConfig(solver=SolverConfig(pseudo_tol=tol, pseudo_maxiter=maxiter))
+        cfg.solver.ppe_preconditioner = pre
+        cfg.solver.ppe_restart = restart
+        return PPESolverFCCDMatrixFree(None, cfg, g, None, bc_type="wall")
+
+    def solve(s, rhs, rho, reps=2):
+        x = torch.empty_like(rhs)
+        for _ in range(reps):  # the last of reps (the first builds graphs / hierarchy)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            d = s.solve_into(rhs, rho, x)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+        return x, {"seconds": round(t, 4), "iterations": d["iterations"],
+                   "restart_cycles": d["restart_cycles"],
+                   "relative_l2": d["ppe_linear_relative_l2"]}
+
+    def events(fn, n):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(n)]
+        fn()
+        for a, b in ev:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        return sorted(a.elapsed_time(b) for a, b in ev)[n // 2]
+
+    out = {}
+    g, rho, rhs = ppe_case(N=256)
+    s = solver(g)
+    s.prepare_operator(rho)
+    p = torch.randn_like(rho)
+    Lp = torch.empty_like(p)
+    npts = p.numel()
+    ms_apply = events(lambda: s.apply_into(p, Lp), reps)
+    ccd = CCDSolver(g, bc_type="wall")
+    q = torch.empty((3,) + tuple(p.shape), dtype=torch.float64, device="cuda")
+    ms_ccd = events(lambda: ccd.apply_into(p, None, q, 0b111), reps)
+    ms_fused = ms_apply - ms_ccd
+    _, out["solve_257_ratio1000_mg"] = solve(solver(g, maxiter=100, restart=20, pre="mg"),
+                                             rhs, rho)
+    out["apply_257"] = {
+        "ms": round(ms_apply, 4), "timing": "median of per-call HIP events",
+        "algorithmic_GBps": round(80 * npts / (ms_apply * 1e-3) / 1e9, 1),
+        "ccd_q_only_ms": round(ms_ccd, 4),
+        "ccd_q_only_GBps": round(32 * npts / (ms_ccd * 1e-3) / 1e9, 1),
+        "fused_face_divergence_ms": round(ms_fused, 4),
+        "fused_face_divergence_GBps": round(48 * npts / (ms_fused * 1e-3) / 1e9, 1)}
+    del s, q, p, Lp, rho, rhs
+    torch.cuda.empty_cache()
+    g, rho, rhs = ppe_case(N=64)
+    _, out["solve_65_ratio1000_mg"] = solve(solver(g, maxiter=100, restart=20, pre="mg"),
+                                            rhs, rho)
+    s = solver(g)
+    x, out["solve_65_ratio1000_jacobi_gmres80"] = solve(s, rhs, rho, reps=1)
+    y = torch.empty_like(x)
+    ms65 = events(lambda: s.apply_into(x, y), reps)
+    out["apply_65_ms"] = round(ms65, 4)
+    if cpu:
+        from oracle.fccd import OracleFCCDPPE
+        o = OracleFCCDPPE((64,) * 3, (1.0,) * 3, "wall")
+        o.prepare_operator(rho.cpu().numpy())
+        xh = x.cpu().numpy()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            o.apply(xh)
+        cpu_s = (time.perf_counter() - t0) / 3
+        out["cpu_oracle_apply_65"] = {"seconds": round(cpu_s, 4), "kind": "port",
+                                      "sample": "oracle FCCD apply (NumPy/SciPy dense-LU CCD) "
+                                                "on the 65^3 case, 3 calls",
+                                      "gpu_speedup": round(cpu_s / (ms65 * 1e-3), 1)}
+    return out
+
+
 def profile_mode(args):
     import torch
     from twophaseflow_amd import CCDSolver, Grid, GridConfig
@@ -383,6 +475,20 @@ def profile_mode(args):
         torch.cuda.synchronize()
         log(f"profile ccd: {args.steps} fused 3-axis launches and {args.steps} x 3 single-axis "
             f"launches on 257^3, then a 1 GiB copy")
+    elif args.profile == "fccd":
+        from twophaseflow_amd import PPESolverFCCDMatrixFree, SimulationConfig, SolverConfig
+        gp, rho, rhs = ppe_case(N=256)
+        cfg = SimulationConfig(solver=SolverConfig())
+        s = PPESolverFCCDMatrixFree(None, cfg, gp, None, bc_type="wall")
+        s.prepare_operator(rho)
+        out = torch.empty_like(rhs)
+        for _ in range(args.steps):
+            s.apply_into(rhs, out)
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"profile fccd: {args.steps} FCCD PPE matvecs on 257^3, then a 1 GiB copy")
     elif args.profile == "ppe":
         gp, rho, rhs = ppe_case(N=128)
         r = ppe_solve_timed(gp, rho, rhs, "fp64")
@@ -403,11 +509,12 @@ def main():
     ap.add_argument("--ppe-n", type=int, default=256)
     ap.add_argument("--train-problems", type=int, default=100,
                     help="random 129^3 bubble problems used to train the learned coarse space")
-    ap.add_argument("--profile", choices=["ccd", "ppe", "mg"], default=None,
+    ap.add_argument("--profile", choices=["ccd", "ppe", "mg", "fccd"], default=None,
                     help="run only one workload (for rocprofv3): 'ccd' = K all-axes CCD steps "
                          "on 257^3 plus one 1 GiB device copy (counter scale reference); "
                          "'ppe' = one FP64 PCG solve of the 129^3 ratio-1000 case; "
-                         "'mg' = one W(2,2) multigrid-FCG solve of the 257^3 case")
+                         "'mg' = one W(2,2) multigrid-FCG solve of the 257^3 case; "
+                         "'fccd' = K FCCD PPE matvecs on the 257^3 case")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
@@ -588,6 +695,9 @@ def main():
         result["ppe"]["gmres_line_pcr_65"] = ppe_solve_timed(gq, rhoq, rhsq, "fp64", maxiter=500,
                                                              method="gmres")
         del rhoq, rhsq
+        log("FCCD matrix-free PPE: 257^3 matvec, 65^3 Jacobi-GMRES(80) solve")
+        result["fccd_ppe"] = fccd_ppe_bench(cpu=not args.no_cpu)
+        log(f"  {result['fccd_ppe']}")
 
     if not args.no_cpu:
         log("CPU baseline (dense-LU CCD slab)")

@@ -250,7 +250,10 @@ TPF_API tpf_status tpf_fccd_ppe_apply(tpf_fccd_ppe* ppe, const double* p, double
 
 /* Replaces PPESolverFCCDMatrixFree.solve (fccd_matrixfree.py:505-600): SciPy
  * 1.15 GMRES (gmres_helpers.py:16-42) on L p = b, b = rhs with b[pin] = 0 and
- * zero image rows; opts->precond 0 none, 1 Jacobi; opts->restart as SciPy's
+ * zero image rows; opts->precond 0 none, 1 Jacobi, 2 multigrid (no
+ * counterpart in the reference; wall grids: one FP32 W(2,2) cycle of the FVM
+ * operator's hierarchy on -W r per step, the low-order operator preconditioning
+ * the FCCD one); opts->restart as SciPy's
  * (the reference passes ppe_restart, None = 20); opts->use_x0: p holds p_init
  * (images synced, pin 0).  On exit p[pin] = 0 and image nodes equal their
  * sources; rep->true_relres = ||L p - b|| / ||b|| on that field.

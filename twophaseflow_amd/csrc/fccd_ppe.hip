@@ -30,6 +30,7 @@
 #include <string>
 
 #include "gmres.hpp"
+#include "ppe_impl.hpp"
 #include "tpf_common.hpp"
 
 namespace tpf {
@@ -116,97 +117,131 @@ __device__ __forceinline__ double axis_term(const FccdGeom& g, int a, int i, dou
   return (c_hi * g_hi - c_lo * g_lo) * g.inv_w[a];
 }
 
-// Neighbours of node idx along slot a (1 or 2) straight from memory (the
-// rows involved are being read by this and the adjacent workgroups).
-__device__ __forceinline__ void neighbours(const FccdGeom& g, int a, int i, int64_t idx,
-                                           const double* __restrict__ p,
-                                           const double* __restrict__ qa,
-                                           const double* __restrict__ rho, Plane0& lo,
-                                           Plane0& hi, double& rho_lo_partner, double rc) {
+// Neighbours of node idx along slot a (1 or 2), loaded unconditionally from
+// clamped indices so that every load of a plane step issues before the first
+// use (the rows involved are being read by this and the adjacent
+// workgroups).  hi: upper value partner (wrapped on periodic axes) with the
+// upper face's rho partner (never wrapped); lo: lower neighbour (wrapped);
+// rlp: the lower face's upper rho (node N for face N-1 of a periodic axis,
+// else the node itself).
+struct Nbr {
+  Plane0 lo, hi;
+  double rlp;
+};
+
+// x[i] with a 32-bit element index (the field has < 2^29 elements): a
+// uniform base plus a 32-bit byte offset, one address register per load.
+__device__ __forceinline__ double at32(const double* __restrict__ x, uint32_t i) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(x) + (i << 3));
+}
+
+template <typename I>
+__device__ __forceinline__ double at(const double* __restrict__ x, I i) {
+  if constexpr (sizeof(I) == 4) return at32(x, i);
+  else return x[i];
+}
+
+template <typename I>
+__device__ __forceinline__ Nbr load_nbr(const FccdGeom& g, int a, int i, I idx,
+                                        const double* __restrict__ p,
+                                        const double* __restrict__ qa,
+                                        const double* __restrict__ rho) {
   const int N = g.n[a] - 1;
-  const int64_t st = stride_of(g, a);
-  const bool per = g.periodic[a];
-  rho_lo_partner = rc;
-  if (i < N) {
-    const int64_t hv = (per && i + 1 == N) ? idx - (int64_t)i * st : idx + st;
-    hi = Plane0{p[hv], qa[hv], rho[idx + st]};
-  }
-  if (i > 0) {
-    lo = Plane0{p[idx - st], qa[idx - st], rho[idx - st]};
-  } else if (per) {  // face N-1: nodes N-1 and 0, rho at N-1 and N
-    const int64_t lv = idx + (int64_t)(N - 1) * st;
-    lo = Plane0{p[lv], qa[lv], rho[lv]};
-    rho_lo_partner = rho[lv + st];
-  }
+  const I st = (I)stride_of(g, a);
+  const bool per = g.periodic[a] != 0;
+  const I hv = i < N ? ((per && i + 1 == N) ? idx - (I)i * st : idx + st) : idx;
+  const I hr = i < N ? idx + st : idx;
+  const I lv = i > 0 ? idx - st : (per ? idx + (I)(N - 1) * st : idx);
+  const I lr = (i == 0 && per) ? idx + (I)N * st : idx;
+  Nbr b;
+  b.hi = Plane0{at(p, hv), at(qa, hv), at(rho, hr)};
+  b.lo = Plane0{at(p, lv), at(qa, lv), at(rho, lv)};
+  b.rlp = at(rho, lr);
+  return b;
 }
 
 // out = L p with the pin row p[pin] and image rows p_img - p_src.
 // Thread = one node of an axis-0 plane (coalesced over the plane); the
 // workgroup marches kSlab planes along axis 0 keeping the previous, current
 // and next planes' p, q_0 and rho in registers, so each of them is read from
-// HBM once per slab (plus one halo plane on each side).
+// HBM once per slab (plus one halo plane on each side).  Each plane step
+// issues all of its loads (next plane, both neighbours on slots 1 and 2)
+// before any arithmetic: one memory round trip per step.  Workgroups are
+// renumbered so each XCD gets a contiguous range of plane chunks (the axis-1
+// neighbours of a chunk lie in the adjacent chunks).  S0 = 3 - ndim: the
+// first active slot.
 constexpr int kSlab = 32;
 
+template <int S0, typename I>
 __global__ __launch_bounds__(kBlock) void k_fccd_apply(FccdGeom g, const double* __restrict__ p,
                                                        const double* __restrict__ q,
                                                        const double* __restrict__ rho,
-                                                       double* __restrict__ out) {
+                                                       double* __restrict__ out, unsigned nx) {
+  constexpr bool has0 = S0 == 0, has1 = S0 <= 1;
+  const unsigned nb = gridDim.x, per = nb / 8;
+  const unsigned lb = blockIdx.x < per * 8 ? (blockIdx.x % 8) * per + blockIdx.x / 8 : blockIdx.x;
+  const unsigned bx = lb % nx, by = lb / nx;
   const int plane = g.n[1] * g.n[2];
-  const int j = blockIdx.x * kBlock + threadIdx.x;
+  const int j = bx * kBlock + threadIdx.x;
   if (j >= plane) return;
   const int c1 = j / g.n[2], c2 = j - c1 * g.n[2];
   const int n0 = g.n[0], N0 = n0 - 1;
-  const bool has0 = n0 > 1, per0 = g.periodic[0] != 0;
-  const int s0 = blockIdx.y * kSlab;
+  const bool per0 = has0 && g.periodic[0] != 0;
+  const int s0 = by * kSlab;
   const int s1 = min(n0, s0 + kSlab);
   const double* q0 = q + (int64_t)g.qk[0] * g.total;
   const double* q1 = q + (int64_t)g.qk[1] * g.total;
   const double* q2 = q + (int64_t)g.qk[2] * g.total;
   // image along slot 1 or 2 (the first periodic slot wins; slot 0 is checked
   // per plane below and comes first)
-  int64_t img12 = -1;
-  if (g.periodic[1] && g.n[1] > 1 && c1 == g.n[1] - 1) img12 = (int64_t)c1 * g.n[2];
-  else if (g.periodic[2] && g.n[2] > 1 && c2 == g.n[2] - 1) img12 = c2;
+  I img12 = 0;  // 0: not an image along slots 1 / 2
+  if (has1 && g.periodic[1] && c1 == g.n[1] - 1) img12 = (I)c1 * g.n[2];
+  else if (g.periodic[2] && c2 == g.n[2] - 1) img12 = (I)c2;
   auto ld = [&](int vplane, int rplane) {
-    const int64_t v = (int64_t)vplane * plane + j, r = (int64_t)rplane * plane + j;
-    return Plane0{p[v], q0[v], rho[r]};
+    const I v = (I)vplane * plane + j, r = (I)rplane * plane + j;
+    return Plane0{at(p, v), at(q0, v), at(rho, r)};
   };
   Plane0 prev{0.0, 0.0, 0.0}, cur = ld(s0, s0), next{0.0, 0.0, 0.0};
-  if (has0 && s0 > 0) prev = ld(s0 - 1, s0 - 1);
-  else if (has0 && per0) prev = ld(N0 - 1, N0 - 1);
+  double rhoN0 = 0.0;  // rho of plane N (the upper partner of periodic face N-1)
+  if (has0) {
+    if (s0 > 0) prev = ld(s0 - 1, s0 - 1);
+    else if (per0) prev = ld(N0 - 1, N0 - 1);
+    if (per0 && s0 == 0) rhoN0 = at(rho, (I)N0 * plane + j);
+  }
   for (int i0 = s0; i0 < s1; ++i0) {
-    if (has0 && i0 < N0) {
-      const int vn = (per0 && i0 + 1 == N0) ? 0 : i0 + 1;
-      next = ld(vn, i0 + 1);
+    const I idx = (I)i0 * plane + j;
+    // ---- loads
+    if (has0) {
+      const int vn = i0 < N0 ? ((per0 && i0 + 1 == N0) ? 0 : i0 + 1) : i0;
+      next = ld(vn, i0 < N0 ? i0 + 1 : i0);
     }
-    const int64_t idx = (int64_t)i0 * plane + j;
+    Nbr b1{}, b2;
+    double q1c = 0.0;
+    if (has1) {
+      b1 = load_nbr(g, 1, c1, idx, p, q1, rho);
+      q1c = at(q1, idx);
+    }
+    b2 = load_nbr(g, 2, c2, idx, p, q2, rho);
+    const double q2c = at(q2, idx);
+    const double pimg = per0 && i0 == N0 ? at(p, (I)j) : (img12 > 0 ? at(p, idx - img12) : 0.0);
+    // ---- arithmetic
     double res;
-    if (has0 && per0 && i0 == N0) {
-      res = p[idx] - p[j];  // image along axis 0 (cur holds plane 0's wrapped values)
-    } else if (img12 >= 0) {
-      res = cur.p - p[idx - img12];
-    } else if (idx == g.pin) {
+    if (per0 && i0 == N0) {
+      res = at(p, idx) - pimg;  // image along axis 0 (cur holds plane 0's wrapped values)
+    } else if (img12 > 0) {
+      res = cur.p - pimg;
+    } else if ((int64_t)idx == g.pin) {
       res = cur.p;
     } else {
       const double pc = cur.p, rc = cur.rho;
       double acc = 0.0;
-      if (has0) {
-        const double rlp = (per0 && i0 == 0) ? rho[(int64_t)N0 * plane + j] : rc;
-        acc += axis_term(g, 0, i0, pc, cur.q, rc, prev, next, rlp);
-      }
-      Plane0 lo{0.0, 0.0, 0.0}, hi{0.0, 0.0, 0.0};
-      double rlp = rc;
-      if (g.n[1] > 1) {
-        neighbours(g, 1, c1, idx, p, q1, rho, lo, hi, rlp, rc);
-        acc += axis_term(g, 1, c1, pc, q1[idx], rc, lo, hi, rlp);
-      }
-      if (g.n[2] > 1) {
-        neighbours(g, 2, c2, idx, p, q2, rho, lo, hi, rlp, rc);
-        acc += axis_term(g, 2, c2, pc, q2[idx], rc, lo, hi, rlp);
-      }
+      if (has0) acc += axis_term(g, 0, i0, pc, cur.q, rc, prev, next,
+                                 (per0 && i0 == 0) ? rhoN0 : rc);
+      if (has1) acc += axis_term(g, 1, c1, pc, q1c, rc, b1.lo, b1.hi, b1.rlp);
+      acc += axis_term(g, 2, c2, pc, q2c, rc, b2.lo, b2.hi, b2.rlp);
       res = acc;
     }
-    __builtin_nontemporal_store(res, &out[idx]);
+    __builtin_nontemporal_store(res, &out[(int64_t)idx]);
     prev = cur;
     cur = next;
   }
@@ -261,6 +296,32 @@ __global__ __launch_bounds__(kBlock) void k_fccd_gauge(FccdGeom g, const double*
     const int64_t s = image_source(g, idx, c);
     x[idx] = idx == g.pin ? 0.0 : (s >= 0 ? src[s] : src[idx]);
   }
+}
+
+// Multigrid preconditioner input: b32 = -W r, the CG-system form (-W L) z =
+// -W r of the FVM operator's hierarchy (W = product of node widths, half
+// cells at walls; fvm_matrixfree.py:167-177), b32[pin] = 0.
+__global__ __launch_bounds__(kBlock) void k_fccd_to_cg32(FccdGeom g, const double* __restrict__ r,
+                                                         float* __restrict__ b32) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int c[3];
+    coords(g, idx, c);
+    double W = 1.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (g.n[a] > 1) W *= (c[a] == 0 || c[a] == g.n[a] - 1) ? 0.5 / g.inv_H[a] : 1.0 / g.inv_H[a];
+    b32[idx] = idx == g.pin ? 0.0f : (float)(-W * r[idx]);
+  }
+}
+
+// z = z32 in FP64, z[pin] = 0.
+__global__ __launch_bounds__(kBlock) void k_fccd_from32(int64_t n, int64_t pin,
+                                                        const float* __restrict__ z32,
+                                                        double* __restrict__ z) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    z[i] = i == pin ? 0.0 : (double)z32[i];
 }
 
 __global__ __launch_bounds__(kBlock) void k_fccd_fill(int64_t n, double v, double* __restrict__ x) {
@@ -327,6 +388,13 @@ struct tpf_fccd_ppe {
   double* part = nullptr;    // norm partials (+1 result)
   double* host = nullptr;    // pinned result
   void* gmres_work = nullptr;
+  // multigrid preconditioner (wall grids): the FVM operator's FP32 hierarchy
+  // on an FVM handle of the same grid, created on first use
+  tpf_ppe* fvm = nullptr;
+  const double* fvm_rho = nullptr;
+  int64_t n_nodes[3] = {0, 0, 0};
+  double L[3] = {0.0, 0.0, 0.0};
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
 
 using namespace tpf;
@@ -341,6 +409,9 @@ extern "C" tpf_status tpf_fccd_ppe_destroy(tpf_fccd_ppe* s) {
   if (!s) return TPF_OK;
   gmres_work_free(&s->gmres_work);
   if (s->ccd) tpf_ccd_plan_destroy(s->ccd);
+  if (s->fvm) tpf_ppe_destroy(s->fvm);
+  if (s->ev_in) (void)hipEventDestroy(s->ev_in);
+  if (s->ev_out) (void)hipEventDestroy(s->ev_out);
   double* bufs[] = {s->q, s->dinv, s->part};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
@@ -371,6 +442,8 @@ extern "C" tpf_status tpf_fccd_ppe_create(tpf_fccd_ppe** out, int ndim, const in
       return fail(TPF_ERR_VALUE, "FCCD PPE axis needs >= 3 nodes, L > 0 and a wall/periodic bc");
     }
     g.n[slot] = (int)n_nodes[a];
+    s->n_nodes[a] = n_nodes[a];
+    s->L[a] = L[a];
     g.periodic[slot] = bc[a] == TPF_BC_PERIODIC;
     g.qk[slot] = a;
     h[slot] = L[a] / (double)(n_nodes[a] - 1);
@@ -421,6 +494,7 @@ extern "C" tpf_status tpf_fccd_ppe_set_density(tpf_fccd_ppe* s, const double* rh
   if (!s || !rho) return fail(TPF_ERR_VALUE, "null argument");
   s->rho = rho;
   FCCD_LAUNCH(k_fccd_jacobi, s->g, rho, s->dinv);
+  s->fvm_rho = nullptr;  // the multigrid hierarchy follows on next use
   return TPF_OK;
 }
 
@@ -429,11 +503,54 @@ tpf_status fccd_apply(tpf_fccd_ppe* s, const double* p, double* out) {
   const uint32_t mask = (1u << s->ndim) - 1u;
   TPF_TRY(tpf_ccd_apply(s->ccd, p, nullptr, s->q, mask, s->stream));
   const int plane = s->g.n[1] * s->g.n[2];
-  const dim3 grid((unsigned)((plane + kBlock - 1) / kBlock),
-                  (unsigned)((s->g.n[0] + kSlab - 1) / kSlab));
-  hipLaunchKernelGGL(k_fccd_apply, grid, dim3(kBlock), 0, s->stream, s->g, p,
-                     (const double*)s->q, s->rho, out);
+  const unsigned nx = (unsigned)((plane + kBlock - 1) / kBlock);
+  const unsigned ny = (unsigned)((s->g.n[0] + kSlab - 1) / kSlab);
+  auto kern = s->ndim == 3 ? k_fccd_apply<0, int64_t>
+                           : (s->ndim == 2 ? k_fccd_apply<1, int64_t> : k_fccd_apply<2, int64_t>);
+  if (s->g.total < ((int64_t)1 << 29))
+    kern = s->ndim == 3 ? k_fccd_apply<0, uint32_t>
+                        : (s->ndim == 2 ? k_fccd_apply<1, uint32_t> : k_fccd_apply<2, uint32_t>);
+  hipLaunchKernelGGL(kern, dim3(nx * ny), dim3(kBlock), 0, s->stream, s->g, p,
+                     (const double*)s->q, s->rho, out, nx);
   TPF_HIP(hipGetLastError());
+  return TPF_OK;
+}
+
+// Multigrid cycle parameters of the preconditioner: W(2,2), W recursion on
+// the top two levels, 20 coarse sweeps (the FVM multigrid-FCG defaults).
+constexpr int kMgNu = 2, kMgGamma = 2, kMgCoarse = 20, kMgWLevels = 2;
+
+tpf_status fccd_mg_ready(tpf_fccd_ppe* s) {
+  for (int a = 0; a < 3; ++a)
+    if (s->g.n[a] > 1 && s->g.periodic[a])
+      return fail(TPF_ERR_VALUE, "the multigrid preconditioner needs wall axes");
+  if (!s->fvm) {
+    TPF_TRY(tpf_ppe_create(&s->fvm, s->ndim, s->n_nodes, s->L, s->g.pin, s->stream));
+    TPF_HIP(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming));
+    TPF_HIP(hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming));
+  }
+  if (s->fvm_rho != s->rho) {
+    TPF_TRY(tpf_ppe_set_density(s->fvm, s->rho));
+    s->fvm_rho = s->rho;
+  }
+  return mg_ensure(s->fvm, 12);  // synchronous
+}
+
+// z = M(r): one FP32 cycle of the FVM hierarchy on -W r, on the FVM handle's
+// stream, ordered after and before the FCCD stream by events.
+tpf_status fccd_mg_precond(tpf_fccd_ppe* s, const double* r, double* z) {
+  hipStream_t fs = s->fvm->stream;
+  TPF_HIP(hipEventRecord(s->ev_in, s->stream));
+  TPF_HIP(hipStreamWaitEvent(fs, s->ev_in, 0));
+  hipLaunchKernelGGL(k_fccd_to_cg32, dim3(s->grid), dim3(kBlock), 0, fs, s->g, r,
+                     mg_fine_b(s->fvm));
+  TPF_HIP(hipGetLastError());
+  TPF_TRY(mg_cycle(s->fvm, fs, kMgNu, kMgGamma, kMgCoarse, kMgWLevels));
+  hipLaunchKernelGGL(k_fccd_from32, dim3(s->grid), dim3(kBlock), 0, fs, s->g.total, s->g.pin,
+                     (const float*)mg_fine_x(s->fvm), z);
+  TPF_HIP(hipGetLastError());
+  TPF_HIP(hipEventRecord(s->ev_out, fs));
+  TPF_HIP(hipStreamWaitEvent(s->stream, s->ev_out, 0));
   return TPF_OK;
 }
 
@@ -460,8 +577,10 @@ extern "C" tpf_status tpf_fccd_ppe_solve_gmres(tpf_fccd_ppe* s, const double* rh
                                                const tpf_gmres_opts* o, tpf_solve_report* rep) {
   if (!s || !rhs || !x || !o) return fail(TPF_ERR_VALUE, "null argument");
   if (!s->rho) return fail(TPF_ERR_RUNTIME, "FCCD PPE density not set");
-  if (o->precond != 0 && o->precond != 1)
-    return fail(TPF_ERR_VALUE, "FCCD PPE supports preconditioner 0 (none) or 1 (Jacobi)");
+  if (o->precond < 0 || o->precond > 2)
+    return fail(TPF_ERR_VALUE,
+                "FCCD PPE supports preconditioner 0 (none), 1 (Jacobi) or 2 (multigrid)");
+  if (o->precond == 2) TPF_TRY(fccd_mg_ready(s));
   if (!(o->tol > 0.0) || o->maxiter < 1 || o->restart < 1)
     return fail(TPF_ERR_VALUE, "GMRES needs tol > 0, maxiter >= 1, restart >= 1");
   const int64_t n = s->g.total;
@@ -476,13 +595,14 @@ extern "C" tpf_status tpf_fccd_ppe_solve_gmres(tpf_fccd_ppe* s, const double* rh
   } else {
     FCCD_LAUNCH(k_fccd_fill, n, 0.0, x);
   }
-  const bool jac = o->precond == 1;
+  const bool jac = o->precond == 1, mg = o->precond == 2;
   const dim3 grid(s->grid);
   tpf_solve_report r{};
   tpf_status st = gmres_core(
       &s->gmres_work, s->stream, (unsigned)s->grid, n, m,
       [&](const double* in, double* out) { return fccd_apply(s, in, out); },
       [&](const double* in, double* out) -> tpf_status {
+        if (mg) return fccd_mg_precond(s, in, out);
         if (jac) {
           hipLaunchKernelGGL(k_fccd_precond, grid, dim3(kBlock), 0, s->stream, n, s->g.pin,
                              (const double*)s->dinv, in, out);

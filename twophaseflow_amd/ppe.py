@@ -299,7 +299,13 @@ class PPESolverFCCDMatrixFree(IPPESolver):
     ``boundary_face_space = "full_face"``, legacy ``scalar_operator_pairing``,
     uniform grids, wall or periodic axes.  The other schemes raise
     ``NotImplementedError``; invalid names raise ``ValueError`` as in the
-    reference (fccd_matrixfree.py:155-186)."""
+    reference (fccd_matrixfree.py:155-186).
+
+    ``ppe_preconditioner`` takes the reference's "none" (default) and
+    "jacobi", plus this build's "mg" (wall grids): one FP32 W(2,2) cycle of the
+    FVM operator's multigrid hierarchy per GMRES step, the low-order operator
+    as preconditioner of the high-order one (the Krylov form of the
+    reference's defect correction, ppe/defect_correction.py:212-371)."""
 
     scheme_names = ("fccd_iterative",)
     _scheme_aliases = {"fccd_matrixfree": "fccd_iterative", "fccd": "fccd_iterative"}
@@ -351,8 +357,11 @@ class PPESolverFCCDMatrixFree(IPPESolver):
                              "'none'|'jump_decomposition'|'affine_jump'")
         if self.coefficient_scheme == "phase_density" and self.interface_coupling_scheme != "none":
             raise ValueError("phase_density PPE requires interface_coupling='none'")
-        if self.preconditioner not in {"jacobi", "none"}:
-            raise ValueError("FCCD PPE supports preconditioner='jacobi'|'none'")
+        if self.preconditioner not in {"jacobi", "none", "mg"}:
+            raise ValueError("FCCD PPE supports preconditioner='jacobi'|'none' (and this "
+                             "build's 'mg')")
+        if self.preconditioner == "mg" and any(k != "wall" for k in self._kinds):
+            raise ValueError("ppe_preconditioner='mg' needs wall axes")
         if self.coefficient_scheme != "phase_density":
             raise NotImplementedError(
                 "this build's FCCD PPE covers ppe_coefficient_scheme='phase_density'")
@@ -438,7 +447,7 @@ class PPESolverFCCDMatrixFree(IPPESolver):
         # SciPy's gmres takes restart=None as 20 (gmres_helpers.py:16-42)
         restart = 20 if self.restart is None else self.restart
         return nat.GmresOpts(tol=self.tol, maxiter=self.maxiter, restart=restart,
-                             precond=1 if self.preconditioner == "jacobi" else 0,
+                             precond={"none": 0, "jacobi": 1, "mg": 2}[self.preconditioner],
                              pcr_stages=0, use_x0=1 if use_x0 else 0, c_tau=0.0)
 
     def system_rhs(self, rhs: torch.Tensor) -> torch.Tensor:
