@@ -1,4 +1,4 @@
-"""CPU oracle for the CCD operator and the variable-density FVM PPE.
+"""CPU oracle for the CCD operator and the variable-density FVM and FCCD PPEs.
 
 TEST INFRASTRUCTURE ONLY.  This package is a NumPy/SciPy restatement of the
 reference algorithms, used as the checker by ``tests/``, by
@@ -20,6 +20,9 @@ function against something other than its own formula):
 * ``ppe.solve_cg`` / ``ppe.solve_direct`` -- each other, paper table U2-b,
   and the FP64 true residual of the reference's CG system.
 * ``fields``                       -- closed forms (sphere volumes, ψ bounds).
+* ``fccd.OracleFCCDPPE``          -- the closed-form periodic symbol
+  (``fccd.fccd_symbol``), cubic exactness of the wall face gradient and rows,
+  the discrete divergence theorem, and a dense direct solve for its GMRES.
 
 No function here is "parity unpinned" except the GMRES/CG iteration counts,
 which the reference does not fix (SciPy internals).
