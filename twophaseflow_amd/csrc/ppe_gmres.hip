@@ -306,6 +306,21 @@ __global__ __launch_bounds__(kBlock) void k_multidot_fin(const double* partials,
 #pragma unroll
     for (int k = 0; k < kDot; ++k) out[blockIdx.y * kDot + k] = v[k];
 }
+// w += sign * sum_{k < kc} d_k V_k with d_k = dots[(k / 16) * 17 + k % 16],
+// the layout k_multidot_fin writes (the Gram-Schmidt coefficients stay on the
+// device).
+__global__ __launch_bounds__(kBlock) void k_multiaxpy_dots(int64_t n, const double* __restrict__ V,
+                                                           int kc,
+                                                           const double* __restrict__ dots,
+                                                           double sign, double* __restrict__ w) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    double acc = 0.0;
+    for (int k = 0; k < kc; ++k) acc += dots[(k / kChunk) * kDot + k % kChunk] * V[(int64_t)k * n + i];
+    w[i] += sign * acc;
+  }
+}
+
 // w += sign * sum_{k < kc} coef[k] V_k
 __global__ __launch_bounds__(kBlock) void k_multiaxpy(int64_t n, const double* __restrict__ V,
                                                       int kc, const double* __restrict__ coef,
@@ -363,8 +378,9 @@ tpf_status ensure_work(void** slot, int64_t n, int m, int grid_blocks, int nscra
   if (nscratch > 0) TPF_HIP(hipMalloc(&gw->s8, sizeof(double) * n * nscratch));
   TPF_HIP(hipMalloc(&gw->coef, sizeof(double) * (size_t)(m + 1)));
   TPF_HIP(hipMalloc(&gw->part, sizeof(double) * kDot * (size_t)grid_blocks * gw->chunks));
-  TPF_HIP(hipMalloc(&gw->dots, sizeof(double) * kDot * (size_t)gw->chunks));
-  TPF_HIP(hipHostMalloc(&gw->host_dn, sizeof(double) * kDot * (size_t)gw->chunks));
+  // three result sets: the two Gram-Schmidt passes and the norm of one step
+  TPF_HIP(hipMalloc(&gw->dots, sizeof(double) * 3 * kDot * (size_t)gw->chunks));
+  TPF_HIP(hipHostMalloc(&gw->host_dn, sizeof(double) * 3 * kDot * (size_t)gw->chunks));
   TPF_HIP(hipHostMalloc(&gw->host_up, sizeof(double) * (size_t)(m + 1)));
   return TPF_OK;
 }
@@ -459,6 +475,20 @@ tpf_status multidot(const Krylov& c, const double* V, int kcount, const double* 
                          hipMemcpyDeviceToHost, c.st));
   TPF_HIP(hipStreamSynchronize(c.st));
   std::copy(c.gw->host_dn, c.gw->host_dn + kDot * chunks, out);
+  return TPF_OK;
+}
+
+// Result set `set` (0..2) of the step's dot products: out[set][j*17 + k] as
+// multidot; no copy, no synchronisation.
+tpf_status multidot_dev(const Krylov& c, const double* V, int kcount, const double* w, int set) {
+  const int chunks = std::max(1, (kcount + kChunk - 1) / kChunk);
+  double* out = c.gw->dots + (size_t)set * kDot * c.gw->chunks;
+  hipLaunchKernelGGL(k_multidot, dim3(c.grid.x, chunks), dim3(kBlock), 0, c.st, c.n, V, kcount,
+                     w, c.gw->part);
+  TPF_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_multidot_fin, dim3(1, chunks), dim3(kBlock), 0, c.st,
+                     (const double*)c.gw->part, (int)c.grid.x, out);
+  TPF_HIP(hipGetLastError());
   return TPF_OK;
 }
 
@@ -563,7 +593,6 @@ tpf_status gmres_core(void** work, hipStream_t st, unsigned grid, int64_t n, int
   int64_t inner = 0, cycles = 0;
   const size_t ld = (size_t)m + 1;
   std::vector<double> h((size_t)m * ld), S(ld), gc(m), gs(m), y(ld), hc(ld);
-  std::vector<double> dots((size_t)kDot * gw->chunks);
   const bool early = rnorm < atol;  // SciPy's first-cycle check
   while (!early && cycles < maxiter) {
     ++cycles;
@@ -580,17 +609,26 @@ tpf_status gmres_core(void** work, hipStream_t st, unsigned grid, int64_t n, int
       double* vn = V + (int64_t)(col + 1) * n;
       TPF_TRY(apply(vc, gw->t));
       TPF_TRY(psolve(gw->t, vn));  // w, built in place in v_{col+1}
-      std::fill(hc.begin(), hc.begin() + col + 1, 0.0);
-      double h0 = 0.0;
+      // CGS2 with the coefficients kept on the device: two dot / subtract
+      // passes and the norm, then one read-back of the three result sets
+      const size_t set = (size_t)kDot * gw->chunks;
       for (int pass = 0; pass < 2; ++pass) {
-        TPF_TRY(multidot(c, V, col + 1, vn, dots.data()));
-        if (pass == 0) h0 = std::sqrt(dots[kChunk]);
-        for (int k = 0; k <= col; ++k) y[k] = dots[(k / kChunk) * kDot + k % kChunk];
-        for (int k = 0; k <= col; ++k) hc[k] += y[k];
-        TPF_TRY(multiaxpy(c, V, col + 1, y.data(), -1.0, vn));
+        TPF_TRY(multidot_dev(c, V, col + 1, vn, pass));
+        TPF_LAUNCH(k_multiaxpy_dots, c.grid, n, (const double*)V, col + 1,
+                   (const double*)(gw->dots + pass * set), -1.0, vn);
       }
-      double h1 = 0.0;
-      TPF_TRY(norm2(c, vn, h1));
+      TPF_TRY(multidot_dev(c, vn, 0, vn, 2));
+      TPF_HIP(hipMemcpyAsync(gw->host_dn, gw->dots, sizeof(double) * 3 * set,
+                             hipMemcpyDeviceToHost, c.st));
+      TPF_HIP(hipStreamSynchronize(c.st));
+      const double* d0 = gw->host_dn;
+      const double* d1 = gw->host_dn + set;
+      const double h0 = std::sqrt(d0[kChunk]);
+      for (int k = 0; k <= col; ++k) {
+        const size_t at = (size_t)(k / kChunk) * kDot + k % kChunk;
+        hc[k] = d0[at] + d1[at];
+      }
+      const double h1 = std::sqrt(gw->host_dn[2 * set + kChunk]);
       double* hcol = &h[(size_t)col * ld];
       for (int k = 0; k <= col; ++k) hcol[k] = hc[k];
       hcol[col + 1] = h1;
