@@ -261,6 +261,24 @@ TPF_API tpf_status tpf_fccd_ppe_apply(tpf_fccd_ppe* ppe, const double* p, double
 TPF_API tpf_status tpf_fccd_ppe_solve_gmres(tpf_fccd_ppe* ppe, const double* rhs, double* p,
                                             const tpf_gmres_opts* opts, tpf_solve_report* rep);
 
+/* FCCD face primitives of FCCDSolver along `axis` (ccd/fccd.py:165-521;
+ * uniform weights ccd/fccd_helpers.py:17-35, H = L/N).  Nodal arrays have the
+ * field shape; face arrays the field shape with N along `axis` (face j between
+ * nodes j and j+1; periodic: face N-1 pairs node N-1 with node 0).  q is the
+ * nodal CCD second derivative along `axis`.  op:
+ *   0 face_gradient          (u_hi - u_lo)/H - (H/24)(q_hi - q_lo)   :243-277
+ *   1 face_value             (u_lo + u_hi)/2 - (H^2/16)(q_lo + q_hi) :279-299
+ *   2 face_second_derivative (q_lo + q_hi)/2                          :301-317
+ *   3 node_gradient          R4 Hermite, one-sided at walls (nodal)  :392-462
+ *   4 face_divergence        u = face array -> nodal; walls 0         :464-521
+ *   5 upwind_face_value      Taylor state from the upwind node with d
+ *                            (nodal f') and vel (face velocity)       :333-390
+ * Asynchronous on `stream`; d and vel only for op 5, q unused by op 4. */
+TPF_API tpf_status tpf_fccd_face_op(int op, int ndim, const int64_t* n_nodes, const double* L,
+                                    const int* bc, int axis, const double* u, const double* q,
+                                    const double* d, const double* vel, double* out,
+                                    void* stream);
+
 /* ------------------------------------------------------------- fields --- */
 /* Replaces InitialConditionBuilder.build (gas spheres in liquid) +
  * update_properties (simulation/initial_conditions/builder.py:100-172,

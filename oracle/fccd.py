@@ -24,11 +24,18 @@ The nodal second derivative q along each axis is the CCD oracle's
 (``oracle/ccd.py``), as ``FCCDSolver.face_gradient`` takes it from
 ``CCDSolver.second_derivative``.
 
+``OracleFCCD`` restates FCCDSolver's face primitives (``ccd/fccd.py:95-131,
+243-521``): face gradient, face value, face second derivative, upwind Taylor
+face state, R4 Hermite node gradient and face divergence.
+
 Pinning: ``fccd_symbol`` (closed form from the CCD symbol and the face
 stencils) checks the periodic operator; cubic exactness of the face gradient
 at face midpoints and the discrete divergence theorem (sum of w_i out_i over a
 wall line is zero) check the wall operator; the GMRES solve is checked
-against a dense direct solve of the assembled operator.
+against a dense direct solve of the assembled operator.  The face primitives
+are pinned by exactness for cubics (face value, node gradient) and quadratics
+(upwind state), linear face fields (divergence), and the reference's own
+truncated symbol of the periodic face gradient, met to O((wH)^6).
 """
 
 from __future__ import annotations
@@ -229,3 +236,97 @@ def fccd_symbol(theta, h):
     symbol (the face gradient's two terms share the factor e^{i theta} - 1)."""
     _, s2 = periodic_symbol(theta, h)
     return (2.0 * np.cos(theta) - 2.0) / (h * h) * (1.0 - (h * h / 24.0) * s2)
+
+
+class OracleFCCD:
+    """CPU restatement of FCCDSolver's face primitives (ccd/fccd.py:165-521,
+    uniform weights ccd/fccd_helpers.py:17-35), on the CCD oracle's q."""
+
+    def __init__(self, N, L, bc_type="wall"):
+        self.N = tuple(int(n) for n in N)
+        self.ndim = len(self.N)
+        self.L = tuple(float(x) for x in L)
+        self.kinds = axis_kinds(bc_type, self.ndim)
+        self.ccd = OracleCCD(self.N, self.L, bc_type)
+
+    def h(self, axis):
+        return self.L[axis] / self.N[axis]
+
+    def _pair(self, a, axis):
+        """(lo, hi) nodal values of every face (ccd/fccd.py:212-232)."""
+        a = _front(np.asarray(a, dtype=np.float64), axis)
+        if self.kinds[axis] == "periodic":
+            n = self.N[axis]
+            return a[:n], np.roll(a[:n], -1, axis=0)
+        return a[:-1], a[1:]
+
+    def _q(self, u, axis, q):
+        return self.ccd.second_derivative(u, axis) if q is None else q
+
+    def face_gradient(self, u, axis, q=None):
+        q = self._q(u, axis, q)
+        H = self.h(axis)
+        (ul, uh), (ql, qh) = self._pair(u, axis), self._pair(q, axis)
+        return np.moveaxis((uh - ul) * (1.0 / H) - (H / 24.0) * (qh - ql), 0, axis)
+
+    def face_value(self, u, axis, q=None):
+        """ccd/fccd.py:101-109, 279-299."""
+        q = self._q(u, axis, q)
+        H = self.h(axis)
+        (ul, uh), (ql, qh) = self._pair(u, axis), self._pair(q, axis)
+        return np.moveaxis(0.5 * (ul + uh) - (H * H / 16.0) * (ql + qh), 0, axis)
+
+    def face_second_derivative(self, u, axis, q=None):
+        """ccd/fccd.py:112-114, 301-317."""
+        q = self._q(u, axis, q)
+        ql, qh = self._pair(q, axis)
+        return np.moveaxis(0.5 * (ql + qh), 0, axis)
+
+    def upwind_face_value(self, u, vel, axis, d=None, q=None):
+        """ccd/fccd.py:115-125, 333-390."""
+        if d is None:
+            d = self.ccd.first_derivative(u, axis)
+        q = self._q(u, axis, q)
+        H = self.h(axis)
+        (ul, uh), (dl, dh), (ql, qh) = self._pair(u, axis), self._pair(d, axis), self._pair(q, axis)
+        left = ul + 0.5 * H * dl + (H * H / 8.0) * ql
+        right = uh - 0.5 * H * dh + (H * H / 8.0) * qh
+        v = _front(np.asarray(vel, dtype=np.float64), axis)
+        return np.moveaxis(np.where(v >= 0.0, left, right), 0, axis)
+
+    def node_gradient(self, u, axis, q=None):
+        """R4 Hermite, one-sided at wall nodes (ccd/fccd.py:128-131, 392-462)."""
+        q = self._q(u, axis, q)
+        H = self.h(axis)
+        d = _front(self.face_gradient(u, axis, q=q), axis)
+        qq = _front(q, axis)
+        n = self.N[axis]
+        out = np.empty((n + 1,) + d.shape[1:])
+        if self.kinds[axis] == "periodic":
+            qu = qq[:n]
+            g = 0.5 * (np.roll(d, 1, axis=0) + d) - (H / 16.0) * (np.roll(qu, -1, axis=0)
+                                                                   - np.roll(qu, 1, axis=0))
+            out[:n], out[n] = g, g[0]
+        else:
+            out[1:n] = 0.5 * (d[:-1] + d[1:]) - (H / 16.0) * (qq[2:] - qq[:-2])
+            out[0], out[n] = d[0], d[n - 1]
+        return np.moveaxis(out, 0, axis)
+
+    def face_divergence(self, F, axis):
+        """ccd/fccd.py:464-521 (uniform)."""
+        H = self.h(axis)
+        f = _front(np.asarray(F, dtype=np.float64), axis)
+        n = self.N[axis]
+        out = np.zeros((n + 1,) + f.shape[1:])
+        if self.kinds[axis] == "periodic":
+            div = (f - np.roll(f, 1, axis=0)) * (1.0 / H)
+            out[:n], out[n] = div, div[0]
+        else:
+            out[1:n] = (f[1:] - f[:-1]) * (1.0 / H)
+        return np.moveaxis(out, 0, axis)
+
+
+def fccd_face_symbol_truncated(omega, H):
+    """The reference's truncated face-operator symbol i w (1 - 7 (w H)^4 / 5760)
+    (ccd/fccd_helpers.py:65-67)."""
+    return 1j * omega * (1.0 - 7.0 * (omega * H) ** 4 / 5760.0)

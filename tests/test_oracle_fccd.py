@@ -157,3 +157,65 @@ def test_fccd_ppe_config_errors(kw, exc):
     from twophaseflow_amd import PPESolverFCCDMatrixFree
     with pytest.raises(exc):
         PPESolverFCCDMatrixFree(None, _Cfg(**kw), _Grid(), None, bc_type="wall")
+
+
+# --- FCCDSolver face primitives (oracle.fccd.OracleFCCD) ---------------------
+from oracle.fccd import OracleFCCD, fccd_face_symbol_truncated  # noqa: E402
+
+
+def _cubic(x):
+    return 0.7 * x ** 3 - 1.3 * x ** 2 + 0.4 * x + 2.0
+
+
+def _dcubic(x):
+    return 2.1 * x ** 2 - 2.6 * x + 0.4
+
+
+def test_face_value_curvature_node_gradient_exact_for_cubics():
+    o = OracleFCCD((24,), (1.5,), "wall")
+    x = np.linspace(0.0, 1.5, 25)
+    xm = 0.5 * (x[1:] + x[:-1])
+    u = _cubic(x)
+    assert np.abs(o.face_value(u, 0) - _cubic(xm)).max() < 1e-12
+    assert np.abs(o.face_second_derivative(u, 0) - (4.2 * xm - 2.6)).max() < 1e-11
+    g = o.node_gradient(u, 0)
+    assert np.abs(g[1:-1] - _dcubic(x[1:-1])).max() < 1e-11
+    # wall nodes: one-sided, the first / last face gradients (exact at the faces)
+    assert abs(g[0] - _dcubic(xm[0])) < 1e-11 and abs(g[-1] - _dcubic(xm[-1])) < 1e-11
+
+
+def test_upwind_face_value_exact_for_quadratics():
+    o = OracleFCCD((16,), (1.0,), "wall")
+    x = np.linspace(0.0, 1.0, 17)
+    xm = 0.5 * (x[1:] + x[:-1])
+    u = 3.0 * x ** 2 - x + 0.5
+    vel = np.where(np.arange(16) % 2 == 0, 1.0, -1.0)
+    got = o.upwind_face_value(u, vel, 0)
+    assert np.abs(got - (3.0 * xm ** 2 - xm + 0.5)).max() < 1e-12
+
+
+def test_face_divergence_linear_faces_and_periodic_closed_form():
+    o = OracleFCCD((20, 8), (1.0, 1.0), "wall")
+    xm = (np.arange(20) + 0.5) / 20
+    F = np.repeat((2.0 * xm + 1.0)[:, None], 9, axis=1)
+    div = o.face_divergence(F, 0)
+    assert np.abs(div[1:-1] - 2.0).max() < 1e-12 and np.all(div[[0, -1]] == 0.0)
+    op = OracleFCCD((32,), (1.0,), "periodic")
+    th = 2 * np.pi * 3 / 32
+    F = np.sin(th * (np.arange(32) + 0.5))
+    div = op.face_divergence(F, 0)
+    j = np.arange(33) % 32
+    want = 2.0 * np.cos(th * j) * np.sin(th / 2) * 32
+    assert np.abs(div - want).max() < 1e-11
+
+
+@pytest.mark.parametrize("k", [1, 2, 4])
+def test_periodic_face_gradient_meets_the_truncated_symbol(k):
+    N, H = 64, 1.0 / 64
+    o = OracleFCCD((N,), (1.0,), "periodic")
+    w = 2 * np.pi * k
+    x = np.arange(N + 1) * H
+    g = o.face_gradient(np.cos(w * x), 0) + 1j * o.face_gradient(np.sin(w * x), 0)
+    sym = g / np.exp(1j * w * (np.arange(N) + 0.5) * H)
+    err = np.abs(sym - fccd_face_symbol_truncated(w, H)).max() / w
+    assert err <= (w * H) ** 6

@@ -13,12 +13,14 @@ from .config import GridConfig, NumericsConfig, SimulationConfig, SolverConfig
 from .grid import Grid
 from .boundary import BoundarySpec, axis_kinds, pin_dof
 from .ccd import CCDBatch, CCDSolver
+from .fccd import FCCDFaceJet, FCCDSolver
 from .ppe import (IPPESolver, MatrixAssemblyUnavailable, PPESolverFCCDMatrixFree,
                   PPESolverFVMMatrixFree, create_ppe_solver, register_ppe_solver)
 
 __all__ = [
     "GridConfig", "NumericsConfig", "SimulationConfig", "SolverConfig", "Grid",
-    "BoundarySpec", "axis_kinds", "pin_dof", "CCDSolver", "CCDBatch", "IPPESolver",
+    "BoundarySpec", "axis_kinds", "pin_dof", "CCDSolver", "CCDBatch", "FCCDSolver",
+    "FCCDFaceJet", "IPPESolver",
     "MatrixAssemblyUnavailable", "PPESolverFVMMatrixFree", "PPESolverFCCDMatrixFree",
     "create_ppe_solver",
     "register_ppe_solver",
