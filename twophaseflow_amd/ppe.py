@@ -409,6 +409,7 @@ class PPESolverFCCDMatrixFree(IPPESolver):
             self.shape = tuple(int(n) + 1 for n in grid.N)
             self._pin_dof = int(np.ravel_multi_index(tuple(n // 2 for n in grid.N), self.shape))
             self._pin_dofs = (self._pin_dof,)
+            self._images_np = None
         self.__del__()
         self._rho = None
         self._create()
@@ -459,6 +460,56 @@ class PPESolverFCCDMatrixFree(IPPESolver):
             if kind == "periodic":
                 b.select(a, self.shape[a] - 1).zero_()
         return b
+
+    # -- gauge helpers used by defect correction (fccd_matrixfree.py:300-376,
+    #    682-706); NumPy arrays or CUDA tensors -----------------------------------
+    def _image_rows(self, like):
+        """(images, sources) flat indices: the first periodic axis on which a
+        node sits at index N maps it to index 0 (fccd_matrixfree.py:300-345)."""
+        if not any(k == "periodic" for k in self._kinds):
+            return None
+        if getattr(self, "_images_np", None) is None:
+            table = {}
+            idx = np.indices(self.shape).reshape(self.ndim, -1)
+            for a, kind in enumerate(self._kinds):
+                if kind != "periodic":
+                    continue
+                sel = idx[a] == self.shape[a] - 1
+                src = idx[:, sel].copy()
+                src[a] = 0
+                for i, j in zip(np.ravel_multi_index(idx[:, sel], self.shape).tolist(),
+                                np.ravel_multi_index(src, self.shape).tolist()):
+                    table.setdefault(i, j)
+            img = np.array(sorted(table), dtype=np.int64)
+            self._images_np = (img, np.array([table[i] for i in img], dtype=np.int64))
+            self._images_t = None
+        if isinstance(like, torch.Tensor):
+            if self._images_t is None or self._images_t[0].device != like.device:
+                self._images_t = tuple(torch.from_numpy(a).to(like.device)
+                                       for a in self._images_np)
+            return self._images_t
+        return self._images_np
+
+    def _sync_periodic_images(self, arr):
+        rows = self._image_rows(arr)
+        if rows is None:
+            return arr
+        out = arr.copy() if isinstance(arr, np.ndarray) else arr.clone()
+        flat = out.reshape(-1)
+        flat[rows[0]] = flat[rows[1]]
+        return out
+
+    def _zero_periodic_image_rows(self, arr):
+        rows = self._image_rows(arr)
+        if rows is not None:
+            arr.reshape(-1)[rows[0]] = 0.0
+        return arr
+
+    def _project_rhs_compatibility(self, rhs, *, record_stats: bool = True):
+        """phase_density: a copy with rhs[pin] = 0 (fccd_matrixfree_helpers.py:337-375)."""
+        out = rhs.copy() if isinstance(rhs, np.ndarray) else rhs.clone()
+        out.reshape(-1)[self._pin_dof] = 0.0
+        return out
 
     def solve_into(self, rhs: torch.Tensor, rho, p: torch.Tensor, use_x0: bool = False) -> dict:
         """Zero-copy solve on CUDA tensors; p holds p_init when use_x0."""
