@@ -33,6 +33,10 @@ namespace tpf {
 namespace {
 
 constexpr int kChunk = 16;          // basis vectors per multi-dot block row
+// Fused PCR: lines of up to kPcrFusedMaxN nodes, kPcrTileNodes nodes per
+// workgroup tile (8 doubles each in LDS: 32 KB per coefficient set at 1024).
+constexpr int kPcrFusedMaxN = 2048;
+constexpr int kPcrTileNodes = 1024;
 constexpr int kDot = kChunk + 1;    // + <w, w>
 
 __device__ inline void coords3(const PpeGeom& g, int64_t idx, int (&c)[3]) {
@@ -242,6 +246,93 @@ __global__ __launch_bounds__(kBlock) void k_pcr_stage(PpeGeom g, int a, int stri
   }
 }
 
+// All PCR stages of one slot in one launch: a workgroup holds `L` whole
+// lines (L * n nodes) of lo / di / up / rw in LDS, twice (ping-pong), forms
+// the shifted line coefficients, runs the stages there with a barrier between
+// them and accumulates scale * rw / di.  The arithmetic per element is that of
+// k_pcr_init / k_pcr_stage / k_pcr_finish.  Threads run over lines on the
+// strided slots and over nodes on the contiguous slot (coalesced loads).
+__global__ __launch_bounds__(kBlock) void k_pcr_fused(PpeGeom g, LineArgs la, int L, int stages,
+                                                      const double* __restrict__ rho,
+                                                      const double* __restrict__ r,
+                                                      double* __restrict__ out) {
+  extern __shared__ double sh[];
+  const int a = la.a;
+  const int n = slot_n(g, a);
+  const int64_t st = slot_stride(g, a);
+  const int64_t nlines = g.total / n;
+  const int64_t l0 = (int64_t)blockIdx.x * L;
+  const int nl = (int)(nlines - l0 < L ? nlines - l0 : L);
+  const int cnt = nl * n;
+  const int tile = L * n;
+  double* A[4] = {sh, sh + tile, sh + 2 * tile, sh + 3 * tile};
+  double* B[4] = {sh + 4 * tile, sh + 5 * tile, sh + 6 * tile, sh + 7 * tile};
+  auto elem = [&](int e, int& li, int& i) {
+    if (a == 2) { li = e / n; i = e - li * n; }
+    else { i = e / nl; li = e - i * nl; }
+  };
+  auto base_of = [&](int64_t l) -> int64_t {
+    if (a == 0) return l;
+    if (a == 1) return (l / g.n2) * ((int64_t)g.n1 * g.n2) + (l % g.n2);
+    return l * n;
+  };
+  for (int e = threadIdx.x; e < cnt; e += kBlock) {
+    int li, i;
+    elem(e, li, i);
+    const int64_t o = base_of(l0 + li) + (int64_t)i * st;
+    double lo, di, up;
+    line_coef(g, rho, a, o, st, n, i, la.c_tau, la.h_min2, lo, di, up);
+    const int s = li * n + i;
+    A[0][s] = lo;
+    A[1][s] = di;
+    A[2][s] = up;
+    A[3][s] = o == g.pin ? 0.0 : r[o];
+  }
+  __syncthreads();
+  for (int k = 0, stride = 1; k < stages; ++k, stride *= 2) {
+    for (int e = threadIdx.x; e < cnt; e += kBlock) {
+      int li, i;
+      elem(e, li, i);
+      const int s = li * n + i;
+      double alpha = 0.0, beta = 0.0;
+      double uL = 0.0, rL = 0.0, lR = 0.0, rR = 0.0, lL = 0.0, uR = 0.0;
+      if (i >= stride) {
+        const int q = s - stride;
+        const double dL = A[1][q];
+        alpha = -A[0][s] / (fabs(dL) > 0.0 ? dL : 1.0);
+        lL = A[0][q];
+        uL = A[2][q];
+        rL = A[3][q];
+      }
+      if (i < n - stride) {
+        const int q = s + stride;
+        const double dR = A[1][q];
+        beta = -A[2][s] / (fabs(dR) > 0.0 ? dR : 1.0);
+        lR = A[0][q];
+        uR = A[2][q];
+        rR = A[3][q];
+      }
+      B[0][s] = alpha * lL;
+      B[1][s] = A[1][s] + alpha * uL + beta * lR;
+      B[2][s] = beta * uR;
+      B[3][s] = A[3][s] + alpha * rL + beta * rR;
+    }
+    __syncthreads();
+    for (int q = 0; q < 4; ++q) {
+      double* t = A[q];
+      A[q] = B[q];
+      B[q] = t;
+    }
+  }
+  for (int e = threadIdx.x; e < cnt; e += kBlock) {
+    int li, i;
+    elem(e, li, i);
+    const int s = li * n + i;
+    const int64_t o = base_of(l0 + li) + (int64_t)i * st;
+    out[o] += la.scale * (A[3][s] / A[1][s]);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_pcr_finish(int64_t n, double scale,
                                                        const double* __restrict__ di,
                                                        const double* __restrict__ rw,
@@ -440,14 +531,26 @@ tpf_status psolve(Ctx& c, const double* r, double* z) {
       TPF_LAUNCH(k_line_thomas, dim3((unsigned)lg), g, la, c.s->rho, r, z, s8, s8 + c.n);
       continue;
     }
-    double* A[4] = {s8, s8 + c.n, s8 + 2 * c.n, s8 + 3 * c.n};
-    double* B[4] = {s8 + 4 * c.n, s8 + 5 * c.n, s8 + 6 * c.n, s8 + 7 * c.n};
-    TPF_LAUNCH(k_pcr_init, c.grid, g, la, c.s->rho, r, A[0], A[1], A[2], A[3]);
     // linalg_backend.py:147-150: min(ceil(log2 n), max(1, max_stages)) stages
     int full = 0;
     while ((1 << full) < na) ++full;
     full = std::max(1, full);
     const int ns = std::min(full, std::max(1, c.stages));
+    if (na <= kPcrFusedMaxN) {  // whole lines in LDS, one launch per slot
+      const int L = std::max(1, kPcrTileNodes / na);
+      const int64_t nlines = c.n / na;
+      const size_t lds = sizeof(double) * 8 * (size_t)L * na;
+      if (lds > 65536)
+        TPF_HIP(hipFuncSetAttribute((const void*)k_pcr_fused,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(k_pcr_fused, dim3((unsigned)((nlines + L - 1) / L)), dim3(kBlock), lds,
+                         c.st, g, la, L, ns, c.s->rho, r, z);
+      TPF_HIP(hipGetLastError());
+      continue;
+    }
+    double* A[4] = {s8, s8 + c.n, s8 + 2 * c.n, s8 + 3 * c.n};
+    double* B[4] = {s8 + 4 * c.n, s8 + 5 * c.n, s8 + 6 * c.n, s8 + 7 * c.n};
+    TPF_LAUNCH(k_pcr_init, c.grid, g, la, c.s->rho, r, A[0], A[1], A[2], A[3]);
     for (int k = 0, stride = 1; k < ns; ++k, stride *= 2) {
       TPF_LAUNCH(k_pcr_stage, c.grid, g, a, stride, A[0], A[1], A[2], A[3], B[0], B[1], B[2],
                  B[3]);
