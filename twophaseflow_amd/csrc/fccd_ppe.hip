@@ -170,7 +170,7 @@ __device__ __forceinline__ Nbr load_nbr(const FccdGeom& g, int a, int i, I idx,
 // renumbered so each XCD gets a contiguous range of plane chunks (the axis-1
 // neighbours of a chunk lie in the adjacent chunks).  S0 = 3 - ndim: the
 // first active slot.
-constexpr int kSlab = 32;
+constexpr int kSlab = 16;
 
 template <int S0, typename I>
 __global__ __launch_bounds__(kBlock) void k_fccd_apply(FccdGeom g, const double* __restrict__ p,
