@@ -474,6 +474,37 @@ def fccd_ppe_bench(reps=20, cpu=True):
     return out
 
 
+def rhie_chow_bench(reps=20):
+    """Rhie-Chow face-velocity divergence (the PPE right-hand side of the
+    projection step) on the 257^3 wall case: one all-axes CCD launch for the
+    pressure gradient (d1 only) plus the fused face-flux divergence kernel.
+    Algorithmic bytes of the kernel: u (3), p, rho, dp (3) in, div out =
+    72 B per node."""
+    import numpy as np
+    import torch
+    from twophaseflow_amd import CCDSolver
+    from twophaseflow_amd.rhie_chow import RhieChowInterpolator
+    g, rho, rhs = ppe_case(N=256)
+    ccd = CCDSolver(g, bc_type="wall")
+    rc = RhieChowInterpolator(None, g, ccd, "wall")
+    rng = np.random.default_rng(9)
+    vel = [torch.from_numpy(rng.standard_normal(tuple(rho.shape))).cuda() for _ in range(3)]
+    p = torch.randn_like(rho)
+    rc.face_velocity_divergence(vel, p, rho, 0.01)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for a, b in ev:
+        a.record()
+        rc.face_velocity_divergence(vel, p, rho, 0.01)
+        b.record()
+    torch.cuda.synchronize()
+    ms = sorted(a.elapsed_time(b) for a, b in ev)[reps // 2]
+    npts = p.numel()
+    return {"ms": round(ms, 4), "timing": "median of per-call HIP events (CCD d1 + divergence "
+            "kernel + the call's allocations)", "Mpts_per_s": round(npts / (ms * 1e-3) / 1e6, 1),
+            "algorithmic_GBps_incl_ccd": round((72 + 32) * npts / (ms * 1e-3) / 1e9, 1)}
+
+
 def profile_mode(args):
     import torch
     from twophaseflow_amd import CCDSolver, Grid, GridConfig
@@ -716,6 +747,9 @@ def main():
         result["ppe"]["gmres_line_pcr_65"] = ppe_solve_timed(gq, rhoq, rhsq, "fp64", maxiter=500,
                                                              method="gmres")
         del rhoq, rhsq
+        log("Rhie-Chow face-velocity divergence, 257^3 wall")
+        result["rhie_chow_257"] = rhie_chow_bench()
+        log(f"  {result['rhie_chow_257']}")
         log("FCCD matrix-free PPE: 257^3 matvec, 65^3 Jacobi-GMRES(80) solve")
         result["fccd_ppe"] = fccd_ppe_bench(cpu=not args.no_cpu)
         log(f"  {result['fccd_ppe']}")

@@ -279,6 +279,22 @@ TPF_API tpf_status tpf_fccd_face_op(int op, int ndim, const int64_t* n_nodes, co
                                     const double* d, const double* vel, double* out,
                                     void* stream);
 
+/* Replaces RhieChowInterpolator.face_velocity_divergence
+ * (spatial/rhie_chow.py:40-284): the divergence of the Rhie-Chow face
+ * velocities u_f = (u_L+u_R)/2 - dt 2/(rho_L+rho_R) [(p_R-p_L)/h - (dp_L+dp_R)/2]
+ * (wall nodes take one-sided dp; face 0 is the periodic wrap or a zero wall
+ * face), nodal (F_{k+1}-F_k)/h with node N = (F_0-F_N)/h (periodic) or -F_N/h
+ * (wall).  vel: ndim device pointers (host array); dp: the CCD first
+ * derivative of p, stacked [ndim][n_total] as tpf_ccd_apply writes it.
+ * kappa, psi, dpsi (dpsi stacked like dp) and we != 0 switch on the
+ * balanced-force bracket (:152-204); all three NULL switch it off. */
+TPF_API tpf_status tpf_rhie_chow_divergence(int ndim, const int64_t* n_nodes, const double* L,
+                                            const int* bc, const double* const* vel,
+                                            const double* p, const double* rho, const double* dp,
+                                            const double* kappa, const double* psi,
+                                            const double* dpsi, double we, double dt,
+                                            double* out, void* stream);
+
 /* ------------------------------------------------------------- fields --- */
 /* Replaces InitialConditionBuilder.build (gas spheres in liquid) +
  * update_properties (simulation/initial_conditions/builder.py:100-172,

@@ -23,6 +23,9 @@ function against something other than its own formula):
 * ``fccd.OracleFCCDPPE``          -- the closed-form periodic symbol
   (``fccd.fccd_symbol``), cubic exactness of the wall face gradient and rows,
   the discrete divergence theorem, and a dense direct solve for its GMRES.
+* ``rhie_chow.OracleRhieChow``    -- closed forms: linear velocity, cubic
+  pressure bracket (interior and wall one-sided values), and the
+  balanced-force equilibrium p = kappa psi / We.
 
 No function here is "parity unpinned" except the GMRES/CG iteration counts,
 which the reference does not fix (SciPy internals).

@@ -15,13 +15,14 @@ from .boundary import BoundarySpec, axis_kinds, pin_dof
 from .ccd import CCDBatch, CCDSolver
 from .fccd import FCCDFaceJet, FCCDSolver
 from .defect_correction import PPESolverDefectCorrection
+from .rhie_chow import RhieChowInterpolator
 from .ppe import (IPPESolver, MatrixAssemblyUnavailable, PPESolverFCCDMatrixFree,
                   PPESolverFVMMatrixFree, create_ppe_solver, register_ppe_solver)
 
 __all__ = [
     "GridConfig", "NumericsConfig", "SimulationConfig", "SolverConfig", "Grid",
     "BoundarySpec", "axis_kinds", "pin_dof", "CCDSolver", "CCDBatch", "FCCDSolver",
-    "FCCDFaceJet", "PPESolverDefectCorrection", "IPPESolver",
+    "FCCDFaceJet", "PPESolverDefectCorrection", "RhieChowInterpolator", "IPPESolver",
     "MatrixAssemblyUnavailable", "PPESolverFVMMatrixFree", "PPESolverFCCDMatrixFree",
     "create_ppe_solver",
     "register_ppe_solver",

@@ -84,6 +84,9 @@ SIGNATURES = {
                                            C.POINTER(SolveReport)]),
     "tpf_fccd_face_op": (C.c_int, [C.c_int, C.c_int, _i64p, _dp, _ip, C.c_int, _vp, _vp, _vp,
                                    _vp, _vp, _vp]),
+    "tpf_rhie_chow_divergence": (C.c_int, [C.c_int, _i64p, _dp, _ip, C.POINTER(_vp), _vp, _vp,
+                                           _vp, _vp, _vp, _vp, C.c_double, C.c_double, _vp,
+                                           _vp]),
     "tpf_fields_bubbles": (C.c_int, [C.c_int, _i64p, _dp, C.c_int, _dp, _dp, C.c_double,
                                      C.c_double, C.c_double, _vp, _vp, _vp]),
     "tpf_fields_mass": (C.c_int, [C.c_int, _i64p, _dp, _ip, _vp, _dp, _vp]),
