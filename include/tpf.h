@@ -295,6 +295,36 @@ TPF_API tpf_status tpf_rhie_chow_divergence(int ndim, const int64_t* n_nodes, co
                                             const double* dpsi, double we, double dt,
                                             double* out, void* stream);
 
+/* Dissipative-CCD level-set advection (DissipativeCCDAdvection,
+ * levelset/advection_dccd.py:20-152) in three calls per TVD-RK3 stage
+ * (time_integration/tvd_rk3.py:26-48); `periodic[a]` = the level-set
+ * boundary of axis a is periodic (else Neumann / wall: end nodes unfiltered).
+ * tpf_dccd_flux: f[a] = q * vel[a], f stacked [ndim][n_total]; vel is a host
+ * array of ndim device pointers. */
+TPF_API tpf_status tpf_dccd_flux(int ndim, const int64_t* n_nodes, const double* q,
+                                 const double* const* vel, double* f, void* stream);
+/* out = clip(ca q0 + cb (q + dt L(q)), lo, hi) (clip = 0: no clip), with
+ * L(q) = -sum_a [fp_a + eps_d (2q-1)^2 (fp_a(+1) - 2 fp_a + fp_a(-1))]
+ * (levelset/advection_kernels.py:31-34, periodic ghosts core/boundary.py:
+ * 273-276), fp = the CCD first derivatives of f stacked like f; periodic
+ * image nodes take their roots' values (core/boundary.py:116-133).  out must
+ * not alias q or q0. */
+TPF_API tpf_status tpf_dccd_stage(int ndim, const int64_t* n_nodes, const int* periodic,
+                                  const double* q0, const double* q, const double* fp,
+                                  double eps_d, double dt, double ca, double cb, int clip,
+                                  double lo, double hi, double* out, void* stream);
+/* out = in with every periodic image node copied from its root (in == out
+ * allowed; core/boundary.py:116-133). */
+TPF_API tpf_status tpf_sync_periodic_images(int ndim, const int64_t* n_nodes,
+                                            const int* periodic, const double* in, double* out,
+                                            void* stream);
+/* Interface-weighted mass correction to m_target (levelset/heaviside.py:
+ * 118-149) with Grid.cell_volumes weights, then the image copies; synchronous
+ * (two FP64 reductions).  mass_before (may be NULL) receives sum q dV. */
+TPF_API tpf_status tpf_dccd_mass_correct(int ndim, const int64_t* n_nodes, const double* L,
+                                         const int* periodic, const double* q, double m_target,
+                                         double* out, double* mass_before, void* stream);
+
 /* ------------------------------------------------------------- fields --- */
 /* Replaces InitialConditionBuilder.build (gas spheres in liquid) +
  * update_properties (simulation/initial_conditions/builder.py:100-172,

@@ -26,6 +26,9 @@ function against something other than its own formula):
 * ``rhie_chow.OracleRhieChow``    -- closed forms: linear velocity, cubic
   pressure bracket (interior and wall one-sided values), and the
   balanced-force equilibrium p = kappa psi / We.
+* ``dccd.OracleDCCD``             -- the TVD-RK3 amplification polynomial of a
+  Fourier mode, the filter's transfer factor, unfiltered wall ends, and
+  mass restoration by the correction.
 
 No function here is "parity unpinned" except the GMRES/CG iteration counts,
 which the reference does not fix (SciPy internals).

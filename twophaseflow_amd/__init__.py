@@ -16,13 +16,15 @@ from .ccd import CCDBatch, CCDSolver
 from .fccd import FCCDFaceJet, FCCDSolver
 from .defect_correction import PPESolverDefectCorrection
 from .rhie_chow import RhieChowInterpolator
+from .dccd import DissipativeCCDAdvection
 from .ppe import (IPPESolver, MatrixAssemblyUnavailable, PPESolverFCCDMatrixFree,
                   PPESolverFVMMatrixFree, create_ppe_solver, register_ppe_solver)
 
 __all__ = [
     "GridConfig", "NumericsConfig", "SimulationConfig", "SolverConfig", "Grid",
     "BoundarySpec", "axis_kinds", "pin_dof", "CCDSolver", "CCDBatch", "FCCDSolver",
-    "FCCDFaceJet", "PPESolverDefectCorrection", "RhieChowInterpolator", "IPPESolver",
+    "FCCDFaceJet", "PPESolverDefectCorrection", "RhieChowInterpolator",
+    "DissipativeCCDAdvection", "IPPESolver",
     "MatrixAssemblyUnavailable", "PPESolverFVMMatrixFree", "PPESolverFCCDMatrixFree",
     "create_ppe_solver",
     "register_ppe_solver",

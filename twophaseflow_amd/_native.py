@@ -87,6 +87,13 @@ SIGNATURES = {
     "tpf_rhie_chow_divergence": (C.c_int, [C.c_int, _i64p, _dp, _ip, C.POINTER(_vp), _vp, _vp,
                                            _vp, _vp, _vp, _vp, C.c_double, C.c_double, _vp,
                                            _vp]),
+    "tpf_dccd_flux": (C.c_int, [C.c_int, _i64p, _vp, C.POINTER(_vp), _vp, _vp]),
+    "tpf_dccd_stage": (C.c_int, [C.c_int, _i64p, _ip, _vp, _vp, _vp, C.c_double, C.c_double,
+                                 C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, _vp,
+                                 _vp]),
+    "tpf_sync_periodic_images": (C.c_int, [C.c_int, _i64p, _ip, _vp, _vp, _vp]),
+    "tpf_dccd_mass_correct": (C.c_int, [C.c_int, _i64p, _dp, _ip, _vp, C.c_double, _vp, _dp,
+                                        _vp]),
     "tpf_fields_bubbles": (C.c_int, [C.c_int, _i64p, _dp, C.c_int, _dp, _dp, C.c_double,
                                      C.c_double, C.c_double, _vp, _vp, _vp]),
     "tpf_fields_mass": (C.c_int, [C.c_int, _i64p, _dp, _ip, _vp, _dp, _vp]),
