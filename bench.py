@@ -533,6 +533,30 @@ def dccd_bench(reps=10):
             "Mpts_steps_per_s": round(psi.numel() / (ms * 1e-3) / 1e6, 1)}
 
 
+def curvature_bench(reps=10):
+    """CurvatureCalculatorPsi on the 257^3 periodic bubble field (64 bubbles),
+    DCCD filter on (eps 0.05): CCD d1/d2, 6 filtered fields, 3 mixed CCD
+    derivatives, the curvature kernel."""
+    import torch
+    from twophaseflow_amd import CCDSolver, CurvatureCalculatorPsi, Grid, GridConfig, fields
+    N = 256
+    g = Grid(GridConfig(ndim=3, N=(N,) * 3, L=(1.0,) * 3))
+    c, r = fields.random_bubbles(64, seed=3, ndim=3)
+    psi, _ = fields.bubbles_psi_rho(g.N, g.L, c, r, 1.5 / N, 1.0, 1e-3)
+    calc = CurvatureCalculatorPsi(None, CCDSolver(g, bc_type="periodic"))
+    calc.compute(psi)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for a, b in ev:
+        a.record()
+        calc.compute(psi)
+        b.record()
+    torch.cuda.synchronize()
+    ms = sorted(a.elapsed_time(b) for a, b in ev)[reps // 2]
+    return {"ms": round(ms, 4), "timing": "median of per-call HIP events (incl. the call's "
+            "allocations)"}
+
+
 def profile_mode(args):
     import torch
     from twophaseflow_amd import CCDSolver, Grid, GridConfig
@@ -775,6 +799,9 @@ def main():
         result["ppe"]["gmres_line_pcr_65"] = ppe_solve_timed(gq, rhoq, rhsq, "fp64", maxiter=500,
                                                              method="gmres")
         del rhoq, rhsq
+        log("psi curvature, 257^3 periodic")
+        result["curvature_psi_257"] = curvature_bench()
+        log(f"  {result['curvature_psi_257']}")
         log("DCCD level-set TVD-RK3 step, 257^3 periodic")
         result["dccd_advection_257"] = dccd_bench()
         log(f"  {result['dccd_advection_257']}")

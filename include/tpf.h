@@ -325,6 +325,26 @@ TPF_API tpf_status tpf_dccd_mass_correct(int ndim, const int64_t* n_nodes, const
                                          const int* periodic, const double* q, double m_target,
                                          double* out, double* mass_before, void* stream);
 
+/* Curvature straight from psi (CurvatureCalculatorPsi,
+ * levelset/curvature_psi.py:45-196).  tpf_dccd_filter_nd: the DCCD filter of
+ * levelset/curvature.py:60-128 on f (successive 3-point passes with
+ * coefficient eps (2 psi - 1)^2 along every axis, periodic axes over the
+ * unique nodes, then the wall boundary planes restored from f); out and
+ * scratch are distinct n_total arrays. */
+TPF_API tpf_status tpf_dccd_filter_nd(int ndim, const int64_t* n_nodes, const int* periodic,
+                                      const double* psi, double eps, const double* f,
+                                      double* out, double* scratch, void* stream);
+/* kappa from the stacked CCD derivatives d1, d2 [ndim][n_total] and the mixed
+ * derivatives (2-D: [psi_xy]; 3-D: [xy, xz, yz]); 0 where psi <= psi_min or
+ * psi >= 1 - psi_min.  2-D or 3-D. */
+TPF_API tpf_status tpf_curvature_psi(int ndim, const int64_t* n_nodes, const double* psi,
+                                     const double* d1, const double* d2, const double* mixed,
+                                     double psi_min, double* out, void* stream);
+/* 3x3 Gaussian [1 2 1; 2 4 2; 1 2 1]/16 with edge replication on a 2-D field
+ * (curvature_psi.py:171-196); in and out distinct. */
+TPF_API tpf_status tpf_gaussian_3x3(const int64_t* n_nodes, const double* in, double* out,
+                                    void* stream);
+
 /* ------------------------------------------------------------- fields --- */
 /* Replaces InitialConditionBuilder.build (gas spheres in liquid) +
  * update_properties (simulation/initial_conditions/builder.py:100-172,

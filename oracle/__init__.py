@@ -29,6 +29,9 @@ function against something other than its own formula):
 * ``dccd.OracleDCCD``             -- the TVD-RK3 amplification polynomial of a
   Fourier mode, the filter's transfer factor, unfiltered wall ends, and
   mass restoration by the correction.
+* ``curvature.OracleCurvaturePsi`` -- exact -1/r and -2/r on circle / sphere
+  level sets of quadratic psi, the rotated-ellipse closed form (mixed term),
+  and the filter's action on linear fields and wall planes.
 
 No function here is "parity unpinned" except the GMRES/CG iteration counts,
 which the reference does not fix (SciPy internals).

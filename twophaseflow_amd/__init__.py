@@ -17,6 +17,7 @@ from .fccd import FCCDFaceJet, FCCDSolver
 from .defect_correction import PPESolverDefectCorrection
 from .rhie_chow import RhieChowInterpolator
 from .dccd import DissipativeCCDAdvection
+from .curvature import CurvatureCalculatorPsi
 from .ppe import (IPPESolver, MatrixAssemblyUnavailable, PPESolverFCCDMatrixFree,
                   PPESolverFVMMatrixFree, create_ppe_solver, register_ppe_solver)
 
@@ -24,7 +25,7 @@ __all__ = [
     "GridConfig", "NumericsConfig", "SimulationConfig", "SolverConfig", "Grid",
     "BoundarySpec", "axis_kinds", "pin_dof", "CCDSolver", "CCDBatch", "FCCDSolver",
     "FCCDFaceJet", "PPESolverDefectCorrection", "RhieChowInterpolator",
-    "DissipativeCCDAdvection", "IPPESolver",
+    "DissipativeCCDAdvection", "CurvatureCalculatorPsi", "IPPESolver",
     "MatrixAssemblyUnavailable", "PPESolverFVMMatrixFree", "PPESolverFCCDMatrixFree",
     "create_ppe_solver",
     "register_ppe_solver",

@@ -94,6 +94,10 @@ SIGNATURES = {
     "tpf_sync_periodic_images": (C.c_int, [C.c_int, _i64p, _ip, _vp, _vp, _vp]),
     "tpf_dccd_mass_correct": (C.c_int, [C.c_int, _i64p, _dp, _ip, _vp, C.c_double, _vp, _dp,
                                         _vp]),
+    "tpf_dccd_filter_nd": (C.c_int, [C.c_int, _i64p, _ip, _vp, C.c_double, _vp, _vp, _vp,
+                                     _vp]),
+    "tpf_curvature_psi": (C.c_int, [C.c_int, _i64p, _vp, _vp, _vp, _vp, C.c_double, _vp, _vp]),
+    "tpf_gaussian_3x3": (C.c_int, [_i64p, _vp, _vp, _vp]),
     "tpf_fields_bubbles": (C.c_int, [C.c_int, _i64p, _dp, C.c_int, _dp, _dp, C.c_double,
                                      C.c_double, C.c_double, _vp, _vp, _vp]),
     "tpf_fields_mass": (C.c_int, [C.c_int, _i64p, _dp, _ip, _vp, _dp, _vp]),
