@@ -557,6 +557,28 @@ def curvature_bench(reps=10):
             "allocations)"}
 
 
+def reinit_bench(reps=5):
+    """SplitReinitializer (4 pseudo-steps, mass correction) on the 257^3 wall
+    bubble field, eps = 1.5 h."""
+    import torch
+    from twophaseflow_amd import CCDSolver, Grid, GridConfig, SplitReinitializer, fields
+    N = 256
+    g = Grid(GridConfig(ndim=3, N=(N,) * 3, L=(1.0,) * 3))
+    c, r = fields.random_bubbles(64, seed=3, ndim=3)
+    psi, _ = fields.bubbles_psi_rho(g.N, g.L, c, r, 1.5 / N, 1.0, 1e-3)
+    ri = SplitReinitializer(None, g, CCDSolver(g, bc_type="wall"), 1.5 / N)
+    ri.reinitialize(psi)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for a, b in ev:
+        a.record()
+        ri.reinitialize(psi)
+        b.record()
+    torch.cuda.synchronize()
+    ms = sorted(a.elapsed_time(b) for a, b in ev)[reps // 2]
+    return {"ms": round(ms, 3), "pseudo_steps": 4, "timing": "median of per-call HIP events"}
+
+
 def profile_mode(args):
     import torch
     from twophaseflow_amd import CCDSolver, Grid, GridConfig
@@ -799,6 +821,9 @@ def main():
         result["ppe"]["gmres_line_pcr_65"] = ppe_solve_timed(gq, rhoq, rhsq, "fp64", maxiter=500,
                                                              method="gmres")
         del rhoq, rhsq
+        log("split reinitialisation, 257^3 wall")
+        result["reinit_split_257"] = reinit_bench()
+        log(f"  {result['reinit_split_257']}")
         log("psi curvature, 257^3 periodic")
         result["curvature_psi_257"] = curvature_bench()
         log(f"  {result['curvature_psi_257']}")

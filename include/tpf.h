@@ -345,6 +345,23 @@ TPF_API tpf_status tpf_curvature_psi(int ndim, const int64_t* n_nodes, const dou
 TPF_API tpf_status tpf_gaussian_3x3(const int64_t* n_nodes, const double* in, double* out,
                                     void* stream);
 
+/* Operator-split CLS reinitialisation (SplitReinitializer,
+ * levelset/reinit_split.py:22-102, levelset/reinit_ops.py:36-204): n_steps
+ * pseudo-time steps of DCCD-filtered compression (eps_d_comp) and CN-ADI
+ * diffusion, each clipped to [0, 1], then (mass_correction != 0) the
+ * interface-weighted mass correction.  ccd: a plan of this grid (its own
+ * wall / periodic closures give the derivatives); bc[a]: the level-set
+ * boundary of axis a, TPF_REINIT_BC_* (ghosts of core/boundary.py:249-293;
+ * periodic axes also synchronise their image nodes).  psi and out distinct;
+ * ordered on `stream` (two short host waits for the mass sums when
+ * mass_correction is set). */
+enum { TPF_REINIT_BC_ZERO = 0, TPF_REINIT_BC_PERIODIC = 1, TPF_REINIT_BC_NEUMANN = 2,
+       TPF_REINIT_BC_OUTFLOW = 3 };
+TPF_API tpf_status tpf_reinit_split(tpf_ccd_plan* ccd, int ndim, const int64_t* n_nodes,
+                                    const double* L, const int* bc, double eps, int n_steps,
+                                    double eps_d_comp, int mass_correction, const double* psi,
+                                    double* out, void* stream);
+
 /* ------------------------------------------------------------- fields --- */
 /* Replaces InitialConditionBuilder.build (gas spheres in liquid) +
  * update_properties (simulation/initial_conditions/builder.py:100-172,

@@ -32,6 +32,9 @@ function against something other than its own formula):
 * ``curvature.OracleCurvaturePsi`` -- exact -1/r and -2/r on circle / sphere
   level sets of quadratic psi, the rotated-ellipse closed form (mixed term),
   and the filter's action on linear fields and wall planes.
+* ``reinit.OracleSplitReinit``    -- the CN-ADI closed-form factor on cosine
+  modes, the pseudo-time step formula, mass restoration and the tanh profile
+  as a near fixed point.
 
 No function here is "parity unpinned" except the GMRES/CG iteration counts,
 which the reference does not fix (SciPy internals).

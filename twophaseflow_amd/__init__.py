@@ -18,6 +18,7 @@ from .defect_correction import PPESolverDefectCorrection
 from .rhie_chow import RhieChowInterpolator
 from .dccd import DissipativeCCDAdvection
 from .curvature import CurvatureCalculatorPsi
+from .reinit import SplitReinitializer
 from .ppe import (IPPESolver, MatrixAssemblyUnavailable, PPESolverFCCDMatrixFree,
                   PPESolverFVMMatrixFree, create_ppe_solver, register_ppe_solver)
 
@@ -25,7 +26,7 @@ __all__ = [
     "GridConfig", "NumericsConfig", "SimulationConfig", "SolverConfig", "Grid",
     "BoundarySpec", "axis_kinds", "pin_dof", "CCDSolver", "CCDBatch", "FCCDSolver",
     "FCCDFaceJet", "PPESolverDefectCorrection", "RhieChowInterpolator",
-    "DissipativeCCDAdvection", "CurvatureCalculatorPsi", "IPPESolver",
+    "DissipativeCCDAdvection", "CurvatureCalculatorPsi", "SplitReinitializer", "IPPESolver",
     "MatrixAssemblyUnavailable", "PPESolverFVMMatrixFree", "PPESolverFCCDMatrixFree",
     "create_ppe_solver",
     "register_ppe_solver",

@@ -1,0 +1,289 @@
+// Operator-split CLS reinitialisation (SplitReinitializer,
+// levelset/reinit_split.py:22-102; levelset/reinit_ops.py:36-204):
+// n_steps pseudo-time steps of
+//   compression  q* = clip(q - dtau div[q (1 - q) n], 0, 1),  n = grad q / max(sqrt(max(|grad q|^2, 1e-12)), 1e-6),
+//                div per axis = the CCD derivative of the flux + eps_d (g+ - 2 g + g-) with
+//                the level-set boundary's ghosts (core/boundary.py:249-293);
+//   diffusion    per axis (M2 - mu B2) q_new = (M2 + mu B2) q, M2 = tridiag(-1/8, 1, -1/8),
+//                B2 = 3 tridiag(1, -2, 1), mirrored end rows, mu = eps dtau / (2 h^2);
+//   then clip to [0, 1]; periodic images follow their roots after every stage;
+// then the interface-weighted mass correction (levelset/heaviside.py:118-149).
+// dtau = min(dx^2 / (4 ndim eps), dx / 2) (reinit_ops.py:36-51).
+//
+// The host loop below drives the CCD plan (gradient and flux derivatives)
+// and three kernels per step: the compression flux, the filtered divergence
+// with the clipped update, and one Thomas sweep per line and axis (one
+// thread per line, constant factors computed once on the host as the
+// reference's CPU path does).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "reduce.hpp"
+#include "tpf_common.hpp"
+
+extern "C" tpf_status tpf_sync_periodic_images(int ndim, const int64_t* n_nodes,
+                                               const int* periodic, const double* in, double* out,
+                                               void* stream);
+extern "C" tpf_status tpf_dccd_mass_correct(int ndim, const int64_t* n_nodes, const double* L,
+                                            const int* periodic, const double* q,
+                                            double m_target, double* out, double* mass_before,
+                                            void* stream);
+extern "C" tpf_status tpf_fields_mass(int ndim, const int64_t* n_nodes, const double* L,
+                                      const int* bc, const double* q, double* mass, void* stream);
+
+namespace tpf {
+namespace {
+
+constexpr double kBeta2 = -1.0 / 8.0;  // M2 off-diagonal (ccd_solver.py:74)
+
+struct RGeom {
+  int n[3];
+  int bc[3];  // TPF_REINIT_BC_* of the slot
+  int k[3];
+  int64_t total;
+};
+
+__device__ __forceinline__ int64_t stride_of(const RGeom& g, int a) {
+  return a == 0 ? (int64_t)g.n[1] * g.n[2] : (a == 1 ? (int64_t)g.n[2] : 1);
+}
+
+__device__ __forceinline__ void coords(const RGeom& g, int64_t idx, int (&c)[3]) {
+  const uint32_t plane = (uint32_t)(g.n[1] * g.n[2]);
+  c[0] = (int)((uint32_t)idx / plane);
+  const uint32_t rem = (uint32_t)idx - (uint32_t)c[0] * plane;
+  c[1] = (int)(rem / (uint32_t)g.n[2]);
+  c[2] = (int)(rem - (uint32_t)c[1] * (uint32_t)g.n[2]);
+}
+
+__device__ __forceinline__ int64_t root_of(const RGeom& g, int64_t idx, int (&c)[3]) {
+  int64_t r = idx;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (g.bc[a] == TPF_REINIT_BC_PERIODIC && g.n[a] > 1 && c[a] == g.n[a] - 1) {
+      r -= (int64_t)c[a] * stride_of(g, a);
+      c[a] = 0;
+    }
+  return r;
+}
+
+// flux_a = q (1 - q) g_a / safe_grad (reinit_ops.py:54-75, 106-114)
+__global__ __launch_bounds__(kBlock) void k_reinit_flux(RGeom g, int ndim,
+                                                        const double* __restrict__ q,
+                                                        const double* __restrict__ d1,
+                                                        double* __restrict__ flux) {
+  const int64_t T = g.total;
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < T;
+       idx += (int64_t)gridDim.x * kBlock) {
+    double gs = 0.0;
+    for (int a = 0; a < ndim; ++a) gs += d1[a * T + idx] * d1[a * T + idx];
+    const double safe = fmax(sqrt(fmax(gs, 1e-12)), 1e-6);
+    const double w = q[idx] * (1.0 - q[idx]);
+    for (int a = 0; a < ndim; ++a) flux[a * T + idx] = w * (d1[a * T + idx] / safe);
+  }
+}
+
+// out = clip(q - dtau sum_a filtered(fp_a), 0, 1) at the root of every node
+__global__ __launch_bounds__(kBlock) void k_reinit_compress(RGeom g, const double* __restrict__ q,
+                                                            const double* __restrict__ fp,
+                                                            double eps_d, double dtau,
+                                                            double* __restrict__ out) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int c[3];
+    coords(g, idx, c);
+    const int64_t r = root_of(g, idx, c);
+    double div = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (g.n[a] == 1) continue;
+      const int N = g.n[a] - 1;
+      const int i = c[a];
+      const int64_t st = stride_of(g, a);
+      const double* fa = fp + (int64_t)g.k[a] * g.total;
+      const double f0 = fa[r];
+      double fm, fpl;
+      if (i > 0) fm = fa[r - st];
+      else if (g.bc[a] == TPF_REINIT_BC_PERIODIC) fm = fa[r + (int64_t)(N - 1) * st];
+      else if (g.bc[a] == TPF_REINIT_BC_ZERO) fm = 0.0;
+      else fm = f0;  // neumann (mirror) and outflow (repeat) with one ghost
+      if (i < N) fpl = fa[r + st];
+      else if (g.bc[a] == TPF_REINIT_BC_PERIODIC) fpl = fa[r - (int64_t)(N - 1) * st];
+      else if (g.bc[a] == TPF_REINIT_BC_ZERO) fpl = 0.0;
+      else fpl = f0;
+      div = div + (f0 + eps_d * (fpl - 2.0 * f0 + fm));
+    }
+    out[idx] = fmin(fmax(q[r] - dtau * div, 0.0), 1.0);
+  }
+}
+
+// One CN-ADI sweep along slot a: rhs from q, forward elimination into out,
+// back substitution in place (reinit_ops.py:164-204, CPU Thomas path).
+__global__ __launch_bounds__(kBlock) void k_reinit_cn(RGeom g, int a, double d_R, double c_R,
+                                                      const double* __restrict__ tf,
+                                                      const double* __restrict__ m,
+                                                      const double* __restrict__ sup,
+                                                      const double* __restrict__ q,
+                                                      double* __restrict__ out) {
+  const int n = g.n[a];
+  const int64_t st = stride_of(g, a);
+  const int64_t nlines = g.total / n;
+  for (int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x; l < nlines;
+       l += (int64_t)gridDim.x * kBlock) {
+    int64_t base;
+    if (a == 0) base = l;
+    else if (a == 1) base = (l / g.n[2]) * ((int64_t)g.n[1] * g.n[2]) + (l % g.n[2]);
+    else base = l * n;
+    double dprev = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const int64_t o = base + (int64_t)i * st;
+      double rhs;
+      if (i == 0) rhs = d_R * q[o] + 2.0 * c_R * q[o + st];
+      else if (i == n - 1) rhs = 2.0 * c_R * q[o - st] + d_R * q[o];
+      else rhs = c_R * q[o - st] + d_R * q[o] + c_R * q[o + st];
+      const double d = i == 0 ? rhs : rhs - tf[i - 1] * dprev;
+      out[o] = d;
+      dprev = d;
+    }
+    double x = 0.0;
+    for (int i = n - 1; i >= 0; --i) {
+      const int64_t o = base + (int64_t)i * st;
+      x = i == n - 1 ? out[o] / m[n - 1] : (out[o] - sup[i] * x) / m[i];
+      out[o] = x;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_clip01(int64_t n, const double* in, double* out) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    out[i] = fmin(fmax(in[i], 0.0), 1.0);
+}
+
+unsigned blocks_for(int64_t total) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + kBlock - 1) / kBlock, 2048));
+}
+
+}  // namespace
+}  // namespace tpf
+
+using namespace tpf;
+
+extern "C" tpf_status tpf_reinit_split(tpf_ccd_plan* ccd, int ndim, const int64_t* n_nodes,
+                                       const double* L, const int* bc, double eps, int n_steps,
+                                       double eps_d_comp, int mass_correction,
+                                       const double* psi, double* out, void* stream) {
+  if (!ccd || !n_nodes || !L || !bc || !psi || !out) return fail(TPF_ERR_VALUE, "null argument");
+  if (ndim < 1 || ndim > 3) return fail(TPF_ERR_VALUE, "ndim must be 1, 2 or 3");
+  if (!(eps > 0.0) || n_steps < 0) return fail(TPF_ERR_VALUE, "need eps > 0 and n_steps >= 0");
+  if (out == psi) return fail(TPF_ERR_VALUE, "psi and out must be distinct arrays");
+  RGeom g{};
+  for (int a = 0; a < 3; ++a) g.n[a] = 1;
+  int periodic[3] = {0, 0, 0};
+  double h[3] = {1.0, 1.0, 1.0};
+  for (int a = 0; a < ndim; ++a) {
+    const int s = 3 - ndim + a;
+    if (n_nodes[a] < 3 || !(L[a] > 0.0)) return fail(TPF_ERR_VALUE, "axis needs >= 3 nodes, L > 0");
+    if (bc[a] < TPF_REINIT_BC_ZERO || bc[a] > TPF_REINIT_BC_OUTFLOW)
+      return fail(TPF_ERR_VALUE, "level-set bc must be zero, periodic, neumann or outflow");
+    g.n[s] = (int)n_nodes[a];
+    g.bc[s] = bc[a];
+    g.k[s] = a;
+    periodic[a] = bc[a] == TPF_REINIT_BC_PERIODIC;
+    h[a] = L[a] / (double)(n_nodes[a] - 1);
+  }
+  g.total = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  if (g.total >= (int64_t)0xffffffffLL) return fail(TPF_ERR_VALUE, "grid too large (2^32 nodes)");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t T = g.total;
+  // dtau (reinit_ops.py:36-51) and the CN factors per axis (:117-161)
+  double dx = h[0];
+  for (int a = 1; a < ndim; ++a) dx = std::min(dx, h[a]);
+  const double dtau = std::min(0.5 * dx * dx / (2.0 * ndim * eps), 0.5 * dx);
+  std::vector<double> hostf;
+  std::vector<int64_t> off_tf(ndim), off_m(ndim), off_sup(ndim);
+  std::vector<double> dR(ndim), cR(ndim);
+  for (int a = 0; a < ndim; ++a) {
+    const int n = (int)n_nodes[a];
+    const double mu = eps * dtau / (2.0 * h[a] * h[a]);
+    const double dL = 1.0 + 6.0 * mu, cL = kBeta2 - 3.0 * mu;
+    dR[a] = 1.0 - 6.0 * mu;
+    cR[a] = kBeta2 + 3.0 * mu;
+    std::vector<double> sup(n - 1, cL), sub(n - 1, cL), m(n, dL), tf(n - 1);
+    sup[0] = 2.0 * cL;
+    sub[n - 2] = 2.0 * cL;
+    for (int i = 1; i < n; ++i) {
+      tf[i - 1] = sub[i - 1] / m[i - 1];
+      m[i] -= tf[i - 1] * sup[i - 1];
+    }
+    off_tf[a] = (int64_t)hostf.size();
+    hostf.insert(hostf.end(), tf.begin(), tf.end());
+    off_m[a] = (int64_t)hostf.size();
+    hostf.insert(hostf.end(), m.begin(), m.end());
+    off_sup[a] = (int64_t)hostf.size();
+    hostf.insert(hostf.end(), sup.begin(), sup.end());
+  }
+  double* buf = nullptr;
+  const size_t nbuf = (size_t)T * (2 + 3 * (size_t)ndim) + hostf.size();
+  TPF_HIP(hipMallocAsync(&buf, sizeof(double) * nbuf, st));
+  double* q = buf;
+  double* qs = q + T;
+  double* d1 = qs + T;
+  double* flux = d1 + (int64_t)ndim * T;
+  double* fpd = flux + (int64_t)ndim * T;
+  double* fac = fpd + (int64_t)ndim * T;
+  auto run = [&]() -> tpf_status {
+    TPF_HIP(hipMemcpyAsync(fac, hostf.data(), sizeof(double) * hostf.size(),
+                           hipMemcpyHostToDevice, st));
+    TPF_HIP(hipMemcpyAsync(q, psi, sizeof(double) * T, hipMemcpyDeviceToDevice, st));
+    TPF_TRY(tpf_sync_periodic_images(ndim, n_nodes, periodic, q, q, st));
+    double m_old = 0.0;
+    if (mass_correction) {
+      int bcs[3];
+      for (int a = 0; a < ndim; ++a) bcs[a] = periodic[a] ? TPF_BC_PERIODIC : TPF_BC_WALL;
+      TPF_TRY(tpf_fields_mass(ndim, n_nodes, L, bcs, q, &m_old, st));
+    }
+    const uint32_t mask = (1u << ndim) - 1u;
+    for (int step = 0; step < n_steps; ++step) {
+      TPF_TRY(tpf_ccd_apply(ccd, q, d1, nullptr, mask, st));
+      hipLaunchKernelGGL(k_reinit_flux, dim3(blocks_for(T)), dim3(kBlock), 0, st, g, ndim,
+                         (const double*)q, (const double*)d1, flux);
+      TPF_HIP(hipGetLastError());
+      for (int a = 0; a < ndim; ++a)
+        TPF_TRY(tpf_ccd_differentiate(ccd, flux + (int64_t)a * T, fpd + (int64_t)a * T, nullptr,
+                                      a, st));
+      hipLaunchKernelGGL(k_reinit_compress, dim3(blocks_for(T)), dim3(kBlock), 0, st, g,
+                         (const double*)q, (const double*)fpd, eps_d_comp, dtau, qs);
+      TPF_HIP(hipGetLastError());
+      // CN-ADI sweeps, ping-pong qs <-> q, images after every sweep
+      double* src = qs;
+      double* dst = q;
+      for (int a = 0; a < ndim; ++a) {
+        const int s = 3 - ndim + a;
+        const int64_t nlines = T / g.n[s];
+        hipLaunchKernelGGL(k_reinit_cn, dim3(blocks_for(nlines)), dim3(kBlock), 0, st, g, s,
+                           dR[a], cR[a], (const double*)(fac + off_tf[a]),
+                           (const double*)(fac + off_m[a]), (const double*)(fac + off_sup[a]),
+                           (const double*)src, dst);
+        TPF_HIP(hipGetLastError());
+        TPF_TRY(tpf_sync_periodic_images(ndim, n_nodes, periodic, dst, dst, st));
+        std::swap(src, dst);
+      }
+      hipLaunchKernelGGL(k_clip01, dim3(blocks_for(T)), dim3(kBlock), 0, st, T,
+                         (const double*)src, q);
+      TPF_HIP(hipGetLastError());
+      TPF_TRY(tpf_sync_periodic_images(ndim, n_nodes, periodic, q, q, st));
+    }
+    if (mass_correction)
+      TPF_TRY(tpf_dccd_mass_correct(ndim, n_nodes, L, periodic, q, m_old, out, nullptr, st));
+    else
+      TPF_HIP(hipMemcpyAsync(out, q, sizeof(double) * T, hipMemcpyDeviceToDevice, st));
+    return TPF_OK;
+  };
+  const tpf_status s = run();
+  (void)hipFreeAsync(buf, st);
+  return s;
+}
