@@ -9,10 +9,11 @@
 //   dgrad   : the forward kernel with W2[tap][ci][co] = Wt[26-tap][co][ci]
 //   wgrad   : dW[tap][co][ci] += sum_v dY[v][co] X[v+off(tap)][ci]   (FP32, atomic)
 //
-// MFMA v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)+j],
-// B[4(l/16)+j][l%16] and D[4(l/16)+i][l%16].  The forward GEMM is
-// M = 16 voxels along x, N = 16 output channels per tile, K = 27 x Cin,
-// operands staged in LDS.
+// The forward GEMM is M = 16 voxels along x, N = 16 output channels per
+// tile, K = 27 x Cin.  Up to 64 input channels it runs on gfx950's
+// v_mfma_f32_16x16x32_bf16 (conv_fwd_k32); wider inputs use
+// v_mfma_f32_16x16x16_bf16, whose lane l holds A[l%16][4(l/16)+j],
+// B[4(l/16)+j][l%16] and D[4(l/16)+i][l%16].
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -177,6 +178,143 @@ __global__ __launch_bounds__(64 * kWaves) void conv_fwd(const uint16_t* __restri
   }
 }
 
+// Forward convolution on gfx950's v_mfma_f32_16x16x32_bf16 (K = 32 per
+// instruction, twice the 16x16x16 form's rate).  A workgroup of 8 wavefronts
+// computes 16 consecutive y rows (two per wavefront) x 16 voxels along x x
+// all Cout channels.  Per (dz, dy) the sixteen input rows' 18 voxels and the
+// three taps' (dx = -1, 0, 1) Cout x Cin weight blocks are staged in LDS
+// together with 16 B loads, so a workgroup synchronises twice per (dz, dy),
+// a staged weight block serves 256 output voxels and every weight operand
+// read from LDS feeds both of a wavefront's rows.  Channels are zero-padded
+// in LDS to Kp = 32 or 64 (a template parameter, so the staging index
+// arithmetic is constant); the next (dz, dy) stage is loaded into registers
+// while the current one computes; row pitches are Kp + 8 BF16 (16 B aligned rows for
+// the 16 B operand reads).  Operand lane layout: lane l holds
+// A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j < 8; the result
+// layout is the 16x16x16 form's.
+typedef __bf16 v8b __attribute__((ext_vector_type(8)));
+constexpr int kWaves32 = 8;                        // wavefronts per workgroup
+constexpr int kRowsPerWave32 = 2;
+constexpr int kRows32 = kWaves32 * kRowsPerWave32;  // y rows per workgroup
+constexpr int kThreads32 = 64 * kWaves32;
+
+
+template <int COT, int KPW>
+__global__ __launch_bounds__(kThreads32) void conv_fwd_k32(const uint16_t* __restrict__ X,
+                                                          const uint16_t* __restrict__ Wt,
+                                                          const float* __restrict__ bias,
+                                                          void* __restrict__ Y, int D, int H,
+                                                          int W, int Cin, int relu, int out_f32) {
+  constexpr int Cout = COT * 16;
+  constexpr int Kp = 32 * KPW;        // padded channels (power of two)
+  constexpr int pitch = Kp + 8;
+  constexpr int kChunks = Kp / 8;     // 16 B chunks per padded voxel
+  constexpr int kStep = kThreads32 / kChunks;  // slots staged per pass
+  constexpr int kAIt = (kRows32 * 18 + kStep - 1) / kStep;
+  constexpr int kBIt = (3 * Cout + kStep - 1) / kStep;
+  extern __shared__ uint16_t sm[];
+  uint16_t* As = sm;                          // [kRows32][18 voxels][pitch]
+  uint16_t* Bs = sm + kRows32 * 18 * pitch;   // [3 taps][Cout][pitch]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int xt = (W + 15) >> 4, yb = (H + kRows32 - 1) / kRows32;
+  const int blk = blockIdx.x;
+  const int z = blk / (yb * xt);
+  const int rem = blk - z * yb * xt;
+  const int y0 = (rem / xt) * kRows32;
+  const int x0 = (rem - (rem / xt) * xt) * 16;
+  const int row = lane & 15, kq = lane >> 4;
+  // staging map: 16 B chunk `ch` of slots tid / kChunks + i * kStep
+  const int c = (tid % kChunks) * 8;
+  const int slot0 = tid / kChunks;
+  const bool cin_ok = c < Cin;
+  // The next (dz, dy) stage's operands are loaded into registers while the
+  // current one computes, and written to LDS after the next barrier.
+  uint4 ra[kAIt], rb[kBIt];
+  auto load_stage = [&](int dzy) {
+    const int dz = dzy / 3 - 1, dy = dzy % 3 - 1;
+    const int zz = z + dz;
+    const bool zok = zz >= 0 && zz < D;
+#pragma unroll
+    for (int i = 0; i < kAIt; ++i) {
+      const int sl = slot0 + i * kStep;
+      const int r = sl / 18, v = sl - r * 18;
+      const int yy = y0 + r + dy, xx = x0 - 1 + v;
+      ra[i] = make_uint4(0u, 0u, 0u, 0u);
+      if (sl < kRows32 * 18 && cin_ok && zok && yy >= 0 && yy < H && xx >= 0 && xx < W)
+        ra[i] = *reinterpret_cast<const uint4*>(X + (((int64_t)zz * H + yy) * W + xx) * Cin + c);
+    }
+    const int tap0 = (dz + 1) * 9 + (dy + 1) * 3;
+#pragma unroll
+    for (int i = 0; i < kBIt; ++i) {  // slot = d * Cout + co
+      const int sl = slot0 + i * kStep;
+      rb[i] = make_uint4(0u, 0u, 0u, 0u);
+      if (sl < 3 * Cout && cin_ok)
+        rb[i] = *reinterpret_cast<const uint4*>(Wt + ((int64_t)tap0 * Cout + sl) * Cin + c);
+    }
+  };
+  v4f acc[kRowsPerWave32][COT];
+#pragma unroll
+  for (int r = 0; r < kRowsPerWave32; ++r)
+#pragma unroll
+    for (int t = 0; t < COT; ++t) acc[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
+  load_stage(0);
+  for (int dzy = 0; dzy < 9; ++dzy) {
+    __syncthreads();  // previous operands consumed
+#pragma unroll
+    for (int i = 0; i < kAIt; ++i) {
+      const int sl = slot0 + i * kStep;
+      if (sl < kRows32 * 18) *reinterpret_cast<uint4*>(As + sl * pitch + c) = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < kBIt; ++i) {
+      const int sl = slot0 + i * kStep;
+      if (sl < 3 * Cout) *reinterpret_cast<uint4*>(Bs + sl * pitch + c) = rb[i];
+    }
+    __syncthreads();
+    if (dzy < 8) load_stage(dzy + 1);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+      for (int c0 = 0; c0 < Kp; c0 += 32) {
+        v8b a[kRowsPerWave32];
+#pragma unroll
+        for (int r = 0; r < kRowsPerWave32; ++r)
+          a[r] = *reinterpret_cast<const v8b*>(
+              As + ((wv * kRowsPerWave32 + r) * 18 + row + d) * pitch + c0 + 8 * kq);
+#pragma unroll
+        for (int t = 0; t < COT; ++t) {
+          const v8b b =
+              *reinterpret_cast<const v8b*>(Bs + (d * Cout + t * 16 + row) * pitch + c0 + 8 * kq);
+#pragma unroll
+          for (int r = 0; r < kRowsPerWave32; ++r)
+            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b, acc[r][t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // epilogue: lane holds voxels x0 + 4*kq + i of its rows, channel t*16 + row
+#pragma unroll
+  for (int r = 0; r < kRowsPerWave32; ++r) {
+    const int y = y0 + wv * kRowsPerWave32 + r;
+    if (y >= H) continue;
+#pragma unroll
+    for (int t = 0; t < COT; ++t) {
+      const int co = t * 16 + row;
+      const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = x0 + 4 * kq + i;
+        if (x >= W) continue;
+        float v = acc[r][t][i] + bb;
+        if (relu) v = v > 0.f ? v : 0.f;
+        const int64_t o = (((int64_t)z * H + y) * W + x) * Cout + co;
+        if (out_f32) reinterpret_cast<float*>(Y)[o] = v;
+        else reinterpret_cast<uint16_t*>(Y)[o] = f2bf(v);
+      }
+    }
+  }
+}
+
 // dW[tap][co][ci] += sum_v dY[v][co] * X[v + off(tap)][ci].  One wavefront:
 // one tap, one 16x16 (co, ci) tile and a run of `rows_per` (z, y) rows, K =
 // voxels along x in steps of 16; partial tiles are added with FP32 atomics.
@@ -263,13 +401,35 @@ extern "C" tpf_status tpf_conv3d_fwd(const void* X, const void* Wt, const float*
   static const Fn kDirect[8] = {conv_fwd_direct<1>, conv_fwd_direct<2>, conv_fwd_direct<3>,
                                 conv_fwd_direct<4>, conv_fwd_direct<5>, conv_fwd_direct<6>,
                                 conv_fwd_direct<7>, conv_fwd_direct<8>};
-  // LDS staging pays once a staged operand feeds enough MFMAs: measured
-  // 1.8-4x faster from 32 channels in and out (65^3 x 64: 0.33 ms against
-  // 1.36 ms), 10-20 % slower at 16 channels.
-  if (Cin < 32 || Cout < 32) {
+  // Wide inputs (Cin > 64) on the 16x16x16 form: the direct kernel below 32
+  // output channels, else the LDS-staged one (LDS staging pays once a staged
+  // operand feeds enough MFMAs).
+  if (Cin > 64 && Cout < 32) {
     const int64_t waves = (int64_t)D * H * ((W + 15) / 16);
     hipLaunchKernelGGL(kDirect[Cout / 16 - 1], dim3((waves + kWaves - 1) / kWaves),
                        dim3(64 * kWaves), 0, s, x, w, bias, Y, D, H, W, Cin, Cout, relu, out_f32);
+    TPF_HIP(hipGetLastError());
+    return TPF_OK;
+  }
+  // gfx950 K = 32 form for up to 64 input channels (LDS <= 100 KB; 16
+  // channels are zero-padded to 32); the 16x16x16 kernels stay for the rest.
+  if (Cin <= 64) {
+    using Fn32 = void (*)(const uint16_t*, const uint16_t*, const float*, void*, int, int, int,
+                          int, int, int);
+    static const Fn32 kFns32[2][8] = {
+        {conv_fwd_k32<1, 1>, conv_fwd_k32<2, 1>, conv_fwd_k32<3, 1>, conv_fwd_k32<4, 1>,
+         conv_fwd_k32<5, 1>, conv_fwd_k32<6, 1>, conv_fwd_k32<7, 1>, conv_fwd_k32<8, 1>},
+        {conv_fwd_k32<1, 2>, conv_fwd_k32<2, 2>, conv_fwd_k32<3, 2>, conv_fwd_k32<4, 2>,
+         conv_fwd_k32<5, 2>, conv_fwd_k32<6, 2>, conv_fwd_k32<7, 2>, conv_fwd_k32<8, 2>}};
+    const int kpw = Cin <= 32 ? 1 : 2;
+    const Fn32 k = kFns32[kpw - 1][Cout / 16 - 1];
+    const size_t sh32 = sizeof(uint16_t) * (size_t)(32 * kpw + 8) * (kRows32 * 18 + 3 * Cout);
+    const int64_t blocks32 = (int64_t)D * ((H + kRows32 - 1) / kRows32) * ((W + 15) / 16);
+    if (sh32 > 64 * 1024)
+      TPF_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)sh32));
+    hipLaunchKernelGGL(k, dim3(blocks32), dim3(kThreads32), sh32, s, x, w, bias, Y, D, H, W, Cin,
+                       relu, out_f32);
     TPF_HIP(hipGetLastError());
     return TPF_OK;
   }
