@@ -615,6 +615,11 @@ def profile_mode(args):
         dst.copy_(src)
         torch.cuda.synchronize()
         log(f"profile fccd: {args.steps} FCCD PPE matvecs on 257^3, then a 1 GiB copy")
+    elif args.profile == "n3":
+        log(f"profile n3 Rhie-Chow 257^3: {rhie_chow_bench(reps=args.steps)}")
+        log(f"profile n3 DCCD step 257^3: {dccd_bench(reps=args.steps)}")
+        log(f"profile n3 psi curvature 257^3: {curvature_bench(reps=args.steps)}")
+        log(f"profile n3 split reinit 257^3: {reinit_bench(reps=args.steps)}")
     elif args.profile == "ppe":
         gp, rho, rhs = ppe_case(N=128)
         r = ppe_solve_timed(gp, rho, rhs, "fp64")
@@ -635,12 +640,14 @@ def main():
     ap.add_argument("--ppe-n", type=int, default=256)
     ap.add_argument("--train-problems", type=int, default=100,
                     help="random 129^3 bubble problems used to train the learned coarse space")
-    ap.add_argument("--profile", choices=["ccd", "ppe", "mg", "fccd"], default=None,
+    ap.add_argument("--profile", choices=["ccd", "ppe", "mg", "fccd", "n3"], default=None,
                     help="run only one workload (for rocprofv3): 'ccd' = K all-axes CCD steps "
                          "on 257^3 plus one 1 GiB device copy (counter scale reference); "
                          "'ppe' = one FP64 PCG solve of the 129^3 ratio-1000 case; "
                          "'mg' = one W(2,2) multigrid-FCG solve of the 257^3 case; "
-                         "'fccd' = K FCCD PPE matvecs on the 257^3 case")
+                         "'fccd' = K FCCD PPE matvecs on the 257^3 case; "
+                         "'n3' = K calls each of the Rhie-Chow divergence, the DCCD step, "
+                         "the psi curvature and the split reinitialisation on 257^3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",

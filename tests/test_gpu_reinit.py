@@ -32,6 +32,9 @@ def _psi(shape, eps):
     ((24, 20), "wall", ("outflow", "zero"), True),
     ((16, 12, 10), "wall", "zero", True),
     ((16, 12, 10), "periodic", "periodic", False),
+    # long lines (many chunks of the sweep)
+    ((400,), "wall", "zero", True),
+    ((12, 330), "periodic", "periodic", False),
 ])
 def test_reinit_matches_oracle(N, ccd_bc, ls_bc, mass):
     print(f"[gpu-case] reinit N={N} bc={ls_bc} mass={mass}", file=sys.stderr, flush=True)

@@ -12,9 +12,9 @@
 //
 // The host loop below drives the CCD plan (gradient and flux derivatives)
 // and three kernels per step: the compression flux, the filtered divergence
-// with the clipped update, and one Thomas sweep per line and axis (one
-// thread per line, constant factors computed once on the host as the
-// reference's CPU path does).
+// with the clipped update, and one Thomas sweep per line and axis (one lane
+// per line, walked in chunks; constant factors computed once on the host as
+// the reference's CPU path does).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -122,37 +122,141 @@ __global__ __launch_bounds__(kBlock) void k_reinit_compress(RGeom g, const doubl
 
 // One CN-ADI sweep along slot a: rhs from q, forward elimination into out,
 // back substitution in place (reinit_ops.py:164-204, CPU Thomas path).
-__global__ __launch_bounds__(kBlock) void k_reinit_cn(RGeom g, int a, double d_R, double c_R,
-                                                      const double* __restrict__ tf,
-                                                      const double* __restrict__ m,
-                                                      const double* __restrict__ sup,
-                                                      const double* __restrict__ q,
-                                                      double* __restrict__ out) {
+// One Thomas sweep per line (forward elimination with the host factors tf,
+// back substitution with m and sup), 64 consecutive lines per workgroup (one
+// lane each), the line walked in chunks of kCnChunk nodes so a chunk's loads
+// are all in flight before its recurrence steps run.  Strided axes: the
+// lanes' nodes are adjacent in memory, so each lane loads its chunk directly
+// (coalesced across lanes).  Contiguous axis: the workgroup's lines are one
+// contiguous block; every chunk is staged through an LDS tile with loads and
+// stores that walk memory in order, and the lanes run their recurrences on
+// the tile.  The forward pass parks the eliminated right-hand sides in out.
+constexpr int kCnChunk = 16;
+constexpr int kCnPitch = kCnChunk + 3;  // chunk + 2 halo nodes + 1 pad
+
+__global__ __launch_bounds__(64) void k_reinit_cn(RGeom g, int a, double d_R, double c_R,
+                                                  const double* __restrict__ tf,
+                                                  const double* __restrict__ m,
+                                                  const double* __restrict__ sup,
+                                                  const double* __restrict__ q,
+                                                  double* __restrict__ out) {
+  __shared__ double tile[64 * kCnPitch];
   const int n = g.n[a];
   const int64_t st = stride_of(g, a);
   const int64_t nlines = g.total / n;
-  for (int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x; l < nlines;
-       l += (int64_t)gridDim.x * kBlock) {
-    int64_t base;
-    if (a == 0) base = l;
-    else if (a == 1) base = (l / g.n[2]) * ((int64_t)g.n[1] * g.n[2]) + (l % g.n[2]);
-    else base = l * n;
-    double dprev = 0.0;
-    for (int i = 0; i < n; ++i) {
-      const int64_t o = base + (int64_t)i * st;
-      double rhs;
-      if (i == 0) rhs = d_R * q[o] + 2.0 * c_R * q[o + st];
-      else if (i == n - 1) rhs = 2.0 * c_R * q[o - st] + d_R * q[o];
-      else rhs = c_R * q[o - st] + d_R * q[o] + c_R * q[o + st];
-      const double d = i == 0 ? rhs : rhs - tf[i - 1] * dprev;
-      out[o] = d;
-      dprev = d;
+  const int lane = threadIdx.x;
+  const int64_t l0 = (int64_t)blockIdx.x * 64;
+  const int nl = (int)(nlines - l0 < 64 ? nlines - l0 : 64);
+  const bool active = lane < nl;
+  const int64_t l = l0 + (active ? lane : 0);
+  int64_t base;
+  if (a == 0) base = l;
+  else if (a == 1) base = (l / g.n[2]) * ((int64_t)g.n[1] * g.n[2]) + (l % g.n[2]);
+  else base = l * n;
+  const bool contig = st == 1;
+  double* trow = tile + lane * kCnPitch;
+  // ---- forward elimination
+  double dprev = 0.0;
+  for (int i0 = 0; i0 < n; i0 += kCnChunk) {
+    const int cnt = n - i0 < kCnChunk ? n - i0 : kCnChunk;
+    // v[k] = q at node i0 - 1 + k, k < cnt + 2 (nodes outside [0, n) unused)
+    double v[kCnChunk + 2];
+    if (contig) {
+      __syncthreads();  // previous chunk's tile consumed
+      const int w = cnt + 2;
+      const double* blk = q + l0 * (int64_t)n;
+      for (int e = lane; e < nl * w; e += 64) {
+        const int j = e / w, k = e - j * w, node = i0 - 1 + k;
+        if (node >= 0 && node < n) tile[j * kCnPitch + k] = blk[(int64_t)j * n + node];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kCnChunk + 2; ++k) {
+        const int node = i0 - 1 + k;
+        v[k] = (k < cnt + 2 && node >= 0 && node < n) ? trow[k] : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kCnChunk + 2; ++k) {
+        const int node = i0 - 1 + k;
+        v[k] = (active && k < cnt + 2 && node >= 0 && node < n) ? q[base + (int64_t)node * st] : 0.0;
+      }
     }
-    double x = 0.0;
-    for (int i = n - 1; i >= 0; --i) {
-      const int64_t o = base + (int64_t)i * st;
-      x = i == n - 1 ? out[o] / m[n - 1] : (out[o] - sup[i] * x) / m[i];
-      out[o] = x;
+    double dv[kCnChunk];
+#pragma unroll
+    for (int k = 0; k < kCnChunk; ++k) {
+      if (k < cnt) {
+        const int i = i0 + k;
+        double rhs;
+        if (i == 0) rhs = d_R * v[k + 1] + 2.0 * c_R * v[k + 2];
+        else if (i == n - 1) rhs = 2.0 * c_R * v[k] + d_R * v[k + 1];
+        else rhs = c_R * v[k] + d_R * v[k + 1] + c_R * v[k + 2];
+        const double d = i == 0 ? rhs : rhs - tf[i - 1] * dprev;
+        dv[k] = d;
+        dprev = d;
+      }
+    }
+    if (contig) {
+      __syncthreads();  // every lane has read its q values
+#pragma unroll
+      for (int k = 0; k < kCnChunk; ++k)
+        if (k < cnt) trow[k] = dv[k];
+      __syncthreads();
+      double* blk = out + l0 * (int64_t)n;
+      for (int e = lane; e < nl * cnt; e += 64) {
+        const int j = e / cnt, k = e - j * cnt;
+        blk[(int64_t)j * n + i0 + k] = tile[j * kCnPitch + k];
+      }
+    } else if (active) {
+#pragma unroll
+      for (int k = 0; k < kCnChunk; ++k)
+        if (k < cnt) out[base + (int64_t)(i0 + k) * st] = dv[k];
+    }
+  }
+  // ---- back substitution, chunks from the end of the line
+  double x = 0.0;
+  const int last0 = ((n - 1) / kCnChunk) * kCnChunk;
+  for (int i0 = last0; i0 >= 0; i0 -= kCnChunk) {
+    const int cnt = n - i0 < kCnChunk ? n - i0 : kCnChunk;
+    double dv[kCnChunk];
+    if (contig) {
+      __syncthreads();
+      const double* blk = out + l0 * (int64_t)n;
+      for (int e = lane; e < nl * cnt; e += 64) {
+        const int j = e / cnt, k = e - j * cnt;
+        tile[j * kCnPitch + k] = blk[(int64_t)j * n + i0 + k];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kCnChunk; ++k) dv[k] = k < cnt ? trow[k] : 0.0;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kCnChunk; ++k)
+        dv[k] = (active && k < cnt) ? out[base + (int64_t)(i0 + k) * st] : 0.0;
+    }
+#pragma unroll
+    for (int k = kCnChunk - 1; k >= 0; --k) {
+      if (k < cnt) {
+        const int i = i0 + k;
+        x = i == n - 1 ? dv[k] / m[n - 1] : (dv[k] - sup[i] * x) / m[i];
+        dv[k] = x;
+      }
+    }
+    if (contig) {
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kCnChunk; ++k)
+        if (k < cnt) trow[k] = dv[k];
+      __syncthreads();
+      double* blk = out + l0 * (int64_t)n;
+      for (int e = lane; e < nl * cnt; e += 64) {
+        const int j = e / cnt, k = e - j * cnt;
+        blk[(int64_t)j * n + i0 + k] = tile[j * kCnPitch + k];
+      }
+    } else if (active) {
+#pragma unroll
+      for (int k = 0; k < kCnChunk; ++k)
+        if (k < cnt) out[base + (int64_t)(i0 + k) * st] = dv[k];
     }
   }
 }
@@ -264,8 +368,8 @@ extern "C" tpf_status tpf_reinit_split(tpf_ccd_plan* ccd, int ndim, const int64_
       for (int a = 0; a < ndim; ++a) {
         const int s = 3 - ndim + a;
         const int64_t nlines = T / g.n[s];
-        hipLaunchKernelGGL(k_reinit_cn, dim3(blocks_for(nlines)), dim3(kBlock), 0, st, g, s,
-                           dR[a], cR[a], (const double*)(fac + off_tf[a]),
+        hipLaunchKernelGGL(k_reinit_cn, dim3((unsigned)((nlines + 63) / 64)), dim3(64), 0, st, g,
+                           s, dR[a], cR[a], (const double*)(fac + off_tf[a]),
                            (const double*)(fac + off_m[a]), (const double*)(fac + off_sup[a]),
                            (const double*)src, dst);
         TPF_HIP(hipGetLastError());
