@@ -52,3 +52,37 @@ def test_dccd_step_matches_oracle(N, ccd_bc, ls_bc, mass):
     got_t = adv.advance(t, [torch.from_numpy(v).cuda() for v in vel], 0.002)
     assert np.abs(got_t.cpu().numpy() - want).max() <= 1e-12 * np.abs(want).max()
     assert np.array_equal(t.cpu().numpy(), o.sync(psi.copy()))
+
+
+@pytest.mark.parametrize("shape,per", [
+    ((9,), (1,)), ((9,), (0,)), ((7, 6), (1, 0)), ((7, 6), (0, 1)), ((7, 6), (1, 1)),
+    ((5, 6, 7), (1, 1, 1)), ((5, 6, 7), (0, 1, 1)), ((5, 6, 7), (1, 0, 1)), ((5, 6, 7), (0, 0, 0)),
+])
+def test_sync_periodic_images_in_place_and_copy(shape, per):
+    """tpf_sync_periodic_images against the reference's axis-by-axis copies
+    (core/boundary.py:116-133): out-of-place (every node written) and in place
+    (only the image planes change; no launch without a periodic axis)."""
+    import ctypes as C
+    from twophaseflow_amd import _native as nat
+    print(f"[gpu-case] image sync {shape} periodic={per}", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(len(shape) * 10 + sum(per))
+    a = rng.standard_normal(shape)
+    want = a.copy()
+    for ax, p in enumerate(per):
+        if p:
+            src = [slice(None)] * len(shape)
+            dst = [slice(None)] * len(shape)
+            src[ax], dst[ax] = 0, -1
+            want[tuple(dst)] = want[tuple(src)]
+    lib = nat.lib()
+    dims, pa = nat.i64arr(shape), nat.iarr(per)
+    t_in = torch.from_numpy(a).cuda()
+    t_out = torch.full_like(t_in, float("nan"))
+    nat.check(lib.tpf_sync_periodic_images(len(shape), dims, pa, C.c_void_p(t_in.data_ptr()),
+                                           C.c_void_p(t_out.data_ptr()), None))
+    t_io = torch.from_numpy(a).cuda()
+    nat.check(lib.tpf_sync_periodic_images(len(shape), dims, pa, C.c_void_p(t_io.data_ptr()),
+                                           C.c_void_p(t_io.data_ptr()), None))
+    torch.cuda.synchronize()
+    assert np.array_equal(t_out.cpu().numpy(), want)
+    assert np.array_equal(t_io.cpu().numpy(), want)

@@ -198,6 +198,36 @@ tpf_status make_geom(DGeom& g, int ndim, const int64_t* n_nodes, const double* L
   return TPF_OK;
 }
 
+// In-place image sync: one thread per node of the image planes (c[a] = N on
+// a periodic slot a), the planes of the periodic slots enumerated one after
+// another.  A node on several image planes is written once per plane, with
+// the same root value each time; roots lie on no image plane, so no write
+// changes a value another thread reads.
+__global__ __launch_bounds__(kBlock) void k_dccd_sync_planes(DGeom g, int64_t nplane,
+                                                             double* __restrict__ q) {
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < nplane;
+       t += (int64_t)gridDim.x * kBlock) {
+    int64_t u = t;
+    int a = 0;
+    for (; a < 3; ++a) {
+      if (!g.periodic[a] || g.n[a] == 1) continue;
+      const int64_t pa = g.total / g.n[a];
+      if (u < pa) break;
+      u -= pa;
+    }
+    if (a == 3) continue;  // not reached for t < nplane
+    // the other two slots of the plane, in order
+    const int b0 = a == 0 ? 1 : 0, b1 = a == 2 ? 1 : 2;
+    int c[3];
+    c[a] = g.n[a] - 1;
+    c[b0] = (int)(u / g.n[b1]);
+    c[b1] = (int)(u - (int64_t)c[b0] * g.n[b1]);
+    const int64_t idx = (int64_t)c[0] * g.n[1] * g.n[2] + (int64_t)c[1] * g.n[2] + c[2];
+    const int64_t r = root_of(g, idx, c);
+    q[idx] = q[r];
+  }
+}
+
 unsigned blocks_for(int64_t total) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + kBlock - 1) / kBlock, 2048));
 }
@@ -242,6 +272,16 @@ extern "C" tpf_status tpf_sync_periodic_images(int ndim, const int64_t* n_nodes,
   DGeom g;
   TPF_TRY(make_geom(g, ndim, n_nodes, nullptr, periodic));
   if (!in || !out) return fail(TPF_ERR_VALUE, "null argument");
+  if (in == out) {  // only the image planes change; nothing without a periodic axis
+    int64_t nplane = 0;
+    for (int a = 0; a < 3; ++a)
+      if (g.periodic[a] && g.n[a] > 1) nplane += g.total / g.n[a];
+    if (nplane == 0) return TPF_OK;
+    hipLaunchKernelGGL(k_dccd_sync_planes, dim3(blocks_for(nplane)), dim3(kBlock), 0,
+                       (hipStream_t)stream, g, nplane, out);
+    TPF_HIP(hipGetLastError());
+    return TPF_OK;
+  }
   hipLaunchKernelGGL(k_dccd_sync, dim3(blocks_for(g.total)), dim3(kBlock), 0, (hipStream_t)stream,
                      g, in, out);
   TPF_HIP(hipGetLastError());
