@@ -52,17 +52,22 @@ class CurvatureCalculatorPsi:
         ccd.apply_into(p, d1, d2, (1 << ndim) - 1)
         scratch = torch.empty_like(p) if self.dccd_eps > 0 else None
 
-        def filt(f):
+        def filt(f, out=None):
             if self.dccd_eps <= 0:
                 return f
-            out = torch.empty_like(f)
+            if out is None:
+                out = torch.empty_like(f)
             nat.check(self._lib.tpf_dccd_filter_nd(ndim, dims, per, ptr(p), self.dccd_eps,
                                                    ptr(f), ptr(out), ptr(scratch), st))
             return out
 
         if self.dccd_eps > 0:
-            d1 = torch.stack([filt(d1[a]) for a in range(ndim)])
-            d2 = torch.stack([filt(d2[a]) for a in range(ndim)])
+            # filtered components written straight into stacked outputs
+            d1f, d2f = torch.empty_like(d1), torch.empty_like(d2)
+            for a in range(ndim):
+                filt(d1[a], d1f[a])
+                filt(d2[a], d2f[a])
+            d1, d2 = d1f, d2f
         pairs = [(0, 1)] if ndim == 2 else [(0, 1), (0, 2), (1, 2)]
         mixed = torch.empty((len(pairs),) + shape, dtype=torch.float64, device="cuda")
         for k, (i, j) in enumerate(pairs):
