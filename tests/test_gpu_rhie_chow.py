@@ -23,6 +23,9 @@ pytestmark = pytest.mark.gpu
     ((24, 16), ("periodic", "wall"), (1.0, 1.0), False),
     ((12, 10, 8), "periodic", (1.0, 0.8, 0.6), True),
     ((12, 10, 8), "wall", (1.0, 1.0, 1.0), False),
+    # several 16-plane slabs along axis 0, mixed boundaries
+    ((40, 6, 5), ("wall", "periodic", "wall"), (2.0, 1.0, 1.0), True),
+    ((33, 7, 6), ("periodic", "wall", "periodic"), (1.0, 0.5, 1.0), False),
 ])
 def test_rhie_chow_matches_oracle(N, bc, L, bf):
     print(f"[gpu-case] rhie-chow N={N} bc={bc} bf={bf}", file=sys.stderr, flush=True)

@@ -7,9 +7,10 @@
 // (:152-204); face 0 is the periodic wrap (nodes N and 0) or a zero wall face
 // (:206-234); the nodal divergence is (F_{k+1} - F_k)/h, node N taking
 // (F_0 - F_N)/h on periodic axes and -F_N/h at a wall (:238-284).
-// One streaming kernel: a thread per node evaluates its two faces per axis
-// from p, rho, u and the stacked CCD derivatives (neighbours via L1/L2) and
-// sums the axes in order; the face arrays never reach HBM.
+// 1-D / 2-D: one streaming kernel, a thread per node evaluating its two faces
+// per axis from p, rho, u and the stacked CCD derivatives (neighbours via
+// L1/L2) and summing the axes in order; 3-D: the slab-marching kernel below.
+// The face arrays never reach HBM.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -129,6 +130,108 @@ __global__ __launch_bounds__(kBlock) void k_rc_div(RcGeom g, RcPtrs P, double* _
   }
 }
 
+// 3-D: the workgroups march kSlab planes along slot 0 (one thread per node of
+// a plane).  Each thread carries the slot-0 cell state of its current plane
+// (u_0, p, rho, the cell pressure gradient and, with the balanced force,
+// kappa, psi and the cell force) and the flux of the face below it, so every
+// slot-0 face is evaluated once and every plane's values are read once; the
+// in-plane slots use the per-node face evaluation above.  The arithmetic is
+// the same as k_rc_div's, operation for operation.
+constexpr int kSlab = 16;
+
+struct Cell0 {
+  double u, p, rho, dpc, kappa, psi, fsc;
+};
+
+__device__ __forceinline__ Cell0 cell0(const RcGeom& g, const RcPtrs& P, int64_t b, int64_t st,
+                                       int m) {
+  const int64_t o = b + (int64_t)m * st;
+  Cell0 c;
+  c.u = P.u[0][o];
+  c.p = P.p[o];
+  c.rho = P.rho[o];
+  c.dpc = dp_cell(g, P, 0, b, st, m, P.dp + (int64_t)g.k[0] * g.total);
+  c.kappa = c.psi = c.fsc = 0.0;
+  if (g.bf) {
+    c.kappa = P.kappa[o];
+    c.psi = P.psi[o];
+    c.fsc = fs_cell(g, P, 0, b, st, m, P.dpsi + (int64_t)g.k[0] * g.total);
+  }
+  return c;
+}
+
+// face_flux of slot 0 from the two cell states (same expression order)
+__device__ __forceinline__ double face0(const RcGeom& g, const Cell0& L, const Cell0& R) {
+  const double h = g.h[0];
+  const double u_bar = 0.5 * (L.u + R.u);
+  const double dp_face = (R.p - L.p) / h;
+  const double dp_bar = 0.5 * (L.dpc + R.dpc);
+  const double inv_rho = 2.0 / (L.rho + R.rho);
+  double bracket = dp_face - dp_bar;
+  if (g.bf) {
+    const double kappa_face = 0.5 * (L.kappa + R.kappa);
+    const double f_face = kappa_face * (R.psi - L.psi) / (h * g.we);
+    const double f_bar = 0.5 * (L.fsc + R.fsc);
+    bracket = (dp_face - dp_bar) - (f_face - f_bar);
+  }
+  return u_bar - g.dt * inv_rho * bracket;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rc_div_slab(RcGeom g, RcPtrs P, unsigned nx,
+                                                        double* __restrict__ out) {
+  const unsigned bx = blockIdx.x % nx, by = blockIdx.x / nx;
+  const int plane = g.n[1] * g.n[2];
+  const int j = (int)(bx * kBlock + threadIdx.x);
+  if (j >= plane) return;
+  int c[3];
+  c[1] = j / g.n[2];
+  c[2] = j - c[1] * g.n[2];
+  const int N0 = g.n[0] - 1;
+  const bool per0 = g.periodic[0] != 0;
+  const int64_t st0 = plane;
+  const int s0 = (int)by * kSlab;
+  const int s1 = min(N0 + 1, s0 + kSlab);
+  Cell0 cur = cell0(g, P, j, st0, s0);
+  // flux of the face below plane s0: flux_k(s0)
+  double f_lo;
+  if (s0 == 0) f_lo = per0 ? face0(g, cell0(g, P, j, st0, N0), cur) : 0.0;
+  else f_lo = face0(g, cell0(g, P, j, st0, s0 - 1), cur);
+  for (int i0 = s0; i0 < s1; ++i0) {
+    const int64_t idx = (int64_t)i0 * plane + j;
+    c[0] = i0;
+    // slot 0: the face above (flux_k(i0 + 1), or the wrap / zero face at N)
+    Cell0 next = cur;
+    double f_up;
+    if (i0 < N0) {
+      next = cell0(g, P, j, st0, i0 + 1);
+      f_up = face0(g, cur, next);
+    } else {
+      f_up = per0 ? face0(g, cur, cell0(g, P, j, st0, 0)) : 0.0;
+    }
+    double acc = 0.0;
+    double d;
+    if (i0 < N0) d = (f_up - f_lo) / g.h[0];
+    else if (per0) d = (f_up - f_lo) / g.h[0];
+    else d = -f_lo / g.h[0];
+    acc += d;
+#pragma unroll
+    for (int a = 1; a < 3; ++a) {
+      const int N = g.n[a] - 1;
+      const int i = c[a];
+      const int64_t st = stride_of(g, a);
+      const int64_t b = idx - (int64_t)i * st;
+      double da;
+      if (i < N) da = (flux_k(g, P, a, b, st, i + 1) - flux_k(g, P, a, b, st, i)) / g.h[a];
+      else if (g.periodic[a]) da = (flux_k(g, P, a, b, st, 0) - flux_k(g, P, a, b, st, N)) / g.h[a];
+      else da = -flux_k(g, P, a, b, st, N) / g.h[a];
+      acc += da;
+    }
+    out[idx] = acc;
+    f_lo = f_up;
+    cur = next;
+  }
+}
+
 }  // namespace
 }  // namespace tpf
 
@@ -173,6 +276,14 @@ extern "C" tpf_status tpf_rhie_chow_divergence(int ndim, const int64_t* n_nodes,
   P.kappa = kappa;
   P.psi = psi;
   P.dpsi = dpsi;
+  if (ndim == 3) {
+    const unsigned nx = (unsigned)((g.n[1] * g.n[2] + kBlock - 1) / kBlock);
+    const unsigned ny = (unsigned)((g.n[0] + kSlab - 1) / kSlab);
+    hipLaunchKernelGGL(k_rc_div_slab, dim3(nx * ny), dim3(kBlock), 0, (hipStream_t)stream, g, P,
+                       nx, out);
+    TPF_HIP(hipGetLastError());
+    return TPF_OK;
+  }
   const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((g.total + kBlock - 1) / kBlock, 2048));
   hipLaunchKernelGGL(k_rc_div, dim3((unsigned)blocks), dim3(kBlock), 0, (hipStream_t)stream, g, P,
                      out);
