@@ -70,6 +70,18 @@ TPF_API tpf_status tpf_ccd_differentiate(tpf_ccd_plan* plan, const double* f,
 TPF_API tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double* d1,
                          double* d2, uint32_t axis_mask, void* stream);
 
+/* Replaces the per-axis loops that differentiate a different field along
+ * each axis, `fp_a, _ = ccd.differentiate(f_a, axis=a)`
+ * (levelset/advection_dccd.py:119-121, levelset/reinit_ops.py:86-89):
+ * every axis a whose bit is set in axis_mask differentiates f[a] along a
+ * into d1[a] / d2[a], all in one launch when the axes share a launch shape
+ * (the axes of tpf_ccd_apply's grid, each with its own input).  f, d1, d2
+ * are arrays of ndim pointers indexed by axis; d1 or d2 may be NULL, and so
+ * may their entries.  No output may alias any input or another output. */
+TPF_API tpf_status tpf_ccd_apply_fields(tpf_ccd_plan* plan, const double* const* f,
+                                        double* const* d1, double* const* d2,
+                                        uint32_t axis_mask, void* stream);
+
 /* Launch geometry / reduced-system facts of one axis (for tests, bench and
  * the autotuner): partitions, rows per partition, coupling window, and the
  * largest dropped reduced-system coefficient (0 when untruncated). */

@@ -244,3 +244,38 @@ def test_explicit_inverse_kernel_paths(N, bc):
         assert _rel(d1.cpu(), refs[a][0]) <= 1e-12 and _rel(d2.cpu(), refs[a][1]) <= 1e-12
     with pytest.raises(ValueError, match="strided axes only"):
         ccd.set_launch(nd - 1, "explicit")
+
+
+@pytest.mark.parametrize("N,bc", [((32, 16, 24), "periodic"), ((20, 33, 17), "wall"),
+                                  ((40, 28), "wall"), ((64,), "periodic")])
+def test_apply_fields_matches_per_field_apply(N, bc):
+    """tpf_ccd_apply_fields (a different field per axis, one launch) against
+    tpf_ccd_apply on each field: the same grid configuration, so every axis
+    block is bit-identical; then a subset mask with d2 omitted, and the
+    aliasing check."""
+    _announce(f"ccd apply_fields N={N} bc={bc}")
+    g = _grid(N)
+    s = CCDSolver(g, bc_type=bc)
+    nd = len(N)
+    rng = np.random.default_rng(sum(N))
+    fs = [torch.from_numpy(rng.standard_normal(g.shape)).cuda() for _ in range(nd)]
+    d1s = [torch.empty_like(fs[0]) for _ in range(nd)]
+    d2s = [torch.empty_like(fs[0]) for _ in range(nd)]
+    s.apply_fields_into(fs, d1s, d2s, (1 << nd) - 1)
+    for a in range(nd):
+        r1 = torch.empty((nd,) + g.shape, dtype=torch.float64, device="cuda")
+        r2 = torch.empty_like(r1)
+        s.apply_into(fs[a], r1, r2, (1 << nd) - 1)
+        assert torch.equal(d1s[a], r1[a]) and torch.equal(d2s[a], r2[a]), a
+    if nd > 1:
+        only = nd - 1
+        o1 = [None] * nd
+        o1[only] = torch.full_like(fs[0], float("nan"))
+        s.apply_fields_into(fs, o1, None, 1 << only)
+        ref1 = torch.empty_like(fs[0])
+        s.differentiate_into(fs[only], ref1, None, only)
+        assert _rel(o1[only].cpu().numpy(), ref1.cpu().numpy()) <= 1e-12
+        bad = list(d1s)
+        bad[0] = fs[nd - 1]  # an output aliasing another axis' input
+        with pytest.raises(ValueError):
+            s.apply_fields_into(fs, bad, None, (1 << nd) - 1)

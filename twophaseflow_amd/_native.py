@@ -56,6 +56,7 @@ SIGNATURES = {
     "tpf_ccd_plan_destroy": (C.c_int, [_vp]),
     "tpf_ccd_differentiate": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "tpf_ccd_apply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "tpf_ccd_apply_fields": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
     "tpf_ccd_axis_info": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _dp]),
     "tpf_ccd_axis_kernel": (C.c_int, [_vp, C.c_int, _ip]),
     "tpf_ccd_plan_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),

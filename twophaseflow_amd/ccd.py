@@ -94,6 +94,24 @@ class CCDSolver:
         nat.check(self._lib.tpf_ccd_apply(
             self._plan, ptr(f), ptr(d1), ptr(d2), int(axis_mask), stream_handle()))
 
+    def apply_fields_into(self, fs, d1s, d2s, axis_mask: int) -> None:
+        """Axis a (bit a of ``axis_mask``) differentiates fs[a] along a into
+        d1s[a] / d2s[a], in one launch when the axes share a launch shape.
+        fs, d1s, d2s are sequences of ndim contiguous CUDA float64 tensors of
+        grid.shape (entries of unmasked axes, and d1s / d2s themselves, may
+        be None)."""
+        n = self.ndim
+        arr = C.c_void_p * n
+
+        def pack(ts):
+            if ts is None:
+                return None
+            if len(ts) != n:
+                raise ValueError("apply_fields_into: one entry per axis")
+            return arr(*[ptr(t) if t is not None else None for t in ts])
+        nat.check(self._lib.tpf_ccd_apply_fields(self._plan, pack(fs), pack(d1s), pack(d2s),
+                                                 int(axis_mask), stream_handle()))
+
     # -- reference API -----------------------------------------------------------
     def differentiate(self, data, axis: int, bc_left=None, bc_right=None,
                       apply_metric: bool = True):

@@ -1352,7 +1352,7 @@ tpf_status raise_lds_limit(const void* k, size_t bytes) {
 }
 
 // Launches the given axes in one grid (all must share the rows cap M).
-tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const double* f,
+tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const double* const* f,
                        double* const* d1, double* const* d2, hipStream_t stream) {
   CcdLaunch L{};
   L.nslots = naxes;
@@ -1367,7 +1367,7 @@ tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const dou
     S.ln = lines_for(plan, axes[i]);
     S.contig = S.ln.stride == 1;
     S.tiled = !wave && S.contig && tile_fits(ax.geom, plan->n_nodes[axes[i]]);
-    S.f = f;
+    S.f = f[i];
     S.d1 = d1[i];
     S.d2 = d2[i];
     if (M < 0) M = ax.geom.M;
@@ -1410,6 +1410,21 @@ tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const dou
   TPF_TRY(raise_lds_limit((const void*)k, shmem));
   hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * L.slot[0].g.P), shmem, stream, L);
   TPF_HIP(hipGetLastError());
+  return TPF_OK;
+}
+
+// One grid for the given axes when they share a launch shape (kernel, rows
+// cap, partition count), else one launch per axis.
+tpf_status launch_fused(tpf_ccd_plan* plan, const int* axes, int n, const double* const* f,
+                        double* const* d1, double* const* d2, hipStream_t stream) {
+  bool same = true;
+  for (int i = 1; i < n; ++i)
+    same = same && plan->axes[axes[i]].geom.M == plan->axes[axes[0]].geom.M &&
+           plan->axes[axes[i]].geom.P == plan->axes[axes[0]].geom.P &&
+           (plan->axes[axes[i]].wave == 1) == (plan->axes[axes[0]].wave == 1);
+  if (same) return launch_axes(plan, axes, n, f, d1, d2, stream);
+  for (int i = 0; i < n; ++i)
+    TPF_TRY(launch_axes(plan, &axes[i], 1, &f[i], &d1[i], &d2[i], stream));
   return TPF_OK;
 }
 
@@ -1557,7 +1572,7 @@ extern "C" tpf_status tpf_ccd_differentiate(tpf_ccd_plan* plan, const double* f,
   if ((const double*)d1 == f || (const double*)d2 == f || (d1 && d1 == d2))
     return fail(TPF_ERR_VALUE, "CCD outputs must not alias the input or each other");
   TPF_TRY(ensure_axis(plan, axis));
-  return launch_axes(plan, &axis, 1, f, &d1, &d2, (hipStream_t)stream);
+  return launch_axes(plan, &axis, 1, &f, &d1, &d2, (hipStream_t)stream);
 }
 
 extern "C" tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double* d1, double* d2,
@@ -1567,6 +1582,7 @@ extern "C" tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double*
   int64_t total = 1;
   for (int a = 0; a < plan->ndim; ++a) total *= plan->n_nodes[a];
   int axes[3], n = 0;
+  const double* fs[3];
   double *o1[3], *o2[3];
   for (int a = 0; a < plan->ndim; ++a) {
     if (!(axis_mask & (1u << a))) continue;
@@ -1575,18 +1591,47 @@ extern "C" tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double*
     o2[n] = d2 ? d2 + n * total : nullptr;
     if ((const double*)o1[n] == f || (const double*)o2[n] == f)
       return fail(TPF_ERR_VALUE, "CCD outputs must not alias the input");
+    fs[n] = f;
     axes[n++] = a;
   }
   if (n == 0 || (!d1 && !d2)) return TPF_OK;
-  bool same = true;  // one grid when every axis has the same launch shape
-  for (int i = 1; i < n; ++i)
-    same = same && plan->axes[axes[i]].geom.M == plan->axes[axes[0]].geom.M &&
-           plan->axes[axes[i]].geom.P == plan->axes[axes[0]].geom.P &&
-           (plan->axes[axes[i]].wave == 1) == (plan->axes[axes[0]].wave == 1);
-  if (same) return launch_axes(plan, axes, n, f, o1, o2, (hipStream_t)stream);
-  for (int i = 0; i < n; ++i)
-    TPF_TRY(launch_axes(plan, &axes[i], 1, f, &o1[i], &o2[i], (hipStream_t)stream));
-  return TPF_OK;
+  return launch_fused(plan, axes, n, fs, o1, o2, (hipStream_t)stream);
+}
+
+extern "C" tpf_status tpf_ccd_apply_fields(tpf_ccd_plan* plan, const double* const* f,
+                                           double* const* d1, double* const* d2,
+                                           uint32_t axis_mask, void* stream) {
+  if (!plan || !f) return fail(TPF_ERR_VALUE, "null plan or inputs");
+  if (axis_mask >> plan->ndim) return fail(TPF_ERR_VALUE, "axis_mask names a missing axis");
+  int axes[3], n = 0;
+  const double* fs[3];
+  double *o1[3], *o2[3];
+  for (int a = 0; a < plan->ndim; ++a) {
+    if (!(axis_mask & (1u << a))) continue;
+    if (!f[a]) return fail(TPF_ERR_VALUE, "null input for a masked axis");
+    TPF_TRY(ensure_axis(plan, a));
+    fs[n] = f[a];
+    o1[n] = d1 ? d1[a] : nullptr;
+    o2[n] = d2 ? d2[a] : nullptr;
+    axes[n++] = a;
+  }
+  // the slots of one grid run concurrently: no output may be any slot's
+  // input or another output
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j)
+      if ((const double*)o1[i] == fs[j] || (const double*)o2[i] == fs[j])
+        return fail(TPF_ERR_VALUE, "CCD outputs must not alias any input");
+    for (int j = 0; j < n; ++j) {
+      if (o1[i] && ((j != i && o1[i] == o1[j]) || o1[i] == o2[j]))
+        return fail(TPF_ERR_VALUE, "CCD outputs must be distinct arrays");
+      if (o2[i] && j != i && o2[i] == o2[j])
+        return fail(TPF_ERR_VALUE, "CCD outputs must be distinct arrays");
+    }
+  }
+  bool any = false;
+  for (int i = 0; i < n; ++i) any = any || o1[i] || o2[i];
+  if (n == 0 || !any) return TPF_OK;
+  return launch_fused(plan, axes, n, fs, o1, o2, (hipStream_t)stream);
 }
 
 extern "C" tpf_status tpf_ccd_axis_info(tpf_ccd_plan* plan, int axis, int* partitions,

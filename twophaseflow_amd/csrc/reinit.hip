@@ -356,9 +356,15 @@ extern "C" tpf_status tpf_reinit_split(tpf_ccd_plan* ccd, int ndim, const int64_
       hipLaunchKernelGGL(k_reinit_flux, dim3(blocks_for(T)), dim3(kBlock), 0, st, g, ndim,
                          (const double*)q, (const double*)d1, flux);
       TPF_HIP(hipGetLastError());
-      for (int a = 0; a < ndim; ++a)
-        TPF_TRY(tpf_ccd_differentiate(ccd, flux + (int64_t)a * T, fpd + (int64_t)a * T, nullptr,
-                                      a, st));
+      {  // every axis differentiates its own flux, one launch
+        const double* fa[3];
+        double* fpa[3];
+        for (int a = 0; a < ndim; ++a) {
+          fa[a] = flux + (int64_t)a * T;
+          fpa[a] = fpd + (int64_t)a * T;
+        }
+        TPF_TRY(tpf_ccd_apply_fields(ccd, fa, fpa, nullptr, mask, st));
+      }
       hipLaunchKernelGGL(k_reinit_compress, dim3(blocks_for(T)), dim3(kBlock), 0, st, g,
                          (const double*)q, (const double*)fpd, eps_d_comp, dtau, qs);
       TPF_HIP(hipGetLastError());

@@ -97,8 +97,11 @@ class DissipativeCCDAdvection:
 
         def stage(q, ca, cb):
             nat.check(lib.tpf_dccd_flux(self.ndim, dims, ptr(q), vel, ptr(f), st))
-            for a in range(self.ndim):
-                self._ccd.differentiate_into(f[a], fp[a], None, a)
+            if hasattr(self._ccd, "apply_fields_into"):  # one launch for every axis
+                self._ccd.apply_fields_into(list(f), list(fp), None, (1 << self.ndim) - 1)
+            else:
+                for a in range(self.ndim):
+                    self._ccd.differentiate_into(f[a], fp[a], None, a)
             out = torch.empty_like(q0)
             nat.check(lib.tpf_dccd_stage(self.ndim, dims, per, ptr(q0), ptr(q), ptr(fp),
                                          self._eps_d, float(dt), ca, cb, 1 if clip else 0,
