@@ -346,6 +346,16 @@ TPF_API tpf_status tpf_dccd_mass_correct(int ndim, const int64_t* n_nodes, const
 TPF_API tpf_status tpf_dccd_filter_nd(int ndim, const int64_t* n_nodes, const int* periodic,
                                       const double* psi, double eps, const double* f,
                                       double* out, double* scratch, void* stream);
+
+/* tpf_dccd_filter_nd over nf <= 8 fields that share psi (the derivative
+ * loop of CurvatureCalculatorPsi, levelset/curvature_psi.py:89-104, filters
+ * every first and second derivative with the same psi): one launch per pass
+ * for all fields, each field bit-identical to its own tpf_dccd_filter_nd.
+ * f, out, scratch: arrays of nf device pointers, all 3 nf distinct. */
+TPF_API tpf_status tpf_dccd_filter_nd_multi(int ndim, const int64_t* n_nodes,
+                                            const int* periodic, const double* psi, double eps,
+                                            int nf, const double* const* f, double* const* out,
+                                            double* const* scratch, void* stream);
 /* kappa from the stacked CCD derivatives d1, d2 [ndim][n_total] and the mixed
  * derivatives (2-D: [psi_xy]; 3-D: [xy, xz, yz]); 0 where psi <= psi_min or
  * psi >= 1 - psi_min.  2-D or 3-D. */

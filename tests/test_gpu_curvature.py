@@ -49,3 +49,38 @@ def test_curvature_matches_oracle(N, bc, eps, gauss):
     assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()
     got_t = calc.compute(torch.from_numpy(psi).cuda())
     assert np.abs(got_t.cpu().numpy() - want).max() <= 1e-11 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("shape,per,nf", [((17, 12), (1, 0), 4), ((9, 10, 11), (1, 1, 1), 6),
+                                          ((9, 10, 11), (0, 1, 0), 3), ((33,), (0,), 2)])
+def test_filter_multi_matches_single(shape, per, nf):
+    """tpf_dccd_filter_nd_multi (one launch per pass for nf fields sharing
+    psi) is bit-identical to tpf_dccd_filter_nd on each field; aliased
+    arrays are rejected."""
+    import ctypes as C
+    from twophaseflow_amd import _native as nat
+    print(f"[gpu-case] filter multi {shape} per={per} nf={nf}", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(nf + len(shape))
+    psi = torch.from_numpy(rng.random(shape)).cuda()
+    fs = [torch.from_numpy(rng.standard_normal(shape)).cuda() for _ in range(nf)]
+    outs = [torch.empty_like(psi) for _ in range(nf)]
+    scr = [torch.empty_like(psi) for _ in range(nf)]
+    lib = nat.lib()
+    dims, pa = nat.i64arr(shape), nat.iarr(per)
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    arr = C.c_void_p * nf
+    nat.check(lib.tpf_dccd_filter_nd_multi(len(shape), dims, pa, vp(psi), 0.05, nf,
+                                           arr(*[t.data_ptr() for t in fs]),
+                                           arr(*[t.data_ptr() for t in outs]),
+                                           arr(*[t.data_ptr() for t in scr]), None))
+    for k in range(nf):
+        ref, s1 = torch.empty_like(psi), torch.empty_like(psi)
+        nat.check(lib.tpf_dccd_filter_nd(len(shape), dims, pa, vp(psi), 0.05, vp(fs[k]), vp(ref),
+                                         vp(s1), None))
+        torch.cuda.synchronize()
+        assert torch.equal(outs[k], ref), k
+    with pytest.raises(ValueError):
+        nat.check(lib.tpf_dccd_filter_nd_multi(len(shape), dims, pa, vp(psi), 0.05, nf,
+                                               arr(*[t.data_ptr() for t in fs]),
+                                               arr(*[t.data_ptr() for t in fs]),
+                                               arr(*[t.data_ptr() for t in scr]), None))

@@ -97,6 +97,8 @@ SIGNATURES = {
                                         _vp]),
     "tpf_dccd_filter_nd": (C.c_int, [C.c_int, _i64p, _ip, _vp, C.c_double, _vp, _vp, _vp,
                                      _vp]),
+    "tpf_dccd_filter_nd_multi": (C.c_int, [C.c_int, _i64p, _ip, _vp, C.c_double, C.c_int, _vp, _vp,
+                                          _vp, _vp]),
     "tpf_curvature_psi": (C.c_int, [C.c_int, _i64p, _vp, _vp, _vp, _vp, C.c_double, _vp, _vp]),
     "tpf_gaussian_3x3": (C.c_int, [_i64p, _vp, _vp, _vp]),
     "tpf_reinit_split": (C.c_int, [_vp, C.c_int, _i64p, _dp, _ip, C.c_double, C.c_int,

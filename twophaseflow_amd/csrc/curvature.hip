@@ -76,6 +76,65 @@ __global__ __launch_bounds__(kBlock) void k_filter_pass(KGeom g, int a, double e
   }
 }
 
+// The same pass over up to kMaxFields fields sharing psi: the switch
+// eps (2 psi - 1)^2 is formed once per node and each field takes exactly
+// k_filter_pass's arithmetic.
+constexpr int kMaxFields = 8;
+struct FieldSet {
+  const double* in[kMaxFields];
+  const double* orig[kMaxFields];
+  double* out[kMaxFields];
+};
+
+__global__ __launch_bounds__(kBlock) void k_filter_pass_multi(KGeom g, int a, double eps,
+                                                              const double* __restrict__ psi,
+                                                              FieldSet F, int nf, int restore) {
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int c[3];
+    coords(g, idx, c);
+    const int N = g.n[a] - 1;
+    const int64_t st = stride_of(g, a);
+    const int i = c[a];
+    bool keep = false;  // wall boundary plane restored from the original
+    if (restore) {
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        if (g.n[b] > 1 && !g.periodic[b] && (c[b] == 0 || c[b] == g.n[b] - 1)) keep = true;
+    }
+    int64_t o = idx, om = 0, op = 0;
+    bool filt = true;
+    if (g.periodic[a]) {
+      const int k = i == N ? 0 : i;
+      o = idx - (int64_t)(i - k) * st;
+      om = k == 0 ? o + (int64_t)(N - 1) * st : o - st;
+      op = k == N - 1 ? o - (int64_t)(N - 1) * st : o + st;
+    } else if (i == 0 || i == N) {
+      filt = false;
+    } else {
+      om = idx - st;
+      op = idx + st;
+    }
+    double w = 0.0;
+    if (filt) {
+      const double s = 2.0 * psi[o] - 1.0;
+      w = eps * (s * s);
+    }
+    for (int f = 0; f < nf; ++f) {
+      double v;
+      if (keep) {
+        v = F.orig[f][idx];
+      } else if (filt) {
+        const double* in = F.in[f];
+        v = in[o] + w * (in[op] - 2.0 * in[o] + in[om]);
+      } else {
+        v = F.in[f][idx];
+      }
+      F.out[f][idx] = v;
+    }
+  }
+}
+
 // d1, d2 stacked [ndim][total]; mixed: 2-D [psi_xy], 3-D [xy, xz, yz].
 __global__ __launch_bounds__(kBlock) void k_curvature(KGeom g, int ndim,
                                                       const double* __restrict__ psi,
@@ -171,6 +230,45 @@ extern "C" tpf_status tpf_dccd_filter_nd(int ndim, const int64_t* n_nodes, const
                        psi, src, f, t == ndim - 1 ? 1 : 0, dst);
     TPF_HIP(hipGetLastError());
     src = dst;
+  }
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_dccd_filter_nd_multi(int ndim, const int64_t* n_nodes,
+                                               const int* periodic, const double* psi, double eps,
+                                               int nf, const double* const* f,
+                                               double* const* out, double* const* scratch,
+                                               void* stream) {
+  KGeom g;
+  TPF_TRY(make_geom(g, ndim, n_nodes, periodic));
+  if (!psi || !f || !out || !scratch) return fail(TPF_ERR_VALUE, "null argument");
+  if (nf < 1 || nf > kMaxFields) return fail(TPF_ERR_VALUE, "1 <= nf <= 8 fields per call");
+  const double* all[3 * kMaxFields];
+  for (int i = 0; i < nf; ++i) {
+    if (!f[i] || !out[i] || !scratch[i]) return fail(TPF_ERR_VALUE, "null field");
+    all[3 * i] = f[i];
+    all[3 * i + 1] = out[i];
+    all[3 * i + 2] = scratch[i];
+  }
+  for (int i = 0; i < 3 * nf; ++i)
+    for (int j = i + 1; j < 3 * nf; ++j)
+      if (all[i] == all[j])
+        return fail(TPF_ERR_VALUE, "fields, outputs and scratch arrays must all be distinct");
+  hipStream_t st = (hipStream_t)stream;
+  // the passes of tpf_dccd_filter_nd, every field per launch
+  FieldSet F{};
+  for (int i = 0; i < nf; ++i) {
+    F.in[i] = f[i];
+    F.orig[i] = f[i];
+  }
+  for (int t = 0; t < ndim; ++t) {
+    const int a = 3 - ndim + t;
+    const bool to_out = (ndim - 1 - t) % 2 == 0;
+    for (int i = 0; i < nf; ++i) F.out[i] = to_out ? out[i] : scratch[i];
+    hipLaunchKernelGGL(k_filter_pass_multi, dim3(blocks_for(g.total)), dim3(kBlock), 0, st, g, a,
+                       eps, psi, F, nf, t == ndim - 1 ? 1 : 0);
+    TPF_HIP(hipGetLastError());
+    for (int i = 0; i < nf; ++i) F.in[i] = F.out[i];
   }
   return TPF_OK;
 }

@@ -12,6 +12,8 @@ tensors out.
 
 from __future__ import annotations
 
+import ctypes as C
+
 import torch
 
 from . import _native as nat
@@ -62,11 +64,16 @@ class CurvatureCalculatorPsi:
             return out
 
         if self.dccd_eps > 0:
-            # filtered components written straight into stacked outputs
+            # every first and second derivative through one batched filter
+            # (one launch per pass), straight into stacked outputs
             d1f, d2f = torch.empty_like(d1), torch.empty_like(d2)
-            for a in range(ndim):
-                filt(d1[a], d1f[a])
-                filt(d2[a], d2f[a])
+            scr = torch.empty((2 * ndim,) + shape, dtype=torch.float64, device="cuda")
+            arr = C.c_void_p * (2 * ndim)
+            fin = arr(*([ptr(d1[a]) for a in range(ndim)] + [ptr(d2[a]) for a in range(ndim)]))
+            fout = arr(*([ptr(d1f[a]) for a in range(ndim)] + [ptr(d2f[a]) for a in range(ndim)]))
+            fscr = arr(*[ptr(scr[k]) for k in range(2 * ndim)])
+            nat.check(self._lib.tpf_dccd_filter_nd_multi(ndim, dims, per, ptr(p), self.dccd_eps,
+                                                         2 * ndim, fin, fout, fscr, st))
             d1, d2 = d1f, d2f
         pairs = [(0, 1)] if ndim == 2 else [(0, 1), (0, 2), (1, 2)]
         mixed = torch.empty((len(pairs),) + shape, dtype=torch.float64, device="cuda")
