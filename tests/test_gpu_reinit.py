@@ -35,6 +35,8 @@ def _psi(shape, eps):
     # long lines (many chunks of the sweep)
     ((400,), "wall", "zero", True),
     ((12, 330), "periodic", "periodic", False),
+    ((600,), "wall", "neumann", False),
+    ((8, 700), "wall", "zero", True),
 ])
 def test_reinit_matches_oracle(N, ccd_bc, ls_bc, mass):
     print(f"[gpu-case] reinit N={N} bc={ls_bc} mass={mass}", file=sys.stderr, flush=True)
