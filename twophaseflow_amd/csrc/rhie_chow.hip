@@ -214,16 +214,30 @@ __global__ __launch_bounds__(kBlock) void k_rc_div_slab(RcGeom g, RcPtrs P, unsi
     else if (per0) d = (f_up - f_lo) / g.h[0];
     else d = -f_lo / g.h[0];
     acc += d;
-#pragma unroll
-    for (int a = 1; a < 3; ++a) {
-      const int N = g.n[a] - 1;
-      const int i = c[a];
-      const int64_t st = stride_of(g, a);
+    {  // slot 1, per node
+      const int N = g.n[1] - 1;
+      const int i = c[1];
+      const int64_t st = stride_of(g, 1);
       const int64_t b = idx - (int64_t)i * st;
       double da;
-      if (i < N) da = (flux_k(g, P, a, b, st, i + 1) - flux_k(g, P, a, b, st, i)) / g.h[a];
-      else if (g.periodic[a]) da = (flux_k(g, P, a, b, st, 0) - flux_k(g, P, a, b, st, N)) / g.h[a];
-      else da = -flux_k(g, P, a, b, st, N) / g.h[a];
+      if (i < N) da = (flux_k(g, P, 1, b, st, i + 1) - flux_k(g, P, 1, b, st, i)) / g.h[1];
+      else if (g.periodic[1]) da = (flux_k(g, P, 1, b, st, 0) - flux_k(g, P, 1, b, st, N)) / g.h[1];
+      else da = -flux_k(g, P, 1, b, st, N) / g.h[1];
+      acc += da;
+    }
+    {  // slot 2 (contiguous): the face below node i is the face above node
+       // i - 1, computed by the previous lane (same row when i > 0)
+      const int N = g.n[2] - 1;
+      const int i = c[2];
+      const int64_t b = idx - i;
+      const double up = i < N ? flux_k(g, P, 2, b, 1, i + 1) : flux_k(g, P, 2, b, 1, 0);
+      const double from_prev = __shfl_up(up, 1, 64);
+      const int lane = threadIdx.x & 63;
+      const double lo = (i > 0 && lane > 0) ? from_prev : flux_k(g, P, 2, b, 1, i);
+      double da;
+      if (i < N) da = (up - lo) / g.h[2];
+      else if (g.periodic[2]) da = (up - lo) / g.h[2];
+      else da = -lo / g.h[2];
       acc += da;
     }
     out[idx] = acc;
