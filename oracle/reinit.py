@@ -14,7 +14,12 @@ Pinning (tests/test_oracle_reinit.py): one CN-ADI sweep multiplies the
 mirror-symmetric cosine modes by the closed-form factor (1 + 2 b c - 6 mu (1 -
 c)) / (1 + 2 b c + 6 mu (1 - c)) (b = -1/8, c = cos theta; M2 and B2 share
 those eigenvectors); dtau follows its formula; the mass correction restores
-sum psi dV; a tanh profile of width eps is close to a fixed point.
+sum psi dV; a tanh profile of width eps is close to a fixed point.  The
+compression step: the filter multiplies a cosine mode's CCD derivative by
+1 - 4 eps_d sin^2(theta/2); on linear fluxes the filtered derivative is the
+slope inside and slope * (1 - eps_d) (zero ghost) or the slope (neumann /
+outflow ghost) at the walls; on a monotone tanh profile (n = +1) the
+compression divergence is d/dx[psi (1 - psi)] = (1 - 2 psi) psi'.
 """
 
 from __future__ import annotations
@@ -102,19 +107,26 @@ class OracleSplitReinit:
         z = np.zeros_like(g[sl(0, 1)])
         return np.concatenate([z, g, z], axis=ax)
 
+    def filtered_divergence(self, flux, a):
+        """CCD f' of one flux component plus the 3-point dissipative filter
+        on one ghost per side (reinit_ops.py:85-108, uniform grids)."""
+        gp = self.ccd.first_derivative(flux, a)
+        pad = self._pad(gp, a)
+        n = gp.shape[a]
+        sl = lambda s, e: tuple(slice(s, e) if k == a else slice(None) for k in range(flux.ndim))
+        c = pad[sl(1, n + 1)]
+        return c + self.eps_d * (pad[sl(2, n + 2)] - 2.0 * c + pad[sl(0, n)])
+
     def compression_div(self, q):
+        """sum_a filtered D1[q (1 - q) n_a], n = grad q / max(|grad q|, floors)
+        (reinit_ops.py:53-83, 111-119)."""
         d = [self.ccd.first_derivative(q, a) for a in range(self.ndim)]
         gs = sum(g * g for g in d)
         safe = np.maximum(np.sqrt(np.maximum(gs, 1e-12)), 1e-6)
         w = q * (1.0 - q)
         div = np.zeros_like(q)
         for a in range(self.ndim):
-            gp = self.ccd.first_derivative(w * (d[a] / safe), a)
-            pad = self._pad(gp, a)
-            n = gp.shape[a]
-            sl = lambda s, e: tuple(slice(s, e) if k == a else slice(None) for k in range(q.ndim))
-            c = pad[sl(1, n + 1)]
-            div = div + (c + self.eps_d * (pad[sl(2, n + 2)] - 2.0 * c + pad[sl(0, n)]))
+            div = div + self.filtered_divergence(w * (d[a] / safe), a)
         return div
 
     def dV(self):

@@ -279,3 +279,43 @@ def test_apply_fields_matches_per_field_apply(N, bc):
         bad[0] = fs[nd - 1]  # an output aliasing another axis' input
         with pytest.raises(ValueError):
             s.apply_fields_into(fs, bad, None, (1 << nd) - 1)
+
+
+@pytest.mark.parametrize("N,bc", [((8, 1024), "wall"), ((24, 1024), "periodic"),
+                                  ((6, 700), "wall_periodic"), ((3, 5, 520), "periodic"),
+                                  ((10, 400), "wall")])
+@pytest.mark.parametrize("mode,cap", [("auto", 0), ("wave", 8), ("wave", 16)])
+def test_long_contiguous_axis(N, bc, mode, cap):
+    """Long lines on the contiguous (last long) axis: the wavefront-per-line
+    kernel's contiguous branch, as used by the 2-D 1025^2 runs, against the
+    dense-LU oracle (f'' on 1024-node periodic lines: max(1e-12, 2 x the
+    dense LU's own error against the closed-form spectral result))."""
+    _announce(f"ccd long contiguous axis N={N} bc={bc} mode={mode} cap={cap}")
+    g = _grid(N)
+    o = occd.OracleCCD(N, [1.0] * len(N), bc)
+    axis = int(np.argmax(N))
+    rng = np.random.default_rng(31)
+    if o.kinds[axis] == "periodic":
+        line = band_limited_periodic(N[axis], 1, kmax=32, seed=2)
+        f_np = np.broadcast_to(line, g.shape).copy() * (1.0 + 0.1 * rng.standard_normal(
+            g.shape[:axis] + (1,) + g.shape[axis + 1:]))
+    else:
+        f_np = rng.standard_normal(g.shape)
+    ccd = CCDSolver(g, bc_type=bc)
+    unknowns = N[axis] if o.kinds[axis] == "periodic" else N[axis] - 1
+    if mode == "wave" and unknowns > 64 * cap:
+        # a wavefront holds at most 64 partitions of `cap` rows
+        with pytest.raises(ValueError, match="too long"):
+            ccd.set_launch(axis, mode, cap)
+        return
+    ccd.set_launch(axis, mode, cap)
+    d1, d2 = ccd.differentiate(torch.from_numpy(f_np).cuda(), axis)
+    d1, d2 = d1.cpu().numpy(), d2.cpu().numpy()
+    r1, r2 = o.differentiate(f_np, axis)
+    assert _rel(d1, r1) <= 1e-12, _rel(d1, r1)
+    bound = 1e-12
+    if o.kinds[axis] == "periodic" and N[axis] >= 1024:
+        s1, s2 = occd.periodic_spectral(f_np, axis, 1.0 / N[axis])
+        bound = max(1e-12, 2.0 * _rel(r2, s2))
+        assert _rel(d2, s2) <= bound, (_rel(d2, s2), bound)
+    assert _rel(d2, r2) <= bound, (_rel(d2, r2), bound)

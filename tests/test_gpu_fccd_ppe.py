@@ -223,6 +223,14 @@ def test_defect_correction_fccd_over_fvm_multigrid_65cubed():
     h = dc.last_residual_history
     assert dc.last_diagnostics["ppe_dc_converged"] == 1.0, h
     assert all(b < a for a, b in zip(h, h[1:])), h
+    # the oracle's FP64 residual of the FCCD system judges the DC result
+    o = OracleFCCDPPE(N, (1.0,) * 3, "wall")
+    o.prepare_operator(rho.cpu().numpy())
+    b = o.system_rhs(rhs.cpu().numpy())
+    rel = np.linalg.norm(o.apply(p.cpu().numpy()) - b) / np.linalg.norm(b)
+    assert rel <= 1.01e-8, rel
+    reported = dc.last_diagnostics["ppe_dc_final_relative_l2"]
+    assert abs(rel - reported) <= 0.05 * rel, (rel, reported)
     pg = torch.empty_like(rhs)
     op.solve_into(rhs, rho, pg)
     assert float((p - pg).abs().max()) <= 1e-5 * float(pg.abs().max())

@@ -302,3 +302,70 @@ def test_native_mg_fcg_nonconvergence_raises():
         c2 = _mg_cfg((8, 8))
         c2.solver.ppe_iteration_method = "gmres"
         PPESolverFVMMatrixFree(None, c2, Grid(c2.grid))
+
+
+def test_stagnating_fcg_falls_back_to_jacobi_pcg():
+    """A preconditioner that returns noise unrelated to r stalls FCG: the
+    stall rule restarts it from the true residual, gives up, and the plain
+    Jacobi-PCG fallback (tpf_ppe_solve, precond=1) finishes the solve.  The
+    FP64 residual of the reference CG system, recomputed by the oracle,
+    judges the result."""
+    _announce("fcg stagnation -> Jacobi-PCG fallback, 24^3")
+    N = (24, 24, 24)
+    s, H, O, rho = _setup(N)
+    rhs = np.random.default_rng(21).standard_normal(s.shape)
+    rhs_t = torch.from_numpy(rhs).cuda()
+    gen = torch.Generator(device="cuda").manual_seed(5)
+
+    def garbage(r):
+        return torch.randn(r.shape, device=r.device, dtype=torch.float64, generator=gen)
+
+    p, info = L.fcg_solve(s, rhs_t, garbage, tol=1e-8, maxiter=2000, m=1, stall_window=20,
+                          restarts=2, fallback=True, fallback_maxiter=20000)
+    assert info["fcg_exit"] == "stagnation", info["fcg_exit"]
+    assert info["fcg_restarts"] == 2
+    assert info["fallback_iterations"] > 0 and info["fallback_converged"]
+    o = oppe.OraclePPE(N)
+    o.prepare_operator(rho)
+    judged = o.relres(rhs, p.cpu().numpy())
+    assert judged <= 1e-8 * 1.01, judged
+    assert abs(info["true_relres"] - judged) <= 1e-3 * judged
+
+
+def test_fallback_budget_exhausted_reports_without_raising():
+    """A fallback that runs out of iterations leaves the iterate in place and
+    reports it: no exception, fallback_converged False, and a true residual
+    above tol that the oracle confirms."""
+    _announce("fcg fallback budget exhausted, 24^3")
+    N = (24, 24, 24)
+    s, H, O, rho = _setup(N)
+    rhs = np.random.default_rng(22).standard_normal(s.shape)
+    rhs_t = torch.from_numpy(rhs).cuda()
+    p, info = L.fcg_solve(s, rhs_t, L.MGPreconditioner(H, nu=2, gamma=1), tol=1e-12,
+                          maxiter=2, m=1, fallback=True, fallback_maxiter=3)
+    assert info["fcg_exit"] == "maxiter"
+    assert not info["fallback_converged"]
+    o = oppe.OraclePPE(N)
+    o.prepare_operator(rho)
+    judged = o.relres(rhs, p.cpu().numpy())
+    assert judged > 1e-12 and abs(info["true_relres"] - judged) <= 1e-3 * judged
+
+
+def test_native_learned_fcg_fallback_engages():
+    """The device learned FCG stopped after two iterations hands its iterate
+    to the Jacobi-PCG fallback, which converges; the oracle's FP64 residual
+    judges it."""
+    _announce("native learned FCG maxiter=2 -> fallback, 64^3")
+    N = (64, 64, 64)
+    s, H, O, rho = _setup(N)
+    net = L.UNet(seed=3).cuda()
+    level = L.level_of(H, 33)
+    M = L.LearnedDeflationPreconditioner(H, net, level, gamma=1)
+    rhs = np.random.default_rng(23).standard_normal(s.shape)
+    p, info = L.fcg_solve_native(s, torch.from_numpy(rhs).cuda(), M, tol=1e-8, maxiter=2,
+                                 gamma=1, w_levels=0, fallback=True)
+    assert info["fcg_exit"] == "maxiter"
+    assert info["fallback_iterations"] > 0 and info["fallback_converged"]
+    o = oppe.OraclePPE(N)
+    o.prepare_operator(rho)
+    assert o.relres(rhs, p.cpu().numpy()) <= 1e-8 * 1.01

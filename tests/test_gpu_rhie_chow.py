@@ -26,6 +26,11 @@ pytestmark = pytest.mark.gpu
     # several 16-plane slabs along axis 0, mixed boundaries
     ((40, 6, 5), ("wall", "periodic", "wall"), (2.0, 1.0, 1.0), True),
     ((33, 7, 6), ("periodic", "wall", "periodic"), (1.0, 0.5, 1.0), False),
+    # planes of n1*n2 > 256 nodes over several workgroups, rows longer than a
+    # wavefront (the 257^3 launch shape's paths)
+    ((20, 4, 80), ("wall", "periodic", "wall"), (1.0, 0.4, 1.0), True),
+    ((20, 5, 80), ("periodic", "wall", "periodic"), (1.0, 0.4, 1.0), False),
+    ((17, 9, 130), "wall", (1.0, 0.5, 2.0), True),
 ])
 def test_rhie_chow_matches_oracle(N, bc, L, bf):
     print(f"[gpu-case] rhie-chow N={N} bc={bc} bf={bf}", file=sys.stderr, flush=True)

@@ -75,3 +75,43 @@ def test_filter_keeps_linear_fields_and_wall_planes(bc):
         assert np.array_equal(out[0], g[0]) and np.array_equal(out[:, -1], g[:, -1])
     else:
         assert np.array_equal(out[-1], out[0]) and np.array_equal(out[:, -1], out[:, 0])
+
+
+@pytest.mark.parametrize("psi0", [0.8, 0.15, 0.5])
+def test_filter_transfer_factor_constant_psi(psi0):
+    """Constant psi: the filter coefficient eps (2 psi - 1)^2 is a constant s,
+    so a periodic cosine mode cos(t1 i) cos(t2 j) is multiplied by
+    (1 - 4 s sin^2(t1/2)) (1 - 4 s sin^2(t2/2)); psi = 0.5 switches it off."""
+    N = (32, 24)
+    eps = 0.05
+    i, j = np.meshgrid(np.arange(33), np.arange(25), indexing="ij")
+    t1, t2 = 2 * np.pi * 3 / 32, 2 * np.pi * 5 / 24
+    f = np.cos(t1 * i) * np.cos(t2 * j)
+    psi = np.full(f.shape, psi0)
+    out = dccd_filter_nd(f, N, ("periodic", "periodic"), eps, psi)
+    s = eps * (2 * psi0 - 1) ** 2
+    G = (1 - 4 * s * np.sin(t1 / 2) ** 2) * (1 - 4 * s * np.sin(t2 / 2) ** 2)
+    assert np.abs(out - G * f).max() < 1e-14
+    # wall axes: interior nodes of a mirror-symmetric mode carry the same
+    # factor, the wall planes keep the input
+    k1, k2 = np.pi * 3 / 32, np.pi * 4 / 24
+    g = np.cos(k1 * i) * np.cos(k2 * j)
+    out = dccd_filter_nd(g, N, ("wall", "wall"), eps, psi)
+    Gw = (1 - 4 * s * np.sin(k1 / 2) ** 2) * (1 - 4 * s * np.sin(k2 / 2) ** 2)
+    assert np.abs(out[1:-1, 1:-1] - Gw * g[1:-1, 1:-1]).max() < 1e-14
+    assert np.array_equal(out[0], g[0]) and np.array_equal(out[:, -1], g[:, -1])
+
+
+def test_filter_coefficient_is_pointwise():
+    """psi varying along axis 0 only and f = x^2 along axis 0: the filter
+    adds eps (2 psi_i - 1)^2 * 2 h^2 at interior node i (the axis-1 pass
+    sees a field constant along axis 1)."""
+    N = (20, 6)
+    eps, h = 0.05, 1.0 / 20
+    x = np.linspace(0, 1, 21)
+    psi = np.broadcast_to((0.5 + 0.45 * np.sin(7 * x))[:, None], (21, 7)).copy()
+    f = np.broadcast_to((x ** 2)[:, None], (21, 7)).copy()
+    out = dccd_filter_nd(f, N, ("wall", "wall"), eps, psi)
+    want = f[1:-1] + eps * (2 * psi[1:-1] - 1) ** 2 * 2 * h * h
+    assert np.abs(out[1:-1, 1:-1] - want[:, 1:-1]).max() < 1e-15
+    assert np.array_equal(out[:, 0], f[:, 0]) and np.array_equal(out[:, -1], f[:, -1])

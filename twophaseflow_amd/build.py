@@ -1,12 +1,14 @@
 """Build recipe for libtpf.so (HIP C++ for gfx950), in-tree.
 
 ``python -m twophaseflow_amd.build`` compiles every source under ``csrc/``
-with hipcc into ``twophaseflow_amd/libtpf.so``.  One hipcc invocation, no
-JIT cache: the built library travels with the source tree.
+with hipcc into ``twophaseflow_amd/libtpf.so``: one object per source,
+compiled concurrently into ``_build/`` at the repository root, then one
+link.  No JIT cache: the built library travels with the source tree.
 """
 
 from __future__ import annotations
 
+import concurrent.futures
 import glob
 import os
 import subprocess
@@ -15,6 +17,7 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libtpf.so")
+OBJDIR = os.path.join(os.path.dirname(PKG), "_build")
 ARCH = os.environ.get("TPF_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -22,7 +25,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-command-line-argument",
@@ -37,11 +39,7 @@ def sources():
                   glob.glob(os.path.join(CSRC, "*.cpp")))
 
 
-def build(verbose: bool = False) -> str:
-    """Compile libtpf.so; raises CalledProcessError with hipcc's output."""
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, *sources()]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -50,6 +48,21 @@ def build(verbose: bool = False) -> str:
         raise subprocess.CalledProcessError(res.returncode, cmd, res.stdout, res.stderr)
     if verbose and res.stderr.strip():
         sys.stderr.write(res.stderr)
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compile libtpf.so; raises CalledProcessError with hipcc's output."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    os.makedirs(OBJDIR, exist_ok=True)
+    srcs = sources()
+    objs = [os.path.join(OBJDIR, os.path.basename(s) + ".o") for s in srcs]
+    cmds = [[hipcc, *HIPCC_FLAGS, "-c", "-o", o, s] for s, o in zip(srcs, objs)]
+    jobs = jobs or min(len(cmds), max(1, min(16, os.cpu_count() or 1)))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        for fut in [ex.submit(_run, c, verbose) for c in cmds]:
+            fut.result()
+    tmp = LIB + ".tmp"
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -31,7 +31,7 @@ __device__ __forceinline__ int64_t stride_of(const KGeom& g, int a) {
 }
 
 __device__ __forceinline__ void coords(const KGeom& g, int64_t idx, int (&c)[3]) {
-  const uint32_t plane = (uint32_t)(g.n[1] * g.n[2]);
+  const uint32_t plane = (uint32_t)g.n[1] * (uint32_t)g.n[2];
   c[0] = (int)((uint32_t)idx / plane);
   const uint32_t rem = (uint32_t)idx - (uint32_t)c[0] * plane;
   c[1] = (int)(rem / (uint32_t)g.n[2]);

@@ -54,7 +54,7 @@ struct FccdGeom {
 };
 
 __device__ __forceinline__ void coords(const FccdGeom& g, int64_t idx, int (&c)[3]) {
-  const uint32_t plane = (uint32_t)(g.n[1] * g.n[2]);
+  const uint32_t plane = (uint32_t)g.n[1] * (uint32_t)g.n[2];
   const uint32_t u = (uint32_t)idx;
   c[0] = (int)(u / plane);
   const uint32_t rem = u - (uint32_t)c[0] * plane;

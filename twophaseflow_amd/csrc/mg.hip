@@ -70,7 +70,7 @@ __host__ __device__ inline int64_t lstride(const LevelGeom& g, int a) {
 }
 
 __device__ inline void lcoords(const LevelGeom& g, int64_t idx, int (&c)[3]) {
-  const uint32_t plane = (uint32_t)(g.n[1] * g.n[2]);
+  const uint32_t plane = (uint32_t)g.n[1] * (uint32_t)g.n[2];
   const uint32_t u = (uint32_t)idx;
   c[0] = (int)(u / plane);
   const uint32_t rem = u - (uint32_t)c[0] * plane;

@@ -106,7 +106,7 @@ __device__ __forceinline__ double flux_k(const RcGeom& g, const RcPtrs& P, int a
 __global__ __launch_bounds__(kBlock) void k_rc_div(RcGeom g, RcPtrs P, double* __restrict__ out) {
   for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
        idx += (int64_t)gridDim.x * kBlock) {
-    const uint32_t plane = (uint32_t)(g.n[1] * g.n[2]);
+    const uint32_t plane = (uint32_t)g.n[1] * (uint32_t)g.n[2];
     int c[3];
     c[0] = (int)((uint32_t)idx / plane);
     const uint32_t rem = (uint32_t)idx - (uint32_t)c[0] * plane;

@@ -377,15 +377,36 @@ def deflation_loss(hier, net, level, samples, ridge=1e-6):
 
 
 # ----------------------------------------------------------- FCG solve ---
+def pcg_fallback(ppe_solver, rhs, x, tol, maxiter):
+    """Plain Jacobi-PCG (``tpf_ppe_solve`` with precond=1, FP64) warm-started
+    from x, in place, with its own iteration budget.  Whatever route
+    ``ppe_solver`` is configured for (GMRES, multigrid-FCG), the fallback is
+    this one.  Non-convergence within `maxiter` is reported, not raised: the
+    caller's FP64 true residual judges the result.  Returns (iterations,
+    converged)."""
+    lib = _lib()
+    opts = nat.SolveOpts(tol=tol, maxiter=int(maxiter), precond=1,
+                         check_every=int(getattr(ppe_solver, "check_every", 50)), use_x0=1,
+                         mixed=0, inner_tol=0.0)
+    rep = nat.SolveReport()
+    status = lib.tpf_ppe_solve(ppe_solver._handle, _p(rhs), _p(x), C.byref(opts),
+                               C.byref(rep))
+    if status not in (nat.TPF_OK, nat.TPF_ERR_RUNTIME):
+        nat.check(status)
+    return int(rep.iterations), status == nat.TPF_OK
+
+
 def fcg_solve(ppe_solver, rhs, precond, tol=1e-8, maxiter=2000, m=8, stall_window=30,
-              stall_factor=0.7, restarts=3, fallback=True, record=()):
+              stall_factor=0.7, restarts=3, fallback=True, record=(),
+              fallback_maxiter=20000):
     """Flexible CG on the CG system in FP64 with a nonlinear preconditioner.
 
     FCG(m) (Notay 2000): each new direction is A-orthogonalised against the
     last m directions.  Progress is tracked on the best residual seen; when
     it fails to improve by `stall_factor` over `stall_window` iterations the
     recursion restarts from the FP64 true residual (at most `restarts` times)
-    and then plain Jacobi-PCG (warm-started from x) takes over.  The FP64
+    and then plain Jacobi-PCG (warm-started from x, at most
+    `fallback_maxiter` iterations, ``pcg_fallback``) takes over.  The FP64
     true residual judges the result.  Residuals of the iterations listed in
     `record` are kept in info["records"].  Returns (p, info)."""
     lib = _lib()
@@ -443,8 +464,8 @@ def fcg_solve(ppe_solver, rhs, precond, tol=1e-8, maxiter=2000, m=8, stall_windo
     info = {"fcg_iterations": it, "fcg_exit": reason, "fcg_restarts": nrestart,
             "fcg_seconds": time.perf_counter() - t0, "fallback_iterations": 0}
     if reason != "converged" and fallback:
-        ppe_solver.solve_into(rhs, None, x, use_x0=True)
-        info["fallback_iterations"] = ppe_solver.last_diagnostics["iterations"]
+        info["fallback_iterations"], info["fallback_converged"] = pcg_fallback(
+            ppe_solver, rhs, x, tol, fallback_maxiter)
     info["true_relres"] = ppe_solver.true_relres(rhs, x)
     info["history"] = hist
     info["records"] = records
@@ -452,13 +473,14 @@ def fcg_solve(ppe_solver, rhs, precond, tol=1e-8, maxiter=2000, m=8, stall_windo
 
 
 def fcg_solve_native(ppe_solver, rhs, precond, tol=1e-8, maxiter=500, nu=2, gamma=2,
-                     coarse_sweeps=20, w_levels=2, fallback=True):
+                     coarse_sweeps=20, w_levels=2, fallback=True, fallback_maxiter=20000):
     """The device FCG(1) of ``tpf_ppe_solve_mg`` (graph-replayed multigrid
     cycle, FP64 vectors and reductions, FP64 true-residual judge) with the
     learned coarse correction of ``precond`` (a LearnedDeflationPreconditioner)
     applied after every cycle on the device (``tpf_ppe_solve_mg_deflated``):
     z = V(r); z += P W E^-1 W^T R (r - A z).  Plain Jacobi-PCG from the FCG
-    iterate is the fallback when FCG stops above tol."""
+    iterate (``pcg_fallback``) is the fallback when FCG stops above tol; the
+    FP64 true residual judges the result."""
     if precond.net is None:
         raise ValueError("fcg_solve_native needs a learned coarse space (net is None)")
     lib = _lib()
@@ -482,8 +504,8 @@ def fcg_solve_native(ppe_solver, rhs, precond, tol=1e-8, maxiter=500, nu=2, gamm
     if status not in (nat.TPF_OK, nat.TPF_ERR_RUNTIME):
         nat.check(status)
     if status != nat.TPF_OK and fallback:
-        ppe_solver.solve_into(rhs, None, p, use_x0=True)
-        info["fallback_iterations"] = ppe_solver.last_diagnostics["iterations"]
+        info["fallback_iterations"], info["fallback_converged"] = pcg_fallback(
+            ppe_solver, rhs, p, tol, fallback_maxiter)
     info["true_relres"] = ppe_solver.true_relres(rhs, p)
     return p, info
 
