@@ -336,6 +336,21 @@ TPF_API tpf_status tpf_sync_periodic_images(int ndim, const int64_t* n_nodes,
 TPF_API tpf_status tpf_dccd_mass_correct(int ndim, const int64_t* n_nodes, const double* L,
                                          const int* periodic, const double* q, double m_target,
                                          double* out, double* mass_before, void* stream);
+/* tpf_dccd_mass_correct without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under stream capture, tight stepping loops, harnesses that forbid
+ * hipMalloc).  Its partial sums and correction factor live in a fixed
+ * device array of 2 * 2048 + 8 doubles (33 KB, one per device, kept until
+ * the process ends); the factor is formed on the device.  With mass_before
+ * NULL the call is asynchronous on `stream` and synchronises nothing; with
+ * mass_before set it copies sum q dV back and synchronises the stream.
+ * Every call on one device shares the array, so calls that overlap on a
+ * device (other streams, other threads) race; nothing in it needs clearing
+ * between calls. */
+TPF_API tpf_status tpf_dccd_mass_correct_noalloc(int ndim, const int64_t* n_nodes,
+                                                 const double* L, const int* periodic,
+                                                 const double* q, double m_target, double* out,
+                                                 double* mass_before, void* stream);
 
 /* Curvature straight from psi (CurvatureCalculatorPsi,
  * levelset/curvature_psi.py:45-196).  tpf_dccd_filter_nd: the DCCD filter of

@@ -95,6 +95,8 @@ SIGNATURES = {
     "tpf_sync_periodic_images": (C.c_int, [C.c_int, _i64p, _ip, _vp, _vp, _vp]),
     "tpf_dccd_mass_correct": (C.c_int, [C.c_int, _i64p, _dp, _ip, _vp, C.c_double, _vp, _dp,
                                         _vp]),
+    "tpf_dccd_mass_correct_noalloc": (C.c_int, [C.c_int, _i64p, _dp, _ip, _vp, C.c_double, _vp,
+                                                _dp, _vp]),
     "tpf_dccd_filter_nd": (C.c_int, [C.c_int, _i64p, _ip, _vp, C.c_double, _vp, _vp, _vp,
                                      _vp]),
     "tpf_dccd_filter_nd_multi": (C.c_int, [C.c_int, _i64p, _ip, _vp, C.c_double, C.c_int, _vp, _vp,

@@ -16,10 +16,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "reduce.hpp"
 #include "tpf_common.hpp"
+
+// Fixed device scratch of the allocation-free mass correction
+// (tpf_dccd_mass_correct_noalloc): 2 partial sums per workgroup of the
+// reduction (at most 2048 workgroups), the two finished sums and the factor.
+// One copy per device, alive until the process ends; every call on a device
+// shares it.  In host code `scratch` names a host-side handle, not device
+// memory: kernels get the address from hipGetSymbolAddress.
+static constexpr int kMassScratchDoubles = 2 * 2048 + 8;
+static __device__ __align__(256) double scratch[kMassScratchDoubles];
 
 namespace tpf {
 namespace {
@@ -162,6 +172,42 @@ __global__ __launch_bounds__(kBlock) void k_dccd_mass_fin(const double* partials
   }
 }
 
+// The two sums of k_dccd_mass_sums finished as k_dccd_mass_fin does, then the
+// correction factor formed on the device (heaviside.py:138-149):
+// ratio = (m_target - M_new) / W gated at W > 1e-12.  out[0..1] = (M_new, W),
+// out[2] = factor.
+__global__ __launch_bounds__(kBlock) void k_dccd_mass_fin_factor(const double* partials,
+                                                                 int nblocks, double m_target,
+                                                                 double* out) {
+  __shared__ double sh[2 * kBlock / 64];
+  double v[2];
+  finish_sum<2>(v, partials, nblocks, sh);
+  if (threadIdx.x == 0) {
+    const double W = v[1];
+    const double ratio = (m_target - v[0]) / (W > 1e-12 ? W : 1.0);
+    out[0] = v[0];
+    out[1] = W;
+    out[2] = (W > 1e-12 ? 1.0 : 0.0) * ratio;
+  }
+}
+
+// out = clip(q + factor 4 q (1 - q), 0, 1) at the root of every node, the
+// factor read from device memory.
+__global__ __launch_bounds__(kBlock) void k_dccd_mass_apply_dev(DGeom g,
+                                                                const double* __restrict__ q,
+                                                                const double* __restrict__ factor,
+                                                                double* __restrict__ out) {
+  const double fac = *factor;
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < g.total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    int c[3];
+    coords(g, idx, c);
+    const double qi = q[root_of(g, idx, c)];
+    const double w = 4.0 * qi * (1.0 - qi);
+    out[idx] = fmin(fmax(qi + fac * w, 0.0), 1.0);
+  }
+}
+
 // out = clip(q + factor 4 q (1 - q), 0, 1) at the root of every node.
 __global__ __launch_bounds__(kBlock) void k_dccd_mass_apply(DGeom g, const double* __restrict__ q,
                                                             double factor,
@@ -228,8 +274,37 @@ __global__ __launch_bounds__(kBlock) void k_dccd_sync_planes(DGeom g, int64_t np
   }
 }
 
+constexpr int kMaxBlocks = 2048;
+
 unsigned blocks_for(int64_t total) {
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + kBlock - 1) / kBlock, 2048));
+  return (unsigned)std::max<int64_t>(1,
+                                     std::min<int64_t>((total + kBlock - 1) / kBlock, kMaxBlocks));
+}
+
+static_assert(2 * kMaxBlocks + 3 <= kMassScratchDoubles, "mass scratch too small");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.  The symbol's name is only a host-side
+// handle: kernels receive this address.
+tpf_status mass_scratch(double** out) {
+  int dev = 0;
+  TPF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(TPF_ERR_RUNTIME, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(TPF_ERR_RUNTIME, std::string("hipGetSymbolAddress(scratch): ") +
+                                       hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return TPF_OK;
 }
 
 }  // namespace
@@ -316,5 +391,33 @@ extern "C" tpf_status tpf_dccd_mass_correct(int ndim, const int64_t* n_nodes, co
   if (mass_before) *mass_before = M_new;
   hipLaunchKernelGGL(k_dccd_mass_apply, dim3(nb), dim3(kBlock), 0, st, g, q, factor, out);
   TPF_HIP(hipGetLastError());
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_dccd_mass_correct_noalloc(int ndim, const int64_t* n_nodes,
+                                                    const double* L, const int* periodic,
+                                                    const double* q, double m_target, double* out,
+                                                    double* mass_before, void* stream) {
+  DGeom g;
+  if (!L) return fail(TPF_ERR_VALUE, "null argument");
+  TPF_TRY(make_geom(g, ndim, n_nodes, L, periodic));
+  if (!q || !out || out == q) return fail(TPF_ERR_VALUE, "q and out must be distinct arrays");
+  const unsigned nb = blocks_for(g.total);
+  if (2 * (int64_t)nb + 3 > kMassScratchDoubles)
+    return fail(TPF_ERR_VALUE, "mass correction needs more scratch than the fixed maximum");
+  double* sc = nullptr;
+  TPF_TRY(mass_scratch(&sc));
+  hipStream_t st = (hipStream_t)stream;
+  double* sums = sc + 2 * (size_t)nb;  // (M_new, W, factor)
+  hipLaunchKernelGGL(k_dccd_mass_sums, dim3(nb), dim3(kBlock), 0, st, g, q, sc);
+  hipLaunchKernelGGL(k_dccd_mass_fin_factor, dim3(1), dim3(kBlock), 0, st, (const double*)sc,
+                     (int)nb, m_target, sums);
+  hipLaunchKernelGGL(k_dccd_mass_apply_dev, dim3(nb), dim3(kBlock), 0, st, g, q,
+                     (const double*)(sums + 2), out);
+  TPF_HIP(hipGetLastError());
+  if (mass_before) {
+    TPF_HIP(hipMemcpyAsync(mass_before, sums, sizeof(double), hipMemcpyDeviceToHost, st));
+    TPF_HIP(hipStreamSynchronize(st));
+  }
   return TPF_OK;
 }
