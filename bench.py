@@ -579,6 +579,78 @@ def reinit_bench(reps=5):
     return {"ms": round(ms, 3), "pseudo_steps": 4, "timing": "median of per-call HIP events"}
 
 
+def legacy_config1_bench(steps=20, cpu=True):
+    """The 2-D 64 x 64 legacy rising-bubble loop (config 1, extension
+    "none"): CFL time step, DCCD advection + split reinit, properties,
+    curvature, UCCD6 / Picard-CN predictor, Rhie-Chow right-hand side,
+    Jacobi-PCG PPE to 1e-10, corrector, non-finite check; `steps` steps timed
+    by the host clock after a 2-step warm-up on another simulation object.
+    The oracle's run of the same steps on the host is the CPU side."""
+    import numpy as np
+    import torch
+    from twophaseflow_amd import (FluidConfig, GridConfig, NumericsConfig, SimulationBuilder,
+                                  SimulationConfig, SolverConfig)
+    n = 64
+
+    def cfg():
+        c = SimulationConfig(grid=GridConfig(ndim=2, N=(n, n), L=(1.0, 1.0)),
+                             fluid=FluidConfig(Re=100.0, Fr=1.0, We=10.0, rho_ratio=0.1,
+                                               mu_ratio=0.1),
+                             numerics=NumericsConfig(advection_scheme="dissipative_ccd",
+                                                     cfl_number=0.3, extension_method="none"),
+                             solver=SolverConfig(ppe_solver_type="fvm_iterative",
+                                                 pseudo_tol=1e-10))
+        c.solver.ppe_iteration_method = "cg"
+        return c
+
+    x = np.linspace(0.0, 1.0, n + 1)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    psi0 = 1.0 / (1.0 + np.exp(-(np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2) - 0.25) / (1.5 / n)))
+
+    def run(k):
+        sim = SimulationBuilder(cfg()).build()
+        sim.psi.data = psi0
+        sim._update_properties()
+        sim._update_curvature()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its = 0
+        for _ in range(k):
+            dt = sim.cfl_calc.compute([sim.velocity[a] for a in range(2)], sim.mu.data,
+                                      sim.rho.data)
+            sim._step_with_retry(dt)
+            its += sim.ppe_solver.last_diagnostics["iterations"]
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, sim, its
+
+    run(2)
+    sec, sim, its = run(steps)
+    out = {"steps": steps, "seconds": round(sec, 4), "steps_per_s": round(steps / sec, 2),
+           "ms_per_step": round(sec / steps * 1e3, 3), "pcg_iterations": its,
+           "div_max_final": sim._diagnostics.divergence_max(sim),
+           "mass_final": sim._diagnostics.volume(sim)}
+    if cpu:
+        from oracle import legacy_step as ls
+        o = ls.OracleLegacySimulation((n, n), (1.0, 1.0))
+        o.psi = psi0.copy()
+        o.update_properties()
+        o.update_curvature()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o.step_forward(o.cfl_dt())
+        cs = time.perf_counter() - t0
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except Exception:
+            cores = os.cpu_count() or 1
+        out["cpu_baseline"] = {"kind": "port", "seconds": round(cs, 4),
+                               "steps_per_s": round(steps / cs, 2), "cores": cores,
+                               "sample": f"oracle legacy step (NumPy / SciPy, dense-LU CCD, "
+                                         f"SciPy CG), the same {steps} steps"}
+        out["gpu_vs_cpu"] = round(cs / sec, 2)
+    return out
+
+
 def profile_mode(args):
     import torch
     from twophaseflow_amd import CCDSolver, Grid, GridConfig
@@ -620,6 +692,9 @@ def profile_mode(args):
         log(f"profile n3 DCCD step 257^3: {dccd_bench(reps=args.steps)}")
         log(f"profile n3 psi curvature 257^3: {curvature_bench(reps=args.steps)}")
         log(f"profile n3 split reinit 257^3: {reinit_bench(reps=args.steps)}")
+    elif args.profile == "legacy":
+        log(f"profile legacy config-1 64^2, 2 + {args.steps} steps: "
+            f"{legacy_config1_bench(steps=args.steps, cpu=False)}")
     elif args.profile == "ppe":
         gp, rho, rhs = ppe_case(N=128)
         r = ppe_solve_timed(gp, rho, rhs, "fp64")
@@ -640,14 +715,16 @@ def main():
     ap.add_argument("--ppe-n", type=int, default=256)
     ap.add_argument("--train-problems", type=int, default=100,
                     help="random 129^3 bubble problems used to train the learned coarse space")
-    ap.add_argument("--profile", choices=["ccd", "ppe", "mg", "fccd", "n3"], default=None,
+    ap.add_argument("--profile", choices=["ccd", "ppe", "mg", "fccd", "n3", "legacy"],
+                    default=None,
                     help="run only one workload (for rocprofv3): 'ccd' = K all-axes CCD steps "
                          "on 257^3 plus one 1 GiB device copy (counter scale reference); "
                          "'ppe' = one FP64 PCG solve of the 129^3 ratio-1000 case; "
                          "'mg' = one W(2,2) multigrid-FCG solve of the 257^3 case; "
                          "'fccd' = K FCCD PPE matvecs on the 257^3 case; "
                          "'n3' = K calls each of the Rhie-Chow divergence, the DCCD step, "
-                         "the psi curvature and the split reinitialisation on 257^3")
+                         "the psi curvature and the split reinitialisation on 257^3; "
+                         "'legacy' = 2 + K steps of the 2-D 64^2 legacy rising-bubble loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
@@ -843,6 +920,9 @@ def main():
         log("FCCD matrix-free PPE: 257^3 matvec, 65^3 Jacobi-GMRES(80) solve")
         result["fccd_ppe"] = fccd_ppe_bench(cpu=not args.no_cpu)
         log(f"  {result['fccd_ppe']}")
+        log("legacy config-1 loop: 2-D 64^2 rising bubble, 20 steps")
+        result["legacy_config1_64"] = legacy_config1_bench(cpu=not args.no_cpu)
+        log(f"  {result['legacy_config1_64']}")
 
     if not args.no_cpu:
         log("CPU baseline (dense-LU CCD slab)")

@@ -352,6 +352,99 @@ TPF_API tpf_status tpf_dccd_mass_correct_noalloc(int ndim, const int64_t* n_node
                                                  const double* q, double m_target, double* out,
                                                  double* mass_before, void* stream);
 
+/* ------------------------------------------------- legacy time step --- */
+/* Material properties (levelset/heaviside.py:167-187): rho = rho_g +
+ * (rho_l - rho_g) psi, mu likewise; phi (may be NULL) = the inverse
+ * Heaviside eps ln(c / (1 - c)), c = clip(psi, 1e-6, 1 - 1e-6)
+ * (heaviside.py:90-118), which update_legacy_curvature stores as sim.phi
+ * (simulation/legacy_flow_helpers.py:83-86).  Asynchronous. */
+TPF_API tpf_status tpf_ns_properties(int64_t n, const double* psi, double rho_l, double rho_g,
+                                     double mu_l, double mu_g, double eps, double* rho,
+                                     double* mu, double* phi, void* stream);
+/* UCCD6ConvectionTerm.compute (ns_terms/uccd6_convection.py:88-128) on the
+ * plan's grid: out[j] = sum_k [-(u_k D1_k u_j + D1_k(u_k u_j))/2
+ * - sigma h_k^7 max|u_k| D2_k^4 u_j], h_k = L[k]/(n_nodes[k]-1), the
+ * maxima kept on the device.  vel, out: ndim device arrays; work: at least
+ * tpf_ns_uccd6_work(ndim, n_nodes) doubles.  Asynchronous. */
+TPF_API int64_t tpf_ns_uccd6_work(int ndim, const int64_t* n_nodes);
+TPF_API tpf_status tpf_ns_uccd6_convection(tpf_ccd_plan* plan, int ndim, const int64_t* n_nodes,
+                                           const double* L, const double* const* vel,
+                                           double sigma, double* const* out, double* work,
+                                           void* stream);
+/* The viscous body of ViscousTerm with the default ccd_bulk scheme
+ * (ns_terms/viscous_spatial.py:39-73, 93-130, 156-169): out[a] = sum_b
+ * Dlow_b[mu (D1_b u_a + D1_a u_b)] / (Re rho), Dlow first-order one-sided at
+ * both ends of each axis and three-point inside.  work: at least
+ * tpf_ns_viscous_work(ndim, n_nodes) doubles.  Asynchronous. */
+TPF_API int64_t tpf_ns_viscous_work(int ndim, const int64_t* n_nodes);
+TPF_API tpf_status tpf_ns_viscous(tpf_ccd_plan* plan, int ndim, const int64_t* n_nodes,
+                                  const double* L, const double* const* vel, const double* mu,
+                                  const double* rho, double Re, double* const* out, double* work,
+                                  void* stream);
+/* Predictor.compute's explicit right-hand side (time_integration/
+ * ab2_predictor.py:150-190): rhs[c] = rho ab2_c - [c = last] rho/Fr^2 +
+ * kappa dpsi_c / We - dp_c, ab2 = conv (conv_prev NULL: Euler start-up) or
+ * 3/2 conv - 1/2 conv_prev; dp_c (the CCD gradient of p, stacked by axis)
+ * is taken as zero on the wall planes normal to c (coupling/
+ * velocity_corrector.py:39-50).  Also writes the Euler point of the Picard
+ * CN advance u_pred = u + dt (rhs / rho + visc_n) (time_integration/
+ * cn_advance/picard_cn.py:66-84).  Asynchronous. */
+TPF_API tpf_status tpf_ns_predictor_rhs(int ndim, const int64_t* n_nodes, const int* periodic,
+                                        const double* const* vel, const double* const* conv,
+                                        const double* const* conv_prev, const double* rho,
+                                        const double* kappa, const double* dpsi, const double* dp,
+                                        double Fr, double We, double dt,
+                                        const double* const* visc_n, double* const* rhs,
+                                        double* const* u_pred, void* stream);
+/* The Picard CN trapezoid u* = u + dt (rhs / rho + visc_n / 2 + visc_pred / 2)
+ * (picard_cn.py:86-96); periodic is accepted for symmetry with
+ * tpf_ns_predictor_rhs and unused.  Asynchronous. */
+TPF_API tpf_status tpf_ns_predictor_final(int ndim, const int64_t* n_nodes, const int* periodic,
+                                          const double* const* vel, const double* const* rhs,
+                                          const double* rho, const double* const* visc_n,
+                                          const double* const* visc_pred, double dt,
+                                          double* const* u_star, void* stream);
+/* Unit normal nhat[a] = g_a / max(sqrt(max(|g|^2, 1e-28)), 1e-14) of a
+ * gradient stacked by axis (ClosestPointExtender.compute_normal,
+ * levelset/closest_point_extender.py:112-131).  Asynchronous. */
+TPF_API tpf_status tpf_ns_normal(int ndim, int64_t n, const double* grad, double* const* nhat,
+                                 void* stream);
+/* BoundaryConditionHandler.apply (simulation/boundary_condition.py:36-87)
+ * on ncomp velocity components in place: wall axes zero both planes,
+ * periodic axes copy the root plane into the image plane.  Asynchronous. */
+TPF_API tpf_status tpf_ns_wall_bc(int ndim, const int64_t* n_nodes, const int* periodic,
+                                  int ncomp, double* const* comps, void* stream);
+/* Host scalars for CFLCalculator.compute (time_integration/cfl.py:85-137):
+ * max |a_k| of nk <= 3 arrays, or max (a / b); synchronous.  work: at least
+ * 3 * 1024 + 8 doubles of device memory. */
+TPF_API tpf_status tpf_ns_absmax(int nk, const double* const* arrays, int64_t n, double* work,
+                                 double* out_host, void* stream);
+TPF_API tpf_status tpf_ns_ratio_max(const double* a, const double* b, int64_t n, double* work,
+                                    double* out_host, void* stream);
+/* Number of NaN / Inf values over nk <= 64 arrays of n doubles, for the run
+ * loop's rollback test (simulation/legacy_run_loop.py:70-82); synchronous.
+ * work: at least 8 + nk doubles of device memory. */
+TPF_API tpf_status tpf_ns_nonfinite(int nk, const double* const* arrays, int64_t n, double* work,
+                                    int64_t* count_host, void* stream);
+/* out = x / s (the PPE right-hand side div_rc / dt, simulation/
+ * legacy_flow_helpers.py:52) and out = x + y (p^{n+1} = p^n + dp, :57);
+ * out may alias x.  Asynchronous. */
+TPF_API tpf_status tpf_ns_div_scalar(int64_t n, const double* x, double s, double* out,
+                                     void* stream);
+TPF_API tpf_status tpf_ns_add(int64_t n, const double* x, const double* y, double* out,
+                              void* stream);
+/* ClosestPointExtender.extend in 2-D (levelset/closest_point_extender.py:
+ * 136-228): nodes with phi >= 0 take the tensor-product quintic Hermite
+ * interpolant of q at the closest point (x - phi nx, y - phi ny), clipped to
+ * the domain; its cell is found by binary search on the node coordinates x
+ * (n_nodes[0]) and y (n_nodes[1]) and shifted at most three times per axis
+ * until its far node has phi < 0.  The derivatives q_x ... q_xxyy come from
+ * the plan (four CCD solves).  work: at least 8 n doubles.  Asynchronous. */
+TPF_API tpf_status tpf_ns_extend_2d(tpf_ccd_plan* plan, const int64_t* n_nodes, const double* x,
+                                    const double* y, const double* q, const double* phi,
+                                    const double* nx, const double* ny, double* out,
+                                    double* work, void* stream);
+
 /* Curvature straight from psi (CurvatureCalculatorPsi,
  * levelset/curvature_psi.py:45-196).  tpf_dccd_filter_nd: the DCCD filter of
  * levelset/curvature.py:60-128 on f (successive 3-point passes with

@@ -39,3 +39,4 @@ def test_restriction_is_scaled_transpose_of_prolongation_in_the_interior():
     lhs = float((restrict_axis(u, 0) * v).sum())
     rhs = float((u * prolong_axis(v, 0)).sum()) / 2
     assert abs(lhs - rhs) < 1e-12
+

@@ -9,7 +9,7 @@ include/tpf.h), reached through ctypes; PyTorch provides device memory and
 the stream only.  There is no CPU fallback.
 """
 
-from .config import GridConfig, NumericsConfig, SimulationConfig, SolverConfig
+from .config import FluidConfig, GridConfig, NumericsConfig, SimulationConfig, SolverConfig
 from .grid import Grid
 from .boundary import BoundarySpec, axis_kinds, pin_dof
 from .ccd import CCDBatch, CCDSolver
@@ -21,13 +21,19 @@ from .curvature import CurvatureCalculatorPsi
 from .reinit import SplitReinitializer
 from .ppe import (IPPESolver, MatrixAssemblyUnavailable, PPESolverFCCDMatrixFree,
                   PPESolverFVMMatrixFree, create_ppe_solver, register_ppe_solver)
+from .ns_terms import (BoundaryConditionHandler, CFLCalculator, ClosestPointExtender,
+                       NullFieldExtender, Predictor, UCCD6ConvectionTerm, VelocityCorrector,
+                       ViscousTerm)
+from .simulation import SimulationBuilder, TwoPhaseSimulation
 
 __all__ = [
-    "GridConfig", "NumericsConfig", "SimulationConfig", "SolverConfig", "Grid",
+    "FluidConfig", "GridConfig", "NumericsConfig", "SimulationConfig", "SolverConfig", "Grid",
     "BoundarySpec", "axis_kinds", "pin_dof", "CCDSolver", "CCDBatch", "FCCDSolver",
     "FCCDFaceJet", "PPESolverDefectCorrection", "RhieChowInterpolator",
     "DissipativeCCDAdvection", "CurvatureCalculatorPsi", "SplitReinitializer", "IPPESolver",
     "MatrixAssemblyUnavailable", "PPESolverFVMMatrixFree", "PPESolverFCCDMatrixFree",
     "create_ppe_solver",
-    "register_ppe_solver",
+    "register_ppe_solver", "BoundaryConditionHandler", "CFLCalculator", "ClosestPointExtender",
+    "NullFieldExtender", "Predictor", "UCCD6ConvectionTerm", "VelocityCorrector", "ViscousTerm",
+    "SimulationBuilder", "TwoPhaseSimulation",
 ]

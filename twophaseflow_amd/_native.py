@@ -132,6 +132,30 @@ SIGNATURES = {
     "tpf_mg_apply": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "tpf_mg_smooth": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
     "tpf_mg_coeffs": (C.c_int, [_vp, C.c_int, _vp]),
+    "tpf_ns_properties": (C.c_int, [C.c_int64, _vp, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, _vp, _vp, _vp, _vp]),
+    "tpf_ns_uccd6_work": (C.c_int64, [C.c_int, _i64p]),
+    "tpf_ns_uccd6_convection": (C.c_int, [_vp, C.c_int, _i64p, _dp, C.POINTER(_vp), C.c_double,
+                                          C.POINTER(_vp), _vp, _vp]),
+    "tpf_ns_viscous_work": (C.c_int64, [C.c_int, _i64p]),
+    "tpf_ns_viscous": (C.c_int, [_vp, C.c_int, _i64p, _dp, C.POINTER(_vp), _vp, _vp, C.c_double,
+                                 C.POINTER(_vp), _vp, _vp]),
+    "tpf_ns_predictor_rhs": (C.c_int, [C.c_int, _i64p, _ip, C.POINTER(_vp), C.POINTER(_vp),
+                                       C.POINTER(_vp), _vp, _vp, _vp, _vp, C.c_double,
+                                       C.c_double, C.c_double, C.POINTER(_vp), C.POINTER(_vp),
+                                       C.POINTER(_vp), _vp]),
+    "tpf_ns_predictor_final": (C.c_int, [C.c_int, _i64p, _ip, C.POINTER(_vp), C.POINTER(_vp),
+                                         _vp, C.POINTER(_vp), C.POINTER(_vp), C.c_double,
+                                         C.POINTER(_vp), _vp]),
+    "tpf_ns_normal": (C.c_int, [C.c_int, C.c_int64, _vp, C.POINTER(_vp), _vp]),
+    "tpf_ns_wall_bc": (C.c_int, [C.c_int, _i64p, _ip, C.c_int, C.POINTER(_vp), _vp]),
+    "tpf_ns_absmax": (C.c_int, [C.c_int, C.POINTER(_vp), C.c_int64, _vp, _dp, _vp]),
+    "tpf_ns_ratio_max": (C.c_int, [_vp, _vp, C.c_int64, _vp, _dp, _vp]),
+    "tpf_ns_nonfinite": (C.c_int, [C.c_int, C.POINTER(_vp), C.c_int64, _vp,
+                                   C.POINTER(C.c_int64), _vp]),
+    "tpf_ns_div_scalar": (C.c_int, [C.c_int64, _vp, C.c_double, _vp, _vp]),
+    "tpf_ns_add": (C.c_int, [C.c_int64, _vp, _vp, _vp, _vp]),
+    "tpf_ns_extend_2d": (C.c_int, [_vp, _i64p, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None
@@ -180,3 +204,8 @@ def darr(vals):
 
 def iarr(vals):
     return (C.c_int * len(vals))(*[int(v) for v in vals])
+
+
+def parr(ptrs):
+    """ctypes array of device pointers (None stays NULL)."""
+    return (_vp * len(ptrs))(*ptrs)

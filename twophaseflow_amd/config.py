@@ -2,8 +2,8 @@
 CCD and PPE paths read).
 
 Mirrors ``config.py`` of the reference: ``GridConfig`` (grid axes),
-``NumericsConfig.bc_type`` (:239), ``SolverConfig`` (:322-358) and
-``SimulationConfig`` (:363-381).  Extra attributes the reference reads with
+``FluidConfig`` (:193-207), ``NumericsConfig`` (:211-316), ``SolverConfig``
+(:322-358) and ``SimulationConfig`` (:363-381).  Extra attributes the reference reads with
 ``getattr`` (``ppe_preconditioner``, ``ppe_restart``, ``ppe_pcr_stages``,
 fvm_matrixfree.py:104-127) may be set dynamically the same way; this build
 adds ``ppe_precision`` ("fp64" | "mixed") and ``ppe_check_every``.
@@ -28,9 +28,57 @@ class GridConfig:
 
 
 @dataclass
+class FluidConfig:
+    """config.py:193-207: Re, Fr, We, rho_ratio = rho_g / rho_l, mu_ratio."""
+    Re: float = 100.0
+    Fr: float = 1.0
+    We: float = 10.0
+    rho_ratio: float = 0.001
+    mu_ratio: float = 0.01
+
+
+@dataclass
 class NumericsConfig:
-    bc_type: str = "wall"
+    """config.py:211-316 with the reference's defaults and checks.  The
+    legacy step of this build runs advection_scheme "dissipative_ccd",
+    convection_scheme "uccd6", reinit_method "split", surface_tension_model
+    "csf", extension_method "hermite" or "none", cn_mode "picard"; the
+    builder raises NotImplementedError for the other accepted values."""
+    epsilon_factor: float = 1.5
+    reinit_steps: int = 4
+    reinit_method: str = "split"
+    ridge_sigma_0: float = 3.0
     cfl_number: float = 0.3
+    t_end: float = 1.0
+    cn_viscous: bool = True
+    cn_mode: str = "picard"
+    bc_type: str = "wall"
+    advection_scheme: str = "fccd_flux"
+    convection_scheme: str = "uccd6"
+    uccd6_sigma: float = 1.0e-3
+    surface_tension_model: str = "csf"
+    extension_method: str = "hermite"
+    n_extend: int = 5
+
+    def __post_init__(self):
+        if self.advection_scheme == "fccd":
+            self.advection_scheme = "fccd_flux"
+        if self.convection_scheme == "fccd":
+            self.convection_scheme = "fccd_flux"
+        assert self.advection_scheme in ("dissipative_ccd", "fccd_nodal", "fccd_flux"), (
+            f"unknown advection_scheme {self.advection_scheme!r}")
+        assert self.convection_scheme in ("ccd", "fccd_nodal", "fccd_flux", "uccd6"), (
+            f"unknown convection_scheme {self.convection_scheme!r}")
+        if self.convection_scheme == "uccd6":
+            assert self.uccd6_sigma > 0.0, f"uccd6_sigma must be positive, got {self.uccd6_sigma}"
+        assert self.surface_tension_model in ("gfm", "csf"), (
+            f"unknown surface_tension_model {self.surface_tension_model!r}")
+        assert self.extension_method in ("hermite", "upwind", "none"), (
+            f"unknown extension_method {self.extension_method!r}")
+        valid = ("split", "unified", "dgr", "hybrid", "eikonal", "eikonal_xi", "eikonal_fmm",
+                 "ridge_eikonal")
+        assert self.reinit_method in valid, f"unknown reinit_method {self.reinit_method!r}"
+        assert self.ridge_sigma_0 > 0.0, f"ridge_sigma_0 must be positive, got {self.ridge_sigma_0}"
 
 
 @dataclass
@@ -49,6 +97,7 @@ class SolverConfig:
 @dataclass
 class SimulationConfig:
     grid: GridConfig = field(default_factory=GridConfig)
+    fluid: FluidConfig = field(default_factory=FluidConfig)
     numerics: NumericsConfig = field(default_factory=NumericsConfig)
     solver: SolverConfig = field(default_factory=SolverConfig)
     use_gpu: bool = True
