@@ -152,9 +152,10 @@ def test_reduced_system_window_truncation_is_negligible():
 
 def test_launch_mode_selection():
     """Few long lines (2-D 1024^2) take the wavefront-per-line kernel (<= 64
-    partitions), many or shorter lines the lane-per-line kernels (<= 16):
-    the explicit partition inverse on strided axes, recurrences on the
-    contiguous one."""
+    partitions); many lines take the explicit partition inverse on strided
+    axes and, on the contiguous axis, the wavefront kernel with four lines of
+    16 partitions per wavefront (2); a fused multi-axis call runs every axis
+    of one rows cap and partition count in one lane-kernel grid."""
     plan = _plan((1025, 1025), (1, 0))
     P, M, W, t = C.c_int(), C.c_int(), C.c_int(), C.c_double()
     nat.check(nat.lib().tpf_ccd_axis_info(plan, 0, C.byref(P), C.byref(M), C.byref(W), C.byref(t)))
@@ -169,10 +170,12 @@ def test_launch_mode_selection():
     assert (P.value, M.value) == (16, 16)
     k = C.c_int()
     kernels = []
-    for a in range(3):  # strided axes: explicit inverse (3); contiguous: recurrences (1)
+    for a in range(3):  # strided axes: explicit inverse (3); contiguous: wavefront (2)
         nat.check(nat.lib().tpf_ccd_axis_kernel(plan, a, C.byref(k)))
         kernels.append(k.value)
-    assert kernels == [3, 3, 1]
+    assert kernels == [3, 3, 2]
+    nat.check(nat.lib().tpf_ccd_axis_info(plan, 2, C.byref(P), C.byref(M), C.byref(W), C.byref(t)))
+    assert (P.value, M.value) == (16, 16)
     nat.check(nat.lib().tpf_ccd_plan_set_launch(plan, 0, 1, 0))
     nat.check(nat.lib().tpf_ccd_axis_kernel(plan, 0, C.byref(k)))
     assert k.value == 1

@@ -11,6 +11,8 @@ namespace tpf {
 struct CcdAxisHost {
   int wave = 0;                 // 1: one wavefront per line (lane = partition);
                                 // 2: lane per line, explicit partition inverse
+  int lpw = 1;                  // wave == 1: lines per wavefront (64 / P rounded)
+  int wpg = 16;                 // wave == 1: wavefronts per workgroup
   CcdAxisGeom geom{};           // device pointers filled after upload
   std::vector<double> rowtab;   // [P][M][kRowTab]
   std::vector<double> qtab;     // [P][4][NQ*4]
