@@ -798,7 +798,18 @@ def main():
     e2 = torch.empty_like(e1)
     sec2 = time_ccd(ccd2, f2, e1, e2, 0b11, args.steps, args.warmup, 2)
     mpts2 = 2 * f2.numel() / sec2 / 1e6
-    del e1, e2
+    log("timing 2-D CCD, FP32 solves + FP64 refinement")
+    ccd2r = CCDSolver(g2, bc_type="periodic", precision="fp32_refined")
+    q1 = torch.empty_like(e1)
+    q2 = torch.empty_like(e2)
+    sec2r = time_ccd(ccd2r, f2, q1, q2, 0b11, args.steps, args.warmup, 2)
+    refined2 = {"Mpts_per_s": round(2 * f2.numel() / sec2r / 1e6, 2),
+                "ms_per_step": round(sec2r * 1e3, 4),
+                "kernel": "ccd_wave_kernel_ref<16, 1> (64 partitions x 16 rows, two refinements)",
+                "max_rel_err_vs_fp64": {
+                    "d1": float((q1 - e1).abs().max() / e1.abs().max()),
+                    "d2": float((q2 - e2).abs().max() / e2.abs().max())}}
+    del e1, e2, q1, q2
 
     result = {
         "metric": METRIC,
@@ -830,6 +841,7 @@ def main():
                       "kernel": a["info"]["kernel"]} for a in axes],
         "ccd_strided_axes_fused": strided,
         "ccd_2d_1025": {"Mpts_per_s": round(mpts2, 2), "ms_per_step": round(sec2 * 1e3, 4)},
+        "ccd_2d_1025_fp32_refined": refined2,
         "ccd_fp32_refined_257": refined,
         "ccd_batch_uneven_2d": ccd_batch_bench(),
     }

@@ -12,6 +12,8 @@ function against something other than its own formula):
 
 * ``ccd.differentiate_periodic``  -- closed-form Fourier symbol
   (``ccd.periodic_symbol``), paper table U1-a, ``test_ccd.py:65-81``.
+* ``ccd.differentiate_wall_extended`` -- the same wall system in long
+  double: cubic exactness, agreement with the dense LU on random data.
 * ``ccd.differentiate_wall``      -- polynomial exactness (x^0..x^3),
   ``test_ccd.py:201-221``, D2 null space (``test_ccd.py:257-299``),
   convergence slopes, block-Thomas vs dense LU.
@@ -31,10 +33,20 @@ function against something other than its own formula):
   mass restoration by the correction.
 * ``curvature.OracleCurvaturePsi`` -- exact -1/r and -2/r on circle / sphere
   level sets of quadratic psi, the rotated-ellipse closed form (mixed term),
-  and the filter's action on linear fields and wall planes.
+  the filter's action on linear fields and wall planes, and its transfer
+  factor (1 - 4 eps (2 psi - 1)^2 sin^2(theta/2)) with a pointwise
+  coefficient.
 * ``reinit.OracleSplitReinit``    -- the CN-ADI closed-form factor on cosine
   modes, the pseudo-time step formula, mass restoration and the tanh profile
-  as a near fixed point.
+  as a near fixed point; the compression step by the filter's transfer
+  factor, exact linear fluxes with each ghost rule and the monotone tanh
+  divergence (three planted mistakes fail them).
+* ``legacy_step`` (the config-1 step) -- the reference's own outputs on
+  seeded inputs for every building block and its 64 x 64 trajectory
+  (tests/golden/, written by make_legacy_step_fixtures.py from the
+  reference source), and closed forms (the extension of a quadratic field,
+  the viscous body of linear fields, the UCCD6 skew weights and
+  hyperviscosity symbol).
 
 No function here is "parity unpinned" except the GMRES/CG iteration counts,
 which the reference does not fix (SciPy internals).

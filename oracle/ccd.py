@@ -144,6 +144,60 @@ def differentiate_wall(data, axis: int, info: dict):
     return _from_lines(out[0], shape, axis), _from_lines(out[1], shape, axis)
 
 
+def differentiate_wall_extended(data, axis: int, info: dict):
+    """The wall system of ``differentiate_wall`` solved in 80-bit long double
+    (block Thomas, right-hand side and closure rows formed in long double from
+    the FP64 input): the exact discrete answer to ~1e-18, the yardstick on
+    long wall lines where the FP64 dense LU's own error (second-difference
+    cancellation, the closure rows) reaches 1e-10 of max|f''| at N = 1024.
+    Returns FP64 (d1, d2)."""
+    LD = np.longdouble
+    N, n_int = info["N"], info["n_int"]
+    hL = LD(info["h"])
+    a1, a2 = LD(15) / 16, LD(3)
+    al, b1, be, b2 = LD(7) / 16, LD(1) / 16, LD(-1) / 8, LD(-9) / 8
+    Lb = np.array([[al, b1 * hL], [b2 / hL, be]], dtype=LD)
+    Ub = np.array([[al, -b1 * hL], [-b2 / hL, be]], dtype=LD)
+    bl, br = info["bc_left"], info["bc_right"]
+    Ml, Mr = np.array(bl["M"], dtype=LD), np.array(br["M"], dtype=LD)
+    f, shape = _as_lines(data, axis)
+    fL = f.astype(LD)
+    rhs = np.empty((n_int, 2, fL.shape[1]), dtype=LD)
+    rhs[:, 0] = (a1 / hL) * (fL[2:] - fL[:-2])
+    rhs[:, 1] = (a2 / (hL * hL)) * (fL[:-2] - 2 * fL[1:-1] + fL[2:])
+    bc_lo = np.stack([np.array(bl["c_I"], dtype=LD) @ fL[:4],
+                      np.array(bl["c_II"], dtype=LD) @ fL[:6]])
+    fr = fL[::-1]
+    bc_hi = np.stack([np.array(br["c_I"], dtype=LD) @ fr[:4],
+                      np.array(br["c_II"], dtype=LD) @ fr[:6]])
+    rhs[0] -= Lb @ bc_lo
+    rhs[-1] -= Ub @ bc_hi
+
+    def inv2(a):
+        det = a[0, 0] * a[1, 1] - a[0, 1] * a[1, 0]
+        return np.array([[a[1, 1], -a[0, 1]], [-a[1, 0], a[0, 0]]], dtype=LD) / det
+
+    D = [np.eye(2, dtype=LD) for _ in range(n_int)]
+    D[0] = D[0] + Lb @ Ml
+    D[-1] = D[-1] + Ub @ Mr
+    Dp, y = [D[0]], np.empty_like(rhs)
+    y[0] = rhs[0]
+    for i in range(1, n_int):
+        Lm = Lb @ inv2(Dp[i - 1])
+        Dp.append(D[i] - Lm @ Ub)
+        y[i] = rhs[i] - Lm @ y[i - 1]
+    x = np.empty_like(rhs)
+    x[-1] = inv2(Dp[-1]) @ y[-1]
+    for i in range(n_int - 2, -1, -1):
+        x[i] = inv2(Dp[i]) @ (y[i] - Ub @ x[i + 1])
+    out = np.empty((2, N + 1, fL.shape[1]), dtype=LD)
+    out[:, 1:-1] = x.transpose(1, 0, 2)
+    out[:, 0] = Ml @ out[:, 1] + bc_lo
+    out[:, N] = Mr @ out[:, N - 1] + bc_hi
+    out = out.astype(np.float64)
+    return _from_lines(out[0], shape, axis), _from_lines(out[1], shape, axis)
+
+
 # --------------------------------------------------------------------------
 # Periodic axis
 # --------------------------------------------------------------------------

@@ -33,6 +33,7 @@ namespace {
 
 // Wavefront-uniform table reads through the scalar cache.
 typedef const __attribute__((address_space(4))) double* kdptr;
+typedef const __attribute__((address_space(4))) float* kfptr;
 
 constexpr int kStageRows = 16;              // rows per LDS re-staging chunk
 constexpr int kStagePitch = kStageRows + 1; // padded line pitch (doubles)
@@ -927,6 +928,286 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
   }
 }
 
+// FP32 partition solve of the wavefront mapping (lane = partition): block-LU
+// sweeps with the FP32 tables, partition ends -> Gw ([4][64] per wavefront),
+// barrier, coupling through the reduced rows, spike correction.  Every
+// thread of the workgroup calls it (it holds a barrier).
+template <int M, bool UNI>
+__device__ __forceinline__ void wave_solve32(const CcdAxisGeom& g, int pp, int lane, int lbase,
+                                             int m, bool periodic, float (&v)[M][2], float* Gw) {
+  using TabF = typename std::conditional<UNI, kfptr, const float*>::type;
+  const TabF tab = UNI ? (TabF)g.rowtab32 : (TabF)(g.rowtab32 + (size_t)pp * M * kRowTab);
+#pragma unroll
+  for (int r = 1; r < M; ++r) {
+    if (r < m) {
+      const TabF t = tab + r * kRowTab;
+      v[r][0] -= t[0] * v[r - 1][0] + t[1] * v[r - 1][1];
+      v[r][1] -= t[2] * v[r - 1][0] + t[3] * v[r - 1][1];
+    }
+  }
+#pragma unroll
+  for (int r = M - 1; r >= 0; --r) {
+    if (r < m) {
+      float a = v[r][0], c = v[r][1];
+      if (r + 1 < M && r < m - 1) {
+        a -= (float)kAlpha1 * v[r + 1][0] - (float)kB1 * v[r + 1][1];
+        c -= -(float)kB2 * v[r + 1][0] + (float)kBeta2 * v[r + 1][1];
+      }
+      const TabF t = tab + r * kRowTab + 4;
+      v[r][0] = t[0] * a + t[1] * c;
+      v[r][1] = t[2] * a + t[3] * c;
+      if (r == m - 1) {
+        Gw[2 * 64 + lane] = v[r][0];
+        Gw[3 * 64 + lane] = v[r][1];
+      }
+    }
+  }
+  Gw[0 * 64 + lane] = v[0][0];
+  Gw[1 * 64 + lane] = v[0][1];
+  __syncthreads();
+  float lr0 = 0.f, lr1 = 0.f, lr2 = 0.f, lr3 = 0.f;
+  const int NQ = g.NQ;
+  const TabF q = UNI ? (TabF)g.qtab32 : (TabF)(g.qtab32 + (size_t)pp * 4 * NQ * 4);
+  for (int t = 0; t < NQ; ++t) {
+    int qq = g.rel ? pp + g.d0 + t : t;
+    if (periodic) qq = qq < 0 ? qq + g.P : (qq >= g.P ? qq - g.P : qq);
+    else qq = qq < 0 ? 0 : (qq >= g.P ? g.P - 1 : qq);
+    const float G0 = Gw[0 * 64 + lbase + qq], G1 = Gw[1 * 64 + lbase + qq];
+    const float G2 = Gw[2 * 64 + lbase + qq], G3 = Gw[3 * 64 + lbase + qq];
+    const TabF q0 = q + (0 * NQ + t) * 4;
+    const TabF q1 = q + (1 * NQ + t) * 4;
+    const TabF q2 = q + (2 * NQ + t) * 4;
+    const TabF q3 = q + (3 * NQ + t) * 4;
+    lr0 += q0[0] * G0 + q0[1] * G1 + q0[2] * G2 + q0[3] * G3;
+    lr1 += q1[0] * G0 + q1[1] * G1 + q1[2] * G2 + q1[3] * G3;
+    lr2 += q2[0] * G0 + q2[1] * G1 + q2[2] * G2 + q2[3] * G3;
+    lr3 += q3[0] * G0 + q3[1] * G1 + q3[2] * G2 + q3[3] * G3;
+  }
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    if (r < m) {
+      const TabF t = tab + r * kRowTab + 8;
+      v[r][0] -= t[0] * lr0 + t[1] * lr1 + t[4] * lr2 + t[5] * lr3;
+      v[r][1] -= t[2] * lr0 + t[3] * lr1 + t[6] * lr2 + t[7] * lr3;
+    }
+  }
+}
+
+// FP32 solves with FP64 iterative refinement on the wavefront mapping: the
+// FP64 right-hand side rounded to FP32 and solved (x1); then twice, the
+// residual of the exact block system formed in FP64 from the current iterate
+// and the neighbour partitions' end rows (exchanged through LDS) and solved
+// in FP32 (d1, d2).  x = x1 + d1 + d2, kept as three FP32 parts and summed
+// in FP64, is stored through the same tile as the FP64 kernel (same HBM
+// traffic).  The second sweep matters on long wall lines: in the scaled
+// unknowns (h f', h^2 f'') one sweep leaves ~(kappa 2^-24)^2 of |x|, which
+// the h f' part dominates, and f'' of a smooth field is h k times smaller.
+// The line values are re-read from the LDS tile for each residual (the
+// exchange buffers lie past it), which keeps the kernel within 128 VGPRs.
+template <int M, bool UNI, int LPW>
+__device__ __forceinline__ void ccd_wave_tile_ref(const CcdAxisGeom& g, const CcdLines& ln,
+                                                  const double* __restrict__ f,
+                                                  double* __restrict__ d1,
+                                                  double* __restrict__ d2,
+                                                  const unsigned block) {
+  extern __shared__ double lds[];
+  constexpr int PW = 64 / LPW;
+  const int lane = threadIdx.x & 63;
+  const int p = lane & (PW - 1);
+  const int lbase = lane & ~(PW - 1);
+  const int nwaves = (int)(blockDim.x >> 6);
+  const int nlw = LPW * nwaves;
+  const int wv = threadIdx.x >> 6;
+  const int w = wv * LPW + lane / PW;
+  const int64_t line0 = (int64_t)block * nlw;
+  const int64_t remain = ln.n_lines - line0;
+  const int nl = (int)(remain < nlw ? remain : nlw);
+  const int nax = g.N + 1;
+  const int pitch = wave_pitch(nax);
+  const bool active = w < nl && p < g.P;
+  const int pp = p < g.P ? p : g.P - 1;
+  const int s0 = (int)((int64_t)g.n * pp / g.P);
+  const int m = (int)((int64_t)g.n * (pp + 1) / g.P) - s0;
+  const bool periodic = g.periodic != 0;
+  const bool wall_first = active && !periodic && p == 0;
+  const bool wall_last = active && !periodic && p == g.P - 1;
+
+  wave_tile_io<true>(ln, nax, pitch, line0, nl, nlw, f, nullptr, lds);
+  __syncthreads();
+  const int lw = w < nl ? w : 0;
+  const int j0 = periodic ? s0 - 1 : s0;
+  auto fv = [&](int k) {
+    int j = j0 + k;
+    if (periodic) j = j < 0 ? j + g.N : (j >= g.N ? j - g.N : j);
+    return lds[tix(pitch, lw, j)];
+  };
+  double cl0 = 0.0, cl1 = 0.0, bl0 = 0.0, bl1 = 0.0;
+  if (wall_first) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) bl0 += g.cIl[k] * fv(k);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bl1 += g.cIIl[k] * fv(k);
+    cl0 = kAlpha1 * bl0 + kB1 * bl1;
+    cl1 = kB2 * bl0 + kBeta2 * bl1;
+  }
+  double cr0 = 0.0, cr1 = 0.0, br0 = 0.0, br1 = 0.0;
+  if (wall_last) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double v = lds[tix(pitch, lw, g.N - k)];
+      if (k < 4) br0 += g.cIr[k] * v;
+      br1 += g.cIIr[k] * v;
+    }
+    cr0 = kAlpha1 * br0 - kB1 * br1;
+    cr1 = -kB2 * br0 + kBeta2 * br1;
+  }
+  // exchange buffers past the tile: Gw (FP32 partition ends), Ew (FP64 end rows)
+  double* xb = lds + (size_t)nlw * pitch + (size_t)wv * 512;
+  float* Gw = reinterpret_cast<float*>(xb);
+  double* Ew = xb + 128;
+  auto rhs = [&](int r, double& r0, double& r1) {
+    const double fa = fv(r), fb = fv(r + 1), fc = fv(r + 2);
+    r0 = kA1 * (fc - fa);
+    r1 = kA2 * ((fa - 2.0 * fb) + fc);
+    if (r == 0) {
+      r0 -= cl0;
+      r1 -= cl1;
+    }
+    if (r == m - 1) {
+      r0 -= cr0;
+      r1 -= cr1;
+    }
+  };
+  float x1[M][2], dv[M][2], dw[M][2];
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    x1[r][0] = x1[r][1] = 0.f;
+    dv[r][0] = dv[r][1] = 0.f;
+    dw[r][0] = dw[r][1] = 0.f;
+    if (r < m) {
+      double r0, r1;
+      rhs(r, r0, r1);
+      x1[r][0] = (float)r0;
+      x1[r][1] = (float)r1;
+    }
+  }
+  wave_solve32<M, UNI>(g, pp, lane, lbase, m, periodic, x1, Gw);
+  auto xo = [&](int r, int c) { return ((double)x1[r][c] + (double)dv[r][c]) + (double)dw[r][c]; };
+  const bool has_left = periodic || p > 0, has_right = periodic || p < g.P - 1;
+  const int pl = pp > 0 ? pp - 1 : g.P - 1, pr = pp < g.P - 1 ? pp + 1 : 0;
+#pragma unroll
+  for (int sweep = 0; sweep < 2; ++sweep) {
+    // end rows of the current iterate for the neighbours' residual rows
+    Ew[0 * 64 + lane] = xo(0, 0);
+    Ew[1 * 64 + lane] = xo(0, 1);
+#pragma unroll
+    for (int r = 0; r < M; ++r)
+      if (r == m - 1) {
+        Ew[2 * 64 + lane] = xo(r, 0);
+        Ew[3 * 64 + lane] = xo(r, 1);
+      }
+    __syncthreads();
+    double lx0 = 0.0, lx1 = 0.0, rx0 = 0.0, rx1 = 0.0;
+    if (has_left) {
+      lx0 = Ew[2 * 64 + lbase + pl];
+      lx1 = Ew[3 * 64 + lbase + pl];
+    }
+    if (has_right) {
+      rx0 = Ew[0 * 64 + lbase + pr];
+      rx1 = Ew[1 * 64 + lbase + pr];
+    }
+    float v[M][2];
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      v[r][0] = v[r][1] = 0.f;
+      if (r < m) {
+        double r0, r1;
+        rhs(r, r0, r1);
+        const double u = xo(r, 0), wq = xo(r, 1);
+        double a0 = u, a1 = wq;
+        if (wall_first && r == 0) {
+          const double mu = g.Ml[0] * u + g.Ml[1] * wq, mv = g.Ml[2] * u + g.Ml[3] * wq;
+          a0 += kAlpha1 * mu + kB1 * mv;
+          a1 += kB2 * mu + kBeta2 * mv;
+        }
+        if (wall_last && r == m - 1) {
+          const double mu = g.Mr[0] * u + g.Mr[1] * wq, mv = g.Mr[2] * u + g.Mr[3] * wq;
+          a0 += kAlpha1 * mu - kB1 * mv;
+          a1 += -kB2 * mu + kBeta2 * mv;
+        }
+        const bool lo = r > 0 || has_left, hi = r < m - 1 || has_right;
+        const double l0 = r > 0 ? xo(r - 1, 0) : lx0, l1 = r > 0 ? xo(r - 1, 1) : lx1;
+        const double h0 = r < m - 1 ? xo(r + 1, 0) : rx0;
+        const double h1 = r < m - 1 ? xo(r + 1, 1) : rx1;
+        if (lo) {
+          a0 += kAlpha1 * l0 + kB1 * l1;
+          a1 += kB2 * l0 + kBeta2 * l1;
+        }
+        if (hi) {
+          a0 += kAlpha1 * h0 - kB1 * h1;
+          a1 += -kB2 * h0 + kBeta2 * h1;
+        }
+        v[r][0] = (float)(r0 - a0);
+        v[r][1] = (float)(r1 - a1);
+      }
+    }
+    __syncthreads();  // every lane has read Ew before the next sweep rewrites it
+    wave_solve32<M, UNI>(g, pp, lane, lbase, m, periodic, v, Gw);
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      if (sweep == 0) {
+        dv[r][0] = v[r][0];
+        dv[r][1] = v[r][1];
+      } else {
+        dw[r][0] = v[r][0];
+        dw[r][1] = v[r][1];
+      }
+    }
+  }
+  const double x00 = xo(0, 0), x01 = xo(0, 1);
+  double xl0 = 0.0, xl1 = 0.0;
+#pragma unroll
+  for (int r = 0; r < M; ++r)
+    if (r == m - 1) {
+      xl0 = xo(r, 0);
+      xl1 = xo(r, 1);
+    }
+  const int off = periodic ? 0 : 1;
+  __syncthreads();  // the tile's inputs are no longer read; it takes the outputs
+#pragma unroll
+  for (int comp = 0; comp < 2; ++comp) {
+    double* dst = comp == 0 ? d1 : d2;
+    const double scale = comp == 0 ? g.inv_h : g.inv_h2;
+    if (dst == nullptr) continue;  // uniform
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < M; ++r)
+        if (r < m) lds[tix(pitch, w, s0 + r + off)] = xo(r, comp) * scale;
+      if (periodic && p == 0) lds[tix(pitch, w, g.N)] = (comp == 0 ? x00 : x01) * scale;
+      if (wall_first)
+        lds[tix(pitch, w, 0)] = (comp == 0 ? g.Ml[0] * x00 + g.Ml[1] * x01 + bl0
+                                           : g.Ml[2] * x00 + g.Ml[3] * x01 + bl1) * scale;
+      if (wall_last)
+        lds[tix(pitch, w, g.N)] = (comp == 0 ? g.Mr[0] * xl0 + g.Mr[1] * xl1 + br0
+                                             : g.Mr[2] * xl0 + g.Mr[3] * xl1 + br1) * scale;
+    }
+    __syncthreads();
+    wave_tile_io<false>(ln, nax, pitch, line0, nl, nlw, nullptr, dst, lds);
+    __syncthreads();
+  }
+}
+
+template <int M, int LPW>
+__global__ __launch_bounds__(1024) void ccd_wave_kernel_ref(CcdLaunch L) {
+  const unsigned blk = blockIdx.x;
+  const int sl = (L.nslots > 1 && blk >= L.block_begin[1]) + (L.nslots > 2 && blk >= L.block_begin[2]);
+  const CcdSlot& S = L.slot[sl];
+  if (S.g.uniform)
+    ccd_wave_tile_ref<M, true, LPW>(S.g, S.ln, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
+  else
+    ccd_wave_tile_ref<M, false, LPW>(S.g, S.ln, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
+}
+
 template <int M, int LPW>
 __global__ __launch_bounds__(1024) void ccd_wave_kernel(CcdLaunch L) {
   const unsigned blk = blockIdx.x;
@@ -936,6 +1217,12 @@ __global__ __launch_bounds__(1024) void ccd_wave_kernel(CcdLaunch L) {
     ccd_wave_tile<M, true, LPW>(S.g, S.ln, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
   else
     ccd_wave_tile<M, false, LPW>(S.g, S.ln, S.f, S.d1, S.d2, blk - L.block_begin[sl]);
+}
+
+// FP32-refined wavefront kernel: the tile, then 4 KB of exchange buffers
+// per wavefront (the tile stays readable for the residuals).
+size_t wave_lds_bytes_ref(const CcdAxisGeom& g, int lines, int waves) {
+  return sizeof(double) * ((size_t)lines * (size_t)wave_pitch(g.N + 1) + (size_t)waves * 512);
 }
 
 // LDS of the wavefront kernel: the [lines][pitch] tile (it also holds the
@@ -1265,7 +1552,14 @@ BatchFn pick_batch_kernel(int M) {
 
 using KernelFn = void (*)(CcdLaunch);
 
-WaveFn pick_wave_kernel(int M, int lpw) {
+WaveFn pick_wave_kernel(int M, int lpw, bool ref = false) {
+  if (ref) {
+    if (M == 8) return lpw == 4 ? ccd_wave_kernel_ref<8, 4> : lpw == 2 ? ccd_wave_kernel_ref<8, 2>
+                                                                       : ccd_wave_kernel_ref<8, 1>;
+    if (M == 16) return lpw == 4 ? ccd_wave_kernel_ref<16, 4>
+                                 : lpw == 2 ? ccd_wave_kernel_ref<16, 2> : ccd_wave_kernel_ref<16, 1>;
+    return nullptr;
+  }
   if (M == 8) return lpw == 4 ? ccd_wave_kernel<8, 4> : lpw == 2 ? ccd_wave_kernel<8, 2>
                                                                  : ccd_wave_kernel<8, 1>;
   if (M == 16) return lpw == 4 ? ccd_wave_kernel<16, 4> : lpw == 2 ? ccd_wave_kernel<16, 2>
@@ -1398,11 +1692,13 @@ tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const dou
   L.block_begin[naxes] = blocks;
   if (wave) {
     const CcdAxisHost& a0 = plan->axes[axes[0]];
-    WaveFn k = pick_wave_kernel(M, a0.lpw);
+    WaveFn k = pick_wave_kernel(M, a0.lpw, plan->precision == TPF_PREC_FP32_REFINED);
     if (!k) return fail(TPF_ERR_VALUE, "unsupported CCD rows cap " + std::to_string(M));
+    const bool refined = plan->precision == TPF_PREC_FP32_REFINED;
     size_t sh = 0;
     for (int i = 0; i < naxes; ++i)
-      sh = std::max(sh, wave_lds_bytes(L.slot[i].g, a0.lpw * a0.wpg, a0.wpg));
+      sh = std::max(sh, refined ? wave_lds_bytes_ref(L.slot[i].g, a0.lpw * a0.wpg, a0.wpg)
+                                : wave_lds_bytes(L.slot[i].g, a0.lpw * a0.wpg, a0.wpg));
     TPF_TRY(raise_lds_limit((const void*)k, sh));
     hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * a0.wpg), sh, stream, L);
     TPF_HIP(hipGetLastError());
@@ -1502,8 +1798,9 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
   const int mode = plan->mode[axis];
   const int cap = plan->rows_cap[axis];
   const bool ref = plan->precision == TPF_PREC_FP32_REFINED;
-  if (ref && (mode >= 2 || cap > 16))
-    return fail(TPF_ERR_VALUE, "FP32-refined CCD runs the lane-per-line kernel with rows cap <= 16");
+  if (ref && (mode == 3 || cap > 16))
+    return fail(TPF_ERR_VALUE, "FP32-refined CCD runs the lane-per-line or wavefront kernel with "
+                               "rows cap <= 16");
   const bool strided = lines_for(plan, axis).stride != 1;
   if (mode == 3) {  // lane per line, explicit partition inverse (strided axes)
     if (!strided)
@@ -1521,7 +1818,8 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
   // <= 256 unknowns on the contiguous axis (four lines of 16 partitions per
   // wavefront: 0.128 against 0.172 ms for the tiled lane kernel on 257^3).
   const bool contig_many = !strided && lines >= 4096 && n > 32 && n <= 256;
-  bool wave = !ref && (mode == 2 || (mode == 0 && ((lines < 4096 && n > 512) || contig_many)));
+  bool wave = mode == 2 || (mode == 0 && ((lines < 4096 && n > 512) || contig_many ||
+                                           (ref && n > 256)));
   if (wave && cap != 0 && cap != 8 && cap != 16)
     return fail(TPF_ERR_VALUE, "the wavefront-per-line CCD kernel takes rows cap 8 or 16");
   if (wave && n >= 1) {
@@ -1541,8 +1839,8 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
       return fail(TPF_ERR_VALUE, "rows cap " + std::to_string(cap) +
                                      " needs more than 16 partitions for the lane-per-line kernel");
     if (ref && ax.geom.M > 16)
-      return fail(TPF_ERR_VALUE, "FP32-refined CCD supports lines of <= 256 unknowns (rows cap "
-                                 "16 x 16 partitions); this axis has " + std::to_string(n));
+      return fail(TPF_ERR_VALUE, "FP32-refined CCD supports lines of <= 1024 unknowns (64 "
+                                 "partitions x 16 rows); this axis has " + std::to_string(n));
   }
   // Wavefront kernel shape: as many lines per wavefront as its partitions
   // leave lanes for, four wavefronts per workgroup.
@@ -1550,7 +1848,9 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
   // measured faster than 8 or 4 (0.096 against 0.110 / 0.099 ms, wall).
   if (wave) {
     ax.lpw = ax.geom.P <= 16 ? 4 : (ax.geom.P <= 32 ? 2 : 1);
-    ax.wpg = ax.lpw == 1 ? 16 : 4;
+    // FP32-refined: 4 wavefronts (the tile stays live next to 4 KB of exchange
+    // buffers per wavefront: <= 51 KB at 1025 nodes)
+    ax.wpg = (ax.lpw == 1 && !ref) ? 16 : 4;
   }
   // Auto: strided FP64 axes of <= 16 partitions x 16 rows take the explicit
   // partition inverse (no recurrence state: two workgroups per CU).
