@@ -47,10 +47,11 @@ typedef struct tpf_ccd_plan tpf_ccd_plan;
  * (ccd/ccd_solver.py:99-120, ccd/ccd_solver_helpers.py:227-314, 406-423).
  * h[a] = L[a]/N[a]; bc[a] in {TPF_BC_WALL, TPF_BC_PERIODIC}.
  * Wall axes need n_nodes >= 6 (TPF_ERR_VALUE otherwise); periodic >= 4.
- * precision: TPF_PREC_FP64, or TPF_PREC_FP32_REFINED (FP32 solves plus one
- * FP64 residual refinement; FP64 inputs and outputs; lines of <= 256
- * unknowns, lane-per-line mapping, rows cap <= 16, TPF_ERR_VALUE beyond;
- * relative error <= 1e-12, no counterpart in the reference). */
+ * precision: TPF_PREC_FP64, or TPF_PREC_FP32_REFINED (FP32 solves under
+ * FP64 residual refinement, one sweep on lines of <= 256 unknowns (a lane
+ * per line) and two on longer lines (a wavefront per line); FP64 inputs and
+ * outputs; lines of <= 1024 unknowns, rows cap <= 16, TPF_ERR_VALUE beyond;
+ * no counterpart in the reference). */
 TPF_API tpf_status tpf_ccd_plan_create(tpf_ccd_plan** plan, int ndim,
                                const int64_t* n_nodes, const double* h,
                                const int* bc, int precision);
@@ -230,6 +231,29 @@ TPF_API tpf_status tpf_ppe_solve_mg_deflated(tpf_ppe* ppe, const double* rhs, do
 /* FP64 ||b - A p||_2 / ||b||_2 on the CG system (the judge of every solve). */
 TPF_API tpf_status tpf_ppe_true_relres(tpf_ppe* ppe, const double* rhs,
                                const double* p, double* relres);
+
+/* Batched Jacobi-PCG over independent PPE handles (uneven grids allowed):
+ * one launch per PCG step covers every problem, iteration chunks replay as
+ * one captured graph, and every problem keeps its own stop test and restart
+ * rule.  Each problem's arithmetic and reduction order are those of
+ * tpf_ppe_solve, so p equals its single solve bit for bit.  Replaces a loop
+ * of PPESolverFVMMatrixFree.solve calls in CG form (fvm_matrixfree.py:
+ * 150-260), one per simulation of a batch; the reference steps such
+ * simulations one after another.  The handles' workspaces are used, so a
+ * handle must not be solved elsewhere during a batched solve; densities are
+ * those of the last tpf_ppe_set_density.  Calls are ordered on `stream`. */
+typedef struct tpf_ppe_batch tpf_ppe_batch;
+TPF_API tpf_status tpf_ppe_batch_create(tpf_ppe_batch** batch, tpf_ppe* const* ppes, int count,
+                                        void* stream);
+/* rhs[i], p[i]: problem i's right-hand side and solution (initial guess
+ * when opts->use_x0); both NULL skip problem i this call (its report is
+ * zeroed).  opts: precond 1, mixed 0.  reports[count] filled.
+ * Synchronous.  TPF_ERR_RUNTIME names the problems that ended above tol
+ * (their reports are filled; the others are solved). */
+TPF_API tpf_status tpf_ppe_batch_solve(tpf_ppe_batch* batch, const double* const* rhs,
+                                       double* const* p, const tpf_solve_opts* opts,
+                                       tpf_solve_report* reports);
+TPF_API tpf_status tpf_ppe_batch_destroy(tpf_ppe_batch* batch);
 
 /* ----------------------------------------------------------- FCCD PPE --- */
 /* The FCCD matrix-free PPE, L p = D_f[(1/rho)_f G_f p] with the FCCD face

@@ -1,0 +1,84 @@
+"""BatchedSimulation: each simulation of a batch follows its own single run
+bit for bit (same CFL dt, same fields after every step), with uneven grids,
+density ratios 10-1000 and We 1-50, and through the dt-halving retry path
+(a failure injected from the host into one simulation of the batch and of
+its single run alike).  The single run is pinned against the reference and
+the oracle in test_gpu_legacy_step.py."""
+
+import sys
+
+import pytest
+import torch
+
+from twophaseflow_amd import BatchedSimulation, bubble_simulation
+
+pytestmark = pytest.mark.gpu
+
+CASES = [{"N": 32, "radius": 0.2, "density_ratio": 1000.0, "We": 1.0},
+         {"N": 48, "radius": 0.27, "density_ratio": 10.0, "We": 50.0},
+         {"N": 64, "radius": 0.16, "density_ratio": 300.0, "We": 10.0},
+         {"N": 40, "radius": 0.3, "density_ratio": 50.0, "We": 3.0}]
+STEPS = 3
+
+
+def _announce(msg):
+    print(f"[gpu-case] {msg}", file=sys.stderr, flush=True)
+
+
+def _state(sim):
+    return {"psi": sim.psi.data.clone(), "rho": sim.rho.data.clone(),
+            "kappa": sim.kappa.data.clone(), "p": sim.pressure.data.clone(),
+            "u": sim.velocity[0].clone(), "v": sim.velocity[1].clone(),
+            "time": sim.time, "step": sim.step}
+
+
+def _inject_failure(sim, at_call):
+    """The at_call-th non-finite check of `sim` reports a failure."""
+    real = sim._has_nonfinite_state
+    calls = [0]
+
+    def check():
+        calls[0] += 1
+        return True if calls[0] == at_call else real()
+    sim._has_nonfinite_state = check
+
+
+def _single_run(case, inject=None):
+    sim = bubble_simulation(case)
+    if inject:
+        _inject_failure(sim, inject)
+    sim._update_properties()
+    sim._update_curvature()
+    dts = []
+    for _ in range(STEPS):
+        dt = sim.cfl_calc.compute([sim.velocity[a] for a in range(2)], sim.mu.data,
+                                  sim.rho.data)
+        dts.append(sim._step_with_retry(dt))
+    return _state(sim), dts
+
+
+def _compare(batched, single, i):
+    for k in ("psi", "rho", "kappa", "p", "u", "v"):
+        a, b = batched[k], single[k]
+        assert torch.equal(a, b), \
+            f"simulation {i} field {k}: max diff {(a - b).abs().max().item():.3e}"
+    assert batched["time"] == single["time"] and batched["step"] == single["step"], i
+
+
+@pytest.mark.parametrize("inject", [None, 2])
+def test_batched_equals_single_runs(inject):
+    _announce(f"batched simulation: {len(CASES)} uneven bubbles x {STEPS} steps, "
+              f"inject={inject}")
+    singles = [_single_run(c, inject if i == 1 else None) for i, c in enumerate(CASES)]
+    sims = [bubble_simulation(c) for c in CASES]
+    if inject:
+        _inject_failure(sims[1], inject)
+    batch = BatchedSimulation(sims)
+    batch.initialize()
+    dts = [batch.step() for _ in range(STEPS)]
+    for i, sim in enumerate(sims):
+        _compare(_state(sim), singles[i][0], i)
+        assert [d[i] for d in dts] == singles[i][1], i
+    assert batch.retries == (1 if inject else 0)
+    assert batch.steps_taken == len(CASES) * STEPS
+    assert batch.step(t_end=0.0) == [None] * len(CASES)

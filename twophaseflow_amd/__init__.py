@@ -19,12 +19,13 @@ from .rhie_chow import RhieChowInterpolator
 from .dccd import DissipativeCCDAdvection
 from .curvature import CurvatureCalculatorPsi
 from .reinit import SplitReinitializer
-from .ppe import (IPPESolver, MatrixAssemblyUnavailable, PPESolverFCCDMatrixFree,
+from .ppe import (IPPESolver, MatrixAssemblyUnavailable, PPEBatch, PPESolverFCCDMatrixFree,
                   PPESolverFVMMatrixFree, create_ppe_solver, register_ppe_solver)
 from .ns_terms import (BoundaryConditionHandler, CFLCalculator, ClosestPointExtender,
                        NullFieldExtender, Predictor, UCCD6ConvectionTerm, VelocityCorrector,
                        ViscousTerm)
-from .simulation import SimulationBuilder, TwoPhaseSimulation
+from .simulation import (BatchedSimulation, SimulationBuilder, TwoPhaseSimulation,
+                         bubble_simulation)
 
 __all__ = [
     "FluidConfig", "GridConfig", "NumericsConfig", "SimulationConfig", "SolverConfig", "Grid",
@@ -35,5 +36,6 @@ __all__ = [
     "create_ppe_solver",
     "register_ppe_solver", "BoundaryConditionHandler", "CFLCalculator", "ClosestPointExtender",
     "NullFieldExtender", "Predictor", "UCCD6ConvectionTerm", "VelocityCorrector", "ViscousTerm",
-    "SimulationBuilder", "TwoPhaseSimulation",
+    "SimulationBuilder", "TwoPhaseSimulation", "PPEBatch", "BatchedSimulation",
+    "bubble_simulation",
 ]

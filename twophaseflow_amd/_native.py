@@ -70,6 +70,10 @@ SIGNATURES = {
     "tpf_ppe_apply": (C.c_int, [_vp, _vp, _vp]),
     "tpf_ppe_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveReport)]),
     "tpf_ppe_true_relres": (C.c_int, [_vp, _vp, _vp, _dp]),
+    "tpf_ppe_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp]),
+    "tpf_ppe_batch_solve": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp),
+                                      C.POINTER(SolveOpts), C.POINTER(SolveReport)]),
+    "tpf_ppe_batch_destroy": (C.c_int, [_vp]),
     "tpf_ppe_solve_gmres": (C.c_int, [_vp, _vp, _vp, C.POINTER(GmresOpts),
                                       C.POINTER(SolveReport)]),
     "tpf_ppe_gmres_precond": (C.c_int, [_vp, _vp, _vp, C.POINTER(GmresOpts)]),

@@ -806,6 +806,13 @@ extern "C" tpf_status tpf_ppe_solve(tpf_ppe* s, const double* rhs, double* x,
     TPF_HIP(hipGetLastError());
     TPF_TRY(ensure_graph(s, x, chunk));
     TPF_TRY(read_state(s));
+    if (s->st_host->bb == 0.0) {  // b = 0: SciPy's cg returns b itself, whatever x0
+      hipLaunchKernelGGL(k_fill_zero, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g.total, x);
+      TPF_HIP(hipGetLastError());
+      TPF_HIP(hipStreamSynchronize(s->stream));
+      rr = bb = tr = 0.0;
+      break;
+    }
     while (!s->st_host->done) {
       TPF_HIP(hipGraphLaunch(s->graph_exec, s->stream));
       TPF_TRY(read_state(s));
@@ -932,6 +939,12 @@ tpf_status tpf_ppe_solve_mixed(tpf_ppe* s, const double* rhs, double* x,
     TPF_TRY(read_inner(s));
     PcgState st = *s->st_in_host;
     const double bb = st.true_bb;
+    if (bb == 0.0) {  // b = 0: SciPy's cg returns b itself, whatever x0
+      hipLaunchKernelGGL(k_fill_zero, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g.total, x);
+      TPF_HIP(hipGetLastError());
+      relres = 0.0;
+      break;
+    }
     relres = bb > 0.0 ? std::sqrt(st.true_rr / bb) : std::sqrt(st.true_rr);
     if (relres <= opts->tol || outer >= kMaxOuter || total_inner >= opts->maxiter) break;
     st.rr = st.true_rr;
@@ -1143,3 +1156,518 @@ static tpf_status solve_mg(tpf_ppe* s, const double* rhs, double* x, const tpf_m
   return TPF_OK;
 }
 }  // namespace tpf
+
+// ---- batched Jacobi-PCG: independent problems iterated together ------------
+//
+// Every kernel of one batched iteration covers all problems: workgroup blk
+// belongs to the problem whose block range [begin[i], begin[i+1]) holds it,
+// and runs that problem's grid-stride loop with the problem's own workgroup
+// count (the count tpf_ppe_solve uses), writing block partials into the
+// problem's buffer; the finishing kernels run one workgroup per problem.
+// Each problem's arithmetic and reduction order are those of tpf_ppe_solve,
+// so its result is bit-identical to a single solve.  Chunks of iterations
+// are captured once as a hipGraph (the descriptor table is read from device
+// memory at run time) and replayed until every problem has converged.
+namespace tpf {
+namespace {
+
+struct PcgProb {
+  PpeGeom g;
+  const double* rho;
+  const double* dinv;
+  const double* rhs;
+  double *x, *b, *r, *p, *q, *partials;
+  PcgState* st;
+  int grid;
+  int active;  // takes part in the current (re)start
+  int zero_x;
+};
+
+// Flags gathered for the host after each chunk.
+struct PcgFlags {
+  double rr, bb, true_rr, true_bb;
+  int64_t iter;
+  int done, pad;
+};
+
+__device__ inline int prob_of(const unsigned* __restrict__ begin, int n, unsigned blk) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {  // last i with begin[i] <= blk
+    const int mid = (lo + hi + 1) >> 1;
+    if (begin[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int K>
+__device__ inline void block_partials_at(double (&v)[K], double* partials, double* sh,
+                                         unsigned blk) {
+  block_sum<K>(v, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) partials[blk * K + k] = v[k];
+}
+
+#define PCG_BATCH_PROLOGUE                                 \
+  const int pi = prob_of(begin, n, blockIdx.x);            \
+  const PcgProb& P = tab[pi];                              \
+  const unsigned blk = blockIdx.x - begin[pi];             \
+  const int64_t nthreads = (int64_t)P.grid * kBlock;       \
+  const PpeGeom& g = P.g;
+
+__global__ __launch_bounds__(kBlock) void kb_init(const PcgProb* __restrict__ tab,
+                                                  const unsigned* __restrict__ begin, int n) {
+  PCG_BATCH_PROLOGUE
+  if (!P.active) return;
+  __shared__ double sh[3 * kBlock / 64];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int64_t idx = (int64_t)blk * kBlock + threadIdx.x; idx < g.total; idx += nthreads) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    const double W = dv(g, 0, i0) * dv(g, 1, i1) * dv(g, 2, i2);
+    const double bi = idx == g.pin ? 0.0 : -W * P.rhs[idx];
+    double ap = 0.0, dg;
+    if (P.zero_x) {
+      P.x[idx] = 0.0;
+    } else {
+      stencil<double>(g, P.x, P.rho, idx, i0, i1, i2, ap, dg);
+      if (idx == g.pin) ap = P.x[idx];
+    }
+    const double ri = bi - ap;
+    P.b[idx] = bi;
+    P.r[idx] = ri;
+    P.p[idx] = 0.0;
+    v[0] += ri * ri;
+    v[1] += ri * P.dinv[idx] * ri;
+    v[2] += bi * bi;
+  }
+  block_partials_at<3>(v, P.partials, sh, blk);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_fin_init(const PcgProb* __restrict__ tab) {
+  const PcgProb& P = tab[blockIdx.x];
+  if (!P.active) return;
+  __shared__ double sh[3 * kBlock / 64];
+  double v[3];
+  finish_sum<3>(v, P.partials, P.grid, sh);
+  if (threadIdx.x == 0) {
+    PcgState* st = P.st;
+    st->rr = v[0];
+    st->rz = v[1];
+    st->bb = v[2];
+    st->beta = 0.0;
+    st->alpha = 0.0;
+    st->iter = 0;
+    st->done = (v[0] <= st->tol2 * v[2] || v[2] == 0.0) ? 1 : 0;  // b = 0: x = 0
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_update_p(const PcgProb* __restrict__ tab,
+                                                      const unsigned* __restrict__ begin, int n) {
+  PCG_BATCH_PROLOGUE
+  const PcgState* st = P.st;
+  if (st->done) return;
+  const double beta = st->beta, alpha = st->alpha;
+  for (int64_t idx = (int64_t)blk * kBlock + threadIdx.x; idx < g.total; idx += nthreads) {
+    const double po = P.p[idx];
+    P.x[idx] += alpha * po;
+    P.p[idx] = P.dinv[idx] * P.r[idx] + beta * po;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_matvec(const PcgProb* __restrict__ tab,
+                                                    const unsigned* __restrict__ begin, int n) {
+  PCG_BATCH_PROLOGUE
+  if (P.st->done) return;
+  __shared__ double sh[kBlock / 64];
+  double v[1] = {0.0};
+  for (int64_t idx = (int64_t)blk * kBlock + threadIdx.x; idx < g.total; idx += nthreads) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    double ap, dg;
+    stencil<double>(g, P.p, P.rho, idx, i0, i1, i2, ap, dg);
+    if (idx == g.pin) ap = P.p[idx];
+    P.q[idx] = ap;
+    v[0] += P.p[idx] * ap;
+  }
+  block_partials_at<1>(v, P.partials, sh, blk);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_fin_alpha(const PcgProb* __restrict__ tab) {
+  const PcgProb& P = tab[blockIdx.x];
+  PcgState* st = P.st;
+  if (st->done) return;
+  __shared__ double sh[kBlock / 64];
+  double v[1];
+  finish_sum<1>(v, P.partials, P.grid, sh);
+  if (threadIdx.x == 0) {
+    st->pq = v[0];
+    st->alpha = st->rz / v[0];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_update_r(const PcgProb* __restrict__ tab,
+                                                      const unsigned* __restrict__ begin, int n) {
+  PCG_BATCH_PROLOGUE
+  const PcgState* st = P.st;
+  if (st->done) return;
+  __shared__ double sh[2 * kBlock / 64];
+  const double alpha = st->alpha;
+  double v[2] = {0.0, 0.0};
+  for (int64_t idx = (int64_t)blk * kBlock + threadIdx.x; idx < g.total; idx += nthreads) {
+    const double ri = P.r[idx] - alpha * P.q[idx];
+    P.r[idx] = ri;
+    v[0] += ri * P.dinv[idx] * ri;
+    v[1] += ri * ri;
+  }
+  block_partials_at<2>(v, P.partials, sh, blk);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_fin_beta(const PcgProb* __restrict__ tab) {
+  const PcgProb& P = tab[blockIdx.x];
+  PcgState* st = P.st;
+  if (st->done) return;
+  __shared__ double sh[2 * kBlock / 64];
+  double v[2];
+  finish_sum<2>(v, P.partials, P.grid, sh);
+  if (threadIdx.x == 0) {
+    st->beta = v[0] / st->rz;
+    st->rz = v[0];
+    st->rr = v[1];
+    st->iter += 1;
+    if (v[1] <= st->tol2 * st->bb || st->iter >= st->maxiter) st->done = 1;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_finish_x(const PcgProb* __restrict__ tab,
+                                                      const unsigned* __restrict__ begin, int n) {
+  PCG_BATCH_PROLOGUE
+  if (!P.active) return;
+  const double alpha = P.st->alpha;
+  const bool zero_b = P.st->bb == 0.0;  // SciPy's cg returns b itself when b = 0
+  for (int64_t idx = (int64_t)blk * kBlock + threadIdx.x; idx < g.total; idx += nthreads) {
+    P.x[idx] += alpha * P.p[idx];
+    if (idx == g.pin || zero_b) P.x[idx] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_true_res(const PcgProb* __restrict__ tab,
+                                                      const unsigned* __restrict__ begin, int n) {
+  PCG_BATCH_PROLOGUE
+  if (!P.active) return;
+  __shared__ double sh[2 * kBlock / 64];
+  double v[2] = {0.0, 0.0};
+  for (int64_t idx = (int64_t)blk * kBlock + threadIdx.x; idx < g.total; idx += nthreads) {
+    int i0, i1, i2;
+    coords(g, idx, i0, i1, i2);
+    const double W = dv(g, 0, i0) * dv(g, 1, i1) * dv(g, 2, i2);
+    const double bi = idx == g.pin ? 0.0 : -W * P.rhs[idx];
+    double ap, dg;
+    stencil<double>(g, P.x, P.rho, idx, i0, i1, i2, ap, dg);
+    if (idx == g.pin) ap = P.x[idx];
+    const double ri = bi - ap;
+    v[0] += ri * ri;
+    v[1] += bi * bi;
+  }
+  block_partials_at<2>(v, P.partials, sh, blk);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_fin_true(const PcgProb* __restrict__ tab) {
+  const PcgProb& P = tab[blockIdx.x];
+  if (!P.active) return;
+  __shared__ double sh[2 * kBlock / 64];
+  double v[2];
+  finish_sum<2>(v, P.partials, P.grid, sh);
+  if (threadIdx.x == 0) {
+    P.st->true_rr = v[0];
+    P.st->true_bb = v[1];
+  }
+}
+
+__global__ void kb_pin_zero(const PcgProb* __restrict__ tab, int n) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n && tab[i].active) tab[i].x[tab[i].g.pin] = 0.0;
+}
+
+__global__ void kb_gather(const PcgProb* __restrict__ tab, int n, PcgFlags* __restrict__ out) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  const PcgState* st = tab[i].st;
+  PcgFlags f;
+  f.rr = st->rr;
+  f.bb = st->bb;
+  f.true_rr = st->true_rr;
+  f.true_bb = st->true_bb;
+  f.iter = st->iter;
+  f.done = st->done;
+  f.pad = 0;
+  out[i] = f;
+}
+
+#undef PCG_BATCH_PROLOGUE
+
+}  // namespace
+}  // namespace tpf
+
+struct tpf_ppe_batch {
+  int n = 0;
+  std::vector<tpf_ppe*> h;
+  hipStream_t stream = nullptr;
+  tpf::PcgProb* d_tab = nullptr;
+  tpf::PcgProb* h_tab = nullptr;      // pinned staging
+  unsigned* d_begin = nullptr;
+  tpf::PcgFlags* d_flags = nullptr;
+  tpf::PcgFlags* h_flags = nullptr;   // pinned
+  unsigned blocks = 0;
+  hipStream_t user_stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_chunk = 0;
+};
+
+namespace tpf {
+namespace {
+
+tpf_status batch_gather(tpf_ppe_batch* b) {
+  hipLaunchKernelGGL(kb_gather, dim3((b->n + 63) / 64), dim3(64), 0, b->stream,
+                     (const PcgProb*)b->d_tab, b->n, b->d_flags);
+  TPF_HIP(hipGetLastError());
+  TPF_HIP(hipMemcpyAsync(b->h_flags, b->d_flags, sizeof(PcgFlags) * b->n, hipMemcpyDeviceToHost,
+                         b->stream));
+  TPF_HIP(hipStreamSynchronize(b->stream));
+  return TPF_OK;
+}
+
+tpf_status batch_upload(tpf_ppe_batch* b) {
+  TPF_HIP(hipMemcpyAsync(b->d_tab, b->h_tab, sizeof(PcgProb) * b->n, hipMemcpyHostToDevice,
+                         b->stream));
+  return TPF_OK;
+}
+
+tpf_status batch_graph(tpf_ppe_batch* b, int chunk) {
+  if (b->graph && b->graph_chunk == chunk) return TPF_OK;
+  if (b->graph) {
+    TPF_HIP(hipGraphExecDestroy(b->graph));
+    b->graph = nullptr;
+  }
+  const PcgProb* tab = b->d_tab;
+  const unsigned* beg = b->d_begin;
+  const int n = b->n;
+  hipGraph_t graph;
+  TPF_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int it = 0; it < chunk; ++it) {
+    hipLaunchKernelGGL(kb_update_p, dim3(b->blocks), dim3(kBlock), 0, b->stream, tab, beg, n);
+    hipLaunchKernelGGL(kb_matvec, dim3(b->blocks), dim3(kBlock), 0, b->stream, tab, beg, n);
+    hipLaunchKernelGGL(kb_fin_alpha, dim3(n), dim3(kBlock), 0, b->stream, tab);
+    hipLaunchKernelGGL(kb_update_r, dim3(b->blocks), dim3(kBlock), 0, b->stream, tab, beg, n);
+    hipLaunchKernelGGL(kb_fin_beta, dim3(n), dim3(kBlock), 0, b->stream, tab);
+  }
+  TPF_HIP(hipStreamEndCapture(b->stream, &graph));
+  hipError_t e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess)
+    return fail(TPF_ERR_RUNTIME, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  b->graph_chunk = chunk;
+  return TPF_OK;
+}
+
+}  // namespace
+}  // namespace tpf
+
+extern "C" tpf_status tpf_ppe_batch_destroy(tpf_ppe_batch* b) {
+  if (!b) return TPF_OK;
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->d_tab) (void)hipFree(b->d_tab);
+  if (b->h_tab) (void)hipHostFree(b->h_tab);
+  if (b->d_begin) (void)hipFree(b->d_begin);
+  if (b->d_flags) (void)hipFree(b->d_flags);
+  if (b->h_flags) (void)hipHostFree(b->h_flags);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ppe_batch_create(tpf_ppe_batch** out, tpf_ppe* const* handles, int n,
+                                           void* stream) {
+  if (!out || !handles || n < 1) return fail(TPF_ERR_VALUE, "need at least one PPE handle");
+  *out = nullptr;
+  auto* b = new tpf_ppe_batch();
+  b->n = n;
+  std::vector<unsigned> begin(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    if (!handles[i]) {
+      delete b;
+      return fail(TPF_ERR_VALUE, "null PPE handle in batch");
+    }
+    for (int j = 0; j < i; ++j)
+      if (handles[j] == handles[i]) {
+        delete b;
+        return fail(TPF_ERR_VALUE, "a PPE handle appears twice in one batch");
+      }
+    b->h.push_back(handles[i]);
+    begin[i + 1] = begin[i] + (unsigned)handles[i]->grid;
+  }
+  b->blocks = begin[n];
+  b->user_stream = (hipStream_t)stream;
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&b->d_tab, sizeof(tpf::PcgProb) * n);
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_tab, sizeof(tpf::PcgProb) * n);
+  if (e == hipSuccess) e = hipMalloc(&b->d_begin, sizeof(unsigned) * (n + 1));
+  if (e == hipSuccess) e = hipMalloc(&b->d_flags, sizeof(tpf::PcgFlags) * n);
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_flags, sizeof(tpf::PcgFlags) * n);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_begin, begin.data(), sizeof(unsigned) * (n + 1), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    tpf_ppe_batch_destroy(b);
+    return fail(TPF_ERR_RUNTIME, std::string("PPE batch allocation: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ppe_batch_solve(tpf_ppe_batch* b, const double* const* rhs,
+                                          double* const* x, const tpf_solve_opts* opts,
+                                          tpf_solve_report* reports) {
+  using namespace tpf;
+  if (!b || !rhs || !x || !opts) return fail(TPF_ERR_VALUE, "null argument");
+  if (opts->precond != 1) return fail(TPF_ERR_VALUE, "batched CG runs Jacobi-PCG (precond 1)");
+  if (opts->mixed) return fail(TPF_ERR_VALUE, "batched CG runs in FP64");
+  if (!(opts->tol > 0.0) || opts->maxiter < 1) return fail(TPF_ERR_VALUE, "bad tol/maxiter");
+  const int n = b->n;
+  int n_on = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!rhs[i] != !x[i]) return fail(TPF_ERR_VALUE, "rhs and x must both be set or both null");
+    if (!rhs[i]) continue;  // problem i sits this solve out
+    ++n_on;
+    if (!b->h[i]->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set for a batch member");
+  }
+  if (n_on == 0) {
+    if (reports)
+      for (int i = 0; i < n; ++i) reports[i] = tpf_solve_report{};
+    return TPF_OK;
+  }
+  {  // order the batch stream after the caller's
+    hipEvent_t ev;
+    TPF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    TPF_HIP(hipEventRecord(ev, b->user_stream));
+    TPF_HIP(hipStreamWaitEvent(b->stream, ev, 0));
+    TPF_HIP(hipEventDestroy(ev));
+  }
+  std::vector<int64_t> iters(n, 0);
+  std::vector<int> active(n, 0), restart(n, 0);
+  for (int i = 0; i < n; ++i) active[i] = rhs[i] ? 1 : 0;
+  std::vector<double> rr(n, 0.0), bb(n, 0.0), tr(n, 0.0);
+  const int chunk = opts->check_every > 0 ? opts->check_every : 32;
+  for (int i = 0; i < n; ++i) {
+    tpf_ppe* s = b->h[i];
+    PcgProb& P = b->h_tab[i];
+    P.g = s->g;
+    P.rho = s->rho;
+    P.dinv = s->dinv;
+    P.rhs = rhs[i];
+    P.x = x[i];
+    P.b = s->b;
+    P.r = s->r;
+    P.p = s->p;
+    P.q = s->q;
+    P.partials = s->partials;
+    P.st = s->st;
+    P.grid = s->grid;
+    P.active = active[i];
+    P.zero_x = opts->use_x0 ? 0 : 1;
+    if (!active[i]) {  // a skipped problem's kernels return at once (done = 1)
+      PcgState idle{};
+      idle.done = 1;
+      *s->st_host = idle;
+      TPF_HIP(hipMemcpyAsync(s->st, s->st_host, sizeof(idle), hipMemcpyHostToDevice, b->stream));
+    }
+  }
+  TPF_TRY(batch_upload(b));
+  if (opts->use_x0) {
+    hipLaunchKernelGGL(kb_pin_zero, dim3((n + 63) / 64), dim3(64), 0, b->stream,
+                       (const PcgProb*)b->d_tab, n);
+    TPF_HIP(hipGetLastError());
+  }
+  TPF_TRY(batch_graph(b, chunk));
+  for (int round = 0;; ++round) {
+    bool any = false;
+    for (int i = 0; i < n; ++i) {
+      if (!active[i]) continue;
+      any = true;
+      PcgState init{};
+      init.tol2 = opts->tol * opts->tol;
+      init.maxiter = opts->maxiter - iters[i];
+      *b->h[i]->st_host = init;
+      TPF_HIP(hipMemcpyAsync(b->h[i]->st, b->h[i]->st_host, sizeof(init), hipMemcpyHostToDevice,
+                             b->stream));
+    }
+    if (!any) break;
+    hipLaunchKernelGGL(kb_init, dim3(b->blocks), dim3(kBlock), 0, b->stream,
+                       (const PcgProb*)b->d_tab, (const unsigned*)b->d_begin, n);
+    hipLaunchKernelGGL(kb_fin_init, dim3(n), dim3(kBlock), 0, b->stream, (const PcgProb*)b->d_tab);
+    TPF_HIP(hipGetLastError());
+    // inactive members keep done = 1 from their last round
+    TPF_TRY(batch_gather(b));
+    for (;;) {
+      bool all_done = true;
+      for (int i = 0; i < n; ++i) all_done = all_done && b->h_flags[i].done;
+      if (all_done) break;
+      TPF_HIP(hipGraphLaunch(b->graph, b->stream));
+      TPF_TRY(batch_gather(b));
+    }
+    hipLaunchKernelGGL(kb_finish_x, dim3(b->blocks), dim3(kBlock), 0, b->stream,
+                       (const PcgProb*)b->d_tab, (const unsigned*)b->d_begin, n);
+    hipLaunchKernelGGL(kb_true_res, dim3(b->blocks), dim3(kBlock), 0, b->stream,
+                       (const PcgProb*)b->d_tab, (const unsigned*)b->d_begin, n);
+    hipLaunchKernelGGL(kb_fin_true, dim3(n), dim3(kBlock), 0, b->stream, (const PcgProb*)b->d_tab);
+    TPF_HIP(hipGetLastError());
+    TPF_TRY(batch_gather(b));
+    bool again = false;
+    for (int i = 0; i < n; ++i) {
+      if (!active[i]) continue;
+      const PcgFlags& f = b->h_flags[i];
+      iters[i] += f.iter;
+      rr[i] = f.rr;
+      bb[i] = f.bb;
+      tr[i] = f.true_bb > 0.0 ? std::sqrt(f.true_rr / f.true_bb) : std::sqrt(f.true_rr);
+      const bool rec_ok = bb[i] > 0 ? std::sqrt(rr[i] / bb[i]) <= opts->tol : true;
+      // residual replacement, as tpf_ppe_solve: restart from x (at most 4 times)
+      const bool more = !(tr[i] <= opts->tol || !rec_ok || iters[i] >= opts->maxiter ||
+                          restart[i] >= 4);
+      active[i] = more ? 1 : 0;
+      if (more) {
+        restart[i] += 1;
+        again = true;
+      }
+      b->h_tab[i].active = active[i];
+      b->h_tab[i].zero_x = 0;
+    }
+    if (!again) break;
+    TPF_TRY(batch_upload(b));
+  }
+  std::string failed;
+  for (int i = 0; i < n; ++i) {
+    tpf_solve_report r{};
+    if (!rhs[i]) {
+      if (reports) reports[i] = r;
+      continue;
+    }
+    r.iterations = iters[i];
+    r.relres = bb[i] > 0 ? std::sqrt(rr[i] / bb[i]) : 0.0;
+    r.true_relres = tr[i];
+    r.converged = tr[i] <= opts->tol ? 1 : 0;
+    if (reports) reports[i] = r;
+    if (r.relres > opts->tol) failed += (failed.empty() ? "" : ", ") + std::to_string(i);
+  }
+  {  // the caller's stream waits for the batch
+    hipEvent_t ev;
+    TPF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    TPF_HIP(hipEventRecord(ev, b->stream));
+    TPF_HIP(hipStreamWaitEvent(b->user_stream, ev, 0));
+    TPF_HIP(hipEventDestroy(ev));
+  }
+  if (!failed.empty())
+    return fail(TPF_ERR_RUNTIME, "batched PPE CG did not converge cleanly for problem(s) " +
+                                     failed + " (maxiter " + std::to_string(opts->maxiter) + ")");
+  return TPF_OK;
+}

@@ -104,3 +104,28 @@ def divergence(ccd, vel) -> torch.Tensor:
         ccd.differentiate_into(u, g, None, ax)
         div += g
     return div
+
+
+def uneven_bubble_batch(seed: int = 5, sizes=(128, 256, 512), per_size: int = 16):
+    """Seeded 2-D rising-bubble cases on [0,1]^2 (wall box, bubble centred):
+    per_size cases per N x N grid in `sizes`, radius ~ U[0.15, 0.3],
+    density ratio rho_l/rho_g log-uniform in [10, 1000], We ~ U[1, 50].
+    Returns dicts with N, radius, density_ratio, We and seed order."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for N in sizes:
+        for _ in range(per_size):
+            cases.append({"N": int(N), "radius": float(rng.uniform(0.15, 0.3)),
+                          "density_ratio": float(10.0 ** rng.uniform(1.0, 3.0)),
+                          "We": float(rng.uniform(1.0, 50.0)), "index": len(cases)})
+    return cases
+
+
+def bubble_psi_2d(N: int, radius: float, centre=(0.5, 0.5), epsilon_factor: float = 1.5):
+    """Host psi of a gas disc in liquid on the (N+1)^2 nodes of [0,1]^2:
+    psi = 1 / (1 + exp(-(|x - c| - radius) / eps)), eps = epsilon_factor / N
+    (psi -> 1 in the liquid)."""
+    x = np.linspace(0.0, 1.0, N + 1)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    d = np.sqrt((X - centre[0]) ** 2 + (Y - centre[1]) ** 2) - radius
+    return 1.0 / (1.0 + np.exp(-d / (epsilon_factor / N)))

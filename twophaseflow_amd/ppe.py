@@ -289,6 +289,103 @@ class PPESolverFVMMatrixFree(IPPESolver):
         return out.value
 
 
+class PPEBatch:
+    """Jacobi-PCG over several ``PPESolverFVMMatrixFree`` solvers at once
+    (CG form, FP64): one launch per PCG step covers every problem and each
+    problem's result equals its own ``solve_into`` bit for bit.  The
+    reference solves such problems one after another
+    (``fvm_matrixfree.py:150-260`` per simulation).
+
+    The solvers must share tol, maxiter and the check period.  Densities are
+    set per solver with ``prepare_operator`` before ``solve_into``."""
+
+    def __init__(self, solvers):
+        solvers = list(solvers)
+        if not solvers:
+            raise ValueError("PPEBatch needs at least one solver")
+        s0 = solvers[0]
+        for s in solvers:
+            if not isinstance(s, PPESolverFVMMatrixFree):
+                raise TypeError("PPEBatch takes PPESolverFVMMatrixFree solvers")
+            if s.iteration_method != "cg" or s.preconditioner != "jacobi" or \
+                    s.precision != "fp64":
+                raise ValueError("PPEBatch runs the CG form with Jacobi preconditioning in "
+                                 "FP64 (ppe_iteration_method='cg', ppe_preconditioner="
+                                 "'jacobi', ppe_precision='fp64')")
+            if (s.tol, s.maxiter, s.check_every) != (s0.tol, s0.maxiter, s0.check_every):
+                raise ValueError("solvers of one PPEBatch must share pseudo_tol, "
+                                 "pseudo_maxiter and ppe_check_every")
+        self.solvers = solvers
+        self._lib = nat.lib()
+        h = C.c_void_p()
+        nat.check(self._lib.tpf_ppe_batch_create(
+            C.byref(h), nat.parr([s._handle.value for s in solvers]), len(solvers),
+            stream_handle()))
+        self._handle = h
+        self.last_reports: list = []
+
+    def __len__(self):
+        return len(self.solvers)
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            self._lib.tpf_ppe_batch_destroy(h)
+            self._handle = None
+
+    def solve_into(self, rhs, p, use_x0: bool = False, raise_on_failure: bool = True):
+        """Solves problem i for p[i] (CUDA tensors of solver i's grid shape;
+        rhs[i] = p[i] = None leaves problem i out of this call).  Returns the per-problem diagnostics (the keys of
+        ``PPESolverFVMMatrixFree.solve_into``) and stores them in each
+        solver's ``last_diagnostics``.  With ``raise_on_failure`` a problem
+        that ended above tol raises RuntimeError naming it; otherwise its
+        diagnostics carry ``converged=False`` and the caller decides."""
+        n = len(self.solvers)
+        if len(rhs) != n or len(p) != n:
+            raise ValueError(f"expected {n} right-hand sides and solutions")
+        for s, r, x in zip(self.solvers, rhs, p):
+            if (r is None) != (x is None):
+                raise ValueError("rhs[i] and p[i] must both be given or both be None")
+            if r is None:
+                continue
+            for t in (r, x):
+                if tuple(t.shape) != s.shape or t.dtype != torch.float64 or not t.is_cuda \
+                        or not t.is_contiguous():
+                    raise ValueError(f"batch tensors must be contiguous CUDA float64 of "
+                                     f"shape {s.shape}")
+            if s._rho is None:
+                raise RuntimeError("prepare_operator(rho) must be called on every solver")
+        s0 = self.solvers[0]
+        opts = nat.SolveOpts(tol=s0.tol, maxiter=s0.maxiter, precond=1,
+                             check_every=s0.check_every, use_x0=1 if use_x0 else 0, mixed=0,
+                             inner_tol=0.0)
+        reps = (nat.SolveReport * n)()
+        status = self._lib.tpf_ppe_batch_solve(self._handle, nat.parr([ptr(t) for t in rhs]),
+                                               nat.parr([ptr(t) for t in p]), C.byref(opts),
+                                               reps)
+        if status not in (nat.TPF_OK, nat.TPF_ERR_RUNTIME):
+            nat.check(status)
+        out = []
+        for s, r, t in zip(self.solvers, reps, rhs):
+            if t is None:
+                out.append(None)
+                continue
+            ok = r.relres <= s.tol
+            d = {"ppe_low_order_cg_info": 0 if ok else int(r.iterations),
+                 "ppe_low_order_cg_tol": s.tol, "ppe_low_order_cg_maxiter": s.maxiter,
+                 "iterations": int(r.iterations), "relres": float(r.relres),
+                 "true_relres": float(r.true_relres), "refinements": 0,
+                 "precision": "fp64", "preconditioner": "jacobi", "converged": ok}
+            s.last_diagnostics = d
+            out.append(d)
+        self.last_reports = out
+        if status == nat.TPF_ERR_RUNTIME and raise_on_failure:
+            msg = self._lib.tpf_last_error().decode(errors="replace")
+            raise RuntimeError(f"PPEBatch did not converge cleanly: {msg}; no alternate PPE "
+                               "solver was applied.")
+        return out
+
+
 class PPESolverFCCDMatrixFree(IPPESolver):
     """Matrix-free FCCD PPE on the GPU: L p = D_f[(1/rho)_f G_f p] with the
     FCCD face gradient (ppe/fccd_matrixfree.py:108-600, ccd/fccd.py:243-277),

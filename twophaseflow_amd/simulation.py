@@ -175,12 +175,17 @@ class TwoPhaseSimulation:
             self.vel_star._c[a] = s["vel_star"][a].clone()
         self.time, self.step = s["time"], s["step"]
 
-    def _step_with_retry(self, dt_nominal: float, verbose: bool = False) -> float:
-        """legacy_run_loop.py:45-67: rollback and halve dt, at most 12 times."""
+    def _step_with_retry(self, dt_nominal: float, verbose: bool = False,
+                         first_retry: int = 0, snap=None) -> float:
+        """legacy_run_loop.py:45-67: rollback and halve dt, at most 12 times.
+        ``first_retry`` > 0 continues a sequence whose earlier attempts ran
+        elsewhere (a batched step): dt_nominal is then the nominal dt and
+        ``snap`` the state before the first attempt."""
         max_retries = 12
-        dt_try = float(dt_nominal)
-        snap = self._snapshot()
-        for retry in range(max_retries + 1):
+        dt_try = float(dt_nominal) * 0.5 ** first_retry
+        if snap is None:
+            snap = self._snapshot()
+        for retry in range(first_retry, max_retries + 1):
             try:
                 self.step_forward(dt_try)
                 if self._has_nonfinite_state():
@@ -210,6 +215,13 @@ class TwoPhaseSimulation:
 
     def step_forward(self, dt: float) -> None:
         """The 7-step algorithm (_core.py:126-166)."""
+        vel_star, rhs = self._step_pre_ppe(dt)
+        # 6: PPE for dp (legacy_flow_helpers.py:38-58)
+        delta_p = self.ppe_solver.solve(rhs, self.rho.data, dt, p_init=None)
+        self._step_post_ppe(delta_p, vel_star, dt)
+
+    def _step_pre_ppe(self, dt: float):
+        """Steps 1-5 and the PPE right-hand side; returns (u*, rhs)."""
         nd = self.config.grid.ndim
         # 1-2: CLS advection + reinitialisation (legacy_flow_helpers.py:23-27)
         vel = [self.velocity[a] for a in range(nd)]
@@ -229,9 +241,23 @@ class TwoPhaseSimulation:
         # 5b: wall BC on u* before the PPE right-hand side reads it (_core.py:151)
         self._bc_handler.apply(self.vel_star)
         vel_star = [self.vel_star[a] for a in range(nd)]
-        # 6: PPE for dp, p += dp (legacy_flow_helpers.py:38-58)
-        delta_p = self._step_ppe(vel_star, dt)
-        # 7: corrector with the extended dp (legacy_flow_helpers.py:61-67)
+        # 6a: Rhie-Chow divergence / dt (legacy_flow_helpers.py:38-52)
+        div_rc = self.rhie_chow.face_velocity_divergence(vel_star, self.pressure.data,
+                                                         self.rho.data, dt)
+        rhs = torch.empty_like(div_rc)
+        nat.check(nat.lib().tpf_ns_div_scalar(div_rc.numel(), ptr(div_rc), float(dt), ptr(rhs),
+                                              stream_handle()))
+        return vel_star, rhs
+
+    def _step_post_ppe(self, delta_p: torch.Tensor, vel_star: List[torch.Tensor],
+                       dt: float) -> None:
+        """p += dp, then step 7: corrector with the extended dp
+        (legacy_flow_helpers.py:55-67) and the wall BC."""
+        nd = self.config.grid.ndim
+        p_new = torch.empty_like(delta_p)
+        nat.check(nat.lib().tpf_ns_add(delta_p.numel(), ptr(self.pressure.data), ptr(delta_p),
+                                       ptr(p_new), stream_handle()))
+        self.pressure._data = p_new
         n_hat = self._field_ext.compute_normal(self.phi.data)
         dp_ext = self._field_ext.extend(delta_p, self.phi.data, n_hat)
         vel_new = self.vel_corrector.correct(vel_star, dp_ext, self.rho.data, dt)
@@ -240,19 +266,6 @@ class TwoPhaseSimulation:
         self._bc_handler.apply(self.velocity)
         self.time += dt
         self.step += 1
-
-    def _step_ppe(self, vel_star: List[torch.Tensor], dt: float) -> torch.Tensor:
-        div_rc = self.rhie_chow.face_velocity_divergence(vel_star, self.pressure.data,
-                                                         self.rho.data, dt)
-        n = div_rc.numel()
-        lib, st = nat.lib(), stream_handle()
-        rhs = torch.empty_like(div_rc)
-        nat.check(lib.tpf_ns_div_scalar(n, ptr(div_rc), float(dt), ptr(rhs), st))
-        delta_p = self.ppe_solver.solve(rhs, self.rho.data, dt, p_init=None)
-        p_new = torch.empty_like(delta_p)
-        nat.check(lib.tpf_ns_add(n, ptr(self.pressure.data), ptr(delta_p), ptr(p_new), st))
-        self.pressure._data = p_new
-        return delta_p
 
     def _update_properties(self) -> None:
         """legacy_flow_helpers.py:70-80 (heaviside.py:167-187)."""
@@ -352,3 +365,127 @@ class SimulationBuilder:
                     _bc_handler=BoundaryConditionHandler(cfg),
                     _diagnostics=DiagnosticsReporter(None, grid), _field_ext=ext)
         return TwoPhaseSimulation._from_components(comp)
+
+
+class BatchedSimulation:
+    """Steps independent legacy simulations together (uneven grids allowed).
+
+    Each step runs, per simulation, the CFL time step and steps 1-5 plus the
+    Rhie-Chow right-hand side (``_step_pre_ppe``), then ONE batched
+    Jacobi-PCG solve of every simulation's PPE (``PPEBatch``: one launch per
+    PCG step covers all problems, chunks of iterations replay as a captured
+    graph), then the corrector (``_step_post_ppe``) and the non-finite check.
+    A simulation whose attempt fails (PPE above tol, non-finite state, an
+    exception) is rolled back and continues the run loop's dt-halving
+    sequence on its own from the second attempt (legacy_run_loop.py:45-67),
+    so every simulation follows exactly the trajectory of its own ``run``:
+    the batched PPE solve equals the single solve bit for bit.
+
+    The reference steps such simulations one after another; the simulations
+    must use the CG form of the FVM matrix-free PPE with Jacobi
+    preconditioning in FP64 and share its tolerance settings."""
+
+    def __init__(self, sims):
+        from .ppe import PPEBatch
+        self.sims = list(sims)
+        if not self.sims:
+            raise ValueError("BatchedSimulation needs at least one simulation")
+        self.ppe_batch = PPEBatch([s.ppe_solver for s in self.sims])
+        self.retries = 0          # simulations that needed the dt-halving path
+        self.steps_taken = 0      # simulation steps completed
+
+    def __len__(self):
+        return len(self.sims)
+
+    def initialize(self) -> None:
+        """Properties and curvature of the initial state (as ``run`` does)."""
+        for s in self.sims:
+            s._update_properties()
+            s._update_curvature()
+
+    def step(self, t_end: Optional[float] = None) -> List[Optional[float]]:
+        """One step of every simulation with its own CFL dt (clamped to
+        ``t_end - time`` when given; simulations at t_end sit out).  Returns
+        the dt each simulation took (None for one that sat out)."""
+        sims = self.sims
+        n = len(sims)
+        dts: List[Optional[float]] = [None] * n
+        snaps = [None] * n
+        rhs: List[Optional[torch.Tensor]] = [None] * n
+        sol: List[Optional[torch.Tensor]] = [None] * n
+        vstar: List[Optional[list]] = [None] * n
+        failed = [False] * n
+        for i, s in enumerate(sims):
+            if t_end is not None and s.time >= t_end:
+                continue
+            nd = s.config.grid.ndim
+            dt = s.cfl_calc.compute([s.velocity[a] for a in range(nd)], s.mu.data, s.rho.data)
+            if t_end is not None:
+                dt = min(dt, t_end - s.time)
+            dts[i] = float(dt)
+            snaps[i] = s._snapshot()
+            try:
+                vstar[i], rhs[i] = s._step_pre_ppe(dts[i])
+                s.ppe_solver.prepare_operator(s.rho.data)
+                sol[i] = torch.empty_like(rhs[i])
+            except Exception:
+                failed[i] = True
+                rhs[i] = sol[i] = None
+        reps = self.ppe_batch.solve_into(rhs, sol, raise_on_failure=False)
+        for i, s in enumerate(sims):
+            if rhs[i] is None:
+                continue
+            if not reps[i]["converged"]:
+                failed[i] = True
+                continue
+            try:
+                s._step_post_ppe(sol[i], vstar[i], dts[i])
+                if s._has_nonfinite_state():
+                    failed[i] = True
+            except Exception:
+                failed[i] = True
+        for i, s in enumerate(sims):
+            if dts[i] is None:
+                continue
+            if failed[i]:
+                s._restore(snaps[i])
+                self.retries += 1
+                dts[i] = s._step_with_retry(dts[i], first_retry=1, snap=snaps[i])
+            self.steps_taken += 1
+        return dts
+
+    def run(self, t_end: float, max_steps: Optional[int] = None) -> None:
+        """Steps until every simulation reaches t_end (or max_steps batched
+        steps)."""
+        k = 0
+        while any(s.time < t_end for s in self.sims):
+            if max_steps is not None and k >= max_steps:
+                break
+            self.step(t_end)
+            k += 1
+
+
+def bubble_simulation(case: dict, pseudo_tol: float = 1e-10,
+                      pseudo_maxiter: int = 20000) -> TwoPhaseSimulation:
+    """A legacy 2-D rising-bubble simulation of one ``uneven_bubble_batch``
+    case: the bubble_2d parameters of the reference (configs/bubble_2d.yaml:
+    Re 35, Fr 1, mu ratio 0.1, CFL 0.25, epsilon_factor 1.5, DCCD
+    advection, PPE tolerance 1e-10) on the case's N x N grid of [0,1]^2,
+    with the case's radius, density ratio and We; the PPE in CG form with
+    Jacobi preconditioning (iteration cap `pseudo_maxiter`), the null field
+    extension."""
+    from .config import FluidConfig, GridConfig, NumericsConfig, SimulationConfig, SolverConfig
+    from .fields import bubble_psi_2d
+    N = int(case["N"])
+    cfg = SimulationConfig(
+        grid=GridConfig(ndim=2, N=(N, N), L=(1.0, 1.0)),
+        fluid=FluidConfig(Re=35.0, Fr=1.0, We=float(case["We"]),
+                          rho_ratio=1.0 / float(case["density_ratio"]), mu_ratio=0.1),
+        numerics=NumericsConfig(advection_scheme="dissipative_ccd", cfl_number=0.25,
+                                extension_method="none"),
+        solver=SolverConfig(ppe_solver_type="fvm_iterative", pseudo_tol=pseudo_tol,
+                            pseudo_maxiter=pseudo_maxiter))
+    cfg.solver.ppe_iteration_method = "cg"
+    sim = SimulationBuilder(cfg).build()
+    sim.psi.data = bubble_psi_2d(N, float(case["radius"]))
+    return sim
