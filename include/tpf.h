@@ -97,9 +97,10 @@ TPF_API tpf_status tpf_ccd_axis_kernel(tpf_ccd_plan* plan, int axis, int* kernel
 
 /* Launch shape of one axis (no counterpart in the reference: the knobs the
  * autotuner in twophaseflow_amd/autotune.py sets).  mode 0: automatic (a
- * wavefront per line for < 4096 lines of > 512 unknowns; else a lane per
- * line, with the explicit partition inverse on strided FP64 axes of
- * <= 16 x 16 rows); 1: a lane per line, block recurrences (<= 16
+ * wavefront per line for < 4096 lines of > 256 unknowns, and for >= 4096
+ * contiguous lines of 33-256 unknowns, several lines per wavefront; else a
+ * lane per line, with the explicit partition inverse on strided FP64 axes
+ * of <= 16 x 16 rows); 1: a lane per line, block recurrences (<= 16
  * partitions, rows cap 8/16/32/64); 2: a wavefront per line (<= 64
  * partitions, rows cap 8/16); 3: a lane per line, explicit partition
  * inverse (strided axes, <= 16 partitions, rows cap 8/16).  An

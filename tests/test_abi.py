@@ -184,3 +184,13 @@ def test_launch_mode_selection():
     nat.check(nat.lib().tpf_ccd_axis_kernel(plan, 0, C.byref(k)))
     assert k.value == 2
     nat.lib().tpf_ccd_plan_destroy(plan)
+    # 2-D wall: 513 lines of 511 unknowns take a wavefront per line (64 x 8
+    # rows); 257 lines of 255 take the lane kernels
+    for shape, want in (((513, 513), [2, 2]), ((257, 257), [3, 1])):
+        plan = _plan(shape, (0, 0))
+        got = []
+        for a in range(2):
+            nat.check(nat.lib().tpf_ccd_axis_kernel(plan, a, C.byref(k)))
+            got.append(k.value)
+        assert got == want, shape
+        nat.lib().tpf_ccd_plan_destroy(plan)

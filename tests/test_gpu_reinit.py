@@ -37,6 +37,9 @@ def _psi(shape, eps):
     ((12, 330), "periodic", "periodic", False),
     ((600,), "wall", "neumann", False),
     ((8, 700), "wall", "zero", True),
+    # 2-D lines staged whole in LDS next to >= 16384 lines of the lane sweep
+    ((128, 128, 8), "wall", "neumann", True),
+    ((512, 40), "wall", "neumann", True),
 ])
 def test_reinit_matches_oracle(N, ccd_bc, ls_bc, mass):
     print(f"[gpu-case] reinit N={N} bc={ls_bc} mass={mass}", file=sys.stderr, flush=True)

@@ -1784,8 +1784,9 @@ namespace tpf {
 // Lazily builds and uploads the tables of one axis (the reference builds
 // wall solvers lazily per used axis, ccd_solver.py:108-111, 294-299).
 // Kernel of one axis: a wavefront per line when the lines are few and long
-// (auto: < 4096 lines of > 512 unknowns, e.g. 2-D 1024^2 grids, where the
-// lane-per-line kernel would need 64 rows per partition), else a lane per
+// (auto: < 4096 lines of > 256 unknowns, e.g. 2-D 512^2 and 1024^2 grids,
+// where the lane-per-line kernel needs 32 or 64 rows per partition: 2-D
+// 513^2 wall, 0.031-0.035 ms per axis against 0.070-0.081), else a lane per
 // line.  The wavefront kernel uses 8 rows per partition when the line fits
 // 64 partitions, else 16.
 tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
@@ -1818,7 +1819,7 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
   // <= 256 unknowns on the contiguous axis (four lines of 16 partitions per
   // wavefront: 0.128 against 0.172 ms for the tiled lane kernel on 257^3).
   const bool contig_many = !strided && lines >= 4096 && n > 32 && n <= 256;
-  bool wave = mode == 2 || (mode == 0 && ((lines < 4096 && n > 512) || contig_many ||
+  bool wave = mode == 2 || (mode == 0 && ((lines < 4096 && n > 256) || contig_many ||
                                            (ref && n > 256)));
   if (wave && cap != 0 && cap != 8 && cap != 16)
     return fail(TPF_ERR_VALUE, "the wavefront-per-line CCD kernel takes rows cap 8 or 16");
@@ -1844,13 +1845,14 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
   }
   // Wavefront kernel shape: as many lines per wavefront as its partitions
   // leave lanes for, four wavefronts per workgroup.
-  // One line per wavefront (1025^2 lines): 16 wavefronts per workgroup
-  // measured faster than 8 or 4 (0.096 against 0.110 / 0.099 ms, wall).
+  // One line per wavefront on >= 1024 lines (1025^2): 16 wavefronts per
+  // workgroup measured faster than 8 or 4 (0.096 against 0.110 / 0.099 ms,
+  // wall); fewer lines keep 4 so that more CUs take part.
   if (wave) {
     ax.lpw = ax.geom.P <= 16 ? 4 : (ax.geom.P <= 32 ? 2 : 1);
     // FP32-refined: 4 wavefronts (the tile stays live next to 4 KB of exchange
     // buffers per wavefront: <= 51 KB at 1025 nodes)
-    ax.wpg = (ax.lpw == 1 && !ref) ? 16 : 4;
+    ax.wpg = (ax.lpw == 1 && !ref && lines >= 1024) ? 16 : 4;
   }
   // Auto: strided FP64 axes of <= 16 partitions x 16 rows take the explicit
   // partition inverse (no recurrence state: two workgroups per CU).

@@ -261,6 +261,104 @@ __global__ __launch_bounds__(64) void k_reinit_cn(RGeom g, int a, double d_R, do
   }
 }
 
+// The same sweep for few lines (2-D grids, where k_reinit_cn would run a few
+// wavefronts whose recurrences wait on global memory at every chunk): a
+// workgroup of 64 threads stages `lb` whole lines and the axis factors in LDS
+// with coalesced loads, lanes 0..lb-1 run the recurrences of k_reinit_cn on
+// LDS with the same arithmetic (rolling q values in registers, eliminated
+// right-hand sides and then the solution written over the line), and the
+// workgroup writes the lines back.  LDS: (3 n + lb pitch) doubles.
+__global__ __launch_bounds__(64) void k_reinit_cn_lds(RGeom g, int a, int lb, int pitch,
+                                                      double d_R, double c_R,
+                                                      const double* __restrict__ tf,
+                                                      const double* __restrict__ m,
+                                                      const double* __restrict__ sup,
+                                                      const double* __restrict__ q,
+                                                      double* __restrict__ out) {
+  extern __shared__ double lds[];
+  const int n = g.n[a];
+  double* ftf = lds;         // n - 1 used
+  double* fm = lds + n;      // n
+  double* fsup = lds + 2 * n;  // n - 1 used
+  double* tile = lds + 3 * n;  // lb x pitch
+  const int64_t st = stride_of(g, a);
+  const int64_t nlines = g.total / n;
+  const int64_t l0 = (int64_t)blockIdx.x * lb;
+  const int nl = (int)(nlines - l0 < lb ? nlines - l0 : lb);
+  const int t = threadIdx.x;
+  for (int i = t; i < n - 1; i += 64) {
+    ftf[i] = tf[i];
+    fsup[i] = sup[i];
+  }
+  for (int i = t; i < n; i += 64) fm[i] = m[i];
+  // node i of line l sits at base(l) + i * st
+  auto base_of = [&](int64_t l) -> int64_t {
+    if (a == 0) return l;
+    if (a == 1) return (l / g.n[2]) * ((int64_t)g.n[1] * g.n[2]) + (l % g.n[2]);
+    return l * n;
+  };
+  const int cnt = nl * n;
+  if (a == 2) {  // the workgroup's lines are one contiguous block
+    const double* src = q + l0 * n;
+    for (int e = t; e < cnt; e += 64) {
+      const int j = e / n, i = e - j * n;
+      tile[j * pitch + i] = src[e];
+    }
+  } else {  // node-major: adjacent lines are adjacent in memory
+    for (int e = t; e < cnt; e += 64) {
+      const int i = e / nl, j = e - i * nl;
+      tile[j * pitch + i] = q[base_of(l0 + j) + (int64_t)i * st];
+    }
+  }
+  __syncthreads();
+  if (t < nl) {
+    double* row = tile + t * pitch;
+    double qm = 0.0, qc = row[0], dprev = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double qp = i + 1 < n ? row[i + 1] : 0.0;
+      double rhs;
+      if (i == 0) rhs = d_R * qc + 2.0 * c_R * qp;
+      else if (i == n - 1) rhs = 2.0 * c_R * qm + d_R * qc;
+      else rhs = c_R * qm + d_R * qc + c_R * qp;
+      const double d = i == 0 ? rhs : rhs - ftf[i - 1] * dprev;
+      row[i] = d;
+      dprev = d;
+      qm = qc;
+      qc = qp;
+    }
+    double x = 0.0;
+    for (int i = n - 1; i >= 0; --i) {
+      x = i == n - 1 ? row[i] / fm[n - 1] : (row[i] - fsup[i] * x) / fm[i];
+      row[i] = x;
+    }
+  }
+  __syncthreads();
+  if (a == 2) {
+    double* dst = out + l0 * n;
+    for (int e = t; e < cnt; e += 64) {
+      const int j = e / n, i = e - j * n;
+      dst[e] = tile[j * pitch + i];
+    }
+  } else {
+    for (int e = t; e < cnt; e += 64) {
+      const int i = e / nl, j = e - i * nl;
+      out[base_of(l0 + j) + (int64_t)i * st] = tile[j * pitch + i];
+    }
+  }
+}
+
+// Lines per workgroup of k_reinit_cn_lds (0: use k_reinit_cn).  Taken when
+// k_reinit_cn would launch fewer than 256 workgroups and up to 8 lines fit
+// 64 KB of LDS next to the factors.
+int cn_lds_lines(int n, int64_t nlines) {
+  if (nlines >= 256 * 64) return 0;
+  const int pitch = n | 1;
+  const int64_t room = 65536 / 8 - 3 * (int64_t)n;
+  if (room < pitch) return 0;
+  const int64_t lb = room / pitch;
+  return (int)(lb < 8 ? lb : 8);
+}
+
 __global__ __launch_bounds__(kBlock) void k_clip01(int64_t n, const double* in, double* out) {
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock)
@@ -374,10 +472,20 @@ extern "C" tpf_status tpf_reinit_split(tpf_ccd_plan* ccd, int ndim, const int64_
       for (int a = 0; a < ndim; ++a) {
         const int s = 3 - ndim + a;
         const int64_t nlines = T / g.n[s];
-        hipLaunchKernelGGL(k_reinit_cn, dim3((unsigned)((nlines + 63) / 64)), dim3(64), 0, st, g,
-                           s, dR[a], cR[a], (const double*)(fac + off_tf[a]),
-                           (const double*)(fac + off_m[a]), (const double*)(fac + off_sup[a]),
-                           (const double*)src, dst);
+        const int lb = cn_lds_lines(g.n[s], nlines);
+        if (lb > 0) {
+          const int pitch = g.n[s] | 1;
+          const size_t sh = sizeof(double) * (3 * (size_t)g.n[s] + (size_t)lb * pitch);
+          hipLaunchKernelGGL(k_reinit_cn_lds, dim3((unsigned)((nlines + lb - 1) / lb)), dim3(64),
+                             sh, st, g, s, lb, pitch, dR[a], cR[a],
+                             (const double*)(fac + off_tf[a]), (const double*)(fac + off_m[a]),
+                             (const double*)(fac + off_sup[a]), (const double*)src, dst);
+        } else {
+          hipLaunchKernelGGL(k_reinit_cn, dim3((unsigned)((nlines + 63) / 64)), dim3(64), 0, st,
+                             g, s, dR[a], cR[a], (const double*)(fac + off_tf[a]),
+                             (const double*)(fac + off_m[a]), (const double*)(fac + off_sup[a]),
+                             (const double*)src, dst);
+        }
         TPF_HIP(hipGetLastError());
         TPF_TRY(tpf_sync_periodic_images(ndim, n_nodes, periodic, dst, dst, st));
         std::swap(src, dst);
