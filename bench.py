@@ -36,7 +36,8 @@ sys.path.insert(0, ROOT)
 METRIC = "256^3 PPE time-to-1e-8: learned-FCG vs plain PCG; CCD Mpts/s & HBM %"
 HBM_PEAK_GBS = 8000.0
 # measured HBM bytes of one all-axes ccd_line_kernel<16> launch on 257^3
-# (KiB per dispatch, mean of 3): FETCH_SIZE 199969 x 2 + WRITE_SIZE 862913 x 1
+# (KiB per dispatch, mean of 3; counter scale from a 1 GiB copy: FETCH_SIZE
+# x 2, WRITE_SIZE x 1): FETCH_SIZE 198213 x 2 + WRITE_SIZE 860282 x 1
 TRAFFIC_BYTES_CCD = int((198_213 * 2 + 860_282) * 1024)
 
 
@@ -77,20 +78,28 @@ def per_axis_event_times(ccd, f, d1, d2, reps):
     return out
 
 
+def host_cores() -> int:
+    """Cores this process may run on, capped at 16: the GPU box gives one
+    GPU's job a 16-core share whatever the machine's core count (and sets
+    OMP_NUM_THREADS to 16); more BLAS threads than that share oversubscribe."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        cores = os.cpu_count() or 1
+    return min(cores, 16)
+
+
 def cpu_baseline_ccd(N=256, seed=2):
     """Reference CPU path (dense LU per axis with SciPy lu_solve, the oracle
     restatement of ccd_solver_helpers.py:406-465) on the full 257^3 field, all
-    three axes, with every host core; plus a 1-thread figure on a 17-plane slab."""
+    three axes, with every host core of this job's share (host_cores); plus a
+    1-thread figure on a 17-plane slab."""
     from threadpoolctl import threadpool_limits
     from oracle import ccd as occd
     from twophaseflow_amd.fields import band_limited_periodic
     f = band_limited_periodic(N, 3, kmax=32, seed=seed)
     info = occd.build_periodic_axis(N + 1, 1.0 / N)
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except Exception:
-        cores = os.cpu_count() or 1
-    cores = min(cores, 16)
+    cores = host_cores()
     with threadpool_limits(limits=cores):
         t0 = time.perf_counter()
         for axis in range(3):
@@ -308,32 +317,77 @@ def copy_bandwidth_GBps(reps=10):
     return 2 * src.numel() * 8 / (ms * 1e-3) / 1e9
 
 
-def cpu_baseline_ppe(rho_dev, rhs_dev, gpu_iterations, k_iters=10):
+def _cpu_cg(rho, rhs, tol, maxiter, progress_every=0, label=""):
+    """The oracle's SciPy Jacobi-CG on host arrays: (seconds, iterations,
+    FP64 true relres).  progress_every > 0 prints a line every that many
+    iterations (long solves)."""
+    import numpy as np
+    import scipy.sparse.linalg as spla
+    from oracle import ppe as oppe
+    N = rho.shape[0] - 1
+    o = oppe.OraclePPE((N,) * 3)
+    o.prepare_operator(rho)
+    n = rho.size
+    A = spla.LinearOperator((n, n), matvec=o.system_matvec, dtype=np.float64)
+    M = spla.LinearOperator((n, n), matvec=o.system_precond, dtype=np.float64)
+    it = [0]
+    t0 = time.perf_counter()
+
+    def cb(_xk):
+        it[0] += 1
+        if progress_every and it[0] % progress_every == 0:
+            log(f"  CPU CG {label}: {it[0]} iterations, {time.perf_counter() - t0:.0f} s")
+    x0 = np.zeros(n)
+    sol, info = spla.cg(A, o.system_rhs(rhs), x0=x0, M=M, maxiter=maxiter, atol=0.0, rtol=tol,
+                        callback=cb)
+    sec = time.perf_counter() - t0
+    p = sol.reshape(rho.shape)
+    p.ravel()[o.pin] = 0.0
+    return sec, it[0], float(o.relres(rhs, p)), info == 0
+
+
+def cpu_baseline_ppe(rho_dev, rhs_dev, gpu_iterations, k_iters=10, full=()):
     """The oracle's SciPy Jacobi-CG (the reference's CG form) on the host:
-    K iterations on the 257^3 case (s/iteration, extrapolated to the GPU's
-    iteration count) and a full solve of the same construction at 65^3."""
+    K iterations on the 257^3 case with every core of the job's share and
+    with one thread (s/iteration, extrapolated to the GPU's iteration count),
+    and full solves to 1e-8 of the cases in `full`: (label, rho_dev, rhs_dev,
+    gpu_result) tuples of the same construction at smaller N.  The CG's
+    vector work and the matrix-free operator are NumPy elementwise code, so
+    the BLAS thread count barely changes the time."""
     import numpy as np
     from threadpoolctl import threadpool_limits
     from oracle import ppe as oppe
-    cores = len(os.sched_getaffinity(0))
+    cores = host_cores()
     rho = rho_dev.cpu().numpy()
     rhs = rhs_dev.cpu().numpy()
     N = rho.shape[0] - 1
-    o = oppe.OraclePPE((N,) * 3)
-    with threadpool_limits(cores):
-        t0 = time.perf_counter()
-        try:
-            o.solve_cg(rhs, rho, tol=1e-30, maxiter=k_iters)
-        except RuntimeError:
-            pass  # stops at maxiter by design
-        t_k = time.perf_counter() - t0
-    per_it = t_k / k_iters
-    return {"kind": "port", "cores": cores,
-            "sample": f"oracle SciPy Jacobi-CG, {k_iters} iterations on the {N + 1}^3 case "
-                      f"({t_k:.1f} s incl. operator setup)",
-            "seconds_per_iteration": round(per_it, 4),
-            "extrapolated_seconds_to_1e-8": round(per_it * gpu_iterations, 1),
-            "extrapolation_iterations": gpu_iterations}
+    per_it = {}
+    for th in (cores, 1):
+        o = oppe.OraclePPE((N,) * 3)
+        with threadpool_limits(th):
+            t0 = time.perf_counter()
+            try:
+                o.solve_cg(rhs, rho, tol=1e-30, maxiter=k_iters)
+            except RuntimeError:
+                pass  # stops at maxiter by design
+            per_it[th] = (time.perf_counter() - t0) / k_iters
+    out = {"kind": "port", "cores": cores,
+           "sample": f"oracle SciPy Jacobi-CG, {k_iters} iterations on the {N + 1}^3 case "
+                     f"(incl. operator setup)",
+           "seconds_per_iteration": round(per_it[cores], 4),
+           "one_thread_seconds_per_iteration": round(per_it[1], 4),
+           "extrapolated_seconds_to_1e-8": round(per_it[cores] * gpu_iterations, 1),
+           "extrapolation_iterations": gpu_iterations}
+    for label, r_dev, b_dev, gpu in full:
+        r = r_dev.cpu().numpy()
+        b = b_dev.cpu().numpy()
+        with threadpool_limits(cores):
+            sec, its, rel, ok = _cpu_cg(r, b, 1e-8, 10 ** 6, progress_every=500, label=label)
+        out[f"full_solve_{label}"] = {
+            "seconds": round(sec, 2), "iterations": its, "true_relres": rel, "converged": ok,
+            "gpu_seconds": round(gpu["seconds"], 4), "gpu_iterations": gpu["iterations"],
+            "gpu_vs_cpu": round(sec / gpu["seconds"], 1)}
+    return out
 
 
 def unet_mfma(reps=10):
@@ -756,6 +810,40 @@ def profile_mode(args):
         log(f"profile mg 257^3 native W(2,2)-FCG: {r}")
 
 
+def cpu_protocol():
+    """Full CPU PPE solves of the 65^3 and 129^3 cases (64 bubbles, ratio
+    1000, the 257^3 case's construction) to ||r||/||b|| <= 1e-8 with the
+    oracle's SciPy Jacobi-CG, beside the GPU's FP64 PCG on the same inputs;
+    65^3 also with one thread.  Prints one JSON line."""
+    from threadpoolctl import threadpool_limits
+    cores = host_cores()
+    out = {"cores": cores}
+    for N in (64, 128):
+        g, rho, rhs = ppe_case(N=N)
+        gpu = ppe_solve_timed(g, rho, rhs, "fp64")
+        r, b = rho.cpu().numpy(), rhs.cpu().numpy()
+        log(f"CPU CG {N + 1}^3 on {cores} cores (GPU: {gpu['iterations']} iterations, "
+            f"{gpu['seconds']:.3f} s)")
+        with threadpool_limits(cores):
+            sec, its, rel, ok = _cpu_cg(r, b, 1e-8, 10 ** 6, progress_every=250,
+                                        label=f"{N + 1}^3")
+        entry = {"cpu_seconds": round(sec, 2), "cpu_iterations": its, "cpu_true_relres": rel,
+                 "cpu_converged": ok, "gpu_seconds": round(gpu["seconds"], 4),
+                 "gpu_iterations": gpu["iterations"], "gpu_true_relres": gpu["true_relres"],
+                 "gpu_vs_cpu": round(sec / gpu["seconds"], 1)}
+        if N == 64:
+            log("CPU CG 65^3 on one thread")
+            with threadpool_limits(1):
+                sec1, its1, _, _ = _cpu_cg(r, b, 1e-8, 10 ** 6, progress_every=250,
+                                           label="65^3 one thread")
+            entry["cpu_one_thread_seconds"] = round(sec1, 2)
+            entry["cpu_one_thread_iterations"] = its1
+        out[f"{N + 1}^3"] = entry
+        log(f"  {entry}")
+        del rho, rhs
+    print(json.dumps({"cpu_protocol": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -777,6 +865,10 @@ def main():
                          "the psi curvature and the split reinitialisation on 257^3; "
                          "'legacy' = 2 + K steps of the 2-D 64^2 legacy rising-bubble loop; "
                          "'batch' = 1 + K batched steps of the 48 uneven 2-D bubbles")
+    ap.add_argument("--cpu-protocol", action="store_true",
+                    help="only the CPU PPE protocol: full oracle Jacobi-CG solves to 1e-8 of the "
+                         "65^3 and 129^3 ratio-1000 cases (all cores of the job's share, then "
+                         "one thread at 65^3) beside the GPU's PCG; minutes of CPU time")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
@@ -789,6 +881,8 @@ def main():
     torch.cuda.set_device(0)
     if args.profile:
         return profile_mode(args)
+    if args.cpu_protocol:
+        return cpu_protocol()
 
     # ---- CCD 3-D 257^3 periodic, all axes, FP64
     N = 256
@@ -958,9 +1052,14 @@ def main():
                 res["mg_fcg_native_W22"]["seconds"] / res["learned_native"]["seconds"], 3),
         }
         if not args.no_cpu:
-            log("CPU baseline (oracle SciPy Jacobi-CG, 10 iterations at 257^3)")
-            result["ppe"]["cpu_baseline"] = cpu_baseline_ppe(rho, rhs,
-                                                             res["fp64"]["iterations"])
+            log("CPU baseline (oracle SciPy Jacobi-CG: 10 iterations at 257^3 with all cores "
+                "and one thread; full solve at 65^3)")
+            gs, rhos, rhss = ppe_case(N=64)
+            gpu65 = ppe_solve_timed(gs, rhos, rhss, "fp64")
+            result["ppe"]["plain_pcg_fp64_65"] = gpu65
+            result["ppe"]["cpu_baseline"] = cpu_baseline_ppe(
+                rho, rhs, res["fp64"]["iterations"], full=[("65^3", rhos, rhss, gpu65)])
+            del rhos, rhss
         del rho, rhs
         log("U-Net BF16 MFMA convolution throughput")
         result["unet_conv_mfma"] = unet_mfma()

@@ -16,8 +16,12 @@
 // Orthogonalisation is classical Gram-Schmidt applied twice (CGS2: one fused
 // multi-dot and one multi-axpy kernel per pass, chunks of 16 basis vectors
 // spread over grid.y) instead of SciPy's modified Gram-Schmidt, so iteration
-// histories agree to rounding, not bit for bit.  The small Hessenberg /
-// Givens work runs on the host in FP64.
+// histories agree to rounding, not bit for bit.  The Hessenberg column, the
+// Givens rotations and the inner stop test of every Arnoldi step run in one
+// single-thread kernel on the device (k_arnoldi), so the host launches the
+// steps ahead and reads the stop flag once every kCheckSteps steps; steps
+// issued past the stop leave the Givens state alone.  The host takes back
+// the Hessenberg matrix once per restart cycle for the triangular solve.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,6 +42,18 @@ constexpr int kChunk = 16;          // basis vectors per multi-dot block row
 constexpr int kPcrFusedMaxN = 2048;
 constexpr int kPcrTileNodes = 1024;
 constexpr int kDot = kChunk + 1;    // + <w, w>
+constexpr int kCheckSteps = 8;      // Arnoldi steps launched between host reads of the stop flag
+
+// Device side of one restart cycle's Hessenberg / Givens work.
+struct GivensDev {
+  double ptol;     // inner tolerance of the cycle (set by the host)
+  double presid;   // |last rotated residual| = preconditioned residual estimate
+  double inv_h1;   // scale of the newest basis vector (1 after a breakdown)
+  int stop;        // the inner loop has ended
+  int col_end;     // the column it ended on
+  int breakdown;   // h_{col+1,col} <= eps h0: exact-solution indicator
+  int inner;       // Arnoldi steps of the cycle
+};
 
 __device__ inline void coords3(const PpeGeom& g, int64_t idx, int (&c)[3]) {
   const uint32_t plane = (uint32_t)(g.n1 * g.n2);
@@ -424,6 +440,88 @@ __global__ __launch_bounds__(kBlock) void k_multiaxpy(int64_t n, const double* _
   }
 }
 
+// Plane rotation with c f + s g = r, -s f + c g = 0 (LAPACK 3.10 dlartg
+// conventions: c >= 0, r carries the sign of f).
+__host__ __device__ inline void lartg(double f, double g, double& cs, double& sn, double& r) {
+  if (g == 0.0) {
+    cs = 1.0;
+    sn = 0.0;
+    r = f;
+  } else if (f == 0.0) {
+    cs = 0.0;
+    sn = g > 0.0 ? 1.0 : -1.0;
+    r = fabs(g);
+  } else {
+    const double d = hypot(f, g);
+    cs = fabs(f) / d;
+    r = f > 0.0 ? d : -d;
+    sn = g / r;
+  }
+}
+
+// Arnoldi step `col` after CGS2: column col of the Hessenberg matrix from the
+// three dot-product sets (two Gram-Schmidt passes, then <w,w>), the
+// breakdown test, the earlier rotations, the new rotation, the rotated
+// right-hand side S and the inner stop test (presid <= ptol or breakdown).
+// One thread; returns at once when an earlier step stopped the cycle.
+__global__ void k_arnoldi(int col, int ld, const double* __restrict__ dots, int set,
+                          double* __restrict__ h, double* __restrict__ S,
+                          double* __restrict__ gc, double* __restrict__ gs, GivensDev* st) {
+  if (blockIdx.x != 0 || threadIdx.x != 0 || st->stop) return;
+  const double eps = 2.220446049250313e-16;
+  const double* d0 = dots;
+  const double* d1 = dots + set;
+  const double h0 = sqrt(d0[kChunk]);
+  double* hcol = h + (size_t)col * ld;
+  for (int k = 0; k <= col; ++k) {
+    const int at = (k / kChunk) * kDot + k % kChunk;
+    hcol[k] = d0[at] + d1[at];
+  }
+  const double h1 = sqrt(dots[2 * set + kChunk]);
+  hcol[col + 1] = h1;
+  int brk = 0;
+  if (h1 <= eps * h0) {
+    hcol[col + 1] = 0.0;
+    brk = 1;
+    st->inv_h1 = 1.0;
+  } else {
+    st->inv_h1 = 1.0 / h1;
+  }
+  for (int k = 0; k < col; ++k) {
+    const double n0 = hcol[k], n1 = hcol[k + 1];
+    hcol[k] = gc[k] * n0 + gs[k] * n1;
+    hcol[k + 1] = -gs[k] * n0 + gc[k] * n1;
+  }
+  double cs, sn, mag;
+  lartg(hcol[col], hcol[col + 1], cs, sn, mag);
+  gc[col] = cs;
+  gs[col] = sn;
+  hcol[col] = mag;
+  hcol[col + 1] = 0.0;
+  const double tmp = -sn * S[col];
+  S[col] = cs * S[col];
+  S[col + 1] = tmp;
+  const double presid = fabs(tmp);
+  st->presid = presid;
+  st->inner += 1;
+  if (presid <= st->ptol || brk) {
+    st->stop = 1;
+    st->col_end = col;
+    st->breakdown = brk;
+  }
+}
+
+// v *= inv_h1 of step `col` (skipped for steps issued after the stop).
+__global__ __launch_bounds__(kBlock) void k_scale_basis(int64_t n, int col,
+                                                        const GivensDev* __restrict__ st,
+                                                        double* __restrict__ v) {
+  if (st->stop && st->col_end < col) return;
+  const double a = st->inv_h1;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    v[i] = 0.0 * v[i] + a * v[i];
+}
+
 // ---- host side -----------------------------------------------------------
 struct GmresWork {
   int m = 0;             // restart the basis was sized for
@@ -440,14 +538,21 @@ struct GmresWork {
   double* dots = nullptr;
   double* host_dn = nullptr;  // pinned: chunks * 17 results
   double* host_up = nullptr;  // pinned: m + 1 coefficients
+  double* hdev = nullptr;     // Hessenberg m x (m + 1), S (m + 1), cosines, sines (m each)
+  double* host_h = nullptr;   // pinned: Hessenberg and S of one cycle
+  GivensDev* gdev = nullptr;
+  GivensDev* ghost = nullptr;  // pinned
 };
 
 void free_work(GmresWork* gw) {
-  double* bufs[] = {gw->V, gw->w, gw->t, gw->b, gw->s8, gw->coef, gw->part, gw->dots};
+  double* bufs[] = {gw->V, gw->w, gw->t, gw->b, gw->s8, gw->coef, gw->part, gw->dots, gw->hdev};
   for (double* p : bufs)
     if (p) (void)hipFree(p);
+  if (gw->gdev) (void)hipFree(gw->gdev);
   if (gw->host_dn) (void)hipHostFree(gw->host_dn);
   if (gw->host_up) (void)hipHostFree(gw->host_up);
+  if (gw->host_h) (void)hipHostFree(gw->host_h);
+  if (gw->ghost) (void)hipHostFree(gw->ghost);
   delete gw;
 }
 
@@ -473,6 +578,11 @@ tpf_status ensure_work(void** slot, int64_t n, int m, int grid_blocks, int nscra
   TPF_HIP(hipMalloc(&gw->dots, sizeof(double) * 3 * kDot * (size_t)gw->chunks));
   TPF_HIP(hipHostMalloc(&gw->host_dn, sizeof(double) * 3 * kDot * (size_t)gw->chunks));
   TPF_HIP(hipHostMalloc(&gw->host_up, sizeof(double) * (size_t)(m + 1)));
+  const size_t hsz = (size_t)m * (m + 1) + (size_t)(m + 1) + 2 * (size_t)m;
+  TPF_HIP(hipMalloc(&gw->hdev, sizeof(double) * hsz));
+  TPF_HIP(hipHostMalloc(&gw->host_h, sizeof(double) * ((size_t)m * (m + 1) + (size_t)(m + 1))));
+  TPF_HIP(hipMalloc(&gw->gdev, sizeof(GivensDev)));
+  TPF_HIP(hipHostMalloc(&gw->ghost, sizeof(GivensDev)));
   return TPF_OK;
 }
 
@@ -614,25 +724,6 @@ tpf_status multiaxpy(const Krylov& c, const double* V, int kc, const double* coe
   return TPF_OK;
 }
 
-// Plane rotation with c f + s g = r, -s f + c g = 0 (LAPACK 3.10 dlartg
-// conventions: c >= 0, r carries the sign of f).
-void lartg(double f, double g, double& cs, double& sn, double& r) {
-  if (g == 0.0) {
-    cs = 1.0;
-    sn = 0.0;
-    r = f;
-  } else if (f == 0.0) {
-    cs = 0.0;
-    sn = g > 0.0 ? 1.0 : -1.0;
-    r = std::fabs(g);
-  } else {
-    const double d = std::hypot(f, g);
-    cs = std::fabs(f) / d;
-    r = f > 0.0 ? d : -d;
-    sn = g / r;
-  }
-}
-
 tpf_status check_opts(tpf_ppe* s, const tpf_gmres_opts* o) {
   if (!s || !o) return fail(TPF_ERR_VALUE, "null argument");
   if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set (call tpf_ppe_set_density)");
@@ -695,74 +786,70 @@ tpf_status gmres_core(void** work, hipStream_t st, unsigned grid, int64_t n, int
   TPF_TRY(norm2(c, gw->w, rnorm));
   int64_t inner = 0, cycles = 0;
   const size_t ld = (size_t)m + 1;
-  std::vector<double> h((size_t)m * ld), S(ld), gc(m), gs(m), y(ld), hc(ld);
+  std::vector<double> y(ld);
+  double* hd = gw->hdev;                  // Hessenberg, column-major with leading dim m + 1
+  double* Sd = hd + (size_t)m * ld;       // rotated right-hand side
+  double* gcd = Sd + ld;                  // rotation cosines
+  double* gsd = gcd + m;                  // rotation sines
+  GivensDev* gd = gw->gdev;
   const bool early = rnorm < atol;  // SciPy's first-cycle check
   while (!early && cycles < maxiter) {
     ++cycles;
     TPF_TRY(psolve(gw->w, V));  // v0 = M r / ||M r||
     double t0 = 0.0;
-    TPF_TRY(norm2(c, V, t0));
+    TPF_TRY(norm2(c, V, t0));   // synchronises: host_up and ghost may be rewritten
     TPF_LAUNCH(k_axpby, c.grid, n, 0.0, (const double*)V, 1.0 / t0, V);
-    std::fill(S.begin(), S.end(), 0.0);
-    S[0] = t0;
-    bool breakdown = false;
+    // S = (t0, 0, ...), fresh Givens state with this cycle's ptol
+    std::fill(gw->host_up, gw->host_up + ld, 0.0);
+    gw->host_up[0] = t0;
+    TPF_HIP(hipMemcpyAsync(Sd, gw->host_up, sizeof(double) * ld, hipMemcpyHostToDevice, c.st));
+    *gw->ghost = GivensDev{ptol, 0.0, 1.0, 0, 0, 0, 0};
+    TPF_HIP(hipMemcpyAsync(gd, gw->ghost, sizeof(GivensDev), hipMemcpyHostToDevice, c.st));
+    const size_t set = (size_t)kDot * gw->chunks;
     int col = 0;
+    bool stopped = false;
     for (col = 0; col < m; ++col) {
       double* vc = V + (int64_t)col * n;
       double* vn = V + (int64_t)(col + 1) * n;
       TPF_TRY(apply(vc, gw->t));
       TPF_TRY(psolve(gw->t, vn));  // w, built in place in v_{col+1}
       // CGS2 with the coefficients kept on the device: two dot / subtract
-      // passes and the norm, then one read-back of the three result sets
-      const size_t set = (size_t)kDot * gw->chunks;
+      // passes and the norm, then the Givens step on the device
       for (int pass = 0; pass < 2; ++pass) {
         TPF_TRY(multidot_dev(c, V, col + 1, vn, pass));
         TPF_LAUNCH(k_multiaxpy_dots, c.grid, n, (const double*)V, col + 1,
                    (const double*)(gw->dots + pass * set), -1.0, vn);
       }
       TPF_TRY(multidot_dev(c, vn, 0, vn, 2));
-      TPF_HIP(hipMemcpyAsync(gw->host_dn, gw->dots, sizeof(double) * 3 * set,
-                             hipMemcpyDeviceToHost, c.st));
-      TPF_HIP(hipStreamSynchronize(c.st));
-      const double* d0 = gw->host_dn;
-      const double* d1 = gw->host_dn + set;
-      const double h0 = std::sqrt(d0[kChunk]);
-      for (int k = 0; k <= col; ++k) {
-        const size_t at = (size_t)(k / kChunk) * kDot + k % kChunk;
-        hc[k] = d0[at] + d1[at];
+      hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(1), 0, c.st, col, (int)ld,
+                         (const double*)gw->dots, (int)set, hd, Sd, gcd, gsd, gd);
+      TPF_HIP(hipGetLastError());
+      TPF_LAUNCH(k_scale_basis, c.grid, n, col, (const GivensDev*)gd, vn);
+      if ((col + 1) % kCheckSteps == 0 || col == m - 1) {
+        TPF_HIP(hipMemcpyAsync(gw->ghost, gd, sizeof(GivensDev), hipMemcpyDeviceToHost, c.st));
+        TPF_HIP(hipStreamSynchronize(c.st));
+        if (gw->ghost->stop) {
+          stopped = true;
+          break;
+        }
       }
-      const double h1 = std::sqrt(gw->host_dn[2 * set + kChunk]);
-      double* hcol = &h[(size_t)col * ld];
-      for (int k = 0; k <= col; ++k) hcol[k] = hc[k];
-      hcol[col + 1] = h1;
-      if (h1 <= eps * h0) {  // exact-solution indicator
-        hcol[col + 1] = 0.0;
-        breakdown = true;
-      } else {
-        TPF_LAUNCH(k_axpby, c.grid, n, 0.0, (const double*)vn, 1.0 / h1, vn);
-      }
-      for (int k = 0; k < col; ++k) {
-        const double n0 = hcol[k], n1 = hcol[k + 1];
-        hcol[k] = gc[k] * n0 + gs[k] * n1;
-        hcol[k + 1] = -gs[k] * n0 + gc[k] * n1;
-      }
-      double cs, sn, mag;
-      lartg(hcol[col], hcol[col + 1], cs, sn, mag);
-      gc[col] = cs;
-      gs[col] = sn;
-      hcol[col] = mag;
-      hcol[col + 1] = 0.0;
-      const double tmp = -sn * S[col];
-      S[col] = cs * S[col];
-      S[col + 1] = tmp;
-      presid = std::fabs(tmp);
-      ++inner;
-      if (presid <= ptol || breakdown) break;
     }
-    if (col == m) col = m - 1;
+    const GivensDev gstate = *gw->ghost;
+    col = stopped ? gstate.col_end : m - 1;
+    const bool breakdown = stopped && gstate.breakdown;
+    presid = gstate.presid;
+    inner += gstate.inner;
+    // the cycle's Hessenberg columns and S back to the host
+    const size_t hcount = (size_t)(col + 1) * ld;
+    TPF_HIP(hipMemcpyAsync(gw->host_h, hd, sizeof(double) * hcount, hipMemcpyDeviceToHost, c.st));
+    TPF_HIP(hipMemcpyAsync(gw->host_h + (size_t)m * ld, Sd, sizeof(double) * ld,
+                           hipMemcpyDeviceToHost, c.st));
+    TPF_HIP(hipStreamSynchronize(c.st));
+    const double* h = gw->host_h;
+    const double* S = gw->host_h + (size_t)m * ld;
     // upper-triangular solve h[:col+1, :col+1]^T y = S[:col+1] (SciPy's loop)
-    if (h[(size_t)col * ld + col] == 0.0) S[col] = 0.0;
     for (int k = 0; k <= col; ++k) y[k] = S[k];
+    if (h[(size_t)col * ld + col] == 0.0) y[col] = 0.0;
     for (int k = col; k > 0; --k) {
       if (y[k] != 0.0) {
         y[k] /= h[(size_t)k * ld + k];
