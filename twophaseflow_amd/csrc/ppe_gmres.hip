@@ -40,9 +40,10 @@ constexpr int kChunk = 16;          // basis vectors per multi-dot block row
 // Fused PCR: lines of up to kPcrFusedMaxN nodes, kPcrTileNodes nodes per
 // workgroup tile (8 doubles each in LDS: 32 KB per coefficient set at 1024).
 constexpr int kPcrFusedMaxN = 2048;
-constexpr int kPcrTileNodes = 1024;
+constexpr int kPcrTileNodes = 512;
 constexpr int kDot = kChunk + 1;    // + <w, w>
 constexpr int kCheckSteps = 8;      // Arnoldi steps launched between host reads of the stop flag
+constexpr unsigned kDotBlocks = 512;  // multi-dot workgroups per block row (fewer partials to finish)
 
 // Device side of one restart cycle's Hessenberg / Givens work.
 struct GivensDev {
@@ -152,6 +153,8 @@ struct LineArgs {
   double c_tau;
   double h_min2;  // h_min^2
   double scale;   // 1 / ndim
+  int first = 0;  // k_pcr_fused: out = 0 + term (the zero fill folded in)
+  int last = 0;   // k_pcr_fused: out[pin] = 0 afterwards (folded in)
 };
 
 // Exact Thomas per line (linalg_backend.py:323-339), one thread per line;
@@ -345,7 +348,9 @@ __global__ __launch_bounds__(kBlock) void k_pcr_fused(PpeGeom g, LineArgs la, in
     elem(e, li, i);
     const int s = li * n + i;
     const int64_t o = base_of(l0 + li) + (int64_t)i * st;
-    out[o] += la.scale * (A[3][s] / A[1][s]);
+    const double v = la.scale * (A[3][s] / A[1][s]);
+    if (la.last && o == g.pin) out[o] = 0.0;
+    else out[o] = (la.first ? 0.0 : out[o]) + v;
   }
 }
 
@@ -463,52 +468,71 @@ __host__ __device__ inline void lartg(double f, double g, double& cs, double& sn
 // three dot-product sets (two Gram-Schmidt passes, then <w,w>), the
 // breakdown test, the earlier rotations, the new rotation, the rotated
 // right-hand side S and the inner stop test (presid <= ptol or breakdown).
-// One thread; returns at once when an earlier step stopped the cycle.
-__global__ void k_arnoldi(int col, int ld, const double* __restrict__ dots, int set,
-                          double* __restrict__ h, double* __restrict__ S,
-                          double* __restrict__ gc, double* __restrict__ gs, GivensDev* st) {
-  if (blockIdx.x != 0 || threadIdx.x != 0 || st->stop) return;
-  const double eps = 2.220446049250313e-16;
+// The column and the rotations are staged in LDS by the workgroup; one
+// thread runs the rotation chain.  Returns at once when an earlier step
+// stopped the cycle.
+constexpr int kArnoldiMax = 1024;  // restart cap of the LDS staging
+__global__ __launch_bounds__(256) void k_arnoldi(int col, int ld, const double* __restrict__ dots,
+                                                 int set, double* __restrict__ h,
+                                                 double* __restrict__ S,
+                                                 double* __restrict__ gc,
+                                                 double* __restrict__ gs, GivensDev* st) {
+  __shared__ double hc[kArnoldiMax + 1], cs_[kArnoldiMax], sn_[kArnoldiMax];
+  __shared__ int stop;
+  if (threadIdx.x == 0) stop = st->stop;
+  __syncthreads();
+  if (stop) return;
   const double* d0 = dots;
   const double* d1 = dots + set;
-  const double h0 = sqrt(d0[kChunk]);
-  double* hcol = h + (size_t)col * ld;
-  for (int k = 0; k <= col; ++k) {
+  for (int k = threadIdx.x; k <= col; k += blockDim.x) {
     const int at = (k / kChunk) * kDot + k % kChunk;
-    hcol[k] = d0[at] + d1[at];
+    hc[k] = d0[at] + d1[at];
+    if (k < col) {
+      cs_[k] = gc[k];
+      sn_[k] = gs[k];
+    }
   }
-  const double h1 = sqrt(dots[2 * set + kChunk]);
-  hcol[col + 1] = h1;
-  int brk = 0;
-  if (h1 <= eps * h0) {
-    hcol[col + 1] = 0.0;
-    brk = 1;
-    st->inv_h1 = 1.0;
-  } else {
-    st->inv_h1 = 1.0 / h1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double eps = 2.220446049250313e-16;
+    const double h0 = sqrt(d0[kChunk]);
+    const double h1 = sqrt(dots[2 * set + kChunk]);
+    int brk = 0;
+    hc[col + 1] = h1;
+    if (h1 <= eps * h0) {
+      hc[col + 1] = 0.0;
+      brk = 1;
+      st->inv_h1 = 1.0;
+    } else {
+      st->inv_h1 = 1.0 / h1;
+    }
+    double carry = hc[0];
+    for (int k = 0; k < col; ++k) {
+      const double n0 = carry, n1 = hc[k + 1];
+      hc[k] = cs_[k] * n0 + sn_[k] * n1;
+      carry = -sn_[k] * n0 + cs_[k] * n1;
+    }
+    double cs, sn, mag;
+    lartg(carry, hc[col + 1], cs, sn, mag);
+    gc[col] = cs;
+    gs[col] = sn;
+    hc[col] = mag;
+    hc[col + 1] = 0.0;
+    const double tmp = -sn * S[col];
+    S[col] = cs * S[col];
+    S[col + 1] = tmp;
+    const double presid = fabs(tmp);
+    st->presid = presid;
+    st->inner += 1;
+    if (presid <= st->ptol || brk) {
+      st->stop = 1;
+      st->col_end = col;
+      st->breakdown = brk;
+    }
   }
-  for (int k = 0; k < col; ++k) {
-    const double n0 = hcol[k], n1 = hcol[k + 1];
-    hcol[k] = gc[k] * n0 + gs[k] * n1;
-    hcol[k + 1] = -gs[k] * n0 + gc[k] * n1;
-  }
-  double cs, sn, mag;
-  lartg(hcol[col], hcol[col + 1], cs, sn, mag);
-  gc[col] = cs;
-  gs[col] = sn;
-  hcol[col] = mag;
-  hcol[col + 1] = 0.0;
-  const double tmp = -sn * S[col];
-  S[col] = cs * S[col];
-  S[col + 1] = tmp;
-  const double presid = fabs(tmp);
-  st->presid = presid;
-  st->inner += 1;
-  if (presid <= st->ptol || brk) {
-    st->stop = 1;
-    st->col_end = col;
-    st->breakdown = brk;
-  }
+  __syncthreads();
+  double* hcol = h + (size_t)col * ld;
+  for (int k = threadIdx.x; k <= col + 1; k += blockDim.x) hcol[k] = hc[k];
 }
 
 // v *= inv_h1 of step `col` (skipped for steps issued after the stop).
@@ -629,12 +653,26 @@ tpf_status psolve(Ctx& c, const double* r, double* z) {
     return TPF_OK;
   }
   const int ndim = (g.n0 > 1) + (g.n1 > 1) + (g.n2 > 1);
-  TPF_LAUNCH(k_fill, c.grid, c.n, 0.0, z);
+  // every axis on the fused PCR kernel: the zero fill and the pin reset ride
+  // in its first and last launches
+  bool fused_all = c.stages > 0;
+  int a_first = -1, a_last = -1;
+  for (int a = 0; a < 3; ++a) {
+    if (slot_n(g, a) == 1) continue;
+    fused_all = fused_all && slot_n(g, a) <= kPcrFusedMaxN;
+    if (a_first < 0) a_first = a;
+    a_last = a;
+  }
+  if (!fused_all) TPF_LAUNCH(k_fill, c.grid, c.n, 0.0, z);
   double* s8 = c.gw->s8;
   for (int a = 0; a < 3; ++a) {
     const int na = slot_n(g, a);
     if (na == 1) continue;
-    const LineArgs la{a, c.c_tau, c.h_min2, 1.0 / ndim};
+    LineArgs la{a, c.c_tau, c.h_min2, 1.0 / ndim};
+    if (fused_all) {
+      la.first = a == a_first;
+      la.last = a == a_last;
+    }
     if (c.stages <= 0) {
       const int64_t nlines = c.n / na;
       const int64_t lg = std::min<int64_t>((nlines + kBlock - 1) / kBlock, 1 << 20);
@@ -668,7 +706,7 @@ tpf_status psolve(Ctx& c, const double* r, double* z) {
     }
     TPF_LAUNCH(k_pcr_finish, c.grid, c.n, la.scale, A[1], A[3], z);
   }
-  TPF_LAUNCH(k_set_pin, dim3(1), z, g.pin);
+  if (!fused_all) TPF_LAUNCH(k_set_pin, dim3(1), z, g.pin);
   return TPF_OK;
 }
 
@@ -678,11 +716,12 @@ tpf_status psolve(Ctx& c, const double* r, double* z) {
 tpf_status multidot(const Krylov& c, const double* V, int kcount, const double* w,
                     double* out) {
   const int chunks = std::max(1, (kcount + kChunk - 1) / kChunk);
-  hipLaunchKernelGGL(k_multidot, dim3(c.grid.x, chunks), dim3(kBlock), 0, c.st, c.n, V, kcount,
+  const unsigned gx = std::min(c.grid.x, kDotBlocks);
+  hipLaunchKernelGGL(k_multidot, dim3(gx, chunks), dim3(kBlock), 0, c.st, c.n, V, kcount,
                      w, c.gw->part);
   TPF_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_multidot_fin, dim3(1, chunks), dim3(kBlock), 0, c.st,
-                     (const double*)c.gw->part, (int)c.grid.x, c.gw->dots);
+                     (const double*)c.gw->part, (int)gx, c.gw->dots);
   TPF_HIP(hipGetLastError());
   TPF_HIP(hipMemcpyAsync(c.gw->host_dn, c.gw->dots, sizeof(double) * kDot * chunks,
                          hipMemcpyDeviceToHost, c.st));
@@ -696,11 +735,12 @@ tpf_status multidot(const Krylov& c, const double* V, int kcount, const double* 
 tpf_status multidot_dev(const Krylov& c, const double* V, int kcount, const double* w, int set) {
   const int chunks = std::max(1, (kcount + kChunk - 1) / kChunk);
   double* out = c.gw->dots + (size_t)set * kDot * c.gw->chunks;
-  hipLaunchKernelGGL(k_multidot, dim3(c.grid.x, chunks), dim3(kBlock), 0, c.st, c.n, V, kcount,
+  const unsigned gx = std::min(c.grid.x, kDotBlocks);
+  hipLaunchKernelGGL(k_multidot, dim3(gx, chunks), dim3(kBlock), 0, c.st, c.n, V, kcount,
                      w, c.gw->part);
   TPF_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_multidot_fin, dim3(1, chunks), dim3(kBlock), 0, c.st,
-                     (const double*)c.gw->part, (int)c.grid.x, out);
+                     (const double*)c.gw->part, (int)gx, out);
   TPF_HIP(hipGetLastError());
   return TPF_OK;
 }
@@ -759,6 +799,7 @@ tpf_status gmres_core(void** work, hipStream_t st, unsigned grid, int64_t n, int
                       const GmresOp& apply, const GmresOp& psolve, const double* b, double* x,
                       double tol, int64_t maxiter, tpf_solve_report& r) {
   GmresWork* gw = static_cast<GmresWork*>(*work);
+  if (m < 1 || m > kArnoldiMax) return fail(TPF_ERR_VALUE, "GMRES restart must be in [1, 1024]");
   if (!gw || gw->m < m || gw->n != n) return fail(TPF_ERR_RUNTIME, "GMRES work not sized");
   const Krylov c{gw, st, dim3(grid), n};
   double* V = gw->V;
@@ -821,7 +862,7 @@ tpf_status gmres_core(void** work, hipStream_t st, unsigned grid, int64_t n, int
                    (const double*)(gw->dots + pass * set), -1.0, vn);
       }
       TPF_TRY(multidot_dev(c, vn, 0, vn, 2));
-      hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(1), 0, c.st, col, (int)ld,
+      hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(256), 0, c.st, col, (int)ld,
                          (const double*)gw->dots, (int)set, hd, Sd, gcd, gsd, gd);
       TPF_HIP(hipGetLastError());
       TPF_LAUNCH(k_scale_basis, c.grid, n, col, (const GivensDev*)gd, vn);
