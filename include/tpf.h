@@ -257,13 +257,17 @@ TPF_API tpf_status tpf_ppe_batch_solve(tpf_ppe_batch* batch, const double* const
 TPF_API tpf_status tpf_ppe_batch_destroy(tpf_ppe_batch* batch);
 
 /* ----------------------------------------------------------- FCCD PPE --- */
-/* The FCCD matrix-free PPE, L p = D_f[(1/rho)_f G_f p] with the FCCD face
+/* The FCCD matrix-free PPE, L p = D_f[alpha_f G_f p] with the FCCD face
  * gradient G_f p = (p_hi - p_lo)/H - (H/24)(q_hi - q_lo), q the nodal CCD
- * second derivative: the reference's default configuration (phase_density
- * coefficients, no interface coupling, full_face boundary space, legacy
- * operator pairing) on uniform grids.  Calls are ordered on the stream given
- * to tpf_fccd_ppe_create. */
+ * second derivative, full_face boundary space, legacy operator pairing,
+ * uniform grids; the coefficient schemes and interface couplings below.
+ * Calls are ordered on the stream given to tpf_fccd_ppe_create. */
 typedef struct tpf_fccd_ppe tpf_fccd_ppe;
+
+/* ppe_coefficient_scheme / ppe_interface_coupling_scheme
+ * (ppe/fccd_matrixfree.py:141-186). */
+enum { TPF_FCCD_PHASE_DENSITY = 0, TPF_FCCD_PHASE_SEPARATED = 1 };
+enum { TPF_FCCD_COUPLING_NONE = 0, TPF_FCCD_JUMP_DECOMPOSITION = 1, TPF_FCCD_AFFINE_JUMP = 2 };
 
 /* Replaces PPESolverFCCDMatrixFree.__init__ with its FCCDSolver / CCDSolver
  * (ppe/fccd_matrixfree.py:108-213, ccd/fccd.py:165-195): bc[a] in
@@ -274,20 +278,70 @@ TPF_API tpf_status tpf_fccd_ppe_create(tpf_fccd_ppe** ppe, int ndim, const int64
                                        void* stream);
 TPF_API tpf_status tpf_fccd_ppe_destroy(tpf_fccd_ppe* ppe);
 
-/* Replaces prepare_operator (ppe/fccd_matrixfree_lifecycle.py:44-96): keeps
- * rho (face coefficients 2/(rho_lo + rho_hi) are formed inside each matvec)
- * and builds the Jacobi inverse (fccd_matrixfree_helpers.py:251-264).  rho
- * must stay valid until the next call. */
+/* Replaces prepare_operator (ppe/fccd_matrixfree_lifecycle.py:44-141): keeps
+ * rho and builds the Jacobi inverse (fccd_matrixfree_helpers.py:251-264; no
+ * pinned row under the mean gauge).  phase_density: the face coefficients
+ * 2/(rho_lo + rho_hi) are formed inside each matvec.  phase_separated: the
+ * face coefficients are built here (helpers.py:207-248) -- zero across the
+ * density threshold 0.5 (rho_min + rho_max), with the per-phase
+ * control-volume mean gauge (helpers.py:56-160; a uniform density keeps the
+ * pin and 2/(rho_lo + rho_hi)), or with affine_jump the cut-face resistance
+ * 1/(theta rho_lo + (1 - theta) rho_hi) where psi crosses the phase threshold
+ * (coupling/interface_stress_closure.py:225-277) and the pin; affine_jump
+ * without interface data is TPF_ERR_RUNTIME.  rho (and the interface arrays)
+ * must stay valid until the next call.  Synchronous for phase_separated. */
 TPF_API tpf_status tpf_fccd_ppe_set_density(tpf_fccd_ppe* ppe, const double* rho);
 
+/* Selects the coefficient scheme and interface coupling (TPF_FCCD_*;
+ * validation of fccd_matrixfree.py:165-183: phase_density takes coupling
+ * none).  Rebuilds the prepared state when a density is set. */
+TPF_API tpf_status tpf_fccd_ppe_set_scheme(tpf_fccd_ppe* ppe, int coefficient_scheme,
+                                           int interface_coupling);
+
+/* Replaces set_interface_jump_context / clear_interface_jump_context
+ * (fccd_matrixfree.py:230-304) for the nodal_cut_face route: psi (NULL
+ * clears) with kappa = kappa_lg (Young-Laplace j = p_gas - p_liquid =
+ * -sigma kappa: the affine drive takes kappa at the cut point, the jump
+ * decomposition J = j (1 - psi) when sigma > 0), or, with kappa NULL, a nodal
+ * jump j (face-averaged on cut faces; the jump decomposition stays off, as
+ * the reference's sigma = 0 context).  phase_threshold: the psi level of the
+ * interface (0.5).  Device arrays of the field shape, kept by pointer. */
+TPF_API tpf_status tpf_fccd_ppe_set_interface(tpf_fccd_ppe* ppe, const double* psi,
+                                              const double* kappa, const double* jump,
+                                              double sigma, double phase_threshold);
+
+/* The right-hand side the solve iterates on (fccd_matrixfree.py:506-517):
+ * rhs - L J (jump decomposition), + D_f[alpha_f B_Gamma(j)] (affine_jump,
+ * fccd_matrixfree.py:462-504, interface_stress_closure.py:285-401), then the
+ * per-phase mean projection or b[pin] = 0 (helpers.py:339-463), zero image
+ * rows.  rhs and b distinct. */
+TPF_API tpf_status tpf_fccd_ppe_system_rhs(tpf_fccd_ppe* ppe, const double* rhs, double* b);
+
+/* The face coefficients alpha_f of `axis` in use (PPESolverFCCDMatrixFree.
+ * _coeff_face), stored at each face's lower node (the last node of a line
+ * holds 0); out has the field shape. */
+TPF_API tpf_status tpf_fccd_ppe_face_coeff(tpf_fccd_ppe* ppe, int axis, double* out);
+
+/* out = p + J of the jump decomposition, or p (apply_interface_jump,
+ * fccd_matrixfree.py:644-654). */
+TPF_API tpf_status tpf_fccd_ppe_interface_jump(tpf_fccd_ppe* ppe, const double* p, double* out);
+
+/* Whether the per-phase mean gauge is active, and its density threshold
+ * (NaN without it; PPESolverFCCDMatrixFree._phase_threshold). */
+TPF_API tpf_status tpf_fccd_ppe_gauge_info(tpf_fccd_ppe* ppe, int* mean_gauge, double* threshold);
+
 /* Replaces PPESolverFCCDMatrixFree.apply (fccd_matrixfree.py:378-397):
- * out = L p, out[pin] = p[pin], periodic image rows out = p_img - p_src.
+ * out = L p, out[pin] = p[pin], or under the per-phase mean gauge
+ * out = P L P p + (I - P) p; periodic image rows out = p_img - p_src.
  * Asynchronous; p and out distinct. */
 TPF_API tpf_status tpf_fccd_ppe_apply(tpf_fccd_ppe* ppe, const double* p, double* out);
 
-/* Replaces PPESolverFCCDMatrixFree.solve (fccd_matrixfree.py:505-600): SciPy
- * 1.15 GMRES (gmres_helpers.py:16-42) on L p = b, b = rhs with b[pin] = 0 and
- * zero image rows; opts->precond 0 none, 1 Jacobi, 2 multigrid (no
+/* Replaces PPESolverFCCDMatrixFree.solve (fccd_matrixfree.py:505-611): SciPy
+ * 1.15 GMRES (gmres_helpers.py:16-42) on the operator of tpf_fccd_ppe_apply
+ * with b = tpf_fccd_ppe_system_rhs(rhs) (for phase_density: rhs with
+ * b[pin] = 0 and zero image rows); under the mean gauge the Jacobi step
+ * projects the phase means and the result is mean-projected, and the jump
+ * decomposition returns p + J; opts->precond 0 none, 1 Jacobi, 2 multigrid (no
  * counterpart in the reference; wall grids: one FP32 W(2,2) cycle of the FVM
  * operator's hierarchy on -W r per step, the low-order operator preconditioning
  * the FCCD one); opts->restart as SciPy's

@@ -24,7 +24,10 @@ function against something other than its own formula):
 * ``fields``                       -- closed forms (sphere volumes, ψ bounds).
 * ``fccd.OracleFCCDPPE``          -- the closed-form periodic symbol
   (``fccd.fccd_symbol``), cubic exactness of the wall face gradient and rows,
-  the discrete divergence theorem, and a dense direct solve for its GMRES.
+  the discrete divergence theorem, and a dense direct solve for its GMRES;
+  its phase-separated forms by the reference's own cut-face examples, the
+  closed-form affine drive of a straight interface, gauge fixed points with
+  zero weighted phase means, exact recovery of J, and planted mistakes.
 * ``rhie_chow.OracleRhieChow``    -- closed forms: linear velocity, cubic
   pressure bracket (interior and wall one-sided values), and the
   balanced-force equilibrium p = kappa psi / We.

@@ -147,7 +147,9 @@ class _Grid:
     ({"ppe_preconditioner": "line_pcr"}, ValueError),
     ({"ppe_coefficient_scheme": "bogus"}, ValueError),
     ({"ppe_interface_coupling_scheme": "affine_jump"}, ValueError),
-    ({"ppe_coefficient_scheme": "phase_separated"}, NotImplementedError),
+    ({"ppe_interface_coupling_scheme": "bogus"}, ValueError),
+    ({"ppe_coefficient_scheme": "phase_density",
+      "ppe_interface_coupling_scheme": "jump_decomposition"}, ValueError),
     ({"scalar_operator_pairing": "variational"}, NotImplementedError),
     ({"boundary_face_space": "impermeable_face"}, NotImplementedError),
     ({"scalar_operator_pairing": "require_certified",

@@ -85,6 +85,12 @@ SIGNATURES = {
     "tpf_fccd_ppe_destroy": (C.c_int, [_vp]),
     "tpf_fccd_ppe_set_density": (C.c_int, [_vp, _vp]),
     "tpf_fccd_ppe_apply": (C.c_int, [_vp, _vp, _vp]),
+    "tpf_fccd_ppe_set_scheme": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "tpf_fccd_ppe_set_interface": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double]),
+    "tpf_fccd_ppe_system_rhs": (C.c_int, [_vp, _vp, _vp]),
+    "tpf_fccd_ppe_face_coeff": (C.c_int, [_vp, C.c_int, _vp]),
+    "tpf_fccd_ppe_interface_jump": (C.c_int, [_vp, _vp, _vp]),
+    "tpf_fccd_ppe_gauge_info": (C.c_int, [_vp, _ip, _dp]),
     "tpf_fccd_ppe_solve_gmres": (C.c_int, [_vp, _vp, _vp, C.POINTER(GmresOpts),
                                            C.POINTER(SolveReport)]),
     "tpf_fccd_face_op": (C.c_int, [C.c_int, C.c_int, _i64p, _dp, _ip, C.c_int, _vp, _vp, _vp,

@@ -17,9 +17,11 @@ Mirrors the reference plugin interface (``ppe/interfaces.py:24-88``,
   residual.
 
 ``PPESolverFCCDMatrixFree`` mirrors the FCCD matrix-free PPE
-(``ppe/fccd_matrixfree.py``) in its default configuration: the operator is
-the CCD second derivative per axis plus one fused face-gradient / weighted
-divergence kernel, solved with the same device GMRES.
+(``ppe/fccd_matrixfree.py``) with the phase_density and phase_separated
+coefficient schemes and the none / jump_decomposition / affine_jump
+interface couplings: the operator is the CCD second derivative per axis plus
+one fused face-gradient / weighted divergence kernel, solved with the same
+device GMRES.
 
 All raise ``RuntimeError`` when they do not converge and return
 ``p[pin] = 0``.  The Krylov loops run on the device (libtpf); nothing
@@ -391,12 +393,16 @@ class PPESolverFCCDMatrixFree(IPPESolver):
     FCCD face gradient (ppe/fccd_matrixfree.py:108-600, ccd/fccd.py:243-277),
     solved by SciPy-semantics GMRES on the device.
 
-    Built for the reference's default configuration: ``ppe_coefficient_scheme
-    = "phase_density"``, ``ppe_interface_coupling_scheme = "none"``,
-    ``boundary_face_space = "full_face"``, legacy ``scalar_operator_pairing``,
-    uniform grids, wall or periodic axes.  The other schemes raise
+    Coefficient schemes ``"phase_density"`` (the class default) and
+    ``"phase_separated"`` (the pipeline default, ns_pipeline.py:195-196) with
+    the interface couplings ``"none"``, ``"jump_decomposition"`` and
+    ``"affine_jump"`` (the pipeline default); interface data through
+    ``set_interface_jump_context`` (nodal_cut_face route).  Built for
+    ``boundary_face_space = "full_face"``, the legacy
+    ``scalar_operator_pairing``, uniform grids, wall or periodic axes; the
+    other face spaces, pairings and curvature routes raise
     ``NotImplementedError``; invalid names raise ``ValueError`` as in the
-    reference (fccd_matrixfree.py:155-186).
+    reference (fccd_matrixfree.py:141-186).
 
     ``ppe_preconditioner`` takes the reference's "none" (default) and
     "jacobi", plus this build's "mg" (wall grids): one FP32 W(2,2) cycle of the
@@ -459,9 +465,6 @@ class PPESolverFCCDMatrixFree(IPPESolver):
                              "build's 'mg')")
         if self.preconditioner == "mg" and any(k != "wall" for k in self._kinds):
             raise ValueError("ppe_preconditioner='mg' needs wall axes")
-        if self.coefficient_scheme != "phase_density":
-            raise NotImplementedError(
-                "this build's FCCD PPE covers ppe_coefficient_scheme='phase_density'")
         if pairing == "variational_operator":
             raise NotImplementedError(
                 "this build's FCCD PPE covers the legacy scalar_operator_pairing")
@@ -481,7 +484,11 @@ class PPESolverFCCDMatrixFree(IPPESolver):
         self._lib = nat.lib()
         self._handle = None
         self._rho = None
+        self._interface = None   # device tensors of the interface data (kept alive)
         self._create()
+
+    _SCHEMES = {"phase_density": 0, "phase_separated": 1}
+    _COUPLINGS = {"none": 0, "jump_decomposition": 1, "affine_jump": 2}
 
     def _create(self):
         require_gpu()
@@ -491,6 +498,83 @@ class PPESolverFCCDMatrixFree(IPPESolver):
                                                 nat.darr(self.grid.L), nat.iarr(bc),
                                                 int(self._pin_dof), stream_handle()))
         self._handle = h
+        nat.check(self._lib.tpf_fccd_ppe_set_scheme(
+            h, self._SCHEMES[self.coefficient_scheme],
+            self._COUPLINGS[self.interface_coupling_scheme]))
+        if self._interface is not None:
+            self._push_interface()
+
+    # -- interface data (fccd_matrixfree.py:230-304) --------------------------------
+    def set_interface_jump_context(self, *, psi, kappa=None, sigma: float = 0.0,
+                                   psi_previous=None, pressure_jump_gas_minus_liquid=None,
+                                   phase_threshold: float = 0.5,
+                                   face_curvature_method: str = "nodal_cut_face",
+                                   transport_variational_nodal_covector=None,
+                                   transport_variational_psi=None,
+                                   transport_variational_previous_surface_energy=None) -> None:
+        """Young-Laplace j_gl = p_gas - p_liquid = -sigma kappa_lg (kappa at the
+        psi cut point on cut faces; J = j (1 - psi) for the jump
+        decomposition), or a given nodal jump (face-averaged; the jump
+        decomposition stays off, as the reference's sigma = 0 context).
+        Only the nodal_cut_face curvature route is built."""
+        if face_curvature_method != "nodal_cut_face":
+            raise NotImplementedError(
+                f"face_curvature_method={face_curvature_method!r} is not built; this build "
+                "covers 'nodal_cut_face'")
+        if any(x is not None for x in (transport_variational_nodal_covector,
+                                       transport_variational_psi,
+                                       transport_variational_previous_surface_energy)):
+            raise NotImplementedError("the transport-variational jump routes are not built")
+        io = Handoff(psi)
+        psi_t = io.to_device(psi).clone()
+        if tuple(psi_t.shape) != self.shape:
+            raise ValueError(f"psi shape {tuple(psi_t.shape)} != grid.shape {self.shape}")
+        if pressure_jump_gas_minus_liquid is None:
+            if kappa is None:
+                raise ValueError("kappa is required without pressure_jump_gas_minus_liquid")
+            self._interface = {"psi": psi_t, "kappa": io.to_device(kappa).clone(), "jump": None,
+                               "sigma": float(sigma), "threshold": float(phase_threshold)}
+        else:
+            self._interface = {"psi": psi_t, "kappa": None,
+                               "jump": io.to_device(pressure_jump_gas_minus_liquid).clone(),
+                               "sigma": 0.0, "threshold": float(phase_threshold)}
+        self._push_interface()
+
+    def _push_interface(self) -> None:
+        c = self._interface
+        nat.check(self._lib.tpf_fccd_ppe_set_interface(
+            self._handle, ptr(c["psi"]), ptr(c["kappa"]), ptr(c["jump"]), c["sigma"],
+            c["threshold"]))
+
+    def clear_interface_jump_context(self) -> None:
+        self._interface = None
+        nat.check(self._lib.tpf_fccd_ppe_set_interface(self._handle, None, None, None, 0.0, 0.5))
+
+    @property
+    def _phase_threshold(self):
+        g, t = C.c_int(), C.c_double()
+        nat.check(self._lib.tpf_fccd_ppe_gauge_info(self._handle, C.byref(g), C.byref(t)))
+        return float(t.value) if g.value else None
+
+    def _uses_phase_mean_gauge(self) -> bool:
+        return self.coefficient_scheme == "phase_separated" and self._phase_threshold is not None
+
+    def face_coefficients(self, axis: int) -> torch.Tensor:
+        """alpha_f of `axis` at each face's lower node (PPESolverFCCDMatrixFree.
+        _coeff_face; the last node of a line holds 0)."""
+        if self._rho is None:
+            raise RuntimeError("prepare_operator(rho) must be called first")
+        out = torch.empty(self.shape, dtype=torch.float64, device="cuda")
+        nat.check(self._lib.tpf_fccd_ppe_face_coeff(self._handle, int(axis), ptr(out)))
+        return out
+
+    def apply_interface_jump(self, pressure):
+        """p + J of the jump decomposition, else p (fccd_matrixfree.py:644-654)."""
+        io = Handoff(pressure)
+        t = io.to_device(pressure)
+        out = torch.empty_like(t)
+        nat.check(self._lib.tpf_fccd_ppe_interface_jump(self._handle, ptr(t), ptr(out)))
+        return io.back(out)
 
     def __del__(self):
         h = getattr(self, "_handle", None)
@@ -549,13 +633,13 @@ class PPESolverFCCDMatrixFree(IPPESolver):
                              pcr_stages=0, use_x0=1 if use_x0 else 0, c_tau=0.0)
 
     def system_rhs(self, rhs: torch.Tensor) -> torch.Tensor:
-        """rhs with rhs[pin] = 0 and zero periodic image rows
-        (fccd_matrixfree.py:514-516, 370-376; fccd_matrixfree_helpers.py:337-451)."""
-        b = rhs.clone()
-        b.view(-1)[self._pin_dof] = 0.0
-        for a, kind in enumerate(self._kinds):
-            if kind == "periodic":
-                b.select(a, self.shape[a] - 1).zero_()
+        """The right-hand side GMRES iterates on (fccd_matrixfree.py:506-517):
+        rhs - L J (jump decomposition), + the affine drive, the per-phase
+        mean projection or rhs[pin] = 0, zero periodic image rows."""
+        if self._rho is None:
+            raise RuntimeError("prepare_operator(rho) must be called first")
+        b = torch.empty_like(rhs)
+        nat.check(self._lib.tpf_fccd_ppe_system_rhs(self._handle, ptr(rhs), ptr(b)))
         return b
 
     # -- gauge helpers used by defect correction (fccd_matrixfree.py:300-376,
@@ -603,7 +687,12 @@ class PPESolverFCCDMatrixFree(IPPESolver):
         return arr
 
     def _project_rhs_compatibility(self, rhs, *, record_stats: bool = True):
-        """phase_density: a copy with rhs[pin] = 0 (fccd_matrixfree_helpers.py:337-375)."""
+        """The pinned gauge: a copy with rhs[pin] = 0
+        (fccd_matrixfree_helpers.py:337-375); the per-phase mean gauge is
+        applied on the device by system_rhs and the solve."""
+        if self._rho is not None and self._uses_phase_mean_gauge():
+            raise NotImplementedError("the per-phase mean gauge projects on the device "
+                                      "(system_rhs); this host helper covers the pinned gauge")
         out = rhs.copy() if isinstance(rhs, np.ndarray) else rhs.clone()
         out.reshape(-1)[self._pin_dof] = 0.0
         return out
@@ -622,15 +711,28 @@ class PPESolverFCCDMatrixFree(IPPESolver):
             nat.check(status)
         msg = self._lib.tpf_last_error().decode(errors="replace") if status else ""
         b = self.system_rhs(rhs)
+        base = p
+        if self.interface_coupling_scheme == "jump_decomposition" and \
+                self._interface is not None and self._interface["sigma"] > 0.0:
+            # the solve returned sync(p + J): the residual is the base pressure's
+            # (image nodes of the base equal their sources)
+            jp = torch.zeros_like(p)
+            nat.check(self._lib.tpf_fccd_ppe_interface_jump(self._handle, ptr(jp), ptr(jp)))
+            base = self._sync_periodic_images(p - jp)
         res = torch.empty_like(p)
-        self.apply_into(p, res)
+        self.apply_into(base.contiguous(), res)
         res -= b
         rhs_l2 = float(torch.linalg.vector_norm(b))
         res_l2 = float(torch.linalg.vector_norm(res))
+        mean_gauge = self._uses_phase_mean_gauge()
         self.last_diagnostics = {
-            "ppe_phase_count": 1.0, "ppe_pin_count": 1.0, "ppe_mean_gauge": 0.0,
-            "ppe_rhs_phase_mean_before_max": 0.0, "ppe_rhs_phase_mean_after_max": 0.0,
-            "ppe_interface_coupling_jump": 0.0, "ppe_interface_coupling_affine_jump": 0.0,
+            "ppe_phase_count": 2.0 if mean_gauge else 1.0,
+            "ppe_pin_count": 0.0 if mean_gauge else 1.0,
+            "ppe_mean_gauge": float(mean_gauge),
+            "ppe_interface_coupling_jump": float(
+                self.interface_coupling_scheme == "jump_decomposition"),
+            "ppe_interface_coupling_affine_jump": float(
+                self.interface_coupling_scheme == "affine_jump"),
             "ppe_linear_info": float(0 if status == nat.TPF_OK else self.maxiter),
             "ppe_linear_converged": float(status == nat.TPF_OK),
             "ppe_linear_rhs_l2": rhs_l2,
@@ -647,7 +749,7 @@ class PPESolverFCCDMatrixFree(IPPESolver):
                 "PPESolverFCCDMatrixFree did not converge: "
                 f"info={self.maxiter}, relative_l2={d['ppe_linear_relative_l2']:.3e}, "
                 f"linf={d['ppe_linear_residual_linf']:.3e}, maxiter={self.maxiter} ({msg})")
-        self.last_base_pressure = p.clone()
+        self.last_base_pressure = base.clone()
         return self.last_diagnostics
 
     def solve(self, rhs, rho, dt: float = 0.0, p_init=None):
