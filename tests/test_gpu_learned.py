@@ -47,7 +47,12 @@ def test_conv_forward_matches_oracle(shape, cout, relu):
 
 
 @pytest.mark.parametrize("shape,cout", [((4, 5, 18, 16), 16), ((3, 3, 11, 32), 16), ((3, 4, 9, 16), 48),
-                                        ((3, 3, 17, 128), 64)])
+                                        ((3, 3, 17, 128), 64),
+                                        # K = 32 wgrad (channels 16 / 32 / 64; voxel counts
+                                        # that leave a partial last chunk)
+                                        ((5, 6, 13, 64), 64), ((4, 7, 9, 32), 64),
+                                        ((3, 5, 21, 64), 16), ((6, 5, 7, 16), 32),
+                                        ((9, 9, 9, 32), 32)])
 def test_conv_gradients_match_oracle(shape, cout):
     _announce(f"conv grads {shape}->{cout}")
     rng = np.random.default_rng(2)

@@ -16,7 +16,9 @@
 // B[4(l/16)+j][l%16] and D[4(l/16)+i][l%16].
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "tpf_common.hpp"
@@ -370,6 +372,117 @@ __global__ __launch_bounds__(64 * kWaves) void conv_wgrad(const uint16_t* __rest
   }
 }
 
+// Weight gradient on gfx950's v_mfma_f32_16x16x32_bf16 for Cin, Cout in
+// {16, 32, 64}.  The GEMM is dW[tap] (Cout x Cin) += dY^T (Cout x K) X_tap
+// (K x Cin) with K = voxels: chunks of 32 consecutive voxels of the
+// flattened (z, y, x) order.  A workgroup of 16 wavefronts takes chunks
+// blockIdx.x, blockIdx.x + gridDim.x, ...; per chunk it stages the dY block
+// and, for each of the 27 taps, the tap-shifted X block (zero outside the
+// grid) in LDS transposed to [channel][voxel] (pitch 40 BF16: an MFMA
+// operand -- 8 consecutive voxels of one channel -- is one 16 B LDS read),
+// from 16 B channel-contiguous global loads.  Wavefront w keeps the output
+// tiles t = w, w + 16, ... of (tap, co tile, ci tile) in registers over all
+// of its chunks and writes them once, as the workgroup's partial dW; a
+// second kernel sums the partials in a fixed order (deterministic, no
+// atomics).
+constexpr int kWgWaves = 16;
+constexpr int kWgPitch = 40;
+
+template <int CIT, int COT>
+__global__ __launch_bounds__(64 * kWgWaves) void conv_wgrad_k32(const uint16_t* __restrict__ X,
+                                                               const uint16_t* __restrict__ dY,
+                                                               float* __restrict__ part, int D,
+                                                               int H, int W) {
+  constexpr int Cin = CIT * 16, Cout = COT * 16;
+  constexpr int T = 27 * COT * CIT;
+  constexpr int TPW = (T + kWgWaves - 1) / kWgWaves;
+  constexpr int YI = 32 * (Cout / 8);        // dY staging items (voxel, 8 channels)
+  constexpr int XI = 27 * 32 * (Cin / 8);    // X staging items (tap, voxel, 8 channels)
+  extern __shared__ uint16_t sm[];
+  uint16_t* Ys = sm;                          // [Cout][pitch]
+  uint16_t* Xs = sm + Cout * kWgPitch;        // [27][Cin][pitch]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int row = lane & 15, kq = lane >> 4;
+  const uint32_t nvox = (uint32_t)D * H * W;
+  const int64_t nch = ((int64_t)nvox + 31) / 32;
+  v4f acc[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) acc[i] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int64_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+    const uint32_t v0 = (uint32_t)ch * 32u;
+    __syncthreads();  // the previous chunk's operands are consumed
+    for (int it = tid; it < YI; it += 64 * kWgWaves) {
+      const int k = it / (Cout / 8), c8 = it - k * (Cout / 8);
+      const uint32_t v = v0 + k;
+      uint4 val = make_uint4(0u, 0u, 0u, 0u);
+      if (v < nvox) val = *reinterpret_cast<const uint4*>(dY + (int64_t)v * Cout + c8 * 8);
+      const uint16_t* e = reinterpret_cast<const uint16_t*>(&val);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Ys[(c8 * 8 + j) * kWgPitch + k] = e[j];
+    }
+    for (int it = tid; it < XI; it += 64 * kWgWaves) {
+      const int c8 = it % (Cin / 8);
+      const int q = it / (Cin / 8);
+      const int k = q & 31, tap = q >> 5;
+      const uint32_t v = v0 + k;
+      uint4 val = make_uint4(0u, 0u, 0u, 0u);
+      if (v < nvox) {
+        const uint32_t r = v / (uint32_t)W;
+        const int x = (int)(v - r * (uint32_t)W);
+        const int y = (int)(r % (uint32_t)H), z = (int)(r / (uint32_t)H);
+        const int zz = z + tap / 9 - 1, yy = y + (tap / 3) % 3 - 1, xx = x + tap % 3 - 1;
+        if (zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W)
+          val = *reinterpret_cast<const uint4*>(X + (((int64_t)zz * H + yy) * W + xx) * Cin +
+                                                c8 * 8);
+      }
+      const uint16_t* e = reinterpret_cast<const uint16_t*>(&val);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Xs[(tap * Cin + c8 * 8 + j) * kWgPitch + k] = e[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      const int t = wv + i * kWgWaves;
+      if (t < T) {  // uniform across the wavefront
+        const int tap = t / (COT * CIT), rem = t - tap * (COT * CIT);
+        const int ot = rem / CIT, ct = rem - ot * CIT;
+#pragma unroll
+        for (int k0 = 0; k0 < 32; k0 += 32) {
+          const v8b a = *reinterpret_cast<const v8b*>(Ys + (ot * 16 + row) * kWgPitch + k0 + 8 * kq);
+          const v8b b = *reinterpret_cast<const v8b*>(Xs + (tap * Cin + ct * 16 + row) * kWgPitch +
+                                                      k0 + 8 * kq);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // lane holds D[co = 4 kq + ii][ci = row] of each of its tiles
+  float* out = part + (size_t)blockIdx.x * T * 256;
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+    const int t = wv + i * kWgWaves;
+    if (t >= T) continue;
+    const int tap = t / (COT * CIT), rem = t - tap * (COT * CIT);
+    const int ot = rem / CIT, ct = rem - ot * CIT;
+#pragma unroll
+    for (int ii = 0; ii < 4; ++ii) {
+      const int co = ot * 16 + 4 * kq + ii, ci = ct * 16 + row;
+      out[((size_t)tap * Cout + co) * Cin + ci] = acc[i][ii];
+    }
+  }
+}
+
+// dW[e] += sum over the nb workgroup partials (fixed order)
+__global__ __launch_bounds__(256) void wgrad_reduce(const float* __restrict__ part, int nb,
+                                                    int64_t n, float* __restrict__ dW) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * 256) {
+    float acc = 0.f;
+    for (int b = 0; b < nb; ++b) acc += part[(int64_t)b * n + e];
+    dW[e] += acc;
+  }
+}
+
 __global__ void to_bf16(const float* __restrict__ a, uint16_t* __restrict__ b, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -380,6 +493,33 @@ __global__ void to_bf16(const float* __restrict__ a, uint16_t* __restrict__ b, i
 }  // namespace tpf
 
 using namespace tpf;
+
+namespace {
+// Workgroup partials of the K = 32 weight gradient: one grow-only buffer per
+// device, kept until the process ends (at most 256 x 27 x 64 x 64 floats,
+// 113 MB).  Calls on one device share it, so wgrad calls that overlap on
+// different streams of one device race; the training loop issues them on
+// one stream.
+std::mutex g_wg_mu;
+float* g_wg_buf[64] = {};
+size_t g_wg_cap[64] = {};
+
+tpf_status wgrad_partials(size_t bytes, float** out) {
+  int dev = 0;
+  TPF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(TPF_ERR_RUNTIME, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_wg_mu);
+  if (g_wg_cap[dev] < bytes) {
+    if (g_wg_buf[dev]) TPF_HIP(hipFree(g_wg_buf[dev]));  // synchronises the device
+    g_wg_buf[dev] = nullptr;
+    g_wg_cap[dev] = 0;
+    TPF_HIP(hipMalloc(&g_wg_buf[dev], bytes));
+    g_wg_cap[dev] = bytes;
+  }
+  *out = g_wg_buf[dev];
+  return TPF_OK;
+}
+}  // namespace
 
 extern "C" tpf_status tpf_conv3d_fwd(const void* X, const void* Wt, const float* bias, void* Y,
                                      int D, int H, int W, int Cin, int Cout, int relu,
@@ -447,6 +587,41 @@ extern "C" tpf_status tpf_conv3d_wgrad(const void* X, const void* dY, float* dW,
   if (!X || !dY || !dW) return fail(TPF_ERR_VALUE, "null argument");
   if (D < 1 || H < 1 || W < 1 || Cin < 16 || Cin % 16 || Cout < 16 || Cout % 16)
     return fail(TPF_ERR_VALUE, "conv3d wgrad: Cin, Cout must be multiples of 16");
+  const int64_t nvox = (int64_t)D * H * W;
+  auto k32_width = [](int c) { return c == 16 || c == 32 || c == 64; };
+  const bool small_grid = nvox < ((int64_t)1 << 31) - 64;  // 32-bit voxel indices
+  // K = 32 form from 32 x 32 channels up (16 x 16 and 16 x 32 stay on the
+  // direct kernel: at 33^3 x 16 x 16 it took 0.025 against 0.143 ms, the
+  // workgroup partials and their reduction outweighing the few MFMAs)
+  if (k32_width(Cin) && k32_width(Cout) && Cin * Cout >= 1024 && small_grid) {
+    // gfx950 K = 32 form: workgroup partials in a stream-ordered scratch buffer
+    using FnW = void (*)(const uint16_t*, const uint16_t*, float*, int, int, int);
+    static const FnW kW[3][3] = {
+        {conv_wgrad_k32<1, 1>, conv_wgrad_k32<1, 2>, conv_wgrad_k32<1, 4>},
+        {conv_wgrad_k32<2, 1>, conv_wgrad_k32<2, 2>, conv_wgrad_k32<2, 4>},
+        {conv_wgrad_k32<4, 1>, conv_wgrad_k32<4, 2>, conv_wgrad_k32<4, 4>}};
+    auto idx = [](int c) { return c == 16 ? 0 : (c == 32 ? 1 : 2); };
+    const FnW k = kW[idx(Cin)][idx(Cout)];
+    const size_t sh = sizeof(uint16_t) * (size_t)kWgPitch * (Cout + 27 * Cin);
+    const int64_t nch = (nvox + 31) / 32;
+    // at least 4 chunks per workgroup; one workgroup per CU at 64 input
+    // channels (143 KB of LDS), two below
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(nch / 4, Cin <= 32 ? 512 : 256));
+    const int64_t n = (int64_t)27 * Cout * Cin;
+    hipStream_t st = (hipStream_t)stream;
+    float* part = nullptr;
+    TPF_TRY(wgrad_partials(sizeof(float) * (size_t)nb * n, &part));
+    if (sh > 64 * 1024)
+      TPF_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)sh));
+    hipLaunchKernelGGL(k, dim3(nb), dim3(64 * kWgWaves), sh, st, (const uint16_t*)X,
+                       (const uint16_t*)dY, part, D, H, W);
+    TPF_HIP(hipGetLastError());
+    const int rb = (int)std::min<int64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(wgrad_reduce, dim3(rb), dim3(256), 0, st, (const float*)part, nb, n, dW);
+    TPF_HIP(hipGetLastError());
+    return TPF_OK;
+  }
   const int nrows = D * H;
   const int rows_per = (nrows + 255) / 256;  // <= 256 partial sums per dW entry
   const int nchunk = (nrows + rows_per - 1) / rows_per;
