@@ -250,6 +250,117 @@ def learned_native_timed(g, rho, rhs, net, tol=1e-8, reps=2):
     return out
 
 
+def bubble_indicator_vectors(hier, level, centres, radii, k):
+    """Per-bubble coarse vectors on one multigrid level (one smoothed sphere
+    indicator per bubble, the k largest bubbles; coarse pin zeroed, unit
+    norm): the hand-built space that deflation for bubbly flows uses, a
+    control for the network's space."""
+    import numpy as np
+    import torch
+    shp = hier.shapes[level]
+    h = 1.0 / (max(shp) - 1)
+    axes = [torch.linspace(0.0, 1.0, n, device="cuda", dtype=torch.float64) for n in shp]
+    X = torch.meshgrid(*axes, indexing="ij")
+    order = np.argsort(-np.asarray(radii))[:k]
+    W = torch.empty((len(order),) + shp, dtype=torch.float32, device="cuda")
+    for j, b in enumerate(order):
+        d = torch.sqrt(sum((X[a] - float(centres[b][a])) ** 2 for a in range(3)))
+        W[j] = (0.5 * (1.0 - torch.tanh((d - float(radii[b])) / h))).float()
+    W.view(W.shape[0], -1)[:, hier.pins[level]] = 0.0
+    return W / (W.reshape(W.shape[0], -1).norm(dim=1).view(-1, 1, 1, 1) + 1e-30)
+
+
+def deflated_native_timed(g, rho, rhs, make, tol=1e-8, reps=2):
+    """Device FCG with a deflation space built by make(hier, level) -> the
+    preconditioner; time includes the hierarchy and the space's setup."""
+    import torch
+    from twophaseflow_amd import PPESolverFVMMatrixFree, SimulationConfig, SolverConfig
+    from twophaseflow_amd import learned as L
+    cfg = SimulationConfig(solver=SolverConfig(pseudo_tol=tol, pseudo_maxiter=10 ** 6))
+    cfg.solver.ppe_iteration_method = "cg"
+    s = PPESolverFVMMatrixFree(None, cfg, g)
+    out = None
+    for _ in range(reps):
+        s.invalidate_cache()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.prepare_operator(rho)
+        H = L.MGHierarchy(s)
+        M = make(H, L.level_of(H, 33))
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        p, info = L.fcg_solve_native(s, rhs, M, tol=tol, maxiter=500, gamma=2, w_levels=2,
+                                     fallback=False)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        out = {"seconds": round(t, 5), "setup_seconds": round(t_setup, 5),
+               "space_seconds": {k: round(v, 5) for k, v in M.setup_times.items()},
+               "iterations": info["fcg_iterations"],
+               "per_iteration_ms": round((t - t_setup) / max(1, info["fcg_iterations"]) * 1e3, 3),
+               "true_relres": info["true_relres"], "converged": bool(info["true_relres"] <= tol)}
+    return out
+
+
+def learned_study(train_problems=400, ks=(16, 64), held_out=(9001, 9002, 9003)):
+    """Controlled study of the learned coarse space against the same device
+    W(2,2) multigrid-FCG without it: networks with k = 16 and 64 output
+    vectors trained on `train_problems` random 129^3 bubble problems, the
+    trained spaces evaluated at 129^3 and 257^3 (the 64-bubble ratio-1000
+    case and held-out random problems) with the exact Galerkin E and with E
+    from the level operator, beside per-bubble indicator vectors (the
+    hand-built deflation space for bubbly flows) and untrained networks."""
+    import numpy as np
+    import torch
+    from twophaseflow_amd import learned as L
+    from twophaseflow_amd import fields
+    res = {"train_problems": train_problems, "train": {}, "cases": []}
+    nets = {}
+    for k in ks:
+        log(f"learned study: training k={k} on {train_problems} problems (129^3)")
+        torch.manual_seed(0)
+        net = L.UNet(seed=0, k_out=k).cuda()
+        tr = L.train_deflation(net, N=128, problems=train_problems, lr=1e-3, seed=1000, gamma=2,
+                               log=log)
+        lh = np.asarray(tr["loss_history"])
+        marks = [n for n in (20, 50, 100, 200, 400, 800, 1600, 2000) if n <= len(lh)]
+        res["train"][f"k{k}"] = {"seconds": round(tr["seconds"], 1),
+                                 "mean_loss_of_20_before": {str(n): round(float(lh[n - 20:n].mean()), 4)
+                                                            for n in marks}}
+        log(f"  {res['train'][f'k{k}']}")
+        nets[k] = net
+    cases = []
+    centres, radii = fields.random_bubbles(64, seed=3, ndim=3)
+    cases.append(("257^3, 64 bubbles, ratio 1000", lambda: ppe_case(N=256) + ((centres, radii, 1000.0),)))
+    for N in (128, 256):
+        for sd in held_out:
+            cases.append((f"{N + 1}^3 held-out seed {sd}",
+                          (lambda N=N, sd=sd: L.make_problem(N, sd, return_bubbles=True))))
+    for name, mk in cases:
+        g, rho, rhs, (cen, rad, ratio) = mk()
+        row = {"case": name, "bubbles": int(len(rad)), "density_ratio": round(float(ratio), 1)}
+        row["no_space"] = mg_native_timed(g, rho, rhs, "W")
+        for k in ks:
+            net = nets[k]
+            row[f"trained_k{k}"] = deflated_native_timed(
+                g, rho, rhs, lambda H, lv, net=net: L.LearnedDeflationPreconditioner(
+                    H, net, lv, nu=2, gamma=2))
+            row[f"trained_k{k}_level_E"] = deflated_native_timed(
+                g, rho, rhs, lambda H, lv, net=net: L.LearnedDeflationPreconditioner(
+                    H, net, lv, nu=2, gamma=2, galerkin="level"))
+        unt = L.UNet(seed=5, k_out=max(ks)).cuda()
+        row[f"untrained_k{max(ks)}"] = deflated_native_timed(
+            g, rho, rhs, lambda H, lv: L.LearnedDeflationPreconditioner(H, unt, lv, nu=2, gamma=2))
+        kb = min(64, len(rad))
+        row[f"bubble_indicators_k{kb}"] = deflated_native_timed(
+            g, rho, rhs, lambda H, lv: L.LearnedDeflationPreconditioner(
+                H, None, lv, nu=2, gamma=2,
+                vectors=bubble_indicator_vectors(H, lv, cen, rad, kb)))
+        log(f"  {json.dumps(row)}")
+        res["cases"].append(row)
+        del g, rho, rhs
+    return res
+
+
 def ccd_batch_bench(reps=30):
     """48 independent 2-D wall problems, 16 each at 129^2, 257^2 and 513^2
     nodes (the uneven-batch sizes), all axes: one batched call (one grid per
@@ -391,27 +502,42 @@ def cpu_baseline_ppe(rho_dev, rhs_dev, gpu_iterations, k_iters=10, full=()):
 
 
 def unet_mfma(reps=10):
-    """BF16 MFMA 3x3x3 convolution throughput (forward kernel) against the
-    dense BF16 peak, at the U-Net's coarse-level shape and at a larger one."""
+    """BF16 MFMA 3x3x3 convolution throughput against the dense BF16 peak, at
+    the U-Net's coarse-level shape and at larger ones: the forward kernel, the
+    input gradient (the forward kernel on dY with flipped, transposed weights)
+    and the weight gradient, and the MFMA fraction of one forward + backward
+    pass (3 x 2 x 27 Cin Cout n^3 flops over the three kernels' time)."""
     import torch
     from twophaseflow_amd import learned as L
-    out = []
-    for (n, cin, cout) in ((33, 16, 16), (65, 64, 64), (129, 32, 32)):
-        x = torch.randn((n, n, n, cin), device="cuda").to(L.BF16).contiguous()
-        w = (torch.randn((27, cout, cin), device="cuda") * 0.05).to(L.BF16).contiguous()
-        b = torch.zeros(cout, device="cuda")
-        L.conv_fwd(x, w, b, True, False)
+
+    def ev(fn):
+        fn()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
-            L.conv_fwd(x, w, b, True, False)
+            fn()
         e1.record()
         torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
+        return e0.elapsed_time(e1) / reps
+
+    out = []
+    for (n, cin, cout) in ((33, 16, 16), (65, 64, 64), (129, 32, 32)):
+        x = torch.randn((n, n, n, cin), device="cuda").to(L.BF16).contiguous()
+        gy = torch.randn((n, n, n, cout), device="cuda").to(L.BF16).contiguous()
+        w = (torch.randn((27, cout, cin), device="cuda") * 0.05).to(L.BF16).contiguous()
+        wf = L.flip_weights(w)
+        b = torch.zeros(cout, device="cuda")
+        gw = torch.zeros((27, cout, cin), device="cuda")
+        t_f = ev(lambda: L.conv_fwd(x, w, b, True, False))
+        t_d = ev(lambda: L.conv_fwd(gy, wf, None, False, False))
+        t_w = ev(lambda: L.conv_wgrad(x, gy, gw))
         flops = 2.0 * 27 * cin * cout * n ** 3
-        tf = flops / (ms * 1e-3) / 1e12
-        out.append({"grid": n, "cin": cin, "cout": cout, "ms": round(ms, 4),
-                    "TFLOPs": round(tf, 1), "frac_of_bf16_dense_peak": round(tf / 2500.0, 4)})
+        tf = flops / (t_f * 1e-3) / 1e12
+        out.append({"grid": n, "cin": cin, "cout": cout, "ms": round(t_f, 4),
+                    "TFLOPs": round(tf, 1), "frac_of_bf16_dense_peak": round(tf / 2500.0, 4),
+                    "dgrad_ms": round(t_d, 4), "wgrad_ms": round(t_w, 4),
+                    "wgrad_TFLOPs": round(flops / (t_w * 1e-3) / 1e12, 1),
+                    "fwd_bwd_frac": round(3 * flops / ((t_f + t_d + t_w) * 1e-3) / 2.5e15, 4)})
     return out
 
 
@@ -885,6 +1011,11 @@ def main():
                     help="only the CPU PPE protocol: full oracle Jacobi-CG solves to 1e-8 of the "
                          "65^3 and 129^3 ratio-1000 cases (all cores of the job's share, then "
                          "one thread at 65^3) beside the GPU's PCG; minutes of CPU time")
+    ap.add_argument("--learned-study", action="store_true",
+                    help="only the learned coarse-space study: k = 16 and 64 networks trained on "
+                         "--train-problems problems, evaluated at 129^3 and 257^3 against the "
+                         "same device multigrid-FCG without them, per-bubble indicator vectors "
+                         "and untrained networks (one JSON line; about ten minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
@@ -899,6 +1030,9 @@ def main():
         return profile_mode(args)
     if args.cpu_protocol:
         return cpu_protocol()
+    if args.learned_study:
+        print(json.dumps({"learned_study": learned_study(args.train_problems)}), flush=True)
+        return None
 
     # ---- CCD 3-D 257^3 periodic, all axes, FP64
     N = 256

@@ -222,7 +222,7 @@ TPF_API tpf_status tpf_ppe_solve_mg(tpf_ppe* ppe, const double* rhs, double* p,
  * counterpart in the reference; the learned preconditioner's device path):
  * z = V(r); z += P W E^-1 W^T R (r - A z), R / P the restriction and
  * prolongation chains between level 0 and `level`, W [k][nodes(level)] FP32
- * device vectors (1 <= k <= 32), einv the k x k FP64 host matrix E^-1.
+ * device vectors (1 <= k <= 64), einv the k x k FP64 host matrix E^-1.
  * Synchronous; TPF_ERR_RUNTIME when maxiter is reached above tol. */
 TPF_API tpf_status tpf_ppe_solve_mg_deflated(tpf_ppe* ppe, const double* rhs, double* p,
                                              const tpf_mg_opts* opts, const float* W, int k,

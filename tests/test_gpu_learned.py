@@ -52,7 +52,7 @@ def test_conv_forward_matches_oracle(shape, cout, relu):
                                         # that leave a partial last chunk)
                                         ((5, 6, 13, 64), 64), ((4, 7, 9, 32), 64),
                                         ((3, 5, 21, 64), 16), ((6, 5, 7, 16), 32),
-                                        ((9, 9, 9, 32), 32)])
+                                        ((9, 9, 9, 32), 32), ((4, 5, 6, 16), 64)])
 def test_conv_gradients_match_oracle(shape, cout):
     _announce(f"conv grads {shape}->{cout}")
     rng = np.random.default_rng(2)

@@ -37,7 +37,7 @@ namespace {
 
 constexpr int kMaxLevels = 12;
 constexpr int kCoarseMax = 1024;  // nodes of the level solved in one block
-constexpr int kMaxDefl = 32;      // learned coarse vectors
+constexpr int kMaxDefl = 64;      // learned coarse vectors
 constexpr int kDeflBlocks = 1024; // dot-product blocks per vector
 
 struct LevelGeom {
@@ -695,7 +695,7 @@ __global__ __launch_bounds__(kBlock) void k_defl_dot(int64_t n, const float* __r
 __global__ __launch_bounds__(kBlock) void k_defl_solve(int k, int nb, const double* __restrict__ part,
                                                        const double* __restrict__ Einv,
                                                        float* __restrict__ y) {
-  __shared__ double c[64];
+  __shared__ double c[kMaxDefl];
   const int j = threadIdx.x;
   if (j < k) {
     double a = 0.0;
@@ -1010,7 +1010,7 @@ tpf_status mg_cycle(tpf_ppe* s, hipStream_t st, int nu, int gamma, int coarse_sw
 tpf_status mg_deflate_setup(tpf_ppe* s, int k, int level, const double* einv_host) {
   MgState* m = mg_of(s);
   if (!m || m->nlev == 0) return fail(TPF_ERR_RUNTIME, "multigrid hierarchy not built");
-  if (k < 1 || k > kMaxDefl) return fail(TPF_ERR_VALUE, "deflation: 1 <= k <= 32 vectors");
+  if (k < 1 || k > kMaxDefl) return fail(TPF_ERR_VALUE, "deflation: 1 <= k <= 64 vectors");
   if (level < 1 || level >= m->nlev) return fail(TPF_ERR_VALUE, "deflation level out of range");
   if (!m->defl_part) {
     TPF_HIP(hipMalloc(&m->defl_part, sizeof(double) * kMaxDefl * kDeflBlocks));

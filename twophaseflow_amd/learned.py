@@ -308,27 +308,52 @@ class LearnedDeflationPreconditioner:
     """Multigrid V-cycle with a learned coarse space (A-DEF2 two-level form).
 
     Once per solve, the U-Net maps the conductance field of one coarse level
-    (no residual) to k = 16 coarse vectors W; Z = P W spans the deflation
-    space, E = Z^T A Z = W^T (R A P) W is factored in FP64.  Each application
-    is linear in r:
+    (no residual) to k coarse vectors W (k = the network's output channels);
+    Z = P W spans the deflation space, E = Z^T A Z = W^T (R A P) W is
+    factored in FP64.  Each application is linear in r:
         z1 = V(r);  t = r - A z1;  z = z1 + Z E^-1 Z^T t
     (Z^T t = W^T R t and Z y = P (W y) are evaluated on the coarse level).
-    With net=None it is the V-cycle alone."""
+    With net=None and no vectors it is the V-cycle alone.
+
+    ``vectors`` [k, *coarse shape] replaces the network's output (any fixed
+    coarse space, e.g. per-bubble indicators).  ``galerkin="level"`` forms
+    E from the hierarchy's own operator on that level (W^T A_l W, no fine
+    work) instead of the exact R A P, which costs k fine-level applications.
+    ``setup_times`` holds the network and E times (device-synchronised)."""
 
     def __init__(self, hier: MGHierarchy, net, level: int, nu=2, coarse_sweeps=20, ridge=1e-6,
-                 gamma=1):
+                 gamma=1, vectors=None, galerkin="exact"):
+        if galerkin not in ("exact", "level"):
+            raise ValueError(f"galerkin must be 'exact' or 'level', got {galerkin!r}")
         self.hier, self.net, self.level = hier, net, level
         self.nu, self.cs, self.gamma = nu, coarse_sweeps, gamma
-        if net is not None:
+        self.setup_times = {}
+        if vectors is not None:
+            self.net = vectors
+        if self.net is not None:
+            t0 = time.perf_counter()
             with torch.no_grad():
-                W = deflation_vectors(hier, net, level)
-                E = (W.reshape(W.shape[0], -1) @ _galerkin(hier, level, W)
-                     .reshape(W.shape[0], -1).T).double()
+                if vectors is not None:
+                    W = vectors.float().clone().contiguous()
+                    W.view(W.shape[0], -1)[:, hier.pins[level]] = 0.0
+                    W = W / (W.reshape(W.shape[0], -1).norm(dim=1).view(-1, 1, 1, 1) + 1e-30)
+                else:
+                    W = deflation_vectors(hier, net, level)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                if galerkin == "exact":
+                    AW = _galerkin(hier, level, W)
+                else:
+                    AW = torch.stack([hier.apply(level, W[j]) for j in range(W.shape[0])])
+                    AW.view(W.shape[0], -1)[:, hier.pins[level]] = 0.0
+                E = (W.reshape(W.shape[0], -1) @ AW.reshape(W.shape[0], -1).T).double()
             E = 0.5 * (E + E.T)
             E = E + ridge * torch.trace(E) / E.shape[0] * torch.eye(E.shape[0], device=E.device,
                                                                  dtype=E.dtype)
             self.W = W.reshape(W.shape[0], -1)
             self.L = torch.linalg.cholesky(E)
+            torch.cuda.synchronize()
+            self.setup_times = {"network": t1 - t0, "coarse_matrix": time.perf_counter() - t1}
 
     def __call__(self, r):
         shape = r.shape
@@ -511,9 +536,10 @@ def fcg_solve_native(ppe_solver, rhs, precond, tol=1e-8, maxiter=500, nu=2, gamm
 
 
 # ------------------------------------------------------------- training ---
-def make_problem(N, seed, nb_range=(8, 128), ratio_range=(10.0, 1000.0)):
+def make_problem(N, seed, nb_range=(8, 128), ratio_range=(10.0, 1000.0), return_bubbles=False):
     """Random bubble density and a divergence right-hand side on an N^3 wall
-    grid (device tensors)."""
+    grid (device tensors); with return_bubbles also the spheres' centres and
+    radii and the density ratio."""
     from . import fields
     from .ccd import CCDSolver
     from .grid import Grid
@@ -531,6 +557,8 @@ def make_problem(N, seed, nb_range=(8, 128), ratio_range=(10.0, 1000.0)):
     vel = [torch.from_numpy(fields.band_limited_periodic(N, 3, kmax=8, seed=seed * 3 + a)).cuda()
            for a in range(3)]
     rhs = fields.divergence(ccd, vel)
+    if return_bubbles:
+        return g, rho, rhs, (centres, radii, ratio)
     return g, rho, rhs
 
 
