@@ -606,7 +606,12 @@ TPF_API tpf_status tpf_fields_correct_velocity(int ndim, const int64_t* n_nodes,
 TPF_API tpf_status tpf_conv3d_fwd(const void* X, const void* Wt, const float* bias, void* Y,
                                   int D, int H, int W, int Cin, int Cout, int relu,
                                   int out_f32, void* stream);
-/* dW[tap][co][ci] += sum_v dY[v][co] X[v+off(tap)][ci] (FP32, accumulates). */
+/* dW[tap][co][ci] += sum_v dY[v][co] X[v+off(tap)][ci] (FP32, accumulates).
+ * With Cin, Cout in {16, 32, 64} and Cin * Cout >= 1024 it runs on the
+ * 16x16x32 BF16 MFMA with workgroup partials reduced in a fixed order
+ * (deterministic); the partials live in one grow-only buffer per device
+ * (<= 113 MB, kept until the process ends), so calls that overlap on
+ * different streams of one device race. */
 TPF_API tpf_status tpf_conv3d_wgrad(const void* X, const void* dY, float* dW, int D, int H,
                                     int W, int Cin, int Cout, void* stream);
 /* b[i] = bf16(a[i]), round to nearest even. */

@@ -473,13 +473,26 @@ __global__ __launch_bounds__(64 * kWgWaves) void conv_wgrad_k32(const uint16_t* 
 }
 
 // dW[e] += sum over the nb workgroup partials (fixed order)
-__global__ __launch_bounds__(256) void wgrad_reduce(const float* __restrict__ part, int nb,
-                                                    int64_t n, float* __restrict__ dW) {
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * 256) {
-    float acc = 0.f;
-    for (int b = 0; b < nb; ++b) acc += part[(int64_t)b * n + e];
-    dW[e] += acc;
+// dW[e] += sum_b part[b][e] in a fixed order: a workgroup owns 64 entries,
+// wavefront w sums partials b = w, w + 16, ... and the 16 wavefront sums are
+// added in wavefront order (deterministic; 16 loads in flight per entry).
+__global__ __launch_bounds__(1024) void wgrad_reduce(const float* __restrict__ part, int nb,
+                                                     int64_t n, float* __restrict__ dW) {
+  __shared__ float sh[16][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t e = (int64_t)blockIdx.x * 64 + lane;
+  float acc = 0.f;
+  if (e < n) {
+#pragma unroll 4
+    for (int b = w; b < nb; b += 16) acc += part[(int64_t)b * n + e];
+  }
+  sh[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && e < n) {
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s += sh[i][lane];
+    dW[e] += s;
   }
 }
 
@@ -617,8 +630,8 @@ extern "C" tpf_status tpf_conv3d_wgrad(const void* X, const void* dY, float* dW,
     hipLaunchKernelGGL(k, dim3(nb), dim3(64 * kWgWaves), sh, st, (const uint16_t*)X,
                        (const uint16_t*)dY, part, D, H, W);
     TPF_HIP(hipGetLastError());
-    const int rb = (int)std::min<int64_t>((n + 255) / 256, 1024);
-    hipLaunchKernelGGL(wgrad_reduce, dim3(rb), dim3(256), 0, st, (const float*)part, nb, n, dW);
+    hipLaunchKernelGGL(wgrad_reduce, dim3((unsigned)((n + 63) / 64)), dim3(1024), 0, st,
+                       (const float*)part, nb, n, dW);
     TPF_HIP(hipGetLastError());
     return TPF_OK;
   }
