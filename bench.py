@@ -2,13 +2,16 @@
 
 Workload (one step): the CCD operator, f' and f'' along every axis of a 3-D
 257^3-node periodic band-limited field (|k| <= 32, amplitude |k|^-2, seed 2)
-in FP64, one launch per axis with all lines of that axis.  `value` is
+in FP64: one launch for the two strided axes and one for the contiguous
+axis, each with all lines of its axes.  `value` is
 Mpts/s = grid points x axes / second over the timed steps, inputs resident in
 HBM.  Also reported on the same line:
 
-* roofline of the dominant CCD kernel (strided-axis line solve): algorithmic
-  bytes per launch (read f, write f' and f'': 24 B per point) over its mean
-  launch time from HIP events on the launch stream;
+* roofline of the dominant CCD kernel (the explicit partition-inverse grid
+  of the two strided axes, the larger of the step's two launches):
+  algorithmic bytes per launch (read f once, write f' and f'' of both axes:
+  40 B per point) over its mean launch time from HIP events on the launch
+  stream;
 * 2-D 1025^2 periodic all-axes CCD throughput;
 * the 257^3 density-ratio-1000 PPE (64 gas spheres, r in [0.03, 0.08],
   eps = 1.5h, rhs = CCD divergence of a seeded smooth field): wall time to
@@ -35,10 +38,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "256^3 PPE time-to-1e-8: learned-FCG vs plain PCG; CCD Mpts/s & HBM %"
 HBM_PEAK_GBS = 8000.0
-# measured HBM bytes of one all-axes ccd_line_kernel<16> launch on 257^3
-# (KiB per dispatch, mean of 3; counter scale from a 1 GiB copy: FETCH_SIZE
-# x 2, WRITE_SIZE x 1): FETCH_SIZE 198213 x 2 + WRITE_SIZE 860282 x 1
-TRAFFIC_BYTES_CCD = int((198_213 * 2 + 860_282) * 1024)
+# measured HBM bytes of one ccd_explicit_kernel<16, 8, true> launch over the
+# strided axes 0 and 1 of 257^3 (KiB per dispatch, mean of 3; counter scale
+# from a 1 GiB copy: FETCH_SIZE x 2, WRITE_SIZE x 1): FETCH_SIZE 161995 x 2 +
+# WRITE_SIZE 692443 x 1
+TRAFFIC_BYTES_CCD = int((161_995 * 2 + 692_443) * 1024)
 
 
 def log(msg):
@@ -55,6 +59,22 @@ def time_ccd(ccd, f, d1, d2, mask, steps, warmup, ndim):
         ccd.apply_into(f, d1, d2, mask)
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / steps
+
+
+def event_ms(fn, reps):
+    """Mean duration of fn's launches from HIP events on torch's current
+    stream (after two untimed calls)."""
+    import torch
+    fn()
+    fn()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for a, b in ev:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in ev) / reps
 
 
 def per_axis_event_times(ccd, f, d1, d2, reps):
@@ -906,17 +926,18 @@ def profile_mode(args):
         ccd = CCDSolver(g, bc_type="periodic")
         d1 = torch.empty((3,) + g.shape, dtype=torch.float64, device="cuda")
         d2 = torch.empty_like(d1)
-        for _ in range(args.steps):
+        for _ in range(args.steps):  # explicit grid (axes 0, 1) + wavefront kernel (axis 2)
             ccd.apply_into(f, d1, d2, 0b111)
-        for _ in range(args.steps):  # single-axis calls: explicit kernel on axes 0, 1
+        for _ in range(args.steps):  # single-axis calls
             for axis in range(3):
                 ccd.differentiate_into(f, d1[axis], d2[axis], axis)
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        log(f"profile ccd: {args.steps} fused 3-axis launches and {args.steps} x 3 single-axis "
-            f"launches on 257^3, then a 1 GiB copy")
+        log(f"profile ccd: {args.steps} all-axes steps (explicit grid of axes 0-1, wavefront "
+            f"kernel on axis 2) and {args.steps} x 3 single-axis launches on 257^3, then a "
+            f"1 GiB copy")
     elif args.profile == "fccd":
         from twophaseflow_amd import PPESolverFCCDMatrixFree, SimulationConfig, SolverConfig
         gp, rho, rhs = ppe_case(N=256)
@@ -1047,31 +1068,30 @@ def main():
     npts = f.numel()
     mpts = 3 * npts / sec / 1e6
     axes = per_axis_event_times(ccd, f, d1, d2, max(args.steps, 5))
-    # dominant kernel: the fused all-axes launch (one ccd_line_kernel<16> grid per
-    # step); algorithmic bytes per launch = 8 (1 + 2d) B x points = 56 B x 257^3
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(max(args.steps, 5))]
-    for a, b in evs:
-        a.record()
-        ccd.apply_into(f, d1, d2, 0b111)
-        b.record()
-    torch.cuda.synchronize()
-    dom_ms = sum(a.elapsed_time(b) for a, b in evs) / len(evs)
-    bytes_per_launch = 8 * (1 + 2 * 3) * npts
+    # One all-axes step is two launches: the explicit partition-inverse grid of
+    # the strided axes 0 and 1 (dominant), then the wavefront kernel on the
+    # contiguous axis 2.  Algorithmic bytes: 8 (1 + 2 x 2) B = 40 B per point
+    # for the strided grid (f read once, f' and f'' of two axes written),
+    # 24 B per point for axis 2.  Mean launch times from HIP events on torch's
+    # stream, the stream both kernels run on.
+    reps = max(args.steps, 5)
+    dom_ms = event_ms(lambda: ccd.apply_into(f, d1[:2], d2[:2], 0b011), reps)
+    ax2_ms = event_ms(lambda: ccd.differentiate_into(f, d1[2], d2[2], 2), reps)
+    step_ms = event_ms(lambda: ccd.apply_into(f, d1, d2, 0b111), reps)
+    bytes_per_launch = 8 * (1 + 2 * 2) * npts
     achieved = bytes_per_launch / (dom_ms * 1e-3) / 1e9
     all_axes_gbs = 8.0 * 7 * npts / sec / 1e9
-    # axes 0 and 1 alone: one grid of the explicit partition-inverse kernel
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(max(args.steps, 5))]
-    for a, b in evs:
-        a.record()
-        ccd.apply_into(f, d1[:2], d2[:2], 0b011)
-        b.record()
-    torch.cuda.synchronize()
-    s_ms = sum(a.elapsed_time(b) for a, b in evs) / len(evs)
-    strided = {"axes": [0, 1], "kernel": "ccd_explicit_kernel<16> (one grid)",
-               "ms": round(s_ms, 4),
-               "algorithmic_GBps": round(8 * (1 + 2 * 2) * npts / (s_ms * 1e-3) / 1e9, 1)}
+    strided = {"axes": [0, 1], "kernel": "ccd_explicit_kernel<16, 8, true> (one grid)",
+               "ms": round(dom_ms, 4),
+               "algorithmic_GBps": round(8 * (1 + 2 * 2) * npts / (dom_ms * 1e-3) / 1e9, 1)}
+    step_kernels = [
+        {"kernel": "ccd_explicit_kernel<16, 8, true>", "axes": [0, 1], "ms": round(dom_ms, 4),
+         "bytes": bytes_per_launch},
+        {"kernel": "ccd_wave_kernel<16, 4>", "axes": [2], "ms": round(ax2_ms, 4),
+         "bytes": 24 * npts,
+         "algorithmic_GBps": round(24 * npts / (ax2_ms * 1e-3) / 1e9, 1)},
+        {"all_axes_step_ms_hip_events": round(step_ms, 4),
+         "algorithmic_GBps": round(56 * npts / (step_ms * 1e-3) / 1e9, 1)}]
     log("timing 3-D CCD, FP32 solves + FP64 refinement")
     ccd_r = CCDSolver(g, bc_type="periodic", precision="fp32_refined")
     r1 = torch.empty_like(d1)
@@ -1125,12 +1145,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
-                     # recorded in profiles/ccd_line_kernel_257cubed_periodic_fp64.md
+                     # recorded in profiles/ccd_all_axes_257cubed_periodic_fp64.md
                      # (scale factors 2 and 1 from a 1 GiB copy in the same runs)
                      "traffic": TRAFFIC_BYTES_CCD,
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/)",
-                     "kernel": "ccd_line_kernel<16> (one grid, all 3 axes)",
+                     "kernel": "ccd_explicit_kernel<16, 8, true> (one grid, strided axes 0 and 1)",
                      "bytes_per_launch": bytes_per_launch, "launch_ms": round(dom_ms, 4)},
+        "ccd_step_kernels": step_kernels,
         "hbm_copy_GBps": round(copy_bandwidth_GBps(), 1),
         "ccd_all_axes_GBps": round(all_axes_gbs, 1),
         "ccd_axes": [{"axis": a["axis"], "ms": round(a["ms"], 4), "GBps": round(a["GBps"], 1),

@@ -111,6 +111,16 @@ TPF_API tpf_status tpf_ccd_axis_kernel(tpf_ccd_plan* plan, int axis, int* kernel
 TPF_API tpf_status tpf_ccd_plan_set_launch(tpf_ccd_plan* plan, int axis, int mode,
                                    int rows_cap);
 
+/* Store chunk of the explicit-inverse kernel on a strided axis (no
+ * counterpart in the reference; a launch knob like tpf_ccd_plan_set_launch).
+ * align 8 or 16: each workgroup writes whole `align`-element aligned chunks
+ * of the outputs, solving `align` lines of the neighbouring group again
+ * (rows of 2^k + 1 node axes start at 8 B offsets, and chunks written by
+ * two workgroups are slow); 0: every workgroup writes its own 64 lines;
+ * -1: the default (8).  Results do not depend on it.  A fused launch takes
+ * the first axis' value. */
+TPF_API tpf_status tpf_ccd_plan_set_store_align(tpf_ccd_plan* plan, int axis, int align);
+
 /* Batches of independent CCD problems (no counterpart in the reference; the
  * batching extension).  A batch fixes `count` plans and the axes to
  * differentiate per plan; every (problem, axis) pair with the same launch

@@ -319,3 +319,56 @@ def test_long_contiguous_axis(N, bc, mode, cap):
         bound = max(1e-12, 2.0 * _rel(r2, s2))
         assert _rel(d2, s2) <= bound, (_rel(d2, s2), bound)
     assert _rel(d2, r2) <= bound, (_rel(d2, r2), bound)
+
+
+@pytest.mark.parametrize("N,bc", [((129, 129, 256), "periodic"), ((129, 129, 256), "wall"),
+                                  ((20, 300, 17), ("periodic", "wall", "periodic")),
+                                  ((70, 5), "wall")])
+def test_explicit_store_ownership_and_split_apply(N, bc):
+    """Aligned store ownership of the explicit kernel (workgroups solve 64
+    lines and store the aligned chunks they own): every chunk size gives the
+    same bits, also for output arrays at odd element offsets (d1 and d2 with
+    different alignments, so d2 follows d1's chunks) and runs of lines that
+    start inside a chunk (axis 1).  130 x 130 x 257 periodic (>= 2^22 nodes)
+    runs the all-axes call as the explicit grid of axes 0-1 followed by the
+    wavefront kernel on axis 2; it must equal single-axis calls bit for bit
+    and the dense-LU oracle to 1e-12."""
+    _announce(f"ccd explicit ownership N={N} bc={bc}")
+    g = _grid(N)
+    o = occd.OracleCCD(N, [1.0] * len(N), bc)
+    f_np = np.random.default_rng(5).standard_normal(g.shape)
+    f = torch.from_numpy(f_np).cuda()
+    nd = len(N)
+    ccd = CCDSolver(g, bc_type=bc)
+    n = f.numel()
+    outs = {}
+    for al in (0, 8, 16):
+        for a in range(nd):
+            ccd.set_store_align(a, al)
+        for a in range(nd - 1):
+            b1 = torch.full((n + 3,), float("nan"), dtype=torch.float64, device="cuda")
+            b2 = torch.full((n + 5,), float("nan"), dtype=torch.float64, device="cuda")
+            d1 = b1[1:1 + n].view(g.shape)
+            d2 = b2[3:3 + n].view(g.shape)
+            ccd.differentiate_into(f, d1, d2, a)
+            torch.cuda.synchronize()
+            assert torch.isnan(b1[0]) and torch.isnan(b1[-2:]).all()
+            assert torch.isnan(b2[:3]).all() and torch.isnan(b2[-2:]).all()
+            outs.setdefault(a, []).append((d1.clone(), d2.clone()))
+    for a, lst in outs.items():
+        for d1, d2 in lst[1:]:
+            assert torch.equal(d1, lst[0][0]) and torch.equal(d2, lst[0][1]), a
+        r1, r2 = o.differentiate(f_np, a)
+        assert _rel(lst[0][0].cpu(), r1) <= 1e-12 and _rel(lst[0][1].cpu(), r2) <= 1e-12, a
+    for a in range(nd):
+        ccd.set_store_align(a, -1)
+    d1 = torch.empty((nd,) + g.shape, dtype=torch.float64, device="cuda")
+    d2 = torch.empty_like(d1)
+    ccd.apply_into(f, d1, d2, (1 << nd) - 1)
+    for a in range(nd):
+        s1, s2 = ccd.differentiate(f, a)
+        assert torch.equal(d1[a], s1) and torch.equal(d2[a], s2), a
+    r1, r2 = o.differentiate(f_np, nd - 1)
+    assert _rel(d1[nd - 1].cpu(), r1) <= 1e-12 and _rel(d2[nd - 1].cpu(), r2) <= 1e-12
+    with pytest.raises(ValueError, match="store alignment"):
+        ccd.set_store_align(0, 4)

@@ -82,6 +82,13 @@ class CCDSolver:
         nat.check(self._lib.tpf_ccd_plan_set_launch(self._plan, int(axis), self.MODES[mode],
                                                      int(rows_cap)))
 
+    def set_store_align(self, axis: int, align: int = -1) -> None:
+        """Store chunk (elements) of the explicit-inverse kernel on a strided
+        axis: 8 or 16 write whole aligned chunks, 0 the lines' own stores,
+        -1 the default.  Results do not depend on it."""
+        self._check_axis(axis)
+        nat.check(self._lib.tpf_ccd_plan_set_store_align(self._plan, int(axis), int(align)))
+
     def differentiate_into(self, f: torch.Tensor, d1, d2, axis: int) -> None:
         """Zero-copy form: f, d1, d2 are contiguous CUDA float64 tensors of
         grid.shape (d1 or d2 may be None).  Asynchronous on torch's stream."""

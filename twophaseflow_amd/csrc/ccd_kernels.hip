@@ -1300,39 +1300,189 @@ __global__ __launch_bounds__(1024) void ccd_line_kernel(CcdLaunch L) {
 // directly (strided axes: every store instruction writes 64 consecutive
 // lines).
 
-template <int M>
-__device__ __forceinline__ void ccd_lines_explicit(const CcdAxisGeom& g, const CcdLines& ln,
-                                                   const double* __restrict__ f,
-                                                   double* __restrict__ d1,
-                                                   double* __restrict__ d2,
-                                                   const unsigned block) {
-  extern __shared__ double lds[];
-  constexpr int K = M + 2;
+// Aligned store ownership (AL > 0): on 2^k + 1 node axes the rows of a line
+// group start at 8 B offsets, so a wavefront store of 64 consecutive lines
+// covers two 128 B lines that the neighbouring groups also write, and those
+// partial-line writes halve the store rate (4.1 against 6.4 TB/s for the
+// bare pattern on 257^3).  With AL > 0 workgroup w solves the 64 lines
+// [w*(64-AL) - AL, w*(64-AL) - AL + 64) and stores element (line b, row r)
+// only when it owns the AL-element aligned chunk of that address: the
+// chunk's first element (clipped to the start of b's run of consecutive
+// lines) is line `lead`, and the owner is (lead + AL) / (64 - AL).  Every
+// element has exactly one owner, the owner's window holds every line of the
+// chunk, and each aligned chunk inside a run is written by one store
+// instruction.  The AL lines shared with the previous window are solved
+// twice (same arithmetic, so the stored values do not depend on AL).
+template <int AL>
+struct ExplicitOwner {
+  static constexpr int OWN = AL ? 64 - AL : 64;
+  static constexpr uint32_t MASK = AL ? AL - 1 : 0;
+  uint32_t lane, kk;  // lane = b - (OWN*w - AL); kk = b's index inside its run
+  bool active;
+  // Whether this lane stores the element whose address (in elements) has
+  // low bits e: lead = b - min(e mod AL, kk) and (lead + AL) / OWN == w,
+  // i.e. 0 <= lane - min(e mod AL, kk) < OWN.
+  __device__ __forceinline__ bool owns(uint32_t e) const {
+    if (AL == 0) return active;
+    uint32_t off = e & MASK;
+    off = off < kk ? off : kk;
+    return active && lane - off < (uint32_t)OWN;
+  }
+};
+
+__device__ __forceinline__ uint32_t elem_bits(const double* p) {
+  return (uint32_t)((uintptr_t)p >> 3);
+}
+
+// Memory access of the explicit kernel.  BUF: raw buffer loads and stores
+// with 32-bit byte offsets, the row part of the offset in the scalar
+// operand; a masked lane has kOOR set in its offset, so its load returns 0
+// and its store is dropped, without a branch.  Every array must be smaller
+// than 2 GiB (checked on the host).  !BUF: 64-bit global addresses and
+// exec-masked stores.
+constexpr uint32_t kOOR = 0x80000000u;
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ccd_rsrc(const double* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double ccd_bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+// non-temporal store (outputs stream past the caches, f stays resident)
+__device__ __forceinline__ void ccd_bstore(__amdgpu_buffer_rsrc_t r, double v, uint32_t voff,
+                                           uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, voff, soff, 2);
+}
+
+// One lane's line in a line group of the explicit kernel.
+template <int AL>
+struct ExplicitLine {
+  ExplicitOwner<AL> own;
+  int64_t base;  // element offset of the line's node 0
+  uint32_t e1;   // low address bits (in elements) of d1's node 0
+  uint32_t vb;   // byte offset of node 0 (kOOR: the lane has no line)
+};
+
+template <int AL>
+__device__ __forceinline__ ExplicitLine<AL> explicit_line(const CcdLines& ln, const double* d1,
+                                                          unsigned block) {
+  ExplicitLine<AL> L;
   const int lane = threadIdx.x & 63;
-  const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t b = (int64_t)block * 64 + lane;
-  const bool active = b < ln.n_lines;
-  int64_t base = 0;
+  const int64_t b = (int64_t)block * ExplicitOwner<AL>::OWN - AL + lane;
+  const bool active = b >= 0 && b < ln.n_lines;
+  L.own.active = active;
+  L.own.lane = (uint32_t)lane;
+  L.own.kk = 0;
+  L.base = 0;
   if (active) {
     const uint32_t bi = (uint32_t)b, in = (uint32_t)ln.inner;
-    base = (int64_t)(bi / in) * ln.outer + (int64_t)(bi % in) * ln.inner_stride;
+    const uint32_t q = bi / in;
+    L.own.kk = bi - q * in;
+    L.base = (int64_t)q * ln.outer + (int64_t)L.own.kk * ln.inner_stride;
   }
+  // Store ownership follows d1's aligned chunks for both outputs (d2 has the
+  // same alignment in the usual case; any chunking is a valid partition).
+  L.e1 = elem_bits(d1) + (uint32_t)L.base;
+  L.vb = active ? (uint32_t)L.base * 8u : kOOR;
+  return L;
+}
+
+// Node j (wavefront-uniform) of a lane's line.
+template <int AL, bool BUF>
+__device__ __forceinline__ double explicit_node(const CcdAxisGeom& g, const CcdLines& ln,
+                                                const double* f, const ExplicitLine<AL>& L, int j) {
+  if constexpr (BUF) {
+    const uint32_t bytes = (uint32_t)(ln.n_lines * (int64_t)(g.N + 1) * 8);
+    return ccd_bload(ccd_rsrc(f, bytes), L.vb, (uint32_t)((int64_t)j * ln.stride) * 8u);
+  } else {
+    return L.own.active ? f[L.base + (int64_t)j * ln.stride] : 0.0;
+  }
+}
+
+// Issues the loads of partition p of a lane's line: fv[k] = node (s0 - 1 + k)
+// periodic (wrapped), (s0 + k) wall; nodes past the partition (short
+// partitions) are clamped into the line and meet zero table columns.
+template <int M, int AL, bool BUF>
+__device__ __forceinline__ void explicit_load(const CcdAxisGeom& g, const CcdLines& ln,
+                                              const double* f, const ExplicitLine<AL>& L, int p,
+                                              double (&fv)[M + 2]) {
+  const int N = g.N;
+  const bool periodic = g.periodic != 0;
+  const int s0 = g.s[p];
+  const int j0 = periodic ? s0 - 1 : s0;
+#pragma unroll
+  for (int k = 0; k < M + 2; ++k) {
+    int j = j0 + k;
+    j = j < 0 ? j + N : j;
+    j = periodic ? (j >= N ? j - N : j) : (j > N ? N : j);
+    fv[k] = explicit_node<AL, BUF>(g, ln, f, L, j);
+  }
+}
+
+// Solves partition p of the lanes' lines from the loaded fv and stores the
+// rows; G is the workgroup's [4P][64] exchange buffer (one barrier).
+template <int M, int AL, bool BUF>
+__device__ __forceinline__ void explicit_solve(const CcdAxisGeom& g, const CcdLines& ln,
+                                               const double* f, double* d1, double* d2,
+                                               const ExplicitLine<AL>& L, int p,
+                                               double (&fv)[M + 2], double* G) {
+  constexpr int K = M + 2;
+  const int lane = threadIdx.x & 63;
   const int64_t st = ln.stride;
+  const int N = g.N;
   const int s0 = g.s[p];
   const int m = g.s[p + 1] - s0;
   const bool periodic = g.periodic != 0;
   const bool wall_first = !periodic && p == 0;
   const bool wall_last = !periodic && p == g.P - 1;
+  // (unused without BUF) a missing output gets an empty range: its stores drop
+  const uint32_t bytes = (uint32_t)(ln.n_lines * (int64_t)(N + 1) * 8);
+  const __amdgpu_buffer_rsrc_t r1 = ccd_rsrc(d1, d1 ? bytes : 0u), r2 = ccd_rsrc(d2, d2 ? bytes : 0u);
+  // stores node jn of both outputs (scaled back from (h f', h^2 f'')) when
+  // `keep` (uniform) holds and this lane owns the node's chunk
+  auto store_node = [&](int jn, double u, double v, bool keep) {
+    const int64_t ro = (int64_t)jn * st;
+    const bool o = keep && L.own.owns(L.e1 + (uint32_t)ro);
+    if constexpr (BUF) {  // the whole offset in the vector operand
+      const uint32_t voff = o ? L.vb + (uint32_t)ro * 8u : kOOR;
+      ccd_bstore(r1, u * g.inv_h, voff, 0);
+      ccd_bstore(r2, v * g.inv_h2, voff, 0);
+    } else if (o) {
+      if (d1) __builtin_nontemporal_store(u * g.inv_h, d1 + L.base + ro);
+      if (d2) __builtin_nontemporal_store(v * g.inv_h2, d2 + L.base + ro);
+    }
+  };
 
-  double fv[K];
-  const int j0 = periodic ? s0 - 1 : s0;
+  // wall closure right-hand sides (ccd_boundary_helpers.py:64-116) from the
+  // raw values: the first partition holds nodes 0..5 in fv[0..5], the last
+  // one nodes N-5..N in fv[m-4..m+1]
+  double bl0 = 0.0, bl1 = 0.0, br0 = 0.0, br1 = 0.0;
+  if (wall_first) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    fv[k] = 0.0;
-    if (k < m + 2) {
-      int j = j0 + k;
-      if (periodic) j = j < 0 ? j + g.N : (j >= g.N ? j - g.N : j);
-      if (active) fv[k] = f[base + (int64_t)j * st];
+    for (int k = 0; k < 6; ++k) {
+      if (k < 4) bl0 += g.cIl[k] * fv[k];
+      bl1 += g.cIIl[k] * fv[k];
+    }
+  }
+  if (wall_last) {
+    if (m < 4) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const double v = explicit_node<AL, BUF>(g, ln, f, L, N - k);
+        if (k < 4) br0 += g.cIr[k] * v;
+        br1 += g.cIIr[k] * v;
+      }
+    }
+#pragma unroll
+    for (int mm = 4; mm <= M; ++mm) {
+      if (m == mm) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          if (k < 4) br0 += g.cIr[k] * fv[mm + 1 - k];
+          br1 += g.cIIr[k] * fv[mm + 1 - k];
+        }
+      }
     }
   }
   // first differences in place (fv[K-1] is unused; the table column is 0)
@@ -1341,10 +1491,9 @@ __device__ __forceinline__ void ccd_lines_explicit(const CcdAxisGeom& g, const C
   const kdptr C = (kdptr)(g.ctab) + (size_t)g.ctype[p] * M * K * 2;
   const kdptr T = (kdptr)(g.rowtab) + (size_t)p * M * kRowTab;
 
-  // ---- partition end values of the local solution -> LDS
-  double* G = lds;  // [4P][64]
+  // ---- local solution at the partition ends (rows 0 and m-1) -> LDS
+  double a0 = 0.0, a1 = 0.0, z0 = 0.0, z1 = 0.0;
   {
-    double a0 = 0.0, a1 = 0.0, z0 = 0.0, z1 = 0.0;
     const kdptr c0 = C, c1 = C + M * K;  // row 0, components 0 and 1
     const kdptr l0 = C + (m - 1) * K, l1 = C + (M + m - 1) * K;  // row m-1
 #pragma unroll
@@ -1354,11 +1503,11 @@ __device__ __forceinline__ void ccd_lines_explicit(const CcdAxisGeom& g, const C
       z0 += l0[k] * fv[k];
       z1 += l1[k] * fv[k];
     }
-    G[(p * 4 + 0) * 64 + lane] = a0;
-    G[(p * 4 + 1) * 64 + lane] = a1;
-    G[(p * 4 + 2) * 64 + lane] = z0;
-    G[(p * 4 + 3) * 64 + lane] = z1;
   }
+  G[(p * 4 + 0) * 64 + lane] = a0;
+  G[(p * 4 + 1) * 64 + lane] = a1;
+  G[(p * 4 + 2) * 64 + lane] = z0;
+  G[(p * 4 + 3) * 64 + lane] = z1;
   __syncthreads();
 
   // ---- coupling: (left, right) end values of the neighbours = Q_p . G
@@ -1386,66 +1535,61 @@ __device__ __forceinline__ void ccd_lines_explicit(const CcdAxisGeom& g, const C
   }
 
   // ---- rows: x_r = C_r . fv - V_r (lr0, lr1) - W_r (lr2, lr3), stored at once
-  const int off = periodic ? 0 : 1;
-  double x00 = 0.0, x01 = 0.0, xl0 = 0.0, xl1 = 0.0;
+  const int off = periodic ? 0 : 1;  // unknown r -> node s0 + r + off
+  auto spike = [&](int r, double& u, double& v) {
+    const kdptr t = T + r * kRowTab + 8;
+    u = __builtin_fma(-t[0], lr0, u);
+    u = __builtin_fma(-t[1], lr1, u);
+    u = __builtin_fma(-t[4], lr2, u);
+    u = __builtin_fma(-t[5], lr3, u);
+    v = __builtin_fma(-t[2], lr0, v);
+    v = __builtin_fma(-t[3], lr1, v);
+    v = __builtin_fma(-t[6], lr2, v);
+    v = __builtin_fma(-t[7], lr3, v);
+  };
+  // the end rows reuse their local solution from above
+  spike(0, a0, a1);
+  store_node(s0 + off, a0, a1, true);
+  // interior rows: one scalar-table row per iteration (rows past a short
+  // partition are computed from zero table rows and not stored)
+#pragma unroll 1
+  for (int r = 1; r < M - 1; ++r) {
+    const kdptr cu = C + r * K, cv = C + (M + r) * K;
+    double u = 0.0, v = 0.0;
 #pragma unroll
-  for (int r = 0; r < M; ++r) {
-    if (r < m) {
-      const kdptr cu = C + r * K, cv = C + (M + r) * K;
-      double u = 0.0, v = 0.0;
-#pragma unroll
-      for (int k = 0; k + 1 < K; ++k) {
-        u += cu[k] * fv[k];
-        v += cv[k] * fv[k];
-      }
-      const kdptr t = T + r * kRowTab + 8;
-      u -= t[0] * lr0 + t[1] * lr1 + t[4] * lr2 + t[5] * lr3;
-      v -= t[2] * lr0 + t[3] * lr1 + t[6] * lr2 + t[7] * lr3;
-      if (r == 0) {
-        x00 = u;
-        x01 = v;
-      }
-      if (r == m - 1) {
-        xl0 = u;
-        xl1 = v;
-      }
-      if (active) {
-        const int64_t a = base + (int64_t)(s0 + r + off) * st;
-        if (d1) __builtin_nontemporal_store(u * g.inv_h, d1 + a);
-        if (d2) __builtin_nontemporal_store(v * g.inv_h2, d2 + a);
-      }
+    for (int k = 0; k + 1 < K; ++k) {
+      u += cu[k] * fv[k];
+      v += cv[k] * fv[k];
     }
+    spike(r, u, v);
+    store_node(s0 + r + off, u, v, r < m - 1);
   }
-  if (!active) return;
-  if (periodic && p == 0) {
-    if (d1) d1[base + (int64_t)g.N * st] = x00 * g.inv_h;
-    if (d2) d2[base + (int64_t)g.N * st] = x01 * g.inv_h2;
+  if (m > 1) spike(m - 1, z0, z1);
+  else {
+    z0 = a0;
+    z1 = a1;
   }
-  if (wall_first) {  // fv holds differences now: the closure reads nodes 0..5 again
-    double bl0 = 0.0, bl1 = 0.0;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const double v = f[base + (int64_t)k * st];
-      if (k < 4) bl0 += g.cIl[k] * v;
-      bl1 += g.cIIl[k] * v;
-    }
-    if (d1) d1[base] = (g.Ml[0] * x00 + g.Ml[1] * x01 + bl0) * g.inv_h;
-    if (d2) d2[base] = (g.Ml[2] * x00 + g.Ml[3] * x01 + bl1) * g.inv_h2;
-  }
-  if (wall_last) {
-    double br0 = 0.0, br1 = 0.0;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const double v = f[base + (int64_t)(g.N - k) * st];
-      if (k < 4) br0 += g.cIr[k] * v;
-      br1 += g.cIIr[k] * v;
-    }
-    if (d1) d1[base + (int64_t)g.N * st] = (g.Mr[0] * xl0 + g.Mr[1] * xl1 + br0) * g.inv_h;
-    if (d2) d2[base + (int64_t)g.N * st] = (g.Mr[2] * xl0 + g.Mr[3] * xl1 + br1) * g.inv_h2;
-  }
+  store_node(s0 + m - 1 + off, z0, z1, m > 1);
+  if (periodic && p == 0) store_node(N, a0, a1, true);  // image node N = node 0
+  if (wall_first)
+    store_node(0, g.Ml[0] * a0 + g.Ml[1] * a1 + bl0, g.Ml[2] * a0 + g.Ml[3] * a1 + bl1, true);
+  if (wall_last)
+    store_node(N, g.Mr[0] * z0 + g.Mr[1] * z1 + br0, g.Mr[2] * z0 + g.Mr[3] * z1 + br1, true);
 }
 
-template <int M>
+template <int M, int AL, bool BUF>
+__device__ __forceinline__ void ccd_lines_explicit(const CcdAxisGeom& g, const CcdLines& ln,
+                                                   const double* f, double* d1, double* d2,
+                                                   const unsigned block) {
+  extern __shared__ double lds[];
+  const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const ExplicitLine<AL> L = explicit_line<AL>(ln, d1, block);
+  double fv[M + 2];
+  explicit_load<M, AL, BUF>(g, ln, f, L, p, fv);
+  explicit_solve<M, AL, BUF>(g, ln, f, d1, d2, L, p, fv, lds);
+}
+
+template <int M, int AL, bool BUF>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
 void ccd_explicit_kernel(CcdLaunch L) {
   int sl;
@@ -1453,8 +1597,9 @@ void ccd_explicit_kernel(CcdLaunch L) {
   slot_of(L, blockIdx.x, sl, local);
   const CcdSlot& S = L.slot[sl];
   local = xcd_local(local, L.block_begin[sl + 1] - L.block_begin[sl]);
-  ccd_lines_explicit<M>(S.g, S.ln, S.f, S.d1, S.d2, local);
+  ccd_lines_explicit<M, AL, BUF>(S.g, S.ln, S.f, S.d1, S.d2, local);
 }
+
 
 
 // Batched launch over many (problem, axis) slots kept in device memory:
@@ -1509,16 +1654,16 @@ __global__ __launch_bounds__(64 * kWaveLines) void ccd_wave_batch_kernel(
     ccd_wave_tile<M, false, 1>(S.g, S.ln, f, d1, d2, blk - begin[lo]);
 }
 
-template <int M>
+template <int M, bool BUF>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
 void ccd_explicit_batch_kernel(const CcdSlot* __restrict__ slots, const unsigned* __restrict__ begin,
                                int nslots, BatchPtrs P) {
   const unsigned blk = blockIdx.x;
   const int lo = batch_slot(begin, nslots, blk);
   const CcdSlot& S = slots[lo];
-  ccd_lines_explicit<M>(S.g, S.ln, P.n ? P.f[lo] : S.f, P.n ? P.d1[lo] : S.d1,
-                        P.n ? P.d2[lo] : S.d2,
-                        xcd_local(blk - begin[lo], begin[lo + 1] - begin[lo]));
+  ccd_lines_explicit<M, 0, BUF>(S.g, S.ln, P.n ? P.f[lo] : S.f, P.n ? P.d1[lo] : S.d1,
+                                P.n ? P.d2[lo] : S.d2,
+                                xcd_local(blk - begin[lo], begin[lo + 1] - begin[lo]));
 }
 
 using BatchFn = void (*)(const CcdSlot*, const unsigned*, int, BatchPtrs);
@@ -1532,12 +1677,18 @@ BatchFn pick_wave_batch_kernel(int M) {
   return nullptr;
 }
 
-BatchFn pick_explicit_batch_kernel(int M) {
+BatchFn pick_explicit_batch_kernel(int M, bool buf) {
   switch (M) {
-    case 8: return ccd_explicit_batch_kernel<8>;
-    case 16: return ccd_explicit_batch_kernel<16>;
+    case 8: return buf ? ccd_explicit_batch_kernel<8, true> : ccd_explicit_batch_kernel<8, false>;
+    case 16: return buf ? ccd_explicit_batch_kernel<16, true> : ccd_explicit_batch_kernel<16, false>;
   }
   return nullptr;
+}
+
+// Whether the explicit kernel may address a slot's arrays with 32-bit buffer
+// offsets (every array of the slot < 2 GiB).
+bool slot_buffer_ok(const CcdSlot& S) {
+  return S.ln.n_lines * (int64_t)(S.g.N + 1) * 8 < ((int64_t)1 << 31);
 }
 
 BatchFn pick_batch_kernel(int M) {
@@ -1708,7 +1859,33 @@ tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const dou
   for (int i = 0; i < naxes; ++i) all_explicit = all_explicit && plan->axes[axes[i]].wave == 2;
   if (all_explicit) {  // strided axes, explicit partition inverse
     if (M != 8 && M != 16) return fail(TPF_ERR_VALUE, "explicit CCD kernel takes rows cap 8 or 16");
-    KernelFn k = M == 16 ? ccd_explicit_kernel<16> : ccd_explicit_kernel<8>;
+    // aligned store ownership: one chunk size per launch (the first axis')
+    const int al = plan->store_align[axes[0]] < 0 ? 8 : plan->store_align[axes[0]];
+    const int own = al ? 64 - al : 64;
+    blocks = 0;
+    for (int i = 0; i < naxes; ++i) {
+      L.block_begin[i] = blocks;
+      blocks += (unsigned)((L.slot[i].ln.n_lines + al + own - 1) / own);
+    }
+    L.block_begin[naxes] = blocks;
+    bool buf = true;
+    for (int i = 0; i < naxes; ++i) buf = buf && slot_buffer_ok(L.slot[i]);
+    KernelFn k = nullptr;
+    if (buf) {
+      if (M == 16) k = al == 16 ? ccd_explicit_kernel<16, 16, true>
+                   : al == 8    ? ccd_explicit_kernel<16, 8, true>
+                                : ccd_explicit_kernel<16, 0, true>;
+      else k = al == 16 ? ccd_explicit_kernel<8, 16, true>
+             : al == 8  ? ccd_explicit_kernel<8, 8, true>
+                        : ccd_explicit_kernel<8, 0, true>;
+    } else {
+      if (M == 16) k = al == 16 ? ccd_explicit_kernel<16, 16, false>
+                   : al == 8    ? ccd_explicit_kernel<16, 8, false>
+                                : ccd_explicit_kernel<16, 0, false>;
+      else k = al == 16 ? ccd_explicit_kernel<8, 16, false>
+             : al == 8  ? ccd_explicit_kernel<8, 8, false>
+                        : ccd_explicit_kernel<8, 0, false>;
+    }
     const size_t sh = sizeof(double) * 4 * L.slot[0].g.P * 64;
     TPF_TRY(raise_lds_limit((const void*)k, sh));
     hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * L.slot[0].g.P), sh, stream, L);
@@ -1725,18 +1902,85 @@ tpf_status launch_axes(tpf_ccd_plan* plan, const int* axes, int naxes, const dou
 }
 
 // One grid for the given axes when they share a launch shape (kernel, rows
-// cap, partition count), else one launch per axis.
+// cap, partition count).  Otherwise: on large grids whose strided axes take
+// the explicit kernel and whose other axes take the wavefront kernel, one
+// explicit grid for the strided axes and then the others (257^3 periodic:
+// 0.30 ms against 0.35 ms for one lane-kernel grid of all three; two streams
+// measured slower; with a wall contiguous axis, which runs the tiled lane
+// kernel, the split measured slower); else, when the axes share one rows cap
+// M <= 16 and <= 16 partitions, one lane-kernel grid for all of them (the
+// lane kernel's tables serve every axis); else one launch per axis.
+constexpr int64_t kSplitFusedNodes = (int64_t)1 << 22;
+
+// The kernel each axis of one tpf_ccd_apply call runs (0 lane recurrences,
+// 1 wavefront per line, 2 explicit inverse) under launch_fused's rules.
+void fused_kinds(const tpf_ccd_plan* plan, const int* axes, int n, int* kind) {
+  bool same = true, lane_ok = true, rest_wave = true;
+  int ne = 0;
+  for (int i = 0; i < n; ++i) {
+    const CcdAxisHost& x = plan->axes[axes[i]];
+    const CcdAxisHost& x0 = plan->axes[axes[0]];
+    kind[i] = x.wave;
+    same = same && x.geom.M == x0.geom.M && x.geom.P == x0.geom.P &&
+           (x.wave == 1) == (x0.wave == 1) && (x.wave == 2) == (x0.wave == 2);
+    lane_ok = lane_ok && x.geom.M == x0.geom.M && x.geom.M <= 16 && x.geom.P <= kMaxParts &&
+              x.geom.P == x0.geom.P;
+    if (x.wave == 2) ++ne;
+    else rest_wave = rest_wave && x.wave == 1;
+  }
+  if (n <= 1 || same) return;
+  int64_t total = 1;
+  for (int a = 0; a < plan->ndim; ++a) total *= plan->n_nodes[a];
+  const bool refined = plan->precision == TPF_PREC_FP32_REFINED;
+  if (ne > 0 && rest_wave && total >= kSplitFusedNodes && !refined) return;
+  if (lane_ok && !refined)
+    for (int i = 0; i < n; ++i) kind[i] = 0;
+}
+
 tpf_status launch_fused(tpf_ccd_plan* plan, const int* axes, int n, const double* const* f,
                         double* const* d1, double* const* d2, hipStream_t stream) {
   bool same = true;
   for (int i = 1; i < n; ++i)
     same = same && plan->axes[axes[i]].geom.M == plan->axes[axes[0]].geom.M &&
            plan->axes[axes[i]].geom.P == plan->axes[axes[0]].geom.P &&
-           (plan->axes[axes[i]].wave == 1) == (plan->axes[axes[0]].wave == 1);
+           (plan->axes[axes[i]].wave == 1) == (plan->axes[axes[0]].wave == 1) &&
+           (plan->axes[axes[i]].wave == 2) == (plan->axes[axes[0]].wave == 2);
   if (same) return launch_axes(plan, axes, n, f, d1, d2, stream);
-  // Axes of one rows cap M <= 16 and <= 16 partitions share the lane
-  // kernel's tables whatever kernel each takes alone: one lane-kernel grid
-  // for all of them (measured faster on 257^3 than a launch per kernel).
+  int64_t total = 1;
+  for (int a = 0; a < plan->ndim; ++a) total *= plan->n_nodes[a];
+  int ex[3], ne = 0, rest[3], nr = 0;
+  bool rest_wave = true;  // the other axes take the wavefront kernel
+  for (int i = 0; i < n; ++i) {
+    if (plan->axes[axes[i]].wave == 2 && (ne == 0 || (plan->axes[axes[i]].geom.M ==
+                                                       plan->axes[axes[ex[0]]].geom.M &&
+                                                   plan->axes[axes[i]].geom.P ==
+                                                       plan->axes[axes[ex[0]]].geom.P))) {
+      ex[ne++] = i;
+    } else {
+      rest[nr++] = i;
+      rest_wave = rest_wave && plan->axes[axes[i]].wave == 1;
+    }
+  }
+  if (ne > 0 && rest_wave && total >= kSplitFusedNodes &&
+      plan->precision != TPF_PREC_FP32_REFINED) {
+    int a1[3], a2[3];
+    const double *f1[3], *f2[3];
+    double *o11[3], *o12[3], *o21[3], *o22[3];
+    for (int i = 0; i < ne; ++i) {
+      a1[i] = axes[ex[i]];
+      f1[i] = f[ex[i]];
+      o11[i] = d1[ex[i]];
+      o12[i] = d2[ex[i]];
+    }
+    for (int i = 0; i < nr; ++i) {
+      a2[i] = axes[rest[i]];
+      f2[i] = f[rest[i]];
+      o21[i] = d1[rest[i]];
+      o22[i] = d2[rest[i]];
+    }
+    TPF_TRY(launch_axes(plan, a1, ne, f1, o11, o12, stream));
+    return nr ? launch_fused(plan, a2, nr, f2, o21, o22, stream) : TPF_OK;
+  }
   if (n > 1 && plan->precision != TPF_PREC_FP32_REFINED) {
     bool lane_ok = true;
     for (int i = 0; i < n; ++i) {
@@ -1818,7 +2062,7 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
   // Auto picks the wavefront kernel for few long lines, and for many lines of
   // <= 256 unknowns on the contiguous axis (four lines of 16 partitions per
   // wavefront: 0.128 against 0.172 ms for the tiled lane kernel on 257^3).
-  const bool contig_many = !strided && lines >= 4096 && n > 32 && n <= 256;
+  const bool contig_many = !strided && periodic && lines >= 4096 && n > 32 && n <= 256;
   bool wave = mode == 2 || (mode == 0 && ((lines < 4096 && n > 256) || contig_many ||
                                            (ref && n > 256)));
   if (wave && cap != 0 && cap != 8 && cap != 16)
@@ -2059,10 +2303,19 @@ extern "C" tpf_status tpf_ccd_plan_set_launch(tpf_ccd_plan* plan, int axis, int 
   return TPF_OK;
 }
 
+extern "C" tpf_status tpf_ccd_plan_set_store_align(tpf_ccd_plan* plan, int axis, int align) {
+  if (!plan || axis < 0 || axis >= plan->ndim) return fail(TPF_ERR_VALUE, "bad plan/axis");
+  if (align != -1 && align != 0 && align != 8 && align != 16)
+    return fail(TPF_ERR_VALUE, "store alignment must be -1 (default), 0, 8 or 16 elements");
+  plan->store_align[axis] = align;
+  return TPF_OK;
+}
+
 // ---- batches of independent problems ---------------------------------------
 struct tpf_ccd_batch {
   struct Group {
     int M, P, wave;  // wave: 0 lane recurrences, 1 wavefront per line, 2 explicit inverse
+    bool buf;          // explicit inverse: every slot's arrays fit 32-bit buffer offsets
     int first, count;  // slots [first, first + count)
     unsigned blocks;
     size_t shmem;
@@ -2114,19 +2367,12 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
       }
       items.push_back({j, k++, a, pl->axes[a].wave});
     }
-    // tpf_ccd_apply's rule: axes of one launch shape go in one grid, and an
-    // explicit-inverse axis grouped with a recurrence axis runs as recurrence
-    bool same = true, all_explicit = true;
-    for (size_t i = first; i < items.size(); ++i) {
-      const CcdAxisHost& x = pl->axes[items[i].axis];
-      const CcdAxisHost& x0 = pl->axes[items[first].axis];
-      same = same && x.geom.M == x0.geom.M && x.geom.P == x0.geom.P &&
-             (x.wave == 1) == (x0.wave == 1);
-      all_explicit = all_explicit && x.wave == 2;
-    }
-    if (same && !all_explicit)
-      for (size_t i = first; i < items.size(); ++i)
-        if (items[i].kind == 2) items[i].kind = 0;
+    // the kernels tpf_ccd_apply runs these axes with (bit-identical results)
+    int ax[3], kinds[3];
+    const int na = (int)(items.size() - first);
+    for (int i = 0; i < na; ++i) ax[i] = items[first + i].axis;
+    fused_kinds(pl, ax, na, kinds);
+    for (int i = 0; i < na; ++i) items[first + i].kind = kinds[i];
   }
   // group by kernel and launch shape (M, P), keeping the input order inside a group
   struct Shape {
@@ -2140,7 +2386,7 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
     if (std::find(shapes.begin(), shapes.end(), sh) == shapes.end()) shapes.push_back(sh);
   }
   for (const auto& sh : shapes) {
-    tpf_ccd_batch::Group G{sh.M, sh.P, sh.wave, (int)b->slots.size(), 0, 0, 0};
+    tpf_ccd_batch::Group G{sh.M, sh.P, sh.wave, true, (int)b->slots.size(), 0, 0, 0};
     b->begin_off.push_back((int)b->begin.size());
     unsigned blocks = 0;
     for (const Item& it : items) {
@@ -2162,6 +2408,7 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
       if (sh.wave == 0) G.shmem = std::max(G.shmem, lds_bytes(g, S.contig, pl->n_nodes[it.axis]));
       else if (sh.wave == 1) G.shmem = std::max(G.shmem, wave_lds_bytes(g, kWaveLines, kWaveLines));
       else G.shmem = std::max(G.shmem, sizeof(double) * 4 * (size_t)g.P * 64);
+      G.buf = G.buf && slot_buffer_ok(S);
       G.count += 1;
     }
     b->begin.push_back(blocks);
@@ -2187,7 +2434,7 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
   std::copy(b->slots.begin(), b->slots.end(), b->h_slots);
   for (const auto& G : b->groups) {
     const void* k = G.wave == 1   ? (const void*)pick_wave_batch_kernel(G.M)
-                    : G.wave == 2 ? (const void*)pick_explicit_batch_kernel(G.M)
+                    : G.wave == 2 ? (const void*)pick_explicit_batch_kernel(G.M, G.buf)
                                   : (const void*)pick_batch_kernel(G.M);
     if (G.shmem == 0) continue;
     const tpf_status st = raise_lds_limit(k, G.shmem);
@@ -2261,7 +2508,7 @@ extern "C" tpf_status tpf_ccd_batch_apply(tpf_ccd_batch* b, const double* const*
       hipLaunchKernelGGL(pick_wave_batch_kernel(G.M), dim3(G.blocks), dim3(64 * kWaveLines),
                          G.shmem, st, slots, begin, G.count, P);
     else
-      hipLaunchKernelGGL(G.wave == 2 ? pick_explicit_batch_kernel(G.M) : pick_batch_kernel(G.M),
+      hipLaunchKernelGGL(G.wave == 2 ? pick_explicit_batch_kernel(G.M, G.buf) : pick_batch_kernel(G.M),
                          dim3(G.blocks), dim3(64 * G.P), G.shmem, st, slots, begin, G.count, P);
     TPF_HIP(hipGetLastError());
   }

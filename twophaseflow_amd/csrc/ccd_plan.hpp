@@ -51,5 +51,6 @@ struct tpf_ccd_plan {
   int precision = 0;
   int rows_cap[3] = {0, 0, 0};  // per-axis rows-per-partition override (0: default)
   int mode[3] = {0, 0, 0};      // per-axis kernel: 0 auto, 1 lane per line, 2 wavefront per line
+  int store_align[3] = {-1, -1, -1};  // explicit kernel's store chunk (elements; -1 default 8)
   tpf::CcdAxisHost axes[3];
 };
