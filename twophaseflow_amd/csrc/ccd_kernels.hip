@@ -729,11 +729,38 @@ __device__ __forceinline__ void wave_tile_io(const CcdLines& ln, int nax, int pi
                                              double* dst, double* tile) {
   const int cnt = nl * nax;
   if (ln.stride == 1) {  // contiguous axis: one block of nl*nax doubles
+    // Batches of kB elements per thread: all global loads of a batch are
+    // issued before any LDS write; (line, node) of element e advance by
+    // blockDim.x = q*nax + rr without a division per element.
+    constexpr int kB = 8;
     const int64_t g0 = line0 * ln.inner_stride;
-    for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
-      const int l = e / nax, j = e - l * nax;
-      if (LOAD) tile[tix(pitch, l, j)] = src[g0 + e];
-      else __builtin_nontemporal_store(tile[tix(pitch, l, j)], dst + g0 + e);
+    const int bd = (int)blockDim.x;
+    const int q = bd / nax, rr = bd - q * nax;
+    int l = (int)threadIdx.x / nax, j = (int)threadIdx.x - l * nax;
+    for (int e0 = (int)threadIdx.x; e0 < cnt; e0 += kB * bd) {
+      double v[kB];
+      int ti[kB];
+#pragma unroll
+      for (int i = 0; i < kB; ++i) {
+        const int e = e0 + i * bd;
+        ti[i] = tix(pitch, l, j);
+        if (LOAD) v[i] = e < cnt ? src[g0 + e] : 0.0;
+        else v[i] = e < cnt ? tile[ti[i]] : 0.0;
+        l += q;
+        j += rr;
+        if (j >= nax) {
+          j -= nax;
+          ++l;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kB; ++i) {
+        const int e = e0 + i * bd;
+        if (e < cnt) {
+          if (LOAD) tile[ti[i]] = v[i];
+          else __builtin_nontemporal_store(v[i], dst + g0 + e);
+        }
+      }
     }
   } else {  // strided axis: consecutive threads take consecutive lines of one node
     for (int e = threadIdx.x; e < nlw * nax; e += blockDim.x) {
@@ -2059,10 +2086,12 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
     ax.wave = 2;
     return TPF_OK;
   }
-  // Auto picks the wavefront kernel for few long lines, and for many lines of
-  // <= 256 unknowns on the contiguous axis (four lines of 16 partitions per
-  // wavefront: 0.128 against 0.172 ms for the tiled lane kernel on 257^3).
-  const bool contig_many = !strided && periodic && lines >= 4096 && n > 32 && n <= 256;
+  // Auto picks the wavefront kernel for few long lines, and for many periodic
+  // lines of 129-256 unknowns on the contiguous axis (four lines of 16
+  // partitions per wavefront: 0.117-0.124 against 0.163 ms for the tiled
+  // lane kernel on 257^3; on 129^3 the tiled lane kernel is faster, 0.027
+  // against 0.075 ms, and on wall lines everywhere).
+  const bool contig_many = !strided && periodic && lines >= 4096 && n > 128 && n <= 256;
   bool wave = mode == 2 || (mode == 0 && ((lines < 4096 && n > 256) || contig_many ||
                                            (ref && n > 256)));
   if (wave && cap != 0 && cap != 8 && cap != 16)
