@@ -42,6 +42,9 @@ struct CcdAxisGeom {
   int periodic;
   int s[kMaxWaveParts + 1];  // partition starts (unknown index), s[P] = n
   int ctype[kMaxParts];      // partition -> ctab type (explicit mode)
+  const double* rowtab_u;    // [ntypes][M][kRowTab]: the distinct partition row tables
+  int ntypes;                // number of distinct partition row tables
+  unsigned char rtype[kMaxWaveParts];  // partition -> its table in rowtab_u
   int uniform;               // 1: every partition has the same rows and relative coupling
                              //    rows (periodic, equal partitions): partition 0's tables serve all
   double inv_h, inv_h2;

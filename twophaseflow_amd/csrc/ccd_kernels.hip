@@ -720,6 +720,22 @@ __device__ __forceinline__ void ccd_lines_ref(const CcdAxisGeom& g, const CcdLin
 constexpr int kWaveLines = 16;  // lines per workgroup of the batch kernels
 
 __host__ __device__ inline int wave_pitch(int nax) { return nax + (nax >> 4) + 1; }
+
+// Non-uniform axes (wall lines) stage their distinct partition row tables in
+// LDS after the tile region when they take at most this many doubles, so the
+// dependent sweeps read them from LDS instead of L2.
+// Each staged table is padded by 2 doubles so lanes of different tables read
+// different LDS banks.
+constexpr int kWaveTabMax = 2048;
+__host__ __device__ inline int wave_tab_stride(int M) { return M * kRowTab + 2; }
+__host__ __device__ inline bool wave_tab_lds(const CcdAxisGeom& g) {
+  return !g.uniform && g.ntypes * wave_tab_stride(g.M) <= kWaveTabMax;
+}
+// Doubles of the tile region: the [lines][pitch] tile, which later holds the
+// 4 x 64 exchange buffer of every wavefront.
+__host__ __device__ inline int wave_region(int lines, int pitch, int waves) {
+  return lines * pitch > waves * 4 * 64 ? lines * pitch : waves * 4 * 64;
+}
 __device__ inline int tix(int pitch, int l, int j) { return l * pitch + j + (j >> 4); }
 
 // Cooperative tile transfer between global lines [line0, line0 + nl) and LDS.
@@ -803,6 +819,11 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
   const bool wall_last = active && !periodic && p == g.P - 1;
 
   wave_tile_io<true>(ln, nax, pitch, line0, nl, nlw, f, nullptr, lds);
+  const bool ltab = !UNI && wave_tab_lds(g);
+  double* tabl = lds + wave_region(nlw, pitch, nwaves);
+  if (ltab)
+    for (int i = threadIdx.x; i < g.ntypes * M * kRowTab; i += blockDim.x)
+      tabl[(i / (M * kRowTab)) * wave_tab_stride(M) + i % (M * kRowTab)] = g.rowtab_u[i];
   __syncthreads();
   const int lw = w < nl ? w : 0;  // rows of inactive waves read line 0 (never stored)
   double fv[M + 2];
@@ -851,7 +872,9 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
   // UNI: every partition's rows are partition 0's, read wavefront-uniform
   // through the scalar cache instead of per lane on the dependent sweeps
   using TabPtr = typename std::conditional<UNI, kdptr, const double*>::type;
-  const TabPtr tab = UNI ? (TabPtr)g.rowtab : (TabPtr)(g.rowtab + (size_t)pp * M * kRowTab);
+  const TabPtr tab = UNI    ? (TabPtr)g.rowtab
+                     : ltab ? (TabPtr)(tabl + (int)g.rtype[pp] * wave_tab_stride(M))
+                            : (TabPtr)(g.rowtab + (size_t)pp * M * kRowTab);
   double x[M][2];
 #pragma unroll
   for (int r = 0; r < M; ++r) {
@@ -1253,10 +1276,11 @@ size_t wave_lds_bytes_ref(const CcdAxisGeom& g, int lines, int waves) {
 }
 
 // LDS of the wavefront kernel: the [lines][pitch] tile (it also holds the
-// 2 KB-per-wavefront exchange buffer once the inputs are read).
+// 2 KB-per-wavefront exchange buffer once the inputs are read), then the
+// staged partition tables of a non-uniform axis.
 size_t wave_lds_bytes(const CcdAxisGeom& g, int lines, int waves) {
-  const size_t tile = sizeof(double) * lines * (size_t)wave_pitch(g.N + 1);
-  return std::max(tile, sizeof(double) * waves * 4 * 64);
+  return sizeof(double) * ((size_t)wave_region(lines, wave_pitch(g.N + 1), waves) +
+                           (wave_tab_lds(g) ? (size_t)g.ntypes * wave_tab_stride(g.M) : 0));
 }
 
 // Slot and slot-local block of workgroup blk.  The slots' first min-count
@@ -2147,6 +2171,10 @@ tpf_status ensure_axis(tpf_ccd_plan* plan, int axis) {
                     hipMemcpyHostToDevice));
   ax.geom.rowtab = ax.d_rowtab;
   ax.geom.qtab = ax.d_qtab;
+  TPF_HIP(hipMalloc(&ax.d_rowtab_u, ax.rowtab_u.size() * sizeof(double)));
+  TPF_HIP(hipMemcpy(ax.d_rowtab_u, ax.rowtab_u.data(), ax.rowtab_u.size() * sizeof(double),
+                    hipMemcpyHostToDevice));
+  ax.geom.rowtab_u = ax.d_rowtab_u;
   if (ax.wave == 2) {
     TPF_HIP(hipMalloc(&ax.d_ctab, ax.ctab.size() * sizeof(double)));
     TPF_HIP(hipMemcpy(ax.d_ctab, ax.ctab.data(), ax.ctab.size() * sizeof(double),
@@ -2170,6 +2198,7 @@ extern "C" tpf_status tpf_ccd_plan_destroy(tpf_ccd_plan* plan) {
   if (!plan) return TPF_OK;
   for (auto& ax : plan->axes) {
     if (ax.d_rowtab) (void)hipFree(ax.d_rowtab);
+    if (ax.d_rowtab_u) (void)hipFree(ax.d_rowtab_u);
     if (ax.d_qtab) (void)hipFree(ax.d_qtab);
     if (ax.d_rowtab32) (void)hipFree(ax.d_rowtab32);
     if (ax.d_qtab32) (void)hipFree(ax.d_qtab32);
@@ -2324,6 +2353,7 @@ extern "C" tpf_status tpf_ccd_plan_set_launch(tpf_ccd_plan* plan, int axis, int 
   }
   CcdAxisHost& ax = plan->axes[axis];
   if (ax.d_rowtab) TPF_HIP(hipFree(ax.d_rowtab));
+  if (ax.d_rowtab_u) TPF_HIP(hipFree(ax.d_rowtab_u));
   if (ax.d_qtab) TPF_HIP(hipFree(ax.d_qtab));
   if (ax.d_rowtab32) TPF_HIP(hipFree(ax.d_rowtab32));
   if (ax.d_qtab32) TPF_HIP(hipFree(ax.d_qtab32));

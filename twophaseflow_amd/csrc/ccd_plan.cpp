@@ -269,6 +269,26 @@ tpf_status build_axis_tables(CcdAxisHost& ax, int n_nodes, double h, int periodi
     }
   }
 
+  // Distinct partition row tables (wall lines: first, last and one or two
+  // interior partition sizes), so kernels with per-partition tables can stage
+  // them in LDS.
+  ax.rowtab_u.clear();
+  {
+    const size_t bsz = size_t(M) * kRowTab;
+    for (int p = 0; p < P; ++p) {
+      const double* blk = &ax.rowtab[size_t(p) * bsz];
+      int type = -1;
+      for (size_t t = 0; t * bsz < ax.rowtab_u.size() && type < 0; ++t)
+        if (std::memcmp(&ax.rowtab_u[t * bsz], blk, bsz * sizeof(double)) == 0) type = (int)t;
+      if (type < 0) {
+        type = (int)(ax.rowtab_u.size() / bsz);
+        ax.rowtab_u.insert(ax.rowtab_u.end(), blk, blk + bsz);
+      }
+      g.rtype[p] = (unsigned char)type;
+    }
+    g.ntypes = (int)(ax.rowtab_u.size() / bsz);
+  }
+
   // Reduced system (I + K) Z = G over Z = [first(p), last(p)]_p, 2 comps each.
   const int nz = 4 * P;
   std::vector<ld> A(size_t(nz) * nz, 0.0L);

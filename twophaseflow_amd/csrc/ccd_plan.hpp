@@ -17,6 +17,8 @@ struct CcdAxisHost {
   std::vector<double> rowtab;   // [P][M][kRowTab]
   std::vector<double> qtab;     // [P][4][NQ*4]
   std::vector<double> ctab;     // [types][2][M][M+2] (M <= 16), geom.ctype maps partitions
+  std::vector<double> rowtab_u; // distinct [M][kRowTab] blocks of rowtab, geom.rtype maps partitions
+  double* d_rowtab_u = nullptr;
   double trunc_coeff = 0.0;     // largest dropped reduced-system coefficient (relative)
   double* d_rowtab = nullptr;
   double* d_qtab = nullptr;
