@@ -812,8 +812,12 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
   const int pitch = wave_pitch(nax);
   const bool active = w < nl && p < g.P;
   const int pp = p < g.P ? p : g.P - 1;  // idle lanes mirror the last partition (no stores)
-  const int s0 = (int)((int64_t)g.n * pp / g.P);
-  const int m = (int)((int64_t)g.n * (pp + 1) / g.P) - s0;
+  // partition pp holds unknowns [n pp / P, n (pp+1) / P); with UNI the
+  // partitions are equal, so m is wavefront-uniform and the row loops below
+  // take uniform branches instead of per-lane masks
+  const int mu = (int)((unsigned)g.n / (unsigned)g.P);
+  const int s0 = UNI ? pp * mu : (int)((unsigned)(g.n * pp) / (unsigned)g.P);
+  const int m = UNI ? mu : (int)((unsigned)(g.n * (pp + 1)) / (unsigned)g.P) - s0;
   const bool periodic = g.periodic != 0;
   const bool wall_first = active && !periodic && p == 0;
   const bool wall_last = active && !periodic && p == g.P - 1;
@@ -1076,8 +1080,9 @@ __device__ __forceinline__ void ccd_wave_tile_ref(const CcdAxisGeom& g, const Cc
   const int pitch = wave_pitch(nax);
   const bool active = w < nl && p < g.P;
   const int pp = p < g.P ? p : g.P - 1;
-  const int s0 = (int)((int64_t)g.n * pp / g.P);
-  const int m = (int)((int64_t)g.n * (pp + 1) / g.P) - s0;
+  const int mu = (int)((unsigned)g.n / (unsigned)g.P);  // equal partitions (UNI)
+  const int s0 = UNI ? pp * mu : (int)((unsigned)(g.n * pp) / (unsigned)g.P);
+  const int m = UNI ? mu : (int)((unsigned)(g.n * (pp + 1)) / (unsigned)g.P) - s0;
   const bool periodic = g.periodic != 0;
   const bool wall_first = active && !periodic && p == 0;
   const bool wall_last = active && !periodic && p == g.P - 1;
@@ -2114,8 +2119,10 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
   // lines of 129-256 unknowns on the contiguous axis (four lines of 16
   // partitions per wavefront: 0.117-0.124 against 0.163 ms for the tiled
   // lane kernel on 257^3; on 129^3 the tiled lane kernel is faster, 0.027
-  // against 0.075 ms, and on wall lines everywhere).
-  const bool contig_many = !strided && periodic && lines >= 4096 && n > 128 && n <= 256;
+  // against 0.075 ms, and on wall lines everywhere; FP32-refined plans keep
+  // the lane kernel, whose fused 3-axis grid is 0.6 ms on 257^3 against
+  // 0.8 ms for per-axis launches with the refined wavefront kernel).
+  const bool contig_many = !strided && periodic && !ref && lines >= 4096 && n > 128 && n <= 256;
   bool wave = mode == 2 || (mode == 0 && ((lines < 4096 && n > 256) || contig_many ||
                                            (ref && n > 256)));
   if (wave && cap != 0 && cap != 8 && cap != 16)
