@@ -43,7 +43,14 @@ constexpr int kPcrFusedMaxN = 2048;
 constexpr int kPcrTileNodes = 512;
 constexpr int kDot = kChunk + 1;    // + <w, w>
 constexpr int kCheckSteps = 8;      // Arnoldi steps launched between host reads of the stop flag
-constexpr unsigned kDotBlocks = 512;  // multi-dot workgroups per block row (fewer partials to finish)
+// Multi-dot workgroups per block row: at most 512 on small problems (fewer
+// partials to finish: 65^3 GMRES step 0.226 -> 0.190 ms), one per 8192
+// unknowns above that (with the cap, a 257^3 multi-dot row of 16 basis
+// vectors streamed at 4.2 TB/s in 357 us).
+inline unsigned dot_blocks(unsigned grid, int64_t n) {
+  const int64_t want = std::max<int64_t>(512, n / 8192);
+  return (unsigned)std::min<int64_t>(grid, want);
+}
 
 // Device side of one restart cycle's Hessenberg / Givens work.
 struct GivensDev {
@@ -716,7 +723,7 @@ tpf_status psolve(Ctx& c, const double* r, double* z) {
 tpf_status multidot(const Krylov& c, const double* V, int kcount, const double* w,
                     double* out) {
   const int chunks = std::max(1, (kcount + kChunk - 1) / kChunk);
-  const unsigned gx = std::min(c.grid.x, kDotBlocks);
+  const unsigned gx = dot_blocks(c.grid.x, c.n);
   hipLaunchKernelGGL(k_multidot, dim3(gx, chunks), dim3(kBlock), 0, c.st, c.n, V, kcount,
                      w, c.gw->part);
   TPF_HIP(hipGetLastError());
@@ -735,7 +742,7 @@ tpf_status multidot(const Krylov& c, const double* V, int kcount, const double* 
 tpf_status multidot_dev(const Krylov& c, const double* V, int kcount, const double* w, int set) {
   const int chunks = std::max(1, (kcount + kChunk - 1) / kChunk);
   double* out = c.gw->dots + (size_t)set * kDot * c.gw->chunks;
-  const unsigned gx = std::min(c.grid.x, kDotBlocks);
+  const unsigned gx = dot_blocks(c.grid.x, c.n);
   hipLaunchKernelGGL(k_multidot, dim3(gx, chunks), dim3(kBlock), 0, c.st, c.n, V, kcount,
                      w, c.gw->part);
   TPF_HIP(hipGetLastError());
