@@ -224,10 +224,10 @@ __global__ __launch_bounds__(kBlock) void k_fccd_apply(FccdGeom g, const double*
   const unsigned nb = gridDim.x, per = nb / 8;
   const unsigned lb = blockIdx.x < per * 8 ? (blockIdx.x % 8) * per + blockIdx.x / 8 : blockIdx.x;
   const unsigned bx = lb % nx, by = lb / nx;
-  const int plane = g.n[1] * g.n[2];
-  const int j = bx * kBlock + threadIdx.x;
+  const unsigned plane = (unsigned)g.n[1] * (unsigned)g.n[2];  // < 2^32 (host guard)
+  const unsigned j = bx * kBlock + threadIdx.x;
   if (j >= plane) return;
-  const int c1 = j / g.n[2], c2 = j - c1 * g.n[2];
+  const int c1 = (int)(j / (unsigned)g.n[2]), c2 = (int)(j - (unsigned)c1 * (unsigned)g.n[2]);
   const int n0 = g.n[0], N0 = n0 - 1;
   const bool per0 = has0 && g.periodic[0] != 0;
   const int s0 = by * kSlab;

@@ -36,15 +36,23 @@ def _dot(a, b) -> float:
     return float((a * b).sum())
 
 
+_OPERATOR_KEYS = ("grid", "coefficient_scheme", "interface_coupling_scheme", "bc_type")
+
+
+def _operator_signature(solver):
+    """What fixes the linear operator a PPE solver discretises: its class and
+    the grid / coefficient / coupling / boundary settings."""
+    return type(solver), tuple(getattr(solver, k, None) for k in _OPERATOR_KEYS)
+
+
 def _can_collapse_same_operator(base_solver, operator) -> bool:
-    """ppe/defect_correction.py:431-439."""
-    if type(base_solver) is not type(operator):
-        return False
-    for name in ("grid", "coefficient_scheme", "interface_coupling_scheme", "bc_type"):
-        if getattr(base_solver, name, None) is not getattr(operator, name, None):
-            if getattr(base_solver, name, None) != getattr(operator, name, None):
-                return False
-    return hasattr(base_solver, "solve") and hasattr(operator, "apply")
+    """True when the low-order base solver and the high-order operator are the
+    same discretisation (a defect correction that would iterate on itself);
+    the rule of ppe/defect_correction.py:431-439."""
+    (kind_a, keys_a), (kind_b, keys_b) = (_operator_signature(base_solver),
+                                          _operator_signature(operator))
+    same = kind_a is kind_b and all(a is b or a == b for a, b in zip(keys_a, keys_b))
+    return same and hasattr(base_solver, "solve") and hasattr(operator, "apply")
 
 
 class PPESolverDefectCorrection(IPPESolver):
