@@ -867,26 +867,29 @@ def legacy_config1_bench(steps=20, cpu=True):
     return out
 
 
-def uneven_bubble_batch_bench(steps=2, sizes=(128, 256, 512), per_size=16):
+def uneven_bubble_batch_bench(steps=2, sizes=(128, 256, 512), per_size=16,
+                              preconditioner="jacobi"):
     """The uneven batch of 48 seeded 2-D rising bubbles (16 each at 128^2,
     256^2, 512^2; radius 0.15-0.3, density ratio 10-1000, We 1-50) stepped
     together by BatchedSimulation: per simulation the CFL dt and steps 1-5,
-    one batched Jacobi-PCG solve of all 48 PPEs (to 1e-10, one launch per
-    PCG step, captured iteration chunks), then the corrector.  One warm-up
-    batched step, then `steps` batched steps timed by the host clock;
-    aggregate simulation steps per second.  The same simulations then take
-    one step each one after another (single PPE solves) for comparison."""
+    one batched solve of all 48 PPEs to 1e-10, then the corrector.  The PPE
+    is the reference's Jacobi-PCG (one launch per PCG step over all problems,
+    captured iteration chunks) or, with preconditioner="mg", multigrid FCG
+    (the 48 solves advance together, one stream each).  One warm-up batched
+    step, then `steps` batched steps timed by the host clock; aggregate
+    simulation steps per second.  The same simulations then take one step
+    each one after another (single PPE solves) for comparison."""
     import torch
     from twophaseflow_amd import BatchedSimulation, bubble_simulation
     from twophaseflow_amd.fields import uneven_bubble_batch
     cases = uneven_bubble_batch(seed=5, sizes=sizes, per_size=per_size)
     t0 = time.perf_counter()
-    sims = [bubble_simulation(c) for c in cases]
+    sims = [bubble_simulation(c, ppe_preconditioner=preconditioner) for c in cases]
     batch = BatchedSimulation(sims)
     batch.initialize()
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
-    batch.step()  # warm-up: graph capture, first-touch allocations
+    batch.step()  # warm-up: graph capture, hierarchies, first-touch allocations
     torch.cuda.synchronize()
     its = []
     t0 = time.perf_counter()
@@ -903,16 +906,18 @@ def uneven_bubble_batch_bench(steps=2, sizes=(128, 256, 512), per_size=16):
     torch.cuda.synchronize()
     seq = time.perf_counter() - t0
     finite = all(not s._has_nonfinite_state() for s in sims)
+    ppe = {"jacobi": "Jacobi-PCG to 1e-10 (CG form)",
+           "mg": "multigrid FCG to 1e-10 (CG form, one W(2,2) FP32 cycle per iteration)"}
     return {"simulations": n, "grids": [f"{N + 1}^2" for N in sizes], "per_grid": per_size,
             "batched_steps": steps, "seconds": round(sec, 3),
             "sim_steps_per_s": round(n * steps / sec, 2),
             "ms_per_batched_step": round(sec / steps * 1e3, 1),
-            "max_pcg_iterations_per_step": its, "retries": batch.retries,
+            "max_ppe_iterations_per_step": its, "retries": batch.retries,
             "sequential_one_step_each_s": round(seq, 3),
             "sequential_sim_steps_per_s": round(n / seq, 2),
             "batched_vs_sequential": round((n * steps / sec) / (n / seq), 2),
             "build_seconds": round(build_s, 1), "all_finite": finite,
-            "ppe": "Jacobi-PCG to 1e-10 (CG form)", "timing": "host clock, synchronized"}
+            "ppe": ppe[preconditioner], "timing": "host clock, synchronized"}
 
 
 def profile_mode(args):
@@ -963,6 +968,9 @@ def profile_mode(args):
     elif args.profile == "batch":
         log(f"profile 48 uneven bubbles, 1 + {args.steps} batched steps: "
             f"{uneven_bubble_batch_bench(steps=args.steps)}")
+    elif args.profile == "batch_mg":
+        log(f"profile 48 uneven bubbles, multigrid FCG, 1 + {args.steps} batched steps: "
+            f"{uneven_bubble_batch_bench(steps=args.steps, preconditioner='mg')}")
     elif args.profile == "ppe":
         gp, rho, rhs = ppe_case(N=128)
         r = ppe_solve_timed(gp, rho, rhs, "fp64")
@@ -1017,7 +1025,8 @@ def main():
     ap.add_argument("--ppe-n", type=int, default=256)
     ap.add_argument("--train-problems", type=int, default=100,
                     help="random 129^3 bubble problems used to train the learned coarse space")
-    ap.add_argument("--profile", choices=["ccd", "ppe", "mg", "fccd", "n3", "legacy", "batch"],
+    ap.add_argument("--profile", choices=["ccd", "ppe", "mg", "fccd", "n3", "legacy", "batch",
+                                         "batch_mg"],
                     default=None,
                     help="run only one workload (for rocprofv3): 'ccd' = K all-axes CCD steps "
                          "on 257^3 plus one 1 GiB device copy (counter scale reference); "
@@ -1027,7 +1036,8 @@ def main():
                          "'n3' = K calls each of the Rhie-Chow divergence, the DCCD step, "
                          "the psi curvature and the split reinitialisation on 257^3; "
                          "'legacy' = 2 + K steps of the 2-D 64^2 legacy rising-bubble loop; "
-                         "'batch' = 1 + K batched steps of the 48 uneven 2-D bubbles")
+                         "'batch' = 1 + K batched steps of the 48 uneven 2-D bubbles "
+                         "(Jacobi-PCG PPE); 'batch_mg' = the same with multigrid FCG")
     ap.add_argument("--cpu-protocol", action="store_true",
                     help="only the CPU PPE protocol: full oracle Jacobi-CG solves to 1e-8 of the "
                          "65^3 and 129^3 ratio-1000 cases (all cores of the job's share, then "
@@ -1260,6 +1270,8 @@ def main():
         log("48 uneven 2-D rising bubbles stepped together (batched PPE)")
         result["uneven_bubble_batch_48"] = uneven_bubble_batch_bench()
         log(f"  {result['uneven_bubble_batch_48']}")
+        result["uneven_bubble_batch_48_mg"] = uneven_bubble_batch_bench(preconditioner="mg")
+        log(f"  {result['uneven_bubble_batch_48_mg']}")
 
     if not args.no_cpu:
         log("CPU baseline (dense-LU CCD slab)")

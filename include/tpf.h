@@ -239,6 +239,20 @@ TPF_API tpf_status tpf_ppe_solve_mg_deflated(tpf_ppe* ppe, const double* rhs, do
                                              int level, const double* einv,
                                              tpf_solve_report* rep);
 
+/* tpf_ppe_solve_mg on `count` distinct handles at once (no counterpart in
+ * the reference; replaces a loop of multigrid solves, one per simulation of
+ * a batch, as in the per-simulation PPE calls of fvm_matrixfree.py:150-260).
+ * Every round enqueues the next step of each unfinished problem on its own
+ * handle's stream and then reads all their scalars, so the solves overlap
+ * on the GPU instead of each waiting on the host once per iteration.  Each
+ * problem runs the kernels of its single solve in the same order: p[i]
+ * equals tpf_ppe_solve_mg(ppes[i], rhs[i], p[i], opts) bit for bit.
+ * reports[count] filled.  Synchronous.  TPF_ERR_RUNTIME names the problems
+ * that ended above tol (the others are solved). */
+TPF_API tpf_status tpf_ppe_solve_mg_many(tpf_ppe* const* ppes, int count,
+                                         const double* const* rhs, double* const* p,
+                                         const tpf_mg_opts* opts, tpf_solve_report* reports);
+
 /* FP64 ||b - A p||_2 / ||b||_2 on the CG system (the judge of every solve). */
 TPF_API tpf_status tpf_ppe_true_relres(tpf_ppe* ppe, const double* rhs,
                                const double* p, double* relres);

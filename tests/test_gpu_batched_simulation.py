@@ -82,3 +82,38 @@ def test_batched_equals_single_runs(inject):
     assert batch.retries == (1 if inject else 0)
     assert batch.steps_taken == len(CASES) * STEPS
     assert batch.step(t_end=0.0) == [None] * len(CASES)
+
+
+def test_batched_multigrid_equals_single_runs():
+    """With ppe_preconditioner="mg" the batch solves every PPE with
+    tpf_ppe_solve_mg_many; each simulation still follows its own run bit for
+    bit, and a failure injected into one of them takes the retry path."""
+    _announce(f"batched simulation, multigrid PPE: {len(CASES)} uneven bubbles x {STEPS} steps")
+    mk = lambda c: bubble_simulation(c, ppe_preconditioner="mg")  # noqa: E731
+    singles = []
+    for i, c in enumerate(CASES):
+        sim = mk(c)
+        if i == 2:
+            _inject_failure(sim, 2)
+        sim._update_properties()
+        sim._update_curvature()
+        dts = []
+        for _ in range(STEPS):
+            dt = sim.cfl_calc.compute([sim.velocity[a] for a in range(2)], sim.mu.data,
+                                      sim.rho.data)
+            dts.append(sim._step_with_retry(dt))
+        singles.append((_state(sim), dts, dict(sim.ppe_solver.last_diagnostics)))
+    sims = [mk(c) for c in CASES]
+    _inject_failure(sims[2], 2)
+    batch = BatchedSimulation(sims)
+    assert batch.ppe_batch.mode == "mg"
+    batch.initialize()
+    dts = [batch.step() for _ in range(STEPS)]
+    for i, sim in enumerate(sims):
+        _compare(_state(sim), singles[i][0], i)
+        assert [d[i] for d in dts] == singles[i][1], i
+        d, ref = sim.ppe_solver.last_diagnostics, singles[i][2]
+        for k in ("iterations", "relres", "true_relres", "refinements"):
+            assert d[k] == ref[k], (i, k, d[k], ref[k])
+        assert d["preconditioner"] == "mg" and d["true_relres"] <= 1e-10
+    assert batch.retries == 1

@@ -1,4 +1,5 @@
-"""Batched Jacobi-PCG (tpf_ppe_batch_*) against single solves.
+"""Batched Jacobi-PCG (tpf_ppe_batch_*) and concurrent multigrid-FCG
+(tpf_ppe_solve_mg_many) against single solves.
 
 Every problem of a batch runs the arithmetic and reduction order of
 tpf_ppe_solve, so its solution and report must equal the single solve's bit
@@ -24,11 +25,12 @@ def _announce(msg):
     print(f"[gpu-case] {msg}", file=sys.stderr, flush=True)
 
 
-def _solver(N, tol=1e-10, maxiter=20000, check_every=32):
+def _solver(N, tol=1e-10, maxiter=20000, check_every=32, pre="jacobi"):
     cfg = SimulationConfig(grid=GridConfig(ndim=len(N), N=tuple(N), L=(1.0,) * len(N)),
                            solver=SolverConfig(pseudo_tol=tol, pseudo_maxiter=maxiter))
     cfg.solver.ppe_iteration_method = "cg"
     cfg.solver.ppe_check_every = check_every
+    cfg.solver.ppe_preconditioner = pre
     return PPESolverFVMMatrixFree(None, cfg, Grid(cfg.grid))
 
 
@@ -134,3 +136,93 @@ def test_batch_rejects_mismatched_solvers():
     a, b = _solver((8, 8)), _solver((8, 8), tol=1e-8)
     with pytest.raises(ValueError, match="share"):
         PPEBatch([a, b])
+
+
+def _single_any(s, rho, rhs, x0=None):
+    """A single solve that may end above tol (its result is still compared)."""
+    p = torch.empty_like(rhs) if x0 is None else x0.clone()
+    try:
+        s.solve_into(rhs, rho, p, use_x0=x0 is not None)
+    except RuntimeError:
+        pass
+    return p, dict(s.last_diagnostics)
+
+
+def test_multigrid_batch_equals_single_solves():
+    """Every problem of a multigrid batch runs the kernels of its own
+    tpf_ppe_solve_mg on its own stream: solutions and reports equal the
+    single solves' bit for bit (uneven 2-D and 3-D grids, ratio up to 1000,
+    a zero right-hand side, initial guesses, a skipped problem, a problem
+    that reaches maxiter above tol)."""
+    _announce("ppe batch, multigrid FCG: uneven problems vs single solves")
+    cases = CASES + [((24, 24), 5.0)]
+    solvers = [_solver(N, pre="mg") for N, _ in cases]
+    probs = [_problem(s.shape, ratio, 20 + i, zero_rhs=(i == len(cases) - 1))
+             for i, (s, (_, ratio)) in enumerate(zip(solvers, cases))]
+    singles = [_single(s, rho, rhs) for s, (rho, rhs) in zip(solvers, probs)]
+    batch = PPEBatch(solvers)
+    assert batch.mode == "mg"
+    for s, (rho, _) in zip(solvers, probs):
+        s.prepare_operator(rho)
+    xs = [torch.full_like(rhs, float("nan")) for _, rhs in probs]
+    reps = batch.solve_into([rhs for _, rhs in probs], xs)
+    for i, ((p, d), x, rep) in enumerate(zip(singles, xs, reps)):
+        assert torch.equal(p, x), f"problem {i}: max diff {(p - x).abs().max().item():.3e}"
+        for k in ("iterations", "relres", "true_relres", "refinements"):
+            assert rep[k] == d[k], (i, k)
+        assert rep["converged"] and rep["true_relres"] <= 1e-10
+    assert torch.count_nonzero(xs[-1]).item() == 0 and reps[-1]["iterations"] == 0
+    # initial guesses, and the second problem sits out
+    gen = torch.Generator(device="cuda").manual_seed(9)
+    x0 = [torch.randn(s.shape, dtype=torch.float64, device="cuda", generator=gen)
+          for s in solvers]
+    ref = [_single(s, rho, rhs, x) for s, (rho, rhs), x in zip(solvers, probs, x0)]
+    for s, (rho, _) in zip(solvers, probs):
+        s.prepare_operator(rho)
+    xs = [x.clone() for x in x0]
+    keep = xs[1].clone()
+    rhs = [r for _, r in probs]
+    rhs[1] = None
+    reps = batch.solve_into(rhs, [x if r is not None else None for x, r in zip(xs, rhs)],
+                            use_x0=True)
+    assert reps[1] is None and torch.equal(xs[1], keep)
+    for i in (0, 2, 3, 4, 5):
+        assert torch.equal(xs[i], ref[i][0]), f"initial guess, problem {i}"
+        assert reps[i]["iterations"] == ref[i][1]["iterations"], i
+
+
+def test_multigrid_batch_reports_failing_problem():
+    """maxiter 3: the easy problem starts from its converged solution (another
+    solver's), the hard one from zero and ends above tol; only the hard one
+    is reported, and both equal their single solves."""
+    _announce("ppe batch, multigrid FCG: maxiter failure isolated to one problem")
+    easy = _solver((16, 16), maxiter=3, pre="mg")
+    hard = _solver((64, 64), maxiter=3, pre="mg")
+    pe, ph = _problem(easy.shape, 2.0, 1), _problem(hard.shape, 1000.0, 2)
+    x0e, _ = _single(_solver((16, 16), maxiter=200, pre="mg"), *pe)
+    x0h = torch.zeros_like(ph[1])
+    se, sh = _single_any(easy, *pe, x0=x0e), _single_any(hard, *ph, x0=x0h)
+    assert se[1]["true_relres"] <= 1e-10 and sh[1]["true_relres"] > 1e-10
+    batch = PPEBatch([easy, hard])
+    easy.prepare_operator(pe[0])
+    hard.prepare_operator(ph[0])
+    xs = [x0e.clone(), x0h.clone()]
+    reps = batch.solve_into([pe[1], ph[1]], xs, use_x0=True, raise_on_failure=False)
+    assert reps[0]["converged"] and not reps[1]["converged"]
+    assert reps[1]["iterations"] == 3 == sh[1]["iterations"]
+    assert torch.equal(se[0], xs[0]) and torch.equal(sh[0], xs[1])
+    xs = [x0e.clone(), x0h.clone()]
+    with pytest.raises(RuntimeError, match="problem\\(s\\) 1"):
+        batch.solve_into([pe[1], ph[1]], xs, use_x0=True)
+
+
+def test_multigrid_batch_rejects_mixed_solvers():
+    with pytest.raises(ValueError, match="'mg'"):
+        PPEBatch([_solver((8, 8), pre="mg"), _solver((8, 8))])
+    with pytest.raises(ValueError, match="'mg'"):
+        PPEBatch([_solver((8, 8)), _solver((8, 8), pre="mg")])
+    with pytest.raises(ValueError, match="share"):
+        PPEBatch([_solver((8, 8), pre="mg"), _solver((8, 8), tol=1e-8, pre="mg")])
+    s = _solver((8, 8), pre="mg")
+    with pytest.raises(ValueError, match="twice"):
+        PPEBatch([s, s])

@@ -81,6 +81,9 @@ SIGNATURES = {
     "tpf_ppe_solve_mg": (C.c_int, [_vp, _vp, _vp, C.POINTER(MgOpts), C.POINTER(SolveReport)]),
     "tpf_ppe_solve_mg_deflated": (C.c_int, [_vp, _vp, _vp, C.POINTER(MgOpts), _vp, C.c_int,
                                             C.c_int, _dp, C.POINTER(SolveReport)]),
+    "tpf_ppe_solve_mg_many": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(_vp),
+                                        C.POINTER(_vp), C.POINTER(MgOpts),
+                                        C.POINTER(SolveReport)]),
     "tpf_fccd_ppe_create": (C.c_int, [C.POINTER(_vp), C.c_int, _i64p, _dp, _ip, C.c_int64,
                                       _vp]),
     "tpf_fccd_ppe_destroy": (C.c_int, [_vp]),

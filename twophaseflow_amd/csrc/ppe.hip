@@ -1157,6 +1157,162 @@ static tpf_status solve_mg(tpf_ppe* s, const double* rhs, double* x, const tpf_m
 }
 }  // namespace tpf
 
+// ---- many multigrid-FCG solves advanced together ----------------------------
+//
+// tpf_ppe_solve_mg on every handle, with the solves interleaved: each round
+// enqueues the next step of every unfinished problem on that problem's own
+// stream (restart set-up, one FCG iteration, or the FP64 true residual) with
+// an asynchronous copy of its scalars, then waits on the streams and updates
+// each problem's state machine.  A problem runs exactly the kernels of its
+// single solve in the same order on its own stream, so its result equals
+// tpf_ppe_solve_mg bit for bit; the streams of different problems overlap,
+// where single solves wait on the host once per iteration each.
+extern "C" tpf_status tpf_ppe_solve_mg_many(tpf_ppe* const* handles, int n,
+                                            const double* const* rhs, double* const* x,
+                                            const tpf_mg_opts* o, tpf_solve_report* reps) {
+  using namespace tpf;
+  if (!handles || n < 0 || !rhs || !x || !o || !reps)
+    return fail(TPF_ERR_VALUE, "null argument");
+  if (!(o->tol > 0.0) || o->maxiter < 1) return fail(TPF_ERR_VALUE, "bad tol/maxiter");
+  if (o->gamma < 1 || o->gamma > 2 || o->nu < 1 || o->coarse_sweeps < 1)
+    return fail(TPF_ERR_VALUE, "bad multigrid cycle options (gamma 1|2, nu >= 1, sweeps >= 1)");
+  enum Phase { kInit, kIter, kTrue, kDone };
+  struct Prob {
+    tpf_ppe* s = nullptr;
+    FcgState* dst = nullptr;
+    FcgState* hst = nullptr;
+    int64_t iters = 0;
+    double rr = 0.0, bb = 0.0, tr = 0.0;
+    int restarts = 0;
+    bool conv = false, pending = false;
+    Phase phase = kDone;
+  };
+  std::vector<Prob> P((size_t)n);
+  const int max_levels = o->max_levels > 0 ? o->max_levels : 12;
+  const double tol2 = o->tol * o->tol;
+  for (int i = 0; i < n; ++i) {
+    reps[i] = tpf_solve_report{};
+    tpf_ppe* s = handles[i];
+    if (!s || !rhs[i] || !x[i])
+      return fail(TPF_ERR_VALUE, "null handle, rhs or solution for problem " + std::to_string(i));
+    for (int j = 0; j < i; ++j)
+      if (handles[j] == s) return fail(TPF_ERR_VALUE, "a handle appears twice in one call");
+    if (!s->rho) return fail(TPF_ERR_RUNTIME, "PPE density not set (call tpf_ppe_set_density)");
+    TPF_TRY(mg_ensure(s, max_levels));
+    TPF_TRY(join_user_stream(s));
+    if (!s->fcg_st) {
+      TPF_HIP(hipMalloc(&s->fcg_st, sizeof(FcgState)));
+      TPF_HIP(hipHostMalloc(&s->fcg_host, sizeof(FcgState)));
+    }
+    Prob& q = P[(size_t)i];
+    q.s = s;
+    q.dst = static_cast<FcgState*>(s->fcg_st);
+    q.hst = static_cast<FcgState*>(s->fcg_host);
+    q.phase = kInit;
+    if (o->use_x0) hipLaunchKernelGGL(k_set_one, dim3(1), dim3(64), 0, s->stream, x[i], s->g.pin, 0.0);
+  }
+  for (;;) {
+    bool any = false;
+    for (int i = 0; i < n; ++i) {  // enqueue every problem's next step
+      Prob& q = P[(size_t)i];
+      if (q.phase == kDone) continue;
+      tpf_ppe* s = q.s;
+      hipStream_t st = s->stream;
+      const int64_t nn = s->g.total;
+      const dim3 grid(s->grid), blk(kBlock);
+      float* r32 = mg_fine_b(s);
+      const float* z32 = mg_fine_x(s);
+      if (q.phase == kInit) {
+        const int zero_x = (q.restarts == 0 && !o->use_x0) ? 1 : 0;
+        if (zero_x) hipLaunchKernelGGL(k_fill_zero, grid, blk, 0, st, nn, x[i]);
+        hipLaunchKernelGGL(k_fcg_init, grid, blk, 0, st, s->g, rhs[i], s->rho, x[i], s->r, r32,
+                           0, s->partials);
+        hipLaunchKernelGGL(k_fcg_fin_init, dim3(1), blk, 0, st, q.dst,
+                           (const double*)s->partials, s->grid);
+        TPF_HIP(hipMemcpyAsync(q.hst, q.dst, sizeof(FcgState), hipMemcpyDeviceToHost, st));
+      } else if (q.phase == kIter) {
+        TPF_TRY(mg_cycle(s, st, o->nu, o->gamma, o->coarse_sweeps, o->w_levels));
+        hipLaunchKernelGGL(k_fcg_zq, grid, blk, 0, st, nn, z32, (const double*)s->q, s->partials);
+        hipLaunchKernelGGL(k_fcg_fin_beta, dim3(1), blk, 0, st, q.dst,
+                           (const double*)s->partials, s->grid);
+        hipLaunchKernelGGL(k_fcg_p, grid, blk, 0, st, nn, z32, s->p, (const FcgState*)q.dst);
+        hipLaunchKernelGGL(k_fcg_matvec, grid, blk, 0, st, s->g, (const double*)s->p, s->rho,
+                           (const double*)s->r, s->q, s->partials);
+        hipLaunchKernelGGL(k_fcg_fin_alpha, dim3(1), blk, 0, st, q.dst,
+                           (const double*)s->partials, s->grid);
+        hipLaunchKernelGGL(k_fcg_update, grid, blk, 0, st, nn, (const double*)s->p,
+                           (const double*)s->q, x[i], s->r, r32, (const FcgState*)q.dst,
+                           s->partials);
+        hipLaunchKernelGGL(k_fcg_fin_r, dim3(1), blk, 0, st, q.dst, (const double*)s->partials,
+                           s->grid);
+        TPF_HIP(hipMemcpyAsync(q.hst, q.dst, sizeof(FcgState), hipMemcpyDeviceToHost, st));
+      } else {  // kTrue: the FP64 true residual of the CG system judges the result
+        hipLaunchKernelGGL(k_set_one, dim3(1), dim3(64), 0, st, x[i], s->g.pin, 0.0);
+        hipLaunchKernelGGL(k_true_res, grid, blk, 0, st, s->g, rhs[i], s->rho,
+                           (const double*)x[i], s->st, s->partials);
+        hipLaunchKernelGGL(k_fin_true, dim3(1), blk, 0, st, s->st, (const double*)s->partials,
+                           s->grid);
+        TPF_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+      }
+      TPF_HIP(hipGetLastError());
+      q.pending = true;
+      any = true;
+    }
+    if (!any) break;
+    for (int i = 0; i < n; ++i) {  // wait and advance every state machine
+      Prob& q = P[(size_t)i];
+      if (!q.pending) continue;
+      q.pending = false;
+      tpf_ppe* s = q.s;
+      TPF_HIP(hipStreamSynchronize(s->stream));
+      if (q.phase == kInit) {
+        q.rr = q.hst->rr;
+        q.bb = q.hst->bb;
+        if (q.bb == 0.0) {  // zero right-hand side: x = 0
+          hipLaunchKernelGGL(k_fill_zero, dim3(s->grid), dim3(kBlock), 0, s->stream, s->g.total,
+                             x[i]);
+          TPF_HIP(hipGetLastError());
+          q.conv = true;
+          q.tr = 0.0;
+          q.phase = kDone;
+          continue;
+        }
+        q.phase = (q.rr <= tol2 * q.bb || q.iters >= o->maxiter) ? kTrue : kIter;
+      } else if (q.phase == kIter) {
+        ++q.iters;
+        q.rr = q.hst->rr;
+        if (q.rr <= tol2 * q.bb || q.iters >= o->maxiter) q.phase = kTrue;
+      } else {
+        q.tr = s->st_host->true_bb > 0.0 ? std::sqrt(s->st_host->true_rr / s->st_host->true_bb)
+                                         : 0.0;
+        q.conv = q.tr <= o->tol;
+        if (q.conv || q.iters >= o->maxiter || q.restarts >= 4) {
+          q.phase = kDone;
+        } else {
+          ++q.restarts;
+          q.phase = kInit;
+        }
+      }
+    }
+  }
+  std::string failed;
+  for (int i = 0; i < n; ++i) {
+    const Prob& q = P[(size_t)i];
+    TPF_HIP(hipStreamSynchronize(q.s->stream));  // zero right-hand sides' fills
+    tpf_solve_report& r = reps[i];
+    r.iterations = q.iters;
+    r.relres = q.bb > 0.0 ? std::sqrt(q.rr / q.bb) : 0.0;
+    r.true_relres = q.tr;
+    r.converged = q.conv ? 1 : 0;
+    r.refinements = q.restarts;
+    if (!q.conv) failed += (failed.empty() ? "" : ", ") + std::to_string(i);
+  }
+  if (!failed.empty())
+    return fail(TPF_ERR_RUNTIME, "multigrid FCG did not reach tol for problem(s) " + failed +
+                                     " (maxiter " + std::to_string(o->maxiter) + ")");
+  return TPF_OK;
+}
+
 // ---- batched Jacobi-PCG: independent problems iterated together ------------
 //
 // Every kernel of one batched iteration covers all problems: workgroup blk

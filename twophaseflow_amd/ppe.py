@@ -292,14 +292,22 @@ class PPESolverFVMMatrixFree(IPPESolver):
 
 
 class PPEBatch:
-    """Jacobi-PCG over several ``PPESolverFVMMatrixFree`` solvers at once
-    (CG form, FP64): one launch per PCG step covers every problem and each
-    problem's result equals its own ``solve_into`` bit for bit.  The
+    """Several ``PPESolverFVMMatrixFree`` solvers (CG form) solved at once;
+    each problem's result equals its own ``solve_into`` bit for bit.  The
     reference solves such problems one after another
     (``fvm_matrixfree.py:150-260`` per simulation).
 
-    The solvers must share tol, maxiter and the check period.  Densities are
-    set per solver with ``prepare_operator`` before ``solve_into``."""
+    Two modes, chosen from the solvers:
+
+    - Jacobi-PCG in FP64 (``ppe_preconditioner="jacobi"``): one launch per
+      PCG step covers every problem (``tpf_ppe_batch_solve``).
+    - multigrid FCG (``ppe_preconditioner="mg"``): every problem keeps its
+      own stream and hierarchy, and the solves advance together round by
+      round (``tpf_ppe_solve_mg_many``).
+
+    The solvers must share tol, maxiter and the rest of their iteration
+    options.  Densities are set per solver with ``prepare_operator`` before
+    ``solve_into``."""
 
     def __init__(self, solvers):
         solvers = list(solvers)
@@ -309,21 +317,37 @@ class PPEBatch:
         for s in solvers:
             if not isinstance(s, PPESolverFVMMatrixFree):
                 raise TypeError("PPEBatch takes PPESolverFVMMatrixFree solvers")
-            if s.iteration_method != "cg" or s.preconditioner != "jacobi" or \
-                    s.precision != "fp64":
-                raise ValueError("PPEBatch runs the CG form with Jacobi preconditioning in "
-                                 "FP64 (ppe_iteration_method='cg', ppe_preconditioner="
-                                 "'jacobi', ppe_precision='fp64')")
-            if (s.tol, s.maxiter, s.check_every) != (s0.tol, s0.maxiter, s0.check_every):
-                raise ValueError("solvers of one PPEBatch must share pseudo_tol, "
-                                 "pseudo_maxiter and ppe_check_every")
+        self.mode = "mg" if s0.preconditioner == "mg" else "jacobi"
+        for s in solvers:
+            if self.mode == "mg":
+                if s.preconditioner != "mg":
+                    raise ValueError("solvers of one PPEBatch must all use ppe_preconditioner"
+                                     "='mg' or none of them")
+                key = lambda t: (t.tol, t.maxiter, t.mg_smooth, t.mg_gamma,  # noqa: E731
+                                 t.mg_coarse_sweeps, t.mg_w_levels)
+                what = ("pseudo_tol, pseudo_maxiter and the ppe_mg_* cycle options")
+            else:
+                if s.iteration_method != "cg" or s.preconditioner != "jacobi" or \
+                        s.precision != "fp64":
+                    raise ValueError("PPEBatch runs the CG form with Jacobi preconditioning in "
+                                     "FP64 (ppe_iteration_method='cg', ppe_preconditioner="
+                                     "'jacobi', ppe_precision='fp64') or multigrid FCG "
+                                     "(ppe_preconditioner='mg')")
+                key = lambda t: (t.tol, t.maxiter, t.check_every)  # noqa: E731
+                what = "pseudo_tol, pseudo_maxiter and ppe_check_every"
+            if key(s) != key(s0):
+                raise ValueError(f"solvers of one PPEBatch must share {what}")
+        if len({id(s) for s in solvers}) != len(solvers):
+            raise ValueError("a solver appears twice in one PPEBatch")
         self.solvers = solvers
         self._lib = nat.lib()
-        h = C.c_void_p()
-        nat.check(self._lib.tpf_ppe_batch_create(
-            C.byref(h), nat.parr([s._handle.value for s in solvers]), len(solvers),
-            stream_handle()))
-        self._handle = h
+        self._handle = None
+        if self.mode == "jacobi":
+            h = C.c_void_p()
+            nat.check(self._lib.tpf_ppe_batch_create(
+                C.byref(h), nat.parr([s._handle.value for s in solvers]), len(solvers),
+                stream_handle()))
+            self._handle = h
         self.last_reports: list = []
 
     def __len__(self):
@@ -337,11 +361,12 @@ class PPEBatch:
 
     def solve_into(self, rhs, p, use_x0: bool = False, raise_on_failure: bool = True):
         """Solves problem i for p[i] (CUDA tensors of solver i's grid shape;
-        rhs[i] = p[i] = None leaves problem i out of this call).  Returns the per-problem diagnostics (the keys of
-        ``PPESolverFVMMatrixFree.solve_into``) and stores them in each
-        solver's ``last_diagnostics``.  With ``raise_on_failure`` a problem
-        that ended above tol raises RuntimeError naming it; otherwise its
-        diagnostics carry ``converged=False`` and the caller decides."""
+        rhs[i] = p[i] = None leaves problem i out of this call).  Returns the
+        per-problem diagnostics (the keys of ``PPESolverFVMMatrixFree.solve_into``
+        plus ``converged``) and stores them in each solver's
+        ``last_diagnostics``.  With ``raise_on_failure`` a problem that ended
+        above tol raises RuntimeError naming it; otherwise its diagnostics
+        carry ``converged=False`` and the caller decides."""
         n = len(self.solvers)
         if len(rhs) != n or len(p) != n:
             raise ValueError(f"expected {n} right-hand sides and solutions")
@@ -358,13 +383,29 @@ class PPEBatch:
             if s._rho is None:
                 raise RuntimeError("prepare_operator(rho) must be called on every solver")
         s0 = self.solvers[0]
-        opts = nat.SolveOpts(tol=s0.tol, maxiter=s0.maxiter, precond=1,
-                             check_every=s0.check_every, use_x0=1 if use_x0 else 0, mixed=0,
-                             inner_tol=0.0)
         reps = (nat.SolveReport * n)()
-        status = self._lib.tpf_ppe_batch_solve(self._handle, nat.parr([ptr(t) for t in rhs]),
-                                               nat.parr([ptr(t) for t in p]), C.byref(opts),
-                                               reps)
+        if self.mode == "mg":
+            on = [i for i in range(n) if rhs[i] is not None]
+            sub = (nat.SolveReport * max(1, len(on)))()
+            status = nat.TPF_OK
+            if on:
+                mopts = nat.MgOpts(tol=s0.tol, maxiter=s0.maxiter, use_x0=1 if use_x0 else 0,
+                                   nu=s0.mg_smooth, gamma=s0.mg_gamma,
+                                   coarse_sweeps=s0.mg_coarse_sweeps, max_levels=0,
+                                   w_levels=s0.mg_w_levels)
+                status = self._lib.tpf_ppe_solve_mg_many(
+                    nat.parr([self.solvers[i]._handle.value for i in on]), len(on),
+                    nat.parr([ptr(rhs[i]) for i in on]), nat.parr([ptr(p[i]) for i in on]),
+                    C.byref(mopts), sub)
+            for j, i in enumerate(on):
+                reps[i] = sub[j]
+        else:
+            opts = nat.SolveOpts(tol=s0.tol, maxiter=s0.maxiter, precond=1,
+                                 check_every=s0.check_every, use_x0=1 if use_x0 else 0,
+                                 mixed=0, inner_tol=0.0)
+            status = self._lib.tpf_ppe_batch_solve(self._handle, nat.parr([ptr(t) for t in rhs]),
+                                                   nat.parr([ptr(t) for t in p]), C.byref(opts),
+                                                   reps)
         if status not in (nat.TPF_OK, nat.TPF_ERR_RUNTIME):
             nat.check(status)
         out = []
@@ -372,12 +413,13 @@ class PPEBatch:
             if t is None:
                 out.append(None)
                 continue
-            ok = r.relres <= s.tol
+            ok = bool(r.converged) if self.mode == "mg" else r.relres <= s.tol
             d = {"ppe_low_order_cg_info": 0 if ok else int(r.iterations),
                  "ppe_low_order_cg_tol": s.tol, "ppe_low_order_cg_maxiter": s.maxiter,
                  "iterations": int(r.iterations), "relres": float(r.relres),
-                 "true_relres": float(r.true_relres), "refinements": 0,
-                 "precision": "fp64", "preconditioner": "jacobi", "converged": ok}
+                 "true_relres": float(r.true_relres), "refinements": int(r.refinements),
+                 "precision": s.precision, "preconditioner": s.preconditioner,
+                 "converged": ok}
             s.last_diagnostics = d
             out.append(d)
         self.last_reports = out

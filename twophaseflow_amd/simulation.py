@@ -371,10 +371,12 @@ class BatchedSimulation:
     """Steps independent legacy simulations together (uneven grids allowed).
 
     Each step runs, per simulation, the CFL time step and steps 1-5 plus the
-    Rhie-Chow right-hand side (``_step_pre_ppe``), then ONE batched
-    Jacobi-PCG solve of every simulation's PPE (``PPEBatch``: one launch per
-    PCG step covers all problems, chunks of iterations replay as a captured
-    graph), then the corrector (``_step_post_ppe``) and the non-finite check.
+    Rhie-Chow right-hand side (``_step_pre_ppe``), then ONE batched solve of
+    every simulation's PPE (``PPEBatch``: with Jacobi-PCG one launch per PCG
+    step covers all problems and chunks of iterations replay as a captured
+    graph; with ``ppe_preconditioner="mg"`` the multigrid-FCG solves advance
+    together, each on its own stream), then the corrector
+    (``_step_post_ppe``) and the non-finite check.
     A simulation whose attempt fails (PPE above tol, non-finite state, an
     exception) is rolled back and continues the run loop's dt-halving
     sequence on its own from the second attempt (legacy_run_loop.py:45-67),
@@ -382,8 +384,9 @@ class BatchedSimulation:
     the batched PPE solve equals the single solve bit for bit.
 
     The reference steps such simulations one after another; the simulations
-    must use the CG form of the FVM matrix-free PPE with Jacobi
-    preconditioning in FP64 and share its tolerance settings."""
+    must use the CG form of the FVM matrix-free PPE, all with Jacobi
+    preconditioning in FP64 or all with multigrid FCG, and share its
+    tolerance settings."""
 
     def __init__(self, sims):
         from .ppe import PPEBatch
@@ -465,15 +468,16 @@ class BatchedSimulation:
             k += 1
 
 
-def bubble_simulation(case: dict, pseudo_tol: float = 1e-10,
-                      pseudo_maxiter: int = 20000) -> TwoPhaseSimulation:
+def bubble_simulation(case: dict, pseudo_tol: float = 1e-10, pseudo_maxiter: int = 20000,
+                      ppe_preconditioner: str = "jacobi") -> TwoPhaseSimulation:
     """A legacy 2-D rising-bubble simulation of one ``uneven_bubble_batch``
     case: the bubble_2d parameters of the reference (configs/bubble_2d.yaml:
     Re 35, Fr 1, mu ratio 0.1, CFL 0.25, epsilon_factor 1.5, DCCD
     advection, PPE tolerance 1e-10) on the case's N x N grid of [0,1]^2,
     with the case's radius, density ratio and We; the PPE in CG form with
-    Jacobi preconditioning (iteration cap `pseudo_maxiter`), the null field
-    extension."""
+    `ppe_preconditioner` ("jacobi": the reference's Jacobi-PCG; "mg":
+    flexible CG with one multigrid cycle per iteration) and iteration cap
+    `pseudo_maxiter`, the null field extension."""
     from .config import FluidConfig, GridConfig, NumericsConfig, SimulationConfig, SolverConfig
     from .fields import bubble_psi_2d
     N = int(case["N"])
@@ -486,6 +490,7 @@ def bubble_simulation(case: dict, pseudo_tol: float = 1e-10,
         solver=SolverConfig(ppe_solver_type="fvm_iterative", pseudo_tol=pseudo_tol,
                             pseudo_maxiter=pseudo_maxiter))
     cfg.solver.ppe_iteration_method = "cg"
+    cfg.solver.ppe_preconditioner = ppe_preconditioner
     sim = SimulationBuilder(cfg).build()
     sim.psi.data = bubble_psi_2d(N, float(case["radius"]))
     return sim
