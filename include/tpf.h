@@ -121,6 +121,19 @@ TPF_API tpf_status tpf_ccd_plan_set_launch(tpf_ccd_plan* plan, int axis, int mod
  * the first axis' value. */
 TPF_API tpf_status tpf_ccd_plan_set_store_align(tpf_ccd_plan* plan, int axis, int align);
 
+/* Grid metric of a non-uniform axis: replaces apply_ccd_metric
+ * (ccd/ccd_solver_helpers.py:509-519) as CCDSolver applies it on grids that
+ * are not uniform (ccd_solver.py:126-130, 169-170).  J[j] = dxi/dx and
+ * dJ[j] = dJ/dxi at the axis' n_nodes[axis] nodes (host arrays, copied);
+ * the plan's h[axis] must then be the computational spacing 1/N.  Every
+ * later call of the plan returns d1 = J d1_xi and d2 = J^2 d2_xi +
+ * J dJ d1_xi on that axis (a d2-only call solves d1_xi into a plan-owned
+ * scratch array, allocated on first use; such calls of one plan on two
+ * streams race).  J = dJ = NULL removes the metric.  Batches reject plans
+ * with a metric. */
+TPF_API tpf_status tpf_ccd_plan_set_metric(tpf_ccd_plan* plan, int axis, const double* J,
+                                           const double* dJ);
+
 /* Batches of independent CCD problems (no counterpart in the reference; the
  * batching extension).  A batch fixes `count` plans and the axes to
  * differentiate per plan; every (problem, axis) pair with the same launch

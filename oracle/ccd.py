@@ -12,6 +12,9 @@ Restates the reference CCD path (paths relative to ``src/twophase``):
 * periodic differentiate ``ccd/ccd_solver_helpers.py:426-465``
 * block-Thomas template  ``ccd/block_tridiag.py:29-175``
 * wall Neumann zeroing   ``ccd/ccd_boundary_helpers.py:52-61``
+* grid metric            ``core/metrics.py:22-88`` (non-uniform branch),
+  ``ccd/ccd_solver_helpers.py:509-519`` (``apply_ccd_metric``) and the
+  computational spacing 1/N of non-uniform grids (``ccd/ccd_solver.py:126-130``)
 
 The dense systems are solved with ``scipy.linalg.lu_factor``/``lu_solve``
 exactly as the reference CPU path does (``ccd_solver_helpers.py:17, 270, 422,
@@ -355,19 +358,52 @@ def axis_kinds(bc_type, ndim: int):
     raise ValueError(f"unsupported boundary type {bc_type!r} for ndim={ndim}")
 
 
-class OracleCCD:
-    """CPU CCD with the reference ``CCDSolver`` call surface (uniform grids)."""
+def metric_from_coords(coords):
+    """J = dxi/dx and dJ/dxi of one axis of a non-uniform grid from its node
+    coordinates (core/metrics.py:55-60): x(xi) differentiated by the wall CCD
+    solve with the computational spacing 1/N (ccd_solver.py:174-203), then
+    J = 1 / x_xi and dJ/dxi = -x_xixi / x_xi^2."""
+    x = np.asarray(coords, dtype=np.float64).reshape(-1)
+    N = x.shape[0] - 1
+    d1, d2 = differentiate_wall(x, 0, build_wall_axis(N + 1, 1.0 / N))
+    return 1.0 / d1, -d2 / (d1 ** 2)
 
-    def __init__(self, N, L, bc_type="wall"):
+
+def apply_ccd_metric(d1_xi, d2_xi, J, dJ, axis: int):
+    """ccd_solver_helpers.py:509-519: d1_x = J d1_xi and
+    d2_x = J J d2_xi + J dJ d1_xi, J and dJ broadcast along ``axis``."""
+    shape = [1] * np.ndim(d1_xi)
+    shape[axis] = -1
+    J = np.asarray(J).reshape(shape)
+    dJ = np.asarray(dJ).reshape(shape)
+    return J * d1_xi, J * J * d2_xi + J * dJ * d1_xi
+
+
+class OracleCCD:
+    """CPU CCD with the reference ``CCDSolver`` call surface.  With
+    ``coords`` (one coordinate vector or None per axis) the grid is
+    non-uniform as the reference's ``alpha_grid > 1``: every axis is solved
+    in xi with spacing 1/N and mapped by its metric (axes given None keep
+    the uniform coordinates, whose metric is 1/L)."""
+
+    def __init__(self, N, L, bc_type="wall", coords=None):
         self.N = tuple(int(n) for n in N)
         self.L = tuple(float(x) for x in L)
         self.ndim = len(self.N)
         self.bc_type = bc_type
         self.kinds = axis_kinds(bc_type, self.ndim)
         self._info = {}
+        self.uniform = coords is None
+        if not self.uniform:
+            self.coords = [np.linspace(0.0, self.L[a], self.N[a] + 1)
+                           if coords[a] is None else np.asarray(coords[a], dtype=np.float64)
+                           for a in range(self.ndim)]
+            m = [metric_from_coords(c) for c in self.coords]
+            self.J = [t[0] for t in m]
+            self.dJ_dxi = [t[1] for t in m]
 
     def h(self, axis):
-        return self.L[axis] / self.N[axis]
+        return self.L[axis] / self.N[axis] if self.uniform else 1.0 / self.N[axis]
 
     def _axis_info(self, axis):
         if axis not in self._info:
@@ -378,17 +414,21 @@ class OracleCCD:
                 self._info[axis] = build_wall_axis(n_pts, self.h(axis))
         return self._info[axis]
 
-    def differentiate(self, data, axis):
+    def differentiate(self, data, axis, apply_metric=True):
         info = self._axis_info(axis)
         if self.kinds[axis] == "periodic":
-            return differentiate_periodic(data, axis, info)
-        return differentiate_wall(data, axis, info)
+            d1, d2 = differentiate_periodic(data, axis, info)
+        else:
+            d1, d2 = differentiate_wall(data, axis, info)
+        if not self.uniform and apply_metric:
+            d1, d2 = apply_ccd_metric(d1, d2, self.J[axis], self.dJ_dxi[axis], axis)
+        return d1, d2
 
-    def first_derivative(self, data, axis):
-        return self.differentiate(data, axis)[0]
+    def first_derivative(self, data, axis, apply_metric=True):
+        return self.differentiate(data, axis, apply_metric)[0]
 
-    def second_derivative(self, data, axis):
-        return self.differentiate(data, axis)[1]
+    def second_derivative(self, data, axis, apply_metric=True):
+        return self.differentiate(data, axis, apply_metric)[1]
 
     def enforce_wall_neumann(self, grad, ax):
         """ccd_boundary_helpers.py:52-61."""

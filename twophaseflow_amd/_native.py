@@ -61,6 +61,7 @@ SIGNATURES = {
     "tpf_ccd_axis_kernel": (C.c_int, [_vp, C.c_int, _ip]),
     "tpf_ccd_plan_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "tpf_ccd_plan_set_store_align": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "tpf_ccd_plan_set_metric": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "tpf_ccd_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.POINTER(C.c_uint32),
                                        C.c_int]),
     "tpf_ccd_batch_apply": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), _vp]),

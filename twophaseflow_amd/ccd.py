@@ -1,9 +1,12 @@
 """``CCDSolver`` on MI355X: the reference call surface over the HIP line kernels.
 
-Mirrors ``ccd/ccd_solver.py:79-403`` for uniform grids:
+Mirrors ``ccd/ccd_solver.py:79-403``:
 ``differentiate(data, axis) -> (d1, d2)``, ``first_derivative``,
-``second_derivative``, ``differentiate_raw``, ``enforce_wall_neumann`` and
-the attributes ``bc_type, grid, backend, xp, ndim``.  Inputs may be NumPy
+``second_derivative`` (each with ``apply_metric``), ``differentiate_raw``,
+``enforce_wall_neumann`` and the attributes ``bc_type, grid, backend, xp,
+ndim``.  On a non-uniform grid (``grid.uniform`` False) the lines are solved
+in xi with spacing 1/N and mapped with the grid's metric ``J``,
+``dJ_dxi`` (``ccd_solver_helpers.py:509-519``).  Inputs may be NumPy
 arrays (copied to and from the device) or CUDA torch tensors (zero copy);
 outputs are new arrays of the input's kind.  Errors follow the reference:
 ``ValueError`` for a wall axis with fewer than 6 nodes
@@ -31,31 +34,68 @@ class CCDSolver:
         self.bc_type = bc_type
         if hasattr(grid, "set_boundary_type"):
             grid.set_boundary_type(bc_type)
-        if not getattr(grid, "uniform", True):
-            raise ValueError("twophaseflow_amd CCDSolver supports uniform grids only")
         self._kinds = axis_kinds(bc_type, self.ndim)
         if precision not in ("fp64", "fp32_refined"):
             raise ValueError("precision must be 'fp64' or 'fp32_refined'")
         self.precision = precision
         self.shape = tuple(int(n) + 1 for n in grid.N)
         self._lib = nat.lib()
-        h = [float(grid.L[a]) / float(grid.N[a]) for a in range(self.ndim)]
-        bcs = [nat.BC_PERIODIC if k == "periodic" else nat.BC_WALL for k in self._kinds]
-        plan = C.c_void_p()
-        nat.check(self._lib.tpf_ccd_plan_create(
-            C.byref(plan), self.ndim, nat.i64arr(self.shape), nat.darr(h), nat.iarr(bcs),
-            nat.PREC_FP64 if precision == "fp64" else nat.PREC_FP32_REFINED))
-        self._plan = plan
+        # compact-stencil spacing (ccd_solver.py:126-130): L/N on uniform
+        # grids, the computational 1/N on non-uniform ones (metric applied)
+        self._uniform = bool(getattr(grid, "uniform", True))
+        self._h = [float(grid.L[a]) / float(grid.N[a]) if self._uniform else 1.0 / float(grid.N[a])
+                   for a in range(self.ndim)]
+        self._bcs = [nat.BC_PERIODIC if k == "periodic" else nat.BC_WALL for k in self._kinds]
+        self._plan = self._new_plan()
+        self._raw = None          # plan without the metric (apply_metric=False)
+        self._metric_key = [None] * self.ndim
         # periodic axes are factorised eagerly in the reference (ccd_solver.py:113-120)
         for a, k in enumerate(self._kinds):
             if k == "periodic":
                 self.axis_info(a)
 
     def __del__(self):
-        plan = getattr(self, "_plan", None)
-        if plan is not None and plan.value:
-            self._lib.tpf_ccd_plan_destroy(plan)
-            self._plan = None
+        for name in ("_plan", "_raw"):
+            plan = getattr(self, name, None)
+            if plan is not None and plan.value:
+                self._lib.tpf_ccd_plan_destroy(plan)
+                setattr(self, name, None)
+
+    def _new_plan(self, shape=None, bcs=None):
+        plan = C.c_void_p()
+        nat.check(self._lib.tpf_ccd_plan_create(
+            C.byref(plan), self.ndim, nat.i64arr(shape or self.shape), nat.darr(self._h),
+            nat.iarr(bcs or self._bcs),
+            nat.PREC_FP64 if self.precision == "fp64" else nat.PREC_FP32_REFINED))
+        return plan
+
+    def _sync_metric(self) -> None:
+        """Non-uniform grids: uploads grid.J / grid.dJ_dxi of every axis whose
+        arrays changed since the last call (the reference reads them on
+        every call, ccd_solver_helpers.py:509-540)."""
+        if self._uniform:
+            return
+        for a in range(self.ndim):
+            J, dJ = self.grid.J[a], self.grid.dJ_dxi[a]
+            key = (id(J), id(dJ))
+            if key == self._metric_key[a]:
+                continue
+            host = [np.ascontiguousarray(np.asarray(t.cpu() if hasattr(t, "cpu") else t,
+                                                    dtype=np.float64).reshape(-1))
+                    for t in (J, dJ)]
+            if any(h.shape[0] != self.shape[a] for h in host):
+                raise ValueError(f"grid metric of axis {a} must have {self.shape[a]} values")
+            dp = C.POINTER(C.c_double)
+            nat.check(self._lib.tpf_ccd_plan_set_metric(
+                self._plan, a, host[0].ctypes.data_as(dp), host[1].ctypes.data_as(dp)))
+            self._metric_key[a] = key
+
+    def _raw_plan(self):
+        if self._uniform:
+            return self._plan
+        if self._raw is None:
+            self._raw = self._new_plan()
+        return self._raw
 
     # -- introspection -------------------------------------------------------
     def axis_info(self, axis: int) -> dict:
@@ -89,15 +129,21 @@ class CCDSolver:
         self._check_axis(axis)
         nat.check(self._lib.tpf_ccd_plan_set_store_align(self._plan, int(axis), int(align)))
 
-    def differentiate_into(self, f: torch.Tensor, d1, d2, axis: int) -> None:
+    def differentiate_into(self, f: torch.Tensor, d1, d2, axis: int,
+                           apply_metric: bool = True) -> None:
         """Zero-copy form: f, d1, d2 are contiguous CUDA float64 tensors of
-        grid.shape (d1 or d2 may be None).  Asynchronous on torch's stream."""
+        grid.shape (d1 or d2 may be None).  Asynchronous on torch's stream.
+        On a non-uniform grid the metric maps the xi derivatives to x unless
+        ``apply_metric`` is False."""
         self._check_axis(axis)
+        self._sync_metric()
+        plan = self._plan if apply_metric else self._raw_plan()
         nat.check(self._lib.tpf_ccd_differentiate(
-            self._plan, ptr(f), ptr(d1), ptr(d2), axis, stream_handle()))
+            plan, ptr(f), ptr(d1), ptr(d2), axis, stream_handle()))
 
     def apply_into(self, f: torch.Tensor, d1, d2, axis_mask: int) -> None:
         """All axes in ``axis_mask`` (bit a = axis a); d1/d2 are (k, *shape)."""
+        self._sync_metric()
         nat.check(self._lib.tpf_ccd_apply(
             self._plan, ptr(f), ptr(d1), ptr(d2), int(axis_mask), stream_handle()))
 
@@ -116,34 +162,38 @@ class CCDSolver:
             if len(ts) != n:
                 raise ValueError("apply_fields_into: one entry per axis")
             return arr(*[ptr(t) if t is not None else None for t in ts])
+        self._sync_metric()
         nat.check(self._lib.tpf_ccd_apply_fields(self._plan, pack(fs), pack(d1s), pack(d2s),
                                                  int(axis_mask), stream_handle()))
 
     # -- reference API -----------------------------------------------------------
     def differentiate(self, data, axis: int, bc_left=None, bc_right=None,
                       apply_metric: bool = True):
-        return self._run(data, axis, bc_left, bc_right, want1=True, want2=True)
+        return self._run(data, axis, bc_left, bc_right, want1=True, want2=True,
+                         apply_metric=apply_metric)
 
     def first_derivative(self, data, axis: int, bc_left=None, bc_right=None,
                          apply_metric: bool = True):
-        return self._run(data, axis, bc_left, bc_right, want1=True, want2=False)[0]
+        return self._run(data, axis, bc_left, bc_right, want1=True, want2=False,
+                         apply_metric=apply_metric)[0]
 
     def second_derivative(self, data, axis: int, bc_left=None, bc_right=None,
                           apply_metric: bool = True):
-        return self._run(data, axis, bc_left, bc_right, want1=False, want2=True)[1]
+        return self._run(data, axis, bc_left, bc_right, want1=False, want2=True,
+                         apply_metric=apply_metric)[1]
 
     def differentiate_raw(self, data, axis: int):
-        """1-D input of length N[axis]+1 (ccd_solver.py:174-203)."""
+        """1-D input of length N[axis]+1: the wall-closure solve in xi with
+        the solver's spacing and no metric, on any axis (ccd_solver.py:
+        174-203; the grid metric is built from it)."""
         arr = data if isinstance(data, torch.Tensor) else np.asarray(data)
         n = self.shape[axis]
         if arr.reshape(-1).shape[0] != n:
             raise ValueError(f"differentiate_raw expects {n} values")
-        if self._kinds[axis] != "wall":
-            raise ValueError("differentiate_raw is the wall-closure path (ccd_solver.py:202)")
         full = [self.shape[a] if a == axis else 1 for a in range(self.ndim)]
-        # A plan over a single line: same spacing and closure along `axis`.
+        # A plan over a single line: same spacing, wall closure along `axis`.
         lib = self._lib
-        h = [float(self.grid.L[a]) / float(self.grid.N[a]) for a in range(self.ndim)]
+        h = list(self._h)
         bcs = [nat.BC_WALL] * self.ndim
         plan = C.c_void_p()
         nat.check(lib.tpf_ccd_plan_create(C.byref(plan), self.ndim, nat.i64arr(full),
@@ -175,7 +225,7 @@ class CCDSolver:
         if not 0 <= axis < self.ndim:
             raise ValueError(f"axis {axis} out of range for ndim={self.ndim}")
 
-    def _run(self, data, axis, bc_left, bc_right, want1, want2):
+    def _run(self, data, axis, bc_left, bc_right, want1, want2, apply_metric=True):
         if bc_left is not None or bc_right is not None:
             raise NotImplementedError(
                 "bc_left/bc_right overrides (testing only in the reference, "
@@ -188,7 +238,7 @@ class CCDSolver:
             raise ValueError(f"data shape {tuple(f.shape)} != grid.shape {self.shape}")
         d1 = torch.empty_like(f) if want1 else None
         d2 = torch.empty_like(f) if want2 else None
-        self.differentiate_into(f, d1, d2, axis)
+        self.differentiate_into(f, d1, d2, axis, apply_metric=apply_metric)
         return (io.back(d1) if want1 else None, io.back(d2) if want2 else None)
 
 

@@ -20,7 +20,7 @@ class GridConfig:
     ndim: int = 2
     N: Tuple[int, ...] = (64, 64)
     L: Tuple[float, ...] = (1.0, 1.0)
-    alpha_grid: float = 1.0  # 1.0 = uniform; other values are not supported here
+    alpha_grid: float = 1.0  # <= 1: uniform; > 1: non-uniform (CCDSolver only)
 
     def __post_init__(self):
         if len(self.N) != self.ndim or len(self.L) != self.ndim:

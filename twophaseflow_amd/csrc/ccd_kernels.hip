@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -2051,7 +2052,82 @@ tpf_status launch_fused(tpf_ccd_plan* plan, const int* axes, int n, const double
   return TPF_OK;
 }
 
+// ---- grid metric of non-uniform axes (ccd_solver_helpers.py:509-519) ------
+// On a non-uniform axis the line solves run in the computational coordinate
+// xi (spacing 1/N) and the metric maps the results to x at node j of the
+// axis: d1 = J d1_xi, d2 = (J J) d2_xi + (J dJ) d1_xi, with J = dxi/dx and
+// dJ = dJ/dxi.  The products J J and J dJ are formed on the host, as the
+// reference forms them elementwise.  In place on d1 / d2; xi1 holds d1_xi
+// (d1 itself, or scratch when only d2 is wanted).  One read and one write of
+// each output per node: the uniform path never runs it.
+__global__ __launch_bounds__(256) void k_ccd_metric(int64_t total, int64_t stride, int n,
+                                                    const double* __restrict__ m,
+                                                    const double* xi1, double* d1, double* d2) {
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += step) {
+    const int j = (int)((i / stride) % n);
+    const double a = xi1[i];
+    if (d2) d2[i] = m[n + j] * d2[i] + m[2 * (int64_t)n + j] * a;
+    if (d1) d1[i] = m[j] * a;
+  }
+}
+
+// The line solves of `axes` (fused launch policy when `fused`), then the
+// metric of every non-uniform one.  A d2-only slot of a non-uniform axis
+// solves d1_xi into the plan's scratch (grown on demand; calls of one plan
+// on different streams that use it race).
+tpf_status run_axes(tpf_ccd_plan* plan, bool fused, const int* axes, int n,
+                    const double* const* f, double* const* d1, double* const* d2,
+                    hipStream_t stream) {
+  int64_t total = 1;
+  for (int a = 0; a < plan->ndim; ++a) total *= plan->n_nodes[a];
+  double* x1[3];
+  int need = 0;
+  bool any = false;
+  for (int i = 0; i < n; ++i) {
+    x1[i] = d1[i];
+    if (plan->d_metric[axes[i]] && (d1[i] || d2[i])) {
+      any = true;
+      if (!d1[i]) ++need;
+    }
+  }
+  if (need) {
+    if (plan->scratch_n < need * total) {
+      if (plan->scratch) TPF_HIP(hipFree(plan->scratch));
+      plan->scratch = nullptr;
+      plan->scratch_n = 0;
+      TPF_HIP(hipMalloc(&plan->scratch, sizeof(double) * (size_t)(need * total)));
+      plan->scratch_n = need * total;
+    }
+    int k = 0;
+    for (int i = 0; i < n; ++i)
+      if (plan->d_metric[axes[i]] && !d1[i] && d2[i]) x1[i] = plan->scratch + (k++) * total;
+  }
+  if (fused) TPF_TRY(launch_fused(plan, axes, n, f, x1, d2, stream));
+  else TPF_TRY(launch_axes(plan, axes, n, f, x1, d2, stream));
+  if (!any) return TPF_OK;
+  const int blocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
+  for (int i = 0; i < n; ++i) {
+    const int a = axes[i];
+    if (!plan->d_metric[a] || !(d1[i] || d2[i])) continue;
+    int64_t stride = 1;
+    for (int b = a + 1; b < plan->ndim; ++b) stride *= plan->n_nodes[b];
+    hipLaunchKernelGGL(k_ccd_metric, dim3(blocks), dim3(256), 0, stream, total, stride,
+                       (int)plan->n_nodes[a], (const double*)plan->d_metric[a],
+                       (const double*)x1[i], d1[i], d2[i]);
+    TPF_HIP(hipGetLastError());
+  }
+  return TPF_OK;
+}
+
 }  // namespace
+
+bool ccd_plan_has_metric(const tpf_ccd_plan* plan) {
+  if (!plan) return false;
+  for (int a = 0; a < 3; ++a)
+    if (plan->d_metric[a]) return true;
+  return false;
+}
 
 }  // namespace tpf
 
@@ -2211,7 +2287,35 @@ extern "C" tpf_status tpf_ccd_plan_destroy(tpf_ccd_plan* plan) {
     if (ax.d_qtab32) (void)hipFree(ax.d_qtab32);
     if (ax.d_ctab) (void)hipFree(ax.d_ctab);
   }
+  for (double* m : plan->d_metric)
+    if (m) (void)hipFree(m);
+  if (plan->scratch) (void)hipFree(plan->scratch);
   delete plan;
+  return TPF_OK;
+}
+
+extern "C" tpf_status tpf_ccd_plan_set_metric(tpf_ccd_plan* plan, int axis, const double* J,
+                                              const double* dJ) {
+  if (!plan || axis < 0 || axis >= plan->ndim) return fail(TPF_ERR_VALUE, "bad plan/axis");
+  if (!J != !dJ) return fail(TPF_ERR_VALUE, "J and dJ must both be given or both be NULL");
+  if (!J) {
+    if (plan->d_metric[axis]) TPF_HIP(hipFree(plan->d_metric[axis]));
+    plan->d_metric[axis] = nullptr;
+    return TPF_OK;
+  }
+  const int64_t n = plan->n_nodes[axis];
+  std::vector<double> m((size_t)(3 * n));
+  for (int64_t j = 0; j < n; ++j) {
+    if (!std::isfinite(J[j]) || !std::isfinite(dJ[j]))
+      return fail(TPF_ERR_VALUE, "grid metric must be finite");
+    m[(size_t)j] = J[j];
+    m[(size_t)(n + j)] = J[j] * J[j];
+    m[(size_t)(2 * n + j)] = J[j] * dJ[j];
+  }
+  if (!plan->d_metric[axis])
+    TPF_HIP(hipMalloc(&plan->d_metric[axis], sizeof(double) * (size_t)(3 * n)));
+  TPF_HIP(hipMemcpy(plan->d_metric[axis], m.data(), sizeof(double) * m.size(),
+                    hipMemcpyHostToDevice));
   return TPF_OK;
 }
 
@@ -2223,7 +2327,7 @@ extern "C" tpf_status tpf_ccd_differentiate(tpf_ccd_plan* plan, const double* f,
   if ((const double*)d1 == f || (const double*)d2 == f || (d1 && d1 == d2))
     return fail(TPF_ERR_VALUE, "CCD outputs must not alias the input or each other");
   TPF_TRY(ensure_axis(plan, axis));
-  return launch_axes(plan, &axis, 1, &f, &d1, &d2, (hipStream_t)stream);
+  return run_axes(plan, false, &axis, 1, &f, &d1, &d2, (hipStream_t)stream);
 }
 
 extern "C" tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double* d1, double* d2,
@@ -2246,7 +2350,7 @@ extern "C" tpf_status tpf_ccd_apply(tpf_ccd_plan* plan, const double* f, double*
     axes[n++] = a;
   }
   if (n == 0 || (!d1 && !d2)) return TPF_OK;
-  return launch_fused(plan, axes, n, fs, o1, o2, (hipStream_t)stream);
+  return run_axes(plan, true, axes, n, fs, o1, o2, (hipStream_t)stream);
 }
 
 extern "C" tpf_status tpf_ccd_apply_fields(tpf_ccd_plan* plan, const double* const* f,
@@ -2282,7 +2386,7 @@ extern "C" tpf_status tpf_ccd_apply_fields(tpf_ccd_plan* plan, const double* con
   bool any = false;
   for (int i = 0; i < n; ++i) any = any || o1[i] || o2[i];
   if (n == 0 || !any) return TPF_OK;
-  return launch_fused(plan, axes, n, fs, o1, o2, (hipStream_t)stream);
+  return run_axes(plan, true, axes, n, fs, o1, o2, (hipStream_t)stream);
 }
 
 extern "C" tpf_status tpf_ccd_axis_info(tpf_ccd_plan* plan, int axis, int* partitions,
@@ -2414,6 +2518,11 @@ extern "C" tpf_status tpf_ccd_batch_create(tpf_ccd_batch** out, tpf_ccd_plan* co
     if (!pl || (axis_masks[j] >> pl->ndim)) {
       delete b;
       return fail(TPF_ERR_VALUE, "batch entry " + std::to_string(j) + ": bad plan or axis mask");
+    }
+    if (tpf::ccd_plan_has_metric(pl)) {
+      delete b;
+      return fail(TPF_ERR_VALUE, "batches take uniform-grid plans (entry " + std::to_string(j) +
+                                     " has a grid metric)");
     }
     if (pl->precision != TPF_PREC_FP64) {
       delete b;

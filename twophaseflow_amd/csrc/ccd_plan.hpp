@@ -55,4 +55,9 @@ struct tpf_ccd_plan {
   int mode[3] = {0, 0, 0};      // per-axis kernel: 0 auto, 1 lane per line, 2 wavefront per line
   int store_align[3] = {-1, -1, -1};  // explicit kernel's store chunk (elements; -1 default 8)
   tpf::CcdAxisHost axes[3];
+  // non-uniform axes: device [3][n_nodes[a]] = J, J*J, J*dJ (J = dxi/dx,
+  // dJ = dJ/dxi), applied after the line solves; nullptr: no metric
+  double* d_metric[3] = {nullptr, nullptr, nullptr};
+  double* scratch = nullptr;  // d1 in xi of d2-only calls on non-uniform axes
+  int64_t scratch_n = 0;
 };

@@ -106,6 +106,8 @@ extern "C" tpf_status tpf_ns_extend_2d(tpf_ccd_plan* plan, const int64_t* n_node
                                        double* out, double* work, void* stream) {
   if (!plan || !n_nodes || !x || !y || !q || !phi || !nx || !ny || !out || !work)
     return fail(TPF_ERR_VALUE, "null argument");
+  if (tpf::ccd_plan_has_metric(plan))
+    return tpf::fail(TPF_ERR_VALUE, "uniform grids only (plan has a metric)");
   if (n_nodes[0] < 2 || n_nodes[1] < 2) return fail(TPF_ERR_VALUE, "axes need >= 2 nodes");
   if (out == q || out == phi) return fail(TPF_ERR_VALUE, "out must not alias q or phi");
   const int64_t n = n_nodes[0] * n_nodes[1];

@@ -366,6 +366,7 @@ extern "C" tpf_status tpf_ns_uccd6_convection(tpf_ccd_plan* plan, int ndim,
   NGeom g;
   TPF_TRY(make_ngeom(g, ndim, n_nodes, nullptr));
   if (!plan || !L || !vel || !out || !work) return fail(TPF_ERR_VALUE, "null argument");
+  if (ccd_plan_has_metric(plan)) return fail(TPF_ERR_VALUE, "uniform grids only (plan has a metric)");
   for (int a = 0; a < ndim; ++a)
     if (!vel[a] || !out[a]) return fail(TPF_ERR_VALUE, "null component");
   hipStream_t st = (hipStream_t)stream;
@@ -434,6 +435,7 @@ extern "C" tpf_status tpf_ns_viscous(tpf_ccd_plan* plan, int ndim, const int64_t
   TPF_TRY(make_ngeom(g, ndim, n_nodes, nullptr));
   if (!plan || !L || !vel || !mu || !rho || !out || !work)
     return fail(TPF_ERR_VALUE, "null argument");
+  if (ccd_plan_has_metric(plan)) return fail(TPF_ERR_VALUE, "uniform grids only (plan has a metric)");
   const int64_t n = g.total;
   const uint32_t all = (1u << ndim) - 1u;
   Ptr3 G{};

@@ -379,6 +379,7 @@ extern "C" tpf_status tpf_reinit_split(tpf_ccd_plan* ccd, int ndim, const int64_
                                        double eps_d_comp, int mass_correction,
                                        const double* psi, double* out, void* stream) {
   if (!ccd || !n_nodes || !L || !bc || !psi || !out) return fail(TPF_ERR_VALUE, "null argument");
+  if (ccd_plan_has_metric(ccd)) return fail(TPF_ERR_VALUE, "uniform grids only (plan has a metric)");
   if (ndim < 1 || ndim > 3) return fail(TPF_ERR_VALUE, "ndim must be 1, 2 or 3");
   if (!(eps > 0.0) || n_steps < 0) return fail(TPF_ERR_VALUE, "need eps > 0 and n_steps >= 0");
   if (out == psi) return fail(TPF_ERR_VALUE, "psi and out must be distinct arrays");

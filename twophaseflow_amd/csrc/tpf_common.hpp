@@ -18,6 +18,9 @@ inline tpf_status fail(tpf_status st, const std::string& msg) {
   return st;
 }
 
+// Whether a CCD plan carries a grid metric (a non-uniform axis).
+bool ccd_plan_has_metric(const tpf_ccd_plan* plan);
+
 }  // namespace tpf
 
 #define TPF_HIP(call)                                                        \

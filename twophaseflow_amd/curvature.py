@@ -19,6 +19,7 @@ import torch
 from . import _native as nat
 from .boundary import axis_kinds
 from .device import Handoff, ptr, require_gpu, stream_handle
+from .grid import require_uniform
 
 DCCD_EPS_D = 0.05  # levelset/curvature.py:43
 
@@ -26,6 +27,7 @@ DCCD_EPS_D = 0.05  # levelset/curvature.py:43
 class CurvatureCalculatorPsi:
     def __init__(self, backend, ccd, psi_min: float = 0.01, gaussian_filter: bool = False,
                  dccd_eps: float = DCCD_EPS_D):
+        require_uniform(getattr(ccd, "grid", None), "CurvatureCalculatorPsi")
         if not hasattr(ccd, "apply_into"):
             raise TypeError("CurvatureCalculatorPsi needs this package's CCDSolver")
         self.backend = backend

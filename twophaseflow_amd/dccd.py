@@ -21,6 +21,7 @@ import torch
 
 from . import _native as nat
 from .device import Handoff, ptr, require_gpu, stream_handle
+from .grid import require_uniform
 
 EPS_D_ADV = 0.05  # levelset/advection_kernels.py:9
 
@@ -33,6 +34,7 @@ class DissipativeCCDAdvection:
 
     def __init__(self, backend, grid, ccd, bc="periodic", eps_d: float = EPS_D_ADV,
                  mass_correction: bool = False):
+        require_uniform(grid, "DissipativeCCDAdvection")
         if not getattr(grid, "uniform", True):
             raise ValueError("this build's DissipativeCCDAdvection needs a uniform grid")
         if not hasattr(ccd, "differentiate_into"):

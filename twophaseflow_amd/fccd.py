@@ -26,6 +26,7 @@ from . import _native as nat
 from .boundary import axis_kinds
 from .ccd import CCDSolver
 from .device import Handoff, ptr, require_gpu, stream_handle
+from .grid import require_uniform
 
 
 @dataclass(frozen=True)
@@ -45,6 +46,7 @@ class FCCDSolver:
     """Face-centred compact-difference differentiator (ccd/fccd.py:138-195)."""
 
     def __init__(self, grid, backend=None, bc_type: str = "wall", ccd_solver=None):
+        require_uniform(grid, "FCCDSolver")
         self.grid = grid
         self.backend = backend
         self.ndim = grid.ndim

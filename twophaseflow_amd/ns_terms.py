@@ -37,6 +37,7 @@ import torch
 from . import _native as nat
 from .boundary import axis_kinds
 from .device import Handoff, ptr, require_gpu, stream_handle
+from .grid import require_uniform
 
 
 def _empty_like_list(t: torch.Tensor, k: int) -> List[torch.Tensor]:
@@ -90,6 +91,7 @@ class UCCD6ConvectionTerm:
     scheme_names = ("uccd6",)
 
     def __init__(self, backend, grid, ccd, sigma: float = 1.0e-3):
+        require_uniform(grid, "UCCD6ConvectionTerm")
         if sigma <= 0.0:
             raise ValueError(f"sigma must be > 0, got {sigma}")
         self._ccd, self._grid, self._sigma = ccd, grid, float(sigma)
@@ -178,6 +180,7 @@ class Predictor:
 
     def __init__(self, backend, config, ccd, convection=None, viscous=None, gravity=None,
                  surface_tension=None, use_gfm: bool = False):
+        require_uniform(getattr(ccd, "grid", None), "Predictor")
         if use_gfm:
             raise NotImplementedError("surface_tension_model='gfm' is not built")
         num, fl = config.numerics, config.fluid
@@ -259,6 +262,7 @@ class CFLCalculator:
 
     def __init__(self, backend, grid, cfl_number: float, We: Optional[float] = None,
                  rho_ratio: float = 0.0, cn_viscous: bool = True):
+        require_uniform(grid, "CFLCalculator")
         self.grid = grid
         self.cfl = float(cfl_number)
         self._h_min = min(float(grid.L[a] / grid.N[a]) for a in range(grid.ndim))
@@ -340,6 +344,7 @@ class ClosestPointExtender:
     is_null_extender = False
 
     def __init__(self, backend, grid, ccd):
+        require_uniform(grid, "ClosestPointExtender")
         self.grid, self.ccd, self.ndim = grid, ccd, grid.ndim
         self._coords = None
         self._work = _Work()
@@ -382,6 +387,7 @@ class VelocityCorrector:
     (coupling/velocity_corrector.py:52-89); returns new arrays."""
 
     def __init__(self, backend, grid, ccd):
+        require_uniform(grid, "VelocityCorrector")
         self.grid, self.ccd = grid, ccd
 
     def correct(self, vel_star, p, rho, dt):

@@ -48,6 +48,7 @@ import torch
 from . import _native as nat
 from .boundary import BoundarySpec, axis_kinds, pin_dof
 from .device import Handoff, ptr, require_gpu, stream_handle
+from .grid import require_uniform
 
 
 class MatrixAssemblyUnavailable(RuntimeError):
@@ -83,6 +84,7 @@ class PPESolverFVMMatrixFree(IPPESolver):
     _scheme_aliases = {"fvm_matrixfree": "fvm_iterative", "matrixfree": "fvm_iterative"}
 
     def __init__(self, backend, config, grid, bc_type: str = "wall", bc_spec=None):
+        require_uniform(grid, "PPESolverFVMMatrixFree")
         self.backend = backend
         self.xp = np
         self.grid = grid
@@ -456,6 +458,7 @@ class PPESolverFCCDMatrixFree(IPPESolver):
     _scheme_aliases = {"fccd_matrixfree": "fccd_iterative", "fccd": "fccd_iterative"}
 
     def __init__(self, backend, config, grid, fccd=None, *, bc_spec=None, bc_type=None):
+        require_uniform(grid, "PPESolverFCCDMatrixFree")
         self.backend = backend
         self.xp = np
         self.grid = grid

@@ -14,6 +14,7 @@ import torch
 
 from . import _native as nat
 from .device import Handoff, ptr, require_gpu, stream_handle
+from .grid import require_uniform
 
 _BC = {"zero": 0, "periodic": 1, "neumann": 2, "outflow": 3}
 
@@ -27,6 +28,7 @@ def compute_dtau(grid, eps: float) -> float:
 class SplitReinitializer:
     def __init__(self, backend, grid, ccd, eps: float, n_steps: int = 4, bc="zero",
                  eps_d_comp: float = 0.05, mass_correction: bool = True):
+        require_uniform(grid, "SplitReinitializer")
         if not hasattr(ccd, "_plan"):
             raise TypeError("SplitReinitializer needs this package's CCDSolver")
         if not getattr(grid, "uniform", True):

@@ -18,12 +18,14 @@ import torch
 from . import _native as nat
 from .boundary import axis_kinds
 from .device import Handoff, ptr, require_gpu, stream_handle
+from .grid import require_uniform
 
 
 class RhieChowInterpolator:
     """Face velocities and Rhie-Chow divergence (spatial/rhie_chow.py:40-60)."""
 
     def __init__(self, backend, grid, ccd, bc_type: str = "wall"):
+        require_uniform(grid, "RhieChowInterpolator")
         self.backend = backend
         self.grid = grid
         self.ndim = grid.ndim

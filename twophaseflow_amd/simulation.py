@@ -32,7 +32,7 @@ from .ccd import CCDSolver
 from .curvature import CurvatureCalculatorPsi
 from .dccd import DissipativeCCDAdvection
 from .device import ptr, require_gpu, stream_handle
-from .grid import Grid
+from .grid import Grid, require_uniform
 from .ns_terms import (BoundaryConditionHandler, CFLCalculator, ClosestPointExtender, FlowState,
                        NullFieldExtender, Predictor, VelocityCorrector)
 from .ppe import create_ppe_solver
@@ -92,6 +92,7 @@ class DiagnosticsReporter:
     vol(psi) = sum psi * prod(L/N), printed."""
 
     def __init__(self, backend, grid):
+        require_uniform(grid, "DiagnosticsReporter")
         self.grid = grid
 
     def divergence_max(self, sim) -> float:
