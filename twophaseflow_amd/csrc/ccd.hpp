@@ -32,6 +32,9 @@ constexpr double kB2 = -9.0 / 8.0;
 struct CcdAxisGeom {
   const double* rowtab;  // [P][M][kRowTab]
   const double* qtab;    // [P][4][NQ*4]: rows (left0,left1,right0,right1)
+  const double* qtab_t;  // [4][NQ*4][P]: qtab with the partition fastest (wavefront
+                         // kernel, lane = partition: one wavefront load covers 64
+                         // consecutive doubles instead of 64 cache lines)
   const float* rowtab32; // FP32 copies (FP32-refined plans only)
   const float* qtab32;
   const double* ctab;    // [types][2][M][M+2]: partition inverse on first differences (explicit mode)

@@ -780,14 +780,39 @@ __device__ __forceinline__ void wave_tile_io(const CcdLines& ln, int nax, int pi
       }
     }
   } else {  // strided axis: consecutive threads take consecutive lines of one node
-    for (int e = threadIdx.x; e < nlw * nax; e += blockDim.x) {
-      const int j = e / nlw, l = e - j * nlw;
-      if (l >= nl) continue;
+    // The workgroup size is a multiple of nlw (64 / LPW threads per line
+    // slot), so a thread keeps its line l and advances its node j by
+    // blockDim.x / nlw: the line's base address is formed once, and batches
+    // of kB elements per thread issue all global accesses before any LDS
+    // access (one load in flight per thread serialised the tile on HBM
+    // latency: 1025^2 strided axis 0.039 ms).
+    constexpr int kB = 8;
+    const int bd = (int)blockDim.x;
+    const int l = (int)threadIdx.x % nlw;
+    const int jstep = bd / nlw;
+    const bool lok = l < nl;
+    int64_t base = 0;
+    if (lok) {
       const uint32_t bi = (uint32_t)(line0 + l), in = (uint32_t)ln.inner;
-      const int64_t a = (int64_t)(bi / in) * ln.outer + (int64_t)(bi % in) * ln.inner_stride +
-                        (int64_t)j * ln.stride;
-      if (LOAD) tile[tix(pitch, l, j)] = src[a];
-      else __builtin_nontemporal_store(tile[tix(pitch, l, j)], dst + a);
+      base = (int64_t)(bi / in) * ln.outer + (int64_t)(bi % in) * ln.inner_stride;
+    }
+    for (int j = (int)threadIdx.x / nlw; j < nax; j += kB * jstep) {
+      double v[kB];
+#pragma unroll
+      for (int i = 0; i < kB; ++i) {
+        const int jj = j + i * jstep;
+        const bool ok = lok && jj < nax;
+        if (LOAD) v[i] = ok ? src[base + (int64_t)jj * ln.stride] : 0.0;
+        else v[i] = ok ? tile[tix(pitch, l, jj)] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < kB; ++i) {
+        const int jj = j + i * jstep;
+        if (lok && jj < nax) {
+          if (LOAD) tile[tix(pitch, l, jj)] = v[i];
+          else __builtin_nontemporal_store(v[i], dst + base + (int64_t)jj * ln.stride);
+        }
+      }
     }
   }
 }
@@ -864,14 +889,23 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
   }
   __syncthreads();  // tile consumed; its space holds the exchange buffer below
   double* Gw = lds + wv * 4 * 64;
-  // reduced-system rows of all partitions -> LDS after the exchange buffer,
+  // reduced-system rows of all partitions (partition fastest, so a lane per
+  // partition reads consecutive doubles) -> LDS after the exchange buffer,
   // when they fit the tile space (the host sized the tile for it)
   const int nq = g.P * 16 * g.NQ;
   const bool qlds = !UNI && nwaves * 4 * 64 + nq <= nlw * wave_pitch(nax);
-  const double* qsrc = g.qtab;
-  if (qlds) {
+  const double* qsrc = g.qtab_t;
+  if (qlds) {  // batches of 8 loads per thread before their LDS stores
     double* qd = lds + nwaves * 4 * 64;
-    for (int i = threadIdx.x; i < nq; i += blockDim.x) qd[i] = g.qtab[i];
+    const int bd = (int)blockDim.x;
+    for (int i0 = threadIdx.x; i0 < nq; i0 += 8 * bd) {
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = i0 + k * bd < nq ? g.qtab_t[i0 + k * bd] : 0.0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (i0 + k * bd < nq) qd[i0 + k * bd] = v[k];
+    }
     qsrc = qd;  // visible after the barrier that precedes the coupling
   }
   // UNI: every partition's rows are partition 0's, read wavefront-uniform
@@ -926,21 +960,24 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
   double lr0 = 0.0, lr1 = 0.0, lr2 = 0.0, lr3 = 0.0;
   {
     const int NQ = g.NQ;
-    const TabPtr q = UNI ? (TabPtr)g.qtab : (TabPtr)(qsrc + (size_t)pp * 4 * NQ * 4);
+    // entry (c, t, k) of this partition's rows: partition 0's [c][NQ*4]
+    // block when UNI, else the transposed table [c][NQ*4][P]
+    const int qs = UNI ? 1 : g.P;
+    const TabPtr q = UNI ? (TabPtr)g.qtab : (TabPtr)(qsrc + pp);
     for (int t = 0; t < NQ; ++t) {
       int qq = g.rel ? pp + g.d0 + t : t;
       if (periodic) qq = qq < 0 ? qq + g.P : (qq >= g.P ? qq - g.P : qq);
       else qq = qq < 0 ? 0 : (qq >= g.P ? g.P - 1 : qq);
       const double G0 = Gw[0 * 64 + lbase + qq], G1 = Gw[1 * 64 + lbase + qq];
       const double G2 = Gw[2 * 64 + lbase + qq], G3 = Gw[3 * 64 + lbase + qq];
-      const TabPtr q0 = q + (0 * NQ + t) * 4;
-      const TabPtr q1 = q + (1 * NQ + t) * 4;
-      const TabPtr q2 = q + (2 * NQ + t) * 4;
-      const TabPtr q3 = q + (3 * NQ + t) * 4;
-      lr0 += q0[0] * G0 + q0[1] * G1 + q0[2] * G2 + q0[3] * G3;
-      lr1 += q1[0] * G0 + q1[1] * G1 + q1[2] * G2 + q1[3] * G3;
-      lr2 += q2[0] * G0 + q2[1] * G1 + q2[2] * G2 + q2[3] * G3;
-      lr3 += q3[0] * G0 + q3[1] * G1 + q3[2] * G2 + q3[3] * G3;
+      const TabPtr q0 = q + (0 * NQ + t) * 4 * qs;
+      const TabPtr q1 = q + (1 * NQ + t) * 4 * qs;
+      const TabPtr q2 = q + (2 * NQ + t) * 4 * qs;
+      const TabPtr q3 = q + (3 * NQ + t) * 4 * qs;
+      lr0 += q0[0] * G0 + q0[qs] * G1 + q0[2 * qs] * G2 + q0[3 * qs] * G3;
+      lr1 += q1[0] * G0 + q1[qs] * G1 + q1[2 * qs] * G2 + q1[3 * qs] * G3;
+      lr2 += q2[0] * G0 + q2[qs] * G1 + q2[2 * qs] * G2 + q2[3 * qs] * G3;
+      lr3 += q3[0] * G0 + q3[qs] * G1 + q3[2 * qs] * G2 + q3[3 * qs] * G3;
     }
   }
   double xl0 = 0.0, xl1 = 0.0;
@@ -2225,14 +2262,16 @@ tpf_status build_axis(tpf_ccd_plan* plan, int axis, CcdAxisHost& ax) {
   }
   // Wavefront kernel shape: as many lines per wavefront as its partitions
   // leave lanes for, four wavefronts per workgroup.
-  // One line per wavefront on >= 1024 lines (1025^2): 16 wavefronts per
-  // workgroup measured faster than 8 or 4 (0.096 against 0.110 / 0.099 ms,
-  // wall); fewer lines keep 4 so that more CUs take part.
+  // One line per wavefront on >= 1024 strided lines (1025^2 axis 0): 16
+  // wavefronts per workgroup, so each node's tile row is 16 lines = 128 B
+  // (8 and 16 measured alike, 4 at 32 B rows 1.3x slower); contiguous lines
+  // keep 4 so that four times as many CUs take part (1025^2 axis 1: 0.021
+  // against 0.027 ms periodic, 0.035 against 0.048 ms wall).
   if (wave) {
     ax.lpw = ax.geom.P <= 16 ? 4 : (ax.geom.P <= 32 ? 2 : 1);
     // FP32-refined: 4 wavefronts (the tile stays live next to 4 KB of exchange
     // buffers per wavefront: <= 51 KB at 1025 nodes)
-    ax.wpg = (ax.lpw == 1 && !ref && lines >= 1024) ? 16 : 4;
+    ax.wpg = (ax.lpw == 1 && !ref && lines >= 1024 && strided) ? 16 : 4;
   }
   // Auto: strided FP64 axes of <= 16 partitions x 16 rows take the explicit
   // partition inverse (no recurrence state: two workgroups per CU).
@@ -2258,6 +2297,15 @@ tpf_status ensure_axis(tpf_ccd_plan* plan, int axis) {
   TPF_HIP(hipMemcpy(ax.d_rowtab_u, ax.rowtab_u.data(), ax.rowtab_u.size() * sizeof(double),
                     hipMemcpyHostToDevice));
   ax.geom.rowtab_u = ax.d_rowtab_u;
+  if (ax.wave == 1 && !ax.geom.uniform) {
+    const int P = ax.geom.P, W = 4 * ax.geom.NQ * 4;
+    std::vector<double> qt((size_t)P * W);
+    for (int p = 0; p < P; ++p)
+      for (int i = 0; i < W; ++i) qt[(size_t)i * P + p] = ax.qtab[(size_t)p * W + i];
+    TPF_HIP(hipMalloc(&ax.d_qtab_t, qt.size() * sizeof(double)));
+    TPF_HIP(hipMemcpy(ax.d_qtab_t, qt.data(), qt.size() * sizeof(double), hipMemcpyHostToDevice));
+    ax.geom.qtab_t = ax.d_qtab_t;
+  }
   if (ax.wave == 2) {
     TPF_HIP(hipMalloc(&ax.d_ctab, ax.ctab.size() * sizeof(double)));
     TPF_HIP(hipMemcpy(ax.d_ctab, ax.ctab.data(), ax.ctab.size() * sizeof(double),
@@ -2283,6 +2331,7 @@ extern "C" tpf_status tpf_ccd_plan_destroy(tpf_ccd_plan* plan) {
     if (ax.d_rowtab) (void)hipFree(ax.d_rowtab);
     if (ax.d_rowtab_u) (void)hipFree(ax.d_rowtab_u);
     if (ax.d_qtab) (void)hipFree(ax.d_qtab);
+    if (ax.d_qtab_t) (void)hipFree(ax.d_qtab_t);
     if (ax.d_rowtab32) (void)hipFree(ax.d_rowtab32);
     if (ax.d_qtab32) (void)hipFree(ax.d_qtab32);
     if (ax.d_ctab) (void)hipFree(ax.d_ctab);
@@ -2466,6 +2515,7 @@ extern "C" tpf_status tpf_ccd_plan_set_launch(tpf_ccd_plan* plan, int axis, int 
   if (ax.d_rowtab) TPF_HIP(hipFree(ax.d_rowtab));
   if (ax.d_rowtab_u) TPF_HIP(hipFree(ax.d_rowtab_u));
   if (ax.d_qtab) TPF_HIP(hipFree(ax.d_qtab));
+  if (ax.d_qtab_t) TPF_HIP(hipFree(ax.d_qtab_t));
   if (ax.d_rowtab32) TPF_HIP(hipFree(ax.d_rowtab32));
   if (ax.d_qtab32) TPF_HIP(hipFree(ax.d_qtab32));
   if (ax.d_ctab) TPF_HIP(hipFree(ax.d_ctab));

@@ -22,6 +22,7 @@ struct CcdAxisHost {
   double trunc_coeff = 0.0;     // largest dropped reduced-system coefficient (relative)
   double* d_rowtab = nullptr;
   double* d_qtab = nullptr;
+  double* d_qtab_t = nullptr;   // wavefront kernel on non-uniform axes: qtab transposed
   float* d_rowtab32 = nullptr;  // FP32-refined plans
   float* d_qtab32 = nullptr;
   double* d_ctab = nullptr;
