@@ -60,6 +60,23 @@ struct CcdLaunch {
 
 constexpr int kTilePer = 20;  // tile doubles per thread when staging a contiguous-axis tile
 
+// Copies n table values to LDS, dst[map(i)] = src[i]: each thread issues 8
+// global loads before their LDS stores, so a staging loop waits on one
+// round trip per 8 values instead of one per value.
+template <typename T, typename Map>
+__device__ __forceinline__ void stage_table(T* dst, const T* __restrict__ src, int n, Map map) {
+  const int bd = (int)blockDim.x;
+  for (int i0 = (int)threadIdx.x; i0 < n; i0 += 8 * bd) {
+    T v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = i0 + k * bd < n ? src[i0 + k * bd] : T(0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (i0 + k * bd < n) dst[map(i0 + k * bd)] = v[k];
+  }
+}
+__device__ __forceinline__ int same_index(int i) { return i; }
+
 template <int M>
 __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& ln,
                                           const bool CONTIG, const bool TILED,
@@ -146,8 +163,8 @@ __device__ __forceinline__ void ccd_lines(const CcdAxisGeom& g, const CcdLines& 
   double* G = lq + (kLdsTab ? np * 16 * NQ : 0);  // [4P][64]
   if (kLdsTab) {
     const int nt = np * M * kRowTab, nq = np * 16 * NQ;
-    for (int i = threadIdx.x; i < nt; i += blockDim.x) ltab[i] = g.rowtab[i];
-    for (int i = threadIdx.x; i < nq; i += blockDim.x) lq[i] = g.qtab[i];
+    stage_table(ltab, g.rowtab, nt, same_index);
+    stage_table(lq, g.qtab, nq, same_index);
     __syncthreads();
   }
 
@@ -505,8 +522,8 @@ __device__ __forceinline__ void ccd_lines_ref(const CcdAxisGeom& g, const CcdLin
   float* E = G + 4 * g.P * 64;
   {
     const int nt = g.P * M * kRowTab, nq = g.P * 16 * g.NQ;
-    for (int i = threadIdx.x; i < nt; i += blockDim.x) ltab[i] = g.rowtab32[i];
-    for (int i = threadIdx.x; i < nq; i += blockDim.x) lq[i] = g.qtab32[i];
+    stage_table(ltab, g.rowtab32, nt, same_index);
+    stage_table(lq, g.qtab32, nq, same_index);
     __syncthreads();
   }
   double cl0 = 0.0, cl1 = 0.0, bl0 = 0.0, bl1 = 0.0;
@@ -852,8 +869,9 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
   const bool ltab = !UNI && wave_tab_lds(g);
   double* tabl = lds + wave_region(nlw, pitch, nwaves);
   if (ltab)
-    for (int i = threadIdx.x; i < g.ntypes * M * kRowTab; i += blockDim.x)
-      tabl[(i / (M * kRowTab)) * wave_tab_stride(M) + i % (M * kRowTab)] = g.rowtab_u[i];
+    stage_table(tabl, g.rowtab_u, g.ntypes * M * kRowTab, [](int i) {
+      return (i / (M * kRowTab)) * wave_tab_stride(M) + i % (M * kRowTab);
+    });
   __syncthreads();
   const int lw = w < nl ? w : 0;  // rows of inactive waves read line 0 (never stored)
   double fv[M + 2];
@@ -895,17 +913,9 @@ __device__ __forceinline__ void ccd_wave_tile(const CcdAxisGeom& g, const CcdLin
   const int nq = g.P * 16 * g.NQ;
   const bool qlds = !UNI && nwaves * 4 * 64 + nq <= nlw * wave_pitch(nax);
   const double* qsrc = g.qtab_t;
-  if (qlds) {  // batches of 8 loads per thread before their LDS stores
+  if (qlds) {
     double* qd = lds + nwaves * 4 * 64;
-    const int bd = (int)blockDim.x;
-    for (int i0 = threadIdx.x; i0 < nq; i0 += 8 * bd) {
-      double v[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) v[k] = i0 + k * bd < nq ? g.qtab_t[i0 + k * bd] : 0.0;
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (i0 + k * bd < nq) qd[i0 + k * bd] = v[k];
-    }
+    stage_table(qd, g.qtab_t, nq, same_index);
     qsrc = qd;  // visible after the barrier that precedes the coupling
   }
   // UNI: every partition's rows are partition 0's, read wavefront-uniform
