@@ -543,13 +543,14 @@ __global__ __launch_bounds__(1024) void k_two_level(
   __shared__ float xcs[kCoarseMax];
   __shared__ float bcs[kCoarseMax];
   const int tid = threadIdx.x;
+  const int bd = (int)blockDim.x;  // >= coarse nodes, >= fine nodes / kTwoLevelPer
   // fine-level coefficients of this thread's nodes
   float thi[kTwoLevelPer][3], tlo[kTwoLevelPer][3], di[kTwoLevelPer], bi[kTwoLevelPer];
   int cf[kTwoLevelPer][3], colf[kTwoLevelPer];
   const float* const Tf[3] = {Tf0, Tf1, Tf2};
 #pragma unroll
   for (int k = 0; k < kTwoLevelPer; ++k) {
-    const int64_t idx = tid + (int64_t)k * 1024;
+    const int64_t idx = tid + (int64_t)k * bd;
     colf[k] = -1;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -597,7 +598,7 @@ __global__ __launch_bounds__(1024) void k_two_level(
         const int color = reverse ? 1 - h : h;
 #pragma unroll
         for (int k = 0; k < kTwoLevelPer; ++k) {
-          const int64_t idx = tid + (int64_t)k * 1024;
+          const int64_t idx = tid + (int64_t)k * bd;
           if (colf[k] == color) {
             float d;
             const float s = nsum(k, idx, d);
@@ -611,7 +612,7 @@ __global__ __launch_bounds__(1024) void k_two_level(
   // residual (pin row 0)
 #pragma unroll
   for (int k = 0; k < kTwoLevelPer; ++k) {
-    const int64_t idx = tid + (int64_t)k * 1024;
+    const int64_t idx = tid + (int64_t)k * bd;
     if (colf[k] >= 0) {
       float d;
       const float s = nsum(k, idx, d);
@@ -663,14 +664,14 @@ __global__ __launch_bounds__(1024) void k_two_level(
   // prolongation (fine pin stays 0) and reverse post-smoothing
 #pragma unroll
   for (int k = 0; k < kTwoLevelPer; ++k) {
-    const int64_t idx = tid + (int64_t)k * 1024;
+    const int64_t idx = tid + (int64_t)k * bd;
     if (colf[k] >= 0 && idx != gf.pin) xs[idx] += prolong_node(gf, gc, xcs, idx);
   }
   __syncthreads();
   smooth_f(true);
 #pragma unroll
   for (int k = 0; k < kTwoLevelPer; ++k) {
-    const int64_t idx = tid + (int64_t)k * 1024;
+    const int64_t idx = tid + (int64_t)k * bd;
     if (colf[k] >= 0) xf[idx] = xs[idx];
   }
 }
@@ -789,7 +790,9 @@ tpf_status smooth(MgLevel& L, hipStream_t st, const float* b, float* x, int swee
 }
 
 tpf_status coarse_solve(MgLevel& L, hipStream_t st, int sweeps) {
-  hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(1024), 0, st, L.g, L.T[0], L.T[1], L.T[2],
+  // one node per thread, whole wavefronts (fewer wavefronts per barrier)
+  const int threads = (int)std::min<int64_t>(1024, (L.g.total + 63) / 64 * 64);
+  hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(threads), 0, st, L.g, L.T[0], L.T[1], L.T[2],
                      L.dinv, L.b, L.x, sweeps);
   TPF_HIP(hipGetLastError());
   return TPF_OK;
@@ -840,7 +843,12 @@ tpf_status cycle(MgState* m, hipStream_t st, int l, int last, int gamma, bool fi
   const int g_here = (w_levels <= 0 || l < w_levels) ? gamma : 1;
   if (l + 1 == last && g_here == 1 && !L.rho32 && nu_pre == nu_post &&
       L.g.total <= kTwoLevelMax && C.g.total <= kCoarseMax) {
-    hipLaunchKernelGGL(k_two_level, dim3(1), dim3(1024), 0, st, L.g, L.T[0], L.T[1], L.T[2],
+    // the fewest whole wavefronts that hold the coarse level at one node and
+    // the fine level at kTwoLevelPer nodes per thread: fewer wavefronts per
+    // barrier on 2-D hierarchies (coarsest 17^2)
+    const int64_t need = std::max<int64_t>(C.g.total, (L.g.total + kTwoLevelPer - 1) / kTwoLevelPer);
+    const int threads = (int)std::min<int64_t>(1024, (need + 63) / 64 * 64);
+    hipLaunchKernelGGL(k_two_level, dim3(1), dim3(threads), 0, st, L.g, L.T[0], L.T[1], L.T[2],
                        L.dinv, L.b, L.x, L.r, C.g, C.T[0], C.T[1], C.T[2], C.dinv, C.b, C.x,
                        nu_pre, coarse_sweeps, first ? 1 : 0);
     TPF_HIP(hipGetLastError());
